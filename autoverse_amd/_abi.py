@@ -1,0 +1,156 @@
+"""ctypes mirror of include/flowlenia_hip.h and the loader of the HIP library.
+
+The product library is ``autoverse_amd/libflowlenia_hip.so`` built in-tree by
+``autoverse_amd/build.py``.  There is no fallback: if the library is missing the
+loader raises, so nothing silently runs on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_int32, c_uint32, c_uint64, c_void_p
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libflowlenia_hip.so")
+
+# hp_status (flowlenia_hip.h); the first five mirror ConfigError (config.rs:169-180)
+HP_OK = 0
+HP_ERR_INVALID_DIMENSIONS = 1
+HP_ERR_INVALID_CHANNELS = 2
+HP_ERR_INVALID_TIME_STEP = 3
+HP_ERR_INVALID_KERNEL_RADIUS = 4
+HP_ERR_INVALID_CHANNEL_INDEX = 5
+HP_ERR_NO_DEVICE = 6
+HP_ERR_HIP = 7
+HP_ERR_SHAPE = 8
+HP_ERR_NULL = 9
+HP_ERR_NONFINITE = 10
+HP_ERR_UNSUPPORTED_SIZE = 11
+
+STATUS_NAMES = {
+    0: "HP_OK", 1: "HP_ERR_INVALID_DIMENSIONS", 2: "HP_ERR_INVALID_CHANNELS",
+    3: "HP_ERR_INVALID_TIME_STEP", 4: "HP_ERR_INVALID_KERNEL_RADIUS",
+    5: "HP_ERR_INVALID_CHANNEL_INDEX", 6: "HP_ERR_NO_DEVICE", 7: "HP_ERR_HIP", 8: "HP_ERR_SHAPE",
+    9: "HP_ERR_NULL", 10: "HP_ERR_NONFINITE", 11: "HP_ERR_UNSUPPORTED_SIZE",
+}
+
+# hp_pattern_kind
+PATTERN_KINDS = {
+    "GaussianBlob": 0, "MultiBlob": 1, "Ring": 2, "GaussianSphere": 3, "MultiSphere": 4,
+    "Shell": 5, "Torus3D": 6, "Noise": 7, "Custom": 8, "Custom3D": 9,
+}
+
+# hp_kernel_class
+KERNEL_CLASSES = ["rows_fwd", "cols_fwd", "spectral", "cols_inv", "rows_inv", "stencil", "general"]
+HP_K_NUM_CLASSES = len(KERNEL_CLASSES)
+
+
+class HpRing(ctypes.Structure):
+    _fields_ = [("amplitude", c_float), ("distance", c_float), ("width", c_float)]
+
+
+class HpKernel(ctypes.Structure):
+    _fields_ = [("radius", c_float), ("weight", c_float), ("mu", c_float), ("sigma", c_float),
+                ("source_channel", c_uint32), ("target_channel", c_uint32),
+                ("ring_offset", c_uint32), ("ring_count", c_uint32)]
+
+
+class HpFlow(ctypes.Structure):
+    _fields_ = [("beta_a", c_float), ("n", c_float), ("distribution_size", c_float)]
+
+
+class HpConfig(ctypes.Structure):
+    _fields_ = [("width", c_uint32), ("height", c_uint32), ("depth", c_uint32), ("channels", c_uint32),
+                ("dt", c_float), ("kernel_radius", c_uint32), ("flow", HpFlow),
+                ("num_kernels", c_uint32), ("kernels", POINTER(HpKernel)),
+                ("num_rings", c_uint32), ("rings", POINTER(HpRing))]
+
+
+class HpBlob(ctypes.Structure):
+    _fields_ = [("center", c_float * 3), ("radius", c_float), ("amplitude", c_float), ("channel", c_uint32)]
+
+
+class HpCustomValue(ctypes.Structure):
+    _fields_ = [("x", c_uint32), ("y", c_uint32), ("z", c_uint32), ("channel", c_uint32), ("value", c_float)]
+
+
+class HpSeed(ctypes.Structure):
+    _fields_ = [("kind", c_int32), ("center", c_float * 3), ("radius", c_float),
+                ("inner_radius", c_float), ("outer_radius", c_float),
+                ("major_radius", c_float), ("minor_radius", c_float), ("amplitude", c_float),
+                ("channel", c_int32), ("noise_seed", c_uint64),
+                ("num_blobs", c_uint32), ("blobs", POINTER(HpBlob)),
+                ("num_values", c_uint32), ("values", POINTER(HpCustomValue))]
+
+
+FloatPP = POINTER(POINTER(c_float))
+FP = POINTER(c_float)
+
+# (name, restype, argtypes) for every entry point declared in include/flowlenia_hip.h
+EXPORTS = [
+    ("hp_validate_config", c_int, [POINTER(HpConfig), POINTER(c_uint32), POINTER(c_uint32)]),
+    ("hp_seed_generate", c_int, [POINTER(HpSeed), c_uint32, c_uint32, c_uint32, c_uint32, FloatPP]),
+    ("hp_create", c_int, [POINTER(HpConfig), c_int, POINTER(c_void_p)]),
+    ("hp_destroy", None, [c_void_p]),
+    ("hp_last_error", c_char_p, [c_void_p]),
+    ("hp_last_global_error", c_char_p, []),
+    ("hp_get_config", c_int, [c_void_p, POINTER(HpConfig)]),
+    ("hp_set_state", c_int, [c_void_p, FloatPP, c_float, c_uint64]),
+    ("hp_get_state", c_int, [c_void_p, FloatPP, POINTER(c_float), POINTER(c_uint64)]),
+    ("hp_step", c_int, [c_void_p, FloatPP, POINTER(c_float), POINTER(c_uint64)]),
+    ("hp_run", c_int, [c_void_p, FloatPP, POINTER(c_float), POINTER(c_uint64), c_uint64]),
+    ("hp_advance", c_int, [c_void_p, c_uint64]),
+    ("hp_mass_f64", c_int, [c_void_p, POINTER(c_double), POINTER(c_double)]),
+    ("hp_stats", c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint64)]),
+    ("hp_profile_enable", c_int, [c_void_p, c_int]),
+    ("hp_profile_read", c_int, [c_void_p, POINTER(c_double), POINTER(c_uint64)]),
+    ("hp_kernel_bytes", c_int, [c_void_p, POINTER(c_double)]),
+    ("hp_advect_mass_2d", c_int, [c_int, FP, FP, FP, FP, c_uint32, c_uint32, c_float, c_float]),
+    ("hp_advect_mass_3d", c_int, [c_int, FP, FP, FP, FP, FP, c_uint32, c_uint32, c_uint32, c_float, c_float]),
+    ("hp_sobel_gradient_2d", c_int, [c_int, FP, FP, FP, c_uint32, c_uint32]),
+    ("hp_compute_flow_2d", c_int, [c_int, FP, FP, FP, FP, FP, c_uint32, HpFlow, FP, FP]),
+    ("hp_growth_accumulate", c_int, [c_int, FP, FP, c_uint32, c_float, c_float, c_float]),
+    ("hp_stencil_with_affinity", c_int, [c_void_p, FloatPP, FloatPP, FloatPP]),
+    ("hp_affinity", c_int, [c_void_p, FloatPP, FloatPP]),
+]
+
+_LIB = None
+
+
+class HipLibraryMissing(RuntimeError):
+    pass
+
+
+def load_library(path: str | None = None) -> ctypes.CDLL:
+    """Load the HIP library (no fallback).  Raises HipLibraryMissing if it was never built."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise HipLibraryMissing(
+            f"{p} not found: build it with `python -m autoverse_amd.build` (hipcc, gfx950)")
+    lib = ctypes.CDLL(p)
+    for name, res, args in EXPORTS:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+class HipError(RuntimeError):
+    def __init__(self, status: int, message: str = ""):
+        self.status = status
+        name = STATUS_NAMES.get(status, f"status {status}")
+        super().__init__(f"{name}: {message}" if message else name)
+
+
+def float_ptr_array(arrays) -> ctypes.Array:
+    """Array of float* built from C-contiguous float32 numpy arrays (kept alive by caller)."""
+    arr = (FP * len(arrays))()
+    for i, a in enumerate(arrays):
+        assert a.dtype.name == "float32" and a.flags["C_CONTIGUOUS"], "float32 C-contiguous required"
+        arr[i] = a.ctypes.data_as(FP)
+    return arr

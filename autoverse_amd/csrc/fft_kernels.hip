@@ -1,0 +1,468 @@
+// fft_kernels.hip -- LDS-staged Stockham FFT passes of the Flow Lenia convolution on gfx950.
+//
+// The reference convolves every kernel with a full complex 2D/3D FFT of the
+// source channel (CachedConvolver::convolve_with_kernel_scratch, fft.rs:288-324;
+// fft3d.rs:308-332) and then applies growth_accumulate per kernel in kernel order
+// (propagator.rs:270-279, growth.rs:44-59).  Here the same mathematics runs as
+// three memory passes over a half spectrum (the input is real):
+//   rows_fwd  : two real rows packed as one complex line, FFT along x, untangled
+//               into two Hermitian half rows (R2C);
+//   spectral  : FFT along the last axis (y in 2D, z in 3D), product with the
+//               kernel spectrum -- real and even in every axis because the ring
+//               kernel is point-symmetric (kernel.rs:41-64), so stored folded as
+//               (D/2+1)(H/2+1)(W/2+1) floats -- and the inverse FFT along the same
+//               axis, written per kernel;
+//   rows_inv  : Hermitian completion of two half rows, inverse FFT along x,
+//               Re * 1/(W*H*D) (fft.rs:389-392), growth G(u) and the weighted
+//               accumulate of every kernel that targets the channel, in kernel
+//               index order, all in registers; one write of the affinity U.
+// 3D adds a forward y pass after rows_fwd and an inverse y pass before rows_inv.
+//
+// Every transform is an in-place Stockham autosort FFT in LDS: each stage loads
+// its radix-R butterfly inputs into registers, synchronises, and stores the
+// outputs; radices 8/4/2/3 cover lengths 2^a * 3^b.  Twiddles come from a table
+// exp(-2 pi i m / L) (host FP64, rounded to f32) staged in LDS.
+#include "hp_common.hpp"
+
+namespace hp {
+
+// ---- compile-time radix plan ------------------------------------------------------
+__host__ __device__ constexpr int pick_radix(int m) {
+    return (m % 8 == 0 && m != 16) ? 8 : (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2 : (m % 3 == 0) ? 3 : 1;
+}
+__host__ __device__ constexpr int num_stages(int L) {
+    int n = 0;
+    while (L > 1) {
+        L /= pick_radix(L);
+        ++n;
+    }
+    return n;
+}
+__host__ __device__ constexpr int radix_at(int L, int s) {
+    for (int i = 0; i < s; ++i) L /= pick_radix(L);
+    return pick_radix(L);
+}
+__host__ __device__ constexpr int ns_at(int L, int s) {  // product of the radices before stage s
+    int ns = 1;
+    for (int i = 0; i < s; ++i) {
+        int r = pick_radix(L);
+        ns *= r;
+        L /= r;
+    }
+    return ns;
+}
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// multiply by -i (DIR = -1, forward) or +i (DIR = +1, inverse)
+template <int DIR>
+__device__ __forceinline__ float2 mul_mi(float2 a) {
+    return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+// In-register DFT of size R with exponent sign DIR.
+template <int R, int DIR>
+__device__ __forceinline__ void dft(float2* v) {
+    if constexpr (R == 2) {
+        float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 3) {
+        const float c1 = -0.5f, s1 = 0.86602540378443864676f;
+        float2 t = cadd(v[1], v[2]);
+        float2 d = csub(v[1], v[2]);
+        float2 m = make_float2(v[0].x + c1 * t.x, v[0].y + c1 * t.y);
+        float2 j = mul_mi<DIR>(make_float2(s1 * d.x, s1 * d.y));  // -i*s*d (fwd)
+        v[0] = cadd(v[0], t);
+        v[1] = cadd(m, j);
+        v[2] = csub(m, j);
+    } else if constexpr (R == 4) {
+        float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+        float2 b0 = cadd(v[1], v[3]), b1 = mul_mi<DIR>(csub(v[1], v[3]));
+        v[0] = cadd(a0, b0);
+        v[2] = csub(a0, b0);
+        v[1] = cadd(a1, b1);
+        v[3] = csub(a1, b1);
+    } else if constexpr (R == 8) {
+        float2 e[4] = {v[0], v[2], v[4], v[6]};
+        float2 o[4] = {v[1], v[3], v[5], v[7]};
+        dft<4, DIR>(e);
+        dft<4, DIR>(o);
+        const float h = 0.70710678118654752440f;
+        // W8^1, W8^2, W8^3 with exponent sign DIR
+        float2 o1 = cmul(o[1], make_float2(h, DIR * h));
+        float2 o2 = mul_mi<DIR>(o[2]);
+        float2 o3 = cmul(o[3], make_float2(-h, DIR * h));
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);
+        v[7] = csub(e[3], o3);
+    }
+}
+
+// One Stockham stage over NFFT transforms of length L at buf[f*LP + i], run by NT threads.
+template <int L, int STAGE, int DIR, int NFFT, int NT, int LP>
+__device__ __forceinline__ void fft_stage(float2* buf, const float2* tw) {
+    constexpr int R = radix_at(L, STAGE);
+    constexpr int NS = ns_at(L, STAGE);
+    constexpr int NB = L / R;
+    constexpr int TOT = NFFT * NB;
+    constexpr int Q = (TOT + NT - 1) / NT;
+    constexpr int TSTEP = L / (NS * R);
+    float2 v[Q][R];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int b = threadIdx.x + q * NT;
+        if (TOT % NT == 0 || b < TOT) {
+            const int f = b / NB, j = b - f * NB;
+            const float2* src = buf + f * LP + j;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = src[r * NB];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int b = threadIdx.x + q * NT;
+        if (TOT % NT == 0 || b < TOT) {
+            const int f = b / NB, j = b - f * NB;
+            const int k = j % NS;
+            if (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    float2 w = tw[r * k * TSTEP];
+                    if (DIR > 0) w.y = -w.y;
+                    v[q][r] = cmul(v[q][r], w);
+                }
+            }
+            dft<R, DIR>(v[q]);
+            float2* dst = buf + f * LP + (j / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) dst[r * NS] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int L, int STAGE, int DIR, int NFFT, int NT, int LP>
+__device__ __forceinline__ void fft_stages(float2* buf, const float2* tw) {
+    if constexpr (STAGE < num_stages(L)) {
+        fft_stage<L, STAGE, DIR, NFFT, NT, LP>(buf, tw);
+        fft_stages<L, STAGE + 1, DIR, NFFT, NT, LP>(buf, tw);
+    }
+}
+
+// NFFT in-place transforms of length L (unnormalised), DIR = -1 forward, +1 inverse.
+// Caller synchronises before (inputs written); returns after a barrier.
+template <int L, int DIR, int NFFT, int NT, int LP>
+__device__ __forceinline__ void lds_fft(float2* buf, const float2* tw) {
+    fft_stages<L, 0, DIR, NFFT, NT, LP>(buf, tw);
+}
+
+template <int NT>
+__device__ __forceinline__ void stage_twiddles(float2* tw, const float2* __restrict__ g, int L) {
+    for (int i = threadIdx.x; i < L; i += NT) tw[i] = g[i];
+}
+
+// ---- pass 1: R2C along x on pairs of real rows ----------------------------------------
+// grid (ceil(rows/2), nsrc); rows = H*D; src_chan[s] = channel of source slot s.
+template <int L, int NT>
+__global__ __launch_bounds__(NT) void k_rows_fwd(const float* __restrict__ A, float2* __restrict__ X,
+                                                 const int* __restrict__ src_chan, int rows, int Wp,
+                                                 long long N, long long spec, const float2* __restrict__ twg) {
+    __shared__ float2 buf[L];
+    __shared__ float2 tw[L];
+    const int s = blockIdx.y;
+    const int y0 = 2 * blockIdx.x;
+    const bool has1 = (y0 + 1) < rows;
+    const float* a = A + (long long)src_chan[s] * N + (long long)y0 * L;
+    stage_twiddles<NT>(tw, twg, L);
+    for (int i = threadIdx.x; i < L; i += NT) buf[i] = make_float2(a[i], has1 ? a[L + i] : 0.0f);
+    __syncthreads();
+    lds_fft<L, -1, 1, NT, L>(buf, tw);
+    float2* x0 = X + (long long)s * spec + (long long)y0 * Wp;
+    float2* x1 = x0 + Wp;
+    for (int k = threadIdx.x; k <= L / 2; k += NT) {
+        const float2 zk = buf[k];
+        const float2 zm = buf[k == 0 ? 0 : L - k];
+        x0[k] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+        if (has1) x1[k] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+    }
+}
+
+// ---- 3D only: FFT along y, in place, forward or inverse ------------------------------------
+// grid (ceil(Wh/CB), D, nspec).  Lines are the kx columns of one z plane.
+template <int L, int CB, int NT, int DIR>
+__global__ __launch_bounds__(NT) void k_cols(float2* __restrict__ X, int Wh, int Wp, long long spec,
+                                             const float2* __restrict__ twg) {
+    constexpr int LP = L + 1;
+    __shared__ float2 buf[CB * LP];
+    __shared__ float2 tw[L];
+    const int kx0 = blockIdx.x * CB;
+    float2* base = X + (long long)blockIdx.z * spec + (long long)blockIdx.y * L * Wp;
+    stage_twiddles<NT>(tw, twg, L);
+    for (int i = threadIdx.x; i < CB * L; i += NT) {
+        const int c = i % CB, e = i / CB;
+        const int kx = kx0 + c;
+        buf[c * LP + e] = kx < Wh ? base[(long long)e * Wp + kx] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    lds_fft<L, DIR, CB, NT, LP>(buf, tw);
+    for (int i = threadIdx.x; i < CB * L; i += NT) {
+        const int c = i % CB, e = i / CB;
+        const int kx = kx0 + c;
+        if (kx < Wh) base[(long long)e * Wp + kx] = buf[c * LP + e];
+    }
+}
+
+// ---- pass 2: FFT along the last axis, x kernel spectrum, inverse FFT ----------------------
+// DIM 2: lines are kx (element stride Wp, L = H).
+// DIM 3: lines are ky*Wp + kx (element stride H*Wp, L = D).
+// grid (ceil(lines/CB), nwork); work[w] names the source slot and its kernels.
+template <int L, int CB, int NT, int DIM>
+__global__ __launch_bounds__(NT) void k_spectral(const float2* __restrict__ X, float2* __restrict__ Y,
+                                                 const float* __restrict__ Kq, const SpecWork* __restrict__ work,
+                                                 const int* __restrict__ klist, int Wh, int Wp, int H, int Hq,
+                                                 long long spec, long long kq_stride,
+                                                 const float2* __restrict__ twg) {
+    constexpr int LP = L + 1;
+    constexpr int PER = CB * L / NT;
+    static_assert((CB * L) % NT == 0, "tile must split evenly over the threads");
+    __shared__ float2 buf[CB * LP];
+    __shared__ float2 tw[L];
+    const SpecWork wk = work[blockIdx.y];
+    const long long nlines = DIM == 2 ? (long long)Wh : (long long)H * Wp;
+    const long long es = DIM == 2 ? (long long)Wp : (long long)H * Wp;
+    const long long line0 = (long long)blockIdx.x * CB;
+    const float2* xs = X + (long long)wk.src_slot * spec;
+    stage_twiddles<NT>(tw, twg, L);
+    float2 reg[PER];
+    int koff[PER];  // offset of this element's folded kernel-spectrum value, or -1 if the line is padding
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int i = threadIdx.x + q * NT;
+        const int c = i % CB, e = i / CB;
+        const long long line = line0 + c;
+        int kx, ky;
+        if (DIM == 2) {
+            kx = (int)line;
+            ky = e;
+        } else {
+            ky = (int)(line / Wp);
+            kx = (int)(line - (long long)ky * Wp);
+        }
+        const bool valid = line < nlines && kx < Wh;
+        const float2 v = valid ? xs[(long long)e * es + line] : make_float2(0.f, 0.f);
+        buf[c * LP + e] = v;
+        if (DIM == 2) {
+            const int kyf = ky <= L / 2 ? ky : L - ky;
+            koff[q] = valid ? kyf * Wh + kx : -1;
+        } else {
+            const int kyf = ky <= H / 2 ? ky : H - ky;
+            const int kzf = e <= L / 2 ? e : L - e;
+            koff[q] = valid ? (kzf * Hq + kyf) * Wh + kx : -1;
+        }
+    }
+    __syncthreads();
+    lds_fft<L, -1, CB, NT, LP>(buf, tw);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int i = threadIdx.x + q * NT;
+        reg[q] = buf[(i % CB) * LP + i / CB];
+    }
+    for (int j = 0; j < wk.k_count; ++j) {
+        const int k = klist[wk.k_first + j];
+        const float* kq = Kq + (long long)k * kq_stride;
+        __syncthreads();  // previous kernel's outputs have been read
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int i = threadIdx.x + q * NT;
+            const float kv = koff[q] >= 0 ? kq[koff[q]] : 0.0f;
+            buf[(i % CB) * LP + i / CB] = make_float2(reg[q].x * kv, reg[q].y * kv);
+        }
+        __syncthreads();
+        lds_fft<L, +1, CB, NT, LP>(buf, tw);
+        float2* ys = Y + (long long)k * spec;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int i = threadIdx.x + q * NT;
+            const int c = i % CB, e = i / CB;
+            if (koff[q] >= 0) ys[(long long)e * es + line0 + c] = buf[c * LP + e];
+        }
+    }
+}
+
+// ---- pass 3: C2R along x on pairs of rows + growth + weighted accumulate -----------------
+// grid (ceil(rows/2), nwork); work[w] is a target channel and its kernels in index order.
+template <int L, int NT>
+__global__ __launch_bounds__(NT) void k_rows_inv(const float2* __restrict__ Y, float* __restrict__ U,
+                                                 const TgtWork* __restrict__ work, const int* __restrict__ klist,
+                                                 const GrowthParams* __restrict__ gp, int rows, int Wp,
+                                                 long long N, long long spec, float scale,
+                                                 const float2* __restrict__ twg) {
+    constexpr int PER = (L + NT - 1) / NT;
+    __shared__ float2 buf[L];
+    __shared__ float2 tw[L];
+    const TgtWork wk = work[blockIdx.y];
+    const int y0 = 2 * blockIdx.x;
+    const bool has1 = (y0 + 1) < rows;
+    stage_twiddles<NT>(tw, twg, L);
+    float acc0[PER], acc1[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) acc0[q] = acc1[q] = 0.0f;
+    for (int j = 0; j < wk.k_count; ++j) {
+        const int k = klist[wk.k_first + j];
+        const float2* r0 = Y + (long long)k * spec + (long long)y0 * Wp;
+        const float2* r1 = r0 + Wp;
+        __syncthreads();  // previous kernel's results have been read
+        for (int n = threadIdx.x; n < L; n += NT) {
+            // Hermitian completion: X0[n] = conj(X0[L-n]) for n > L/2, likewise X1;
+            // the packed line is Z = X0 + i X1, whose inverse is x0 + i x1.
+            float2 a, b;
+            if (n <= L / 2) {
+                a = r0[n];
+                b = has1 ? r1[n] : make_float2(0.f, 0.f);
+                buf[n] = make_float2(a.x - b.y, a.y + b.x);
+            } else {
+                a = r0[L - n];
+                b = has1 ? r1[L - n] : make_float2(0.f, 0.f);
+                buf[n] = make_float2(a.x + b.y, b.x - a.y);
+            }
+        }
+        __syncthreads();
+        lds_fft<L, +1, 1, NT, L>(buf, tw);
+        const GrowthParams p = gp[k];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int m = threadIdx.x + q * NT;
+            if (L % NT == 0 || m < L) {
+                const float2 z = buf[m];
+                {
+#pragma clang fp contract(off)
+                    // growth_accumulate (growth.rs:51-57): g = 2 exp(-d*d*inv) - 1; t += w*g
+                    const float c0 = z.x * scale, c1 = z.y * scale;
+                    const float d0 = c0 - p.mu, d1 = c1 - p.mu;
+                    const float g0 = 2.0f * expf(-d0 * d0 * p.inv) - 1.0f;
+                    const float g1 = 2.0f * expf(-d1 * d1 * p.inv) - 1.0f;
+                    acc0[q] += p.weight * g0;
+                    acc1[q] += p.weight * g1;
+                }
+            }
+        }
+    }
+    float* u0 = U + (long long)wk.target * N + (long long)y0 * L;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int m = threadIdx.x + q * NT;
+        if (L % NT == 0 || m < L) {
+            u0[m] = acc0[q];
+            if (has1) u0[L + m] = acc1[q];
+        }
+    }
+}
+
+// ---- host-side dispatch ------------------------------------------------------------------
+namespace {
+template <int L>
+struct Cfg {
+    static constexpr int ROW_NT = L / 8 < 64 ? 64 : (L / 8 > 256 ? 256 : L / 8);
+    static constexpr int CB = 8192 / L < 1 ? 1 : (8192 / L > 64 ? 64 : 8192 / L);
+    static constexpr int COL_NT = (CB * L) / 32 < 64 ? 64 : ((CB * L) / 32 > 256 ? 256 : (CB * L) / 32);
+};
+
+template <int L>
+void rows_fwd(hipStream_t st, const float* A, float2* X, const int* src_chan, int nsrc, const Geo& g,
+              const float2* tw) {
+    const int rows = g.H * g.D;
+    dim3 grid((rows + 1) / 2, nsrc);
+    hipLaunchKernelGGL((k_rows_fwd<L, Cfg<L>::ROW_NT>), grid, dim3(Cfg<L>::ROW_NT), 0, st, A, X, src_chan, rows,
+                       g.Wp, g.N, g.spec, tw);
+}
+
+template <int L>
+void cols(hipStream_t st, int dir, float2* X, int nspec, const Geo& g, const float2* tw) {
+    constexpr int CB = Cfg<L>::CB, NT = Cfg<L>::COL_NT;
+    dim3 grid((g.Wh + CB - 1) / CB, g.D, nspec);
+    if (dir < 0)
+        hipLaunchKernelGGL((k_cols<L, CB, NT, -1>), grid, dim3(NT), 0, st, X, g.Wh, g.Wp, g.spec, tw);
+    else
+        hipLaunchKernelGGL((k_cols<L, CB, NT, 1>), grid, dim3(NT), 0, st, X, g.Wh, g.Wp, g.spec, tw);
+}
+
+template <int L>
+void spectral(int dim, hipStream_t st, const float2* X, float2* Y, const float* Kq, const SpecWork* work,
+              int nwork, const int* klist, const Geo& g, const float2* tw) {
+    constexpr int CB = Cfg<L>::CB, NT = Cfg<L>::COL_NT;
+    const long long kq_stride = (long long)g.Dq * g.Hq * g.Wh;
+    if (dim == 2) {
+        dim3 grid((g.Wh + CB - 1) / CB, nwork);
+        hipLaunchKernelGGL((k_spectral<L, CB, NT, 2>), grid, dim3(NT), 0, st, X, Y, Kq, work, klist, g.Wh, g.Wp,
+                           g.H, g.Hq, g.spec, kq_stride, tw);
+    } else {
+        const long long lines = (long long)g.H * g.Wp;
+        dim3 grid((unsigned)((lines + CB - 1) / CB), nwork);
+        hipLaunchKernelGGL((k_spectral<L, CB, NT, 3>), grid, dim3(NT), 0, st, X, Y, Kq, work, klist, g.Wh, g.Wp,
+                           g.H, g.Hq, g.spec, kq_stride, tw);
+    }
+}
+
+template <int L>
+void rows_inv(hipStream_t st, const float2* Y, float* U, const TgtWork* work, int nwork, const int* klist,
+              const GrowthParams* gp, const Geo& g, const float2* tw) {
+    const int rows = g.H * g.D;
+    dim3 grid((rows + 1) / 2, nwork);
+    const float scale = 1.0f / (float)(g.N);  // fft.rs:389, fft3d.rs:449: 1 / (W*H*D) as f32
+    hipLaunchKernelGGL((k_rows_inv<L, Cfg<L>::ROW_NT>), grid, dim3(Cfg<L>::ROW_NT), 0, st, Y, U, work, klist, gp,
+                       rows, g.Wp, g.N, g.spec, scale, tw);
+}
+}  // namespace
+
+#define HP_FFT_SIZES(X) \
+    X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(24) X(48) X(96) X(192) X(384) X(768) X(1536)
+
+bool fft_supported(int L) {
+#define X_(n) if (L == n) return true;
+    HP_FFT_SIZES(X_)
+#undef X_
+    return false;
+}
+
+bool launch_rows_fwd(int L, hipStream_t st, const float* A, float2* X, const int* src_chan, int nsrc, const Geo& g,
+                     const float2* tw) {
+#define X_(n) if (L == n) { rows_fwd<n>(st, A, X, src_chan, nsrc, g, tw); return true; }
+    HP_FFT_SIZES(X_)
+#undef X_
+    return false;
+}
+
+bool launch_cols(int L, hipStream_t st, int dir, float2* X, int nspec, const Geo& g, const float2* tw) {
+#define X_(n) if (L == n) { cols<n>(st, dir, X, nspec, g, tw); return true; }
+    HP_FFT_SIZES(X_)
+#undef X_
+    return false;
+}
+
+bool launch_spectral(int L, int dim, hipStream_t st, const float2* X, float2* Y, const float* Kq,
+                     const SpecWork* work, int nwork, const int* klist, const Geo& g, const float2* tw) {
+#define X_(n) if (L == n) { spectral<n>(dim, st, X, Y, Kq, work, nwork, klist, g, tw); return true; }
+    HP_FFT_SIZES(X_)
+#undef X_
+    return false;
+}
+
+bool launch_rows_inv(int L, hipStream_t st, const float2* Y, float* U, const TgtWork* work, int nwork,
+                     const int* klist, const GrowthParams* gp, const Geo& g, const float2* tw) {
+#define X_(n) if (L == n) { rows_inv<n>(st, Y, U, work, nwork, klist, gp, g, tw); return true; }
+    HP_FFT_SIZES(X_)
+#undef X_
+    return false;
+}
+
+}  // namespace hp

@@ -1,0 +1,924 @@
+// host.cpp -- the HipPropagator context behind the C ABI of include/flowlenia_hip.h.
+//
+// One context = one simulation on one GPU (CpuPropagator::new, propagator.rs:157-236).
+// Device memory, all resident in HBM for the life of the context:
+//   A[2]   state ping-pong, [C][D*H*W] f32 (the reference's state.channels / next_channels)
+//   U      affinity [C][N] f32 (the reference's `affinity`)
+//   X      half spectra of the distinct source channels, [nsrc][D*H][Wp] complex64
+//   Y      per-kernel filtered half spectra, [K][D*H][Wp] complex64
+//   Kq     folded real kernel spectra, [K][D/2+1][H/2+1][W/2+1] f32
+//   box    general-path source boxes, 2 x [C][N] float4 (only touched on the rare path)
+// A step is: rows_fwd -> (3D: cols_fwd) -> spectral -> (3D: cols_inv) -> rows_inv ->
+// stencil; steps replay from two captured hipGraphs (A0->A1 and A1->A0).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <mutex>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/flowlenia_hip.h"
+#include "hp_common.hpp"
+#include "host_internal.hpp"
+
+namespace {
+
+thread_local std::string g_global_error;
+
+struct Launch {
+    int cls;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct hp_ctx {
+    hp_config cfg{};
+    std::vector<hp_kernel> kernels;
+    std::vector<hp_ring> rings;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hp::Geo g{};
+    int C = 0, K = 0;
+    bool is3d = false;
+    std::vector<int> src_chan;  // source slot -> channel
+    std::vector<int> tgt_chan;  // target work item -> channel
+    float* A[2] = {nullptr, nullptr};
+    float* U = nullptr;
+    float2* X = nullptr;
+    float2* Y = nullptr;
+    float* Kq = nullptr;
+    std::map<int, float2*> tw;  // FFT length -> twiddle table
+    int* d_src_chan = nullptr;
+    hp::SpecWork* d_spec_work = nullptr;
+    int n_spec_work = 0;
+    int* d_klist_spec = nullptr;
+    hp::TgtWork* d_tgt_work = nullptr;
+    int n_tgt_work = 0;
+    int* d_klist_tgt = nullptr;
+    hp::GrowthParams* d_gp = nullptr;
+    int* d_flags = nullptr;  // [0] stencil flag bits, [1] needed radius, [2] non-finite bits
+    float4* box_lo = nullptr;
+    float4* box_hi = nullptr;
+    double* d_partials = nullptr;
+    double* d_mass = nullptr;
+    int mass_parts = 0;
+    float* snapshot = nullptr;
+    int cur = 0;
+    float time = 0.0f;
+    uint64_t step = 0;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    bool profiling = false;
+    std::vector<Launch> launches;
+    std::vector<hipEvent_t> event_pool;
+    double prof_ms[HP_K_NUM_CLASSES] = {};
+    uint64_t prof_n[HP_K_NUM_CLASSES] = {};
+    uint32_t max_radius = 0;
+    uint64_t general_steps = 0;
+    std::string err;
+
+    hp_status fail(hp_status st, const std::string& msg) {
+        err = msg;
+        return st;
+    }
+};
+
+#define HIP_TRY(ctx, expr)                                                                   \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return (ctx)->fail(HP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+namespace {
+
+hipEvent_t take_event(hp_ctx* c) {
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+// Brackets one kernel launch with events when profiling is on.
+struct Timed {
+    hp_ctx* c;
+    int cls;
+    hipEvent_t a = nullptr;
+    Timed(hp_ctx* ctx, int k) : c(ctx), cls(k) {
+        if (c->profiling) {
+            a = take_event(c);
+            (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~Timed() {
+        if (c->profiling) {
+            hipEvent_t b = take_event(c);
+            (void)hipEventRecord(b, c->stream);
+            c->launches.push_back({cls, a, b});
+        }
+    }
+};
+
+void collect_profile(hp_ctx* c) {
+    for (auto& l : c->launches) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, l.a, l.b) == hipSuccess) {
+            c->prof_ms[l.cls] += ms;
+            c->prof_n[l.cls] += 1;
+        }
+        c->event_pool.push_back(l.a);
+        c->event_pool.push_back(l.b);
+    }
+    c->launches.clear();
+}
+
+template <typename T>
+hp_status dalloc(hp_ctx* c, T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) return c->fail(HP_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return HP_OK;
+}
+
+template <typename T>
+hp_status upload(hp_ctx* c, T* dst, const std::vector<T>& src) {
+    if (src.empty()) return HP_OK;
+    HIP_TRY(c, hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return HP_OK;
+}
+
+hp_status make_twiddles(hp_ctx* c, int L) {
+    if (c->tw.count(L)) return HP_OK;
+    std::vector<float2> t(L);
+    for (int m = 0; m < L; m++) {
+        const double a = -2.0 * M_PI * (double)m / (double)L;
+        t[m] = make_float2((float)cos(a), (float)sin(a));
+    }
+    float2* d = nullptr;
+    hp_status st = dalloc(c, &d, L);
+    if (st != HP_OK) return st;
+    c->tw[L] = d;
+    return upload(c, d, t);
+}
+
+// ---- one step, enqueued on c->stream: A[src] -> A[1-src]; the FFT path leaves U filled ----
+hp_status enqueue_affinity(hp_ctx* c, const float* A) {
+    if (c->K == 0) return HP_OK;  // U stays zero (propagator.rs:245-247)
+    const hp::Geo& g = c->g;
+    {
+        Timed t(c, HP_K_ROWS_FWD);
+        hp::launch_rows_fwd(g.W, c->stream, A, c->X, c->d_src_chan, (int)c->src_chan.size(), g, c->tw[g.W]);
+    }
+    if (c->is3d) {
+        Timed t(c, HP_K_COLS_FWD);
+        hp::launch_cols(g.H, c->stream, -1, c->X, (int)c->src_chan.size(), g, c->tw[g.H]);
+    }
+    {
+        Timed t(c, HP_K_SPECTRAL);
+        const int L = c->is3d ? g.D : g.H;
+        hp::launch_spectral(L, c->is3d ? 3 : 2, c->stream, c->X, c->Y, c->Kq, c->d_spec_work, c->n_spec_work,
+                            c->d_klist_spec, g, c->tw[L]);
+    }
+    if (c->is3d) {
+        Timed t(c, HP_K_COLS_INV);
+        hp::launch_cols(g.H, c->stream, +1, c->Y, c->K, g, c->tw[g.H]);
+    }
+    {
+        Timed t(c, HP_K_ROWS_INV);
+        hp::launch_rows_inv(g.W, c->stream, c->Y, c->U, c->d_tgt_work, c->n_tgt_work, c->d_klist_tgt, c->d_gp, g,
+                            c->tw[g.W]);
+    }
+    HIP_TRY(c, hipGetLastError());
+    return HP_OK;
+}
+
+bool fast_stencil_ok(const hp_ctx* c) {
+    if (c->is3d) return false;
+    const float s = c->cfg.flow.distribution_size;
+    const float ta = (2.0f * s) * (2.0f * s);
+    return ta >= 1e-10f && c->g.W >= 5 && c->g.H >= 5;
+}
+
+hp_status enqueue_fast_stencil(hp_ctx* c, const float* A, const float* U, float* out) {
+    {
+        Timed t(c, HP_K_STENCIL);
+        hp::launch_stencil2d(c->stream, A, U, out, c->C, c->g, c->cfg.dt, c->cfg.flow.beta_a, c->cfg.flow.n,
+                             c->cfg.flow.distribution_size, c->d_flags);
+    }
+    HIP_TRY(c, hipGetLastError());
+    return HP_OK;
+}
+
+// General path for one step whose affinity is in `U`: boxes, sync, radius, gather.
+hp_status general_stencil(hp_ctx* c, const float* A, const float* U, float* out) {
+    hp::launch_clear_ints(c->stream, c->d_flags + 1, 2);
+    {
+        Timed t(c, HP_K_GENERAL);
+        hp::launch_boxes(c->stream, A, U, c->C, c->g, c->cfg.dt, c->cfg.flow.beta_a, c->cfg.flow.n,
+                         c->cfg.flow.distribution_size, c->box_lo, c->box_hi, c->d_flags + 1);
+    }
+    int h[2] = {0, 0};
+    HIP_TRY(c, hipMemcpyAsync(h, c->d_flags + 1, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (h[1]) return c->fail(HP_ERR_NONFINITE, "state or flow holds NaN/Inf");
+    const int R = std::max(h[0], 1);
+    c->max_radius = std::max<uint32_t>(c->max_radius, (uint32_t)R);
+    // the gather visits (2R+1)^d sources per destination (capped at the grid): refuse work that
+    // would keep the GPU busy for minutes rather than run it
+    auto win = [](int R_, int n) { return (double)std::min(2 * R_ + 1, n); };
+    const double visits = win(R, c->g.W) * win(R, c->g.H) * (c->is3d ? win(R, c->g.D) : 1.0) * c->C * c->g.N;
+    if (visits > 2e10)
+        return c->fail(HP_ERR_UNSUPPORTED_SIZE, "flow displacement of " + std::to_string(R) +
+                                                    " cells is beyond the general gather's work bound");
+    {
+        Timed t(c, HP_K_GENERAL);
+        hp::launch_gather_general(c->stream, c->box_lo, c->box_hi, out, c->C, c->g, c->cfg.flow.distribution_size, R);
+    }
+    HIP_TRY(c, hipGetLastError());
+    c->general_steps++;
+    return HP_OK;
+}
+
+// One step with a host check of the speculative radius (used by hp_step and by the replay
+// after a flagged run).
+hp_status safe_step(hp_ctx* c) {
+    float* A = c->A[c->cur];
+    float* out = c->A[1 - c->cur];
+    hp_status st = enqueue_affinity(c, A);
+    if (st != HP_OK) return st;
+    bool need_general = !fast_stencil_ok(c);
+    if (!need_general) {
+        hp::launch_clear_ints(c->stream, c->d_flags, 1);
+        st = enqueue_fast_stencil(c, A, c->U, out);
+        if (st != HP_OK) return st;
+        int f = 0;
+        HIP_TRY(c, hipMemcpyAsync(&f, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        if (f & 2) return c->fail(HP_ERR_NONFINITE, "state or flow holds NaN/Inf");
+        if (f & 1) need_general = true;
+        else c->max_radius = std::max<uint32_t>(c->max_radius, 2);
+    }
+    if (need_general) {
+        st = general_stencil(c, A, c->U, out);
+        if (st != HP_OK) return st;
+    }
+    c->cur = 1 - c->cur;
+    c->time += c->cfg.dt;
+    c->step += 1;
+    return HP_OK;
+}
+
+hp_status build_graphs(hp_ctx* c) {
+    for (int b = 0; b < 2; b++) {
+        if (c->graph[b]) continue;
+        hipGraph_t graph = nullptr;
+        HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_affinity(c, c->A[b]);
+        enqueue_fast_stencil(c, c->A[b], c->U, c->A[1 - b]);
+        HIP_TRY(c, hipStreamEndCapture(c->stream, &graph));
+        hipError_t e = hipGraphInstantiate(&c->graph[b], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return c->fail(HP_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    return HP_OK;
+}
+
+hp_status advance(hp_ctx* c, uint64_t steps) {
+    if (steps == 0) return HP_OK;
+    if (!fast_stencil_ok(c)) {
+        for (uint64_t i = 0; i < steps; i++) {
+            hp_status st = safe_step(c);
+            if (st != HP_OK) return st;
+        }
+        if (c->profiling) collect_profile(c);
+        return HP_OK;
+    }
+    // Speculative run: every step uses the fused stencil; a sticky flag records whether any
+    // source left the fused radius.  If it did, restore the snapshot and replay step by step.
+    const size_t bytes = (size_t)c->C * c->g.N * sizeof(float);
+    const int start_cur = c->cur;
+    const float start_time = c->time;
+    const uint64_t start_step = c->step;
+    HIP_TRY(c, hipMemcpyAsync(c->snapshot, c->A[c->cur], bytes, hipMemcpyDeviceToDevice, c->stream));
+    hp::launch_clear_ints(c->stream, c->d_flags, 1);
+    if (!c->profiling) {
+        hp_status st = build_graphs(c);
+        if (st != HP_OK) return st;
+    }
+    for (uint64_t i = 0; i < steps; i++) {
+        if (c->profiling) {
+            float* A = c->A[c->cur];
+            hp_status st = enqueue_affinity(c, A);
+            if (st == HP_OK) st = enqueue_fast_stencil(c, A, c->U, c->A[1 - c->cur]);
+            if (st != HP_OK) return st;
+        } else {
+            HIP_TRY(c, hipGraphLaunch(c->graph[c->cur], c->stream));
+        }
+        c->cur = 1 - c->cur;
+        c->time += c->cfg.dt;
+        c->step += 1;
+    }
+    int f = 0;
+    HIP_TRY(c, hipMemcpyAsync(&f, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->profiling) collect_profile(c);
+    if (f == 0) {
+        c->max_radius = std::max<uint32_t>(c->max_radius, 2);
+        return HP_OK;
+    }
+    // replay from the snapshot with a per-step check
+    c->cur = start_cur;
+    c->time = start_time;
+    c->step = start_step;
+    HIP_TRY(c, hipMemcpyAsync(c->A[c->cur], c->snapshot, bytes, hipMemcpyDeviceToDevice, c->stream));
+    for (uint64_t i = 0; i < steps; i++) {
+        hp_status st = safe_step(c);
+        if (st != HP_OK) return st;
+    }
+    if (c->profiling) collect_profile(c);
+    return HP_OK;
+}
+
+hp_status check_host_state(hp_ctx* c, const float* const* ch) {
+    if (!ch) return c->fail(HP_ERR_NULL, "channel pointer array is NULL");
+    for (int i = 0; i < c->C; i++)
+        if (!ch[i]) return c->fail(HP_ERR_NULL, "channel pointer is NULL");
+    return HP_OK;
+}
+
+hp_status upload_state(hp_ctx* c, const float* const* ch, float* dst) {
+    for (int i = 0; i < c->C; i++) {
+        for (long long j = 0; j < c->g.N; j++)
+            if (!isfinite(ch[i][j])) return c->fail(HP_ERR_NONFINITE, "input state holds NaN/Inf");
+        HIP_TRY(c, hipMemcpyAsync(dst + (size_t)i * c->g.N, ch[i], c->g.N * sizeof(float), hipMemcpyHostToDevice,
+                                  c->stream));
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return HP_OK;
+}
+
+hp_status download(hp_ctx* c, const float* src, float* const* ch) {
+    for (int i = 0; i < c->C; i++)
+        HIP_TRY(c, hipMemcpyAsync(ch[i], src + (size_t)i * c->g.N, c->g.N * sizeof(float), hipMemcpyDeviceToHost,
+                                  c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return HP_OK;
+}
+
+void release(hp_ctx* c) {
+    if (!c) return;
+    for (int b = 0; b < 2; b++)
+        if (c->graph[b]) (void)hipGraphExecDestroy(c->graph[b]);
+    for (auto& l : c->launches) {
+        (void)hipEventDestroy(l.a);
+        (void)hipEventDestroy(l.b);
+    }
+    for (auto e : c->event_pool) (void)hipEventDestroy(e);
+    void* ptrs[] = {c->A[0], c->A[1], c->U, c->X, c->Y, c->Kq, c->d_src_chan, c->d_spec_work, c->d_klist_spec,
+                    c->d_tgt_work, c->d_klist_tgt, c->d_gp, c->d_flags, c->box_lo, c->box_hi, c->d_partials,
+                    c->d_mass, c->snapshot};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto& kv : c->tw) (void)hipFree(kv.second);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+hp_status create(const hp_config* cfg, int device, hp_ctx** out) {
+    *out = nullptr;
+    hp_status st = hp::validate_config(cfg, nullptr, nullptr);
+    if (st != HP_OK) {
+        g_global_error = "invalid configuration";
+        return st;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        g_global_error = "no HIP device";
+        return HP_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= ndev) {
+        g_global_error = "device ordinal out of range";
+        return HP_ERR_NO_DEVICE;
+    }
+    const bool is3d = cfg->depth > 1;
+    const uint32_t W = cfg->width, H = cfg->height, D = cfg->depth;
+    if (!hp::fft_supported((int)W) || !hp::fft_supported((int)H) || (is3d && !hp::fft_supported((int)D))) {
+        g_global_error = "grid sides must be 2^a * 3^b between 8 and 2048";
+        return HP_ERR_UNSUPPORTED_SIZE;
+    }
+    hp_ctx* c = new hp_ctx();
+    c->device = device;
+    c->cfg = *cfg;
+    c->kernels.assign(cfg->kernels, cfg->kernels + cfg->num_kernels);
+    c->rings.assign(cfg->rings, cfg->rings + cfg->num_rings);
+    c->cfg.kernels = c->kernels.data();
+    c->cfg.rings = c->rings.data();
+    c->C = (int)cfg->channels;
+    c->K = (int)cfg->num_kernels;
+    c->is3d = is3d;
+    hp::Geo& g = c->g;
+    g.W = (int)W;
+    g.H = (int)H;
+    g.D = (int)D;
+    g.Wh = (int)(W / 2 + 1);
+    g.Wp = (g.Wh + 15) / 16 * 16;
+    g.Hq = (int)(H / 2 + 1);
+    g.Dq = is3d ? (int)(D / 2 + 1) : 1;
+    g.N = (long long)W * H * D;
+    g.spec = (long long)D * H * g.Wp;
+    // kernel spectra (host FP64) and work lists
+    std::vector<float> kq((size_t)std::max(1, c->K) * g.Dq * g.Hq * g.Wh, 0.0f);
+    std::vector<int> slot_of(c->C, -1);
+    std::vector<hp::GrowthParams> gp(std::max(1, c->K));
+    for (int k = 0; k < c->K; k++) {
+        const hp_kernel& kc = c->kernels[k];
+        uint32_t size = 0;
+        std::vector<float> data = hp::build_ring_kernel(kc, c->rings.data(), cfg->kernel_radius, is3d ? 3 : 2, &size);
+        if (size > W || size > H || (is3d && size > D)) {
+            release(c);
+            g_global_error = "kernel diameter 2R+1 exceeds a grid side";
+            return HP_ERR_UNSUPPORTED_SIZE;
+        }
+        std::vector<float> f = hp::folded_spectrum(data, size, is3d ? 3 : 2, W, H, D);
+        std::copy(f.begin(), f.end(), kq.begin() + (size_t)k * f.size());
+        if (slot_of[kc.source_channel] < 0) {
+            slot_of[kc.source_channel] = (int)c->src_chan.size();
+            c->src_chan.push_back((int)kc.source_channel);
+        }
+        const float sigma_sq_2 = 2.0f * kc.sigma * kc.sigma;  // growth.rs:51-52
+        gp[k] = {kc.weight, kc.mu, 1.0f / sigma_sq_2, 0.0f};
+    }
+    // spectral work: one item per kernel (reads its source spectrum)
+    std::vector<hp::SpecWork> sw;
+    std::vector<int> kls;
+    for (int k = 0; k < c->K; k++) {
+        sw.push_back({slot_of[c->kernels[k].source_channel], (int)kls.size(), 1, 0});
+        kls.push_back(k);
+    }
+    // inverse-row work: one item per target with its kernels in index order
+    std::vector<hp::TgtWork> tw;
+    std::vector<int> klt;
+    for (int t = 0; t < c->C; t++) {
+        int first = (int)klt.size(), cnt = 0;
+        for (int k = 0; k < c->K; k++)
+            if ((int)c->kernels[k].target_channel == t) {
+                klt.push_back(k);
+                cnt++;
+            }
+        if (cnt) {
+            tw.push_back({t, first, cnt, 0});
+            c->tgt_chan.push_back(t);
+        }
+    }
+    c->n_spec_work = (int)sw.size();
+    c->n_tgt_work = (int)tw.size();
+    c->mass_parts = 256;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        release(c);
+        g_global_error = "hipStreamCreate failed";
+        return HP_ERR_HIP;
+    }
+    const size_t N = (size_t)g.N, CN = (size_t)c->C * N;
+    const size_t nsrc = std::max<size_t>(1, c->src_chan.size());
+#define TRY(x)                                \
+    do {                                      \
+        hp_status s_ = (x);                   \
+        if (s_ != HP_OK) {                    \
+            g_global_error = c->err;          \
+            release(c);                       \
+            return s_;                        \
+        }                                     \
+    } while (0)
+    TRY(dalloc(c, &c->A[0], CN));
+    TRY(dalloc(c, &c->A[1], CN));
+    TRY(dalloc(c, &c->U, CN));
+    TRY(dalloc(c, &c->snapshot, CN));
+    TRY(dalloc(c, &c->X, nsrc * g.spec));
+    TRY(dalloc(c, &c->Y, (size_t)std::max(1, c->K) * g.spec));
+    TRY(dalloc(c, &c->Kq, kq.size()));
+    TRY(dalloc(c, &c->d_src_chan, nsrc));
+    TRY(dalloc(c, &c->d_spec_work, std::max<size_t>(1, sw.size())));
+    TRY(dalloc(c, &c->d_klist_spec, std::max<size_t>(1, kls.size())));
+    TRY(dalloc(c, &c->d_tgt_work, std::max<size_t>(1, tw.size())));
+    TRY(dalloc(c, &c->d_klist_tgt, std::max<size_t>(1, klt.size())));
+    TRY(dalloc(c, &c->d_gp, gp.size()));
+    TRY(dalloc(c, &c->d_flags, 4));
+    TRY(dalloc(c, &c->box_lo, CN));
+    TRY(dalloc(c, &c->box_hi, CN));
+    TRY(dalloc(c, &c->d_partials, (size_t)c->C * c->mass_parts));
+    TRY(dalloc(c, &c->d_mass, (size_t)c->C + 1));
+    TRY(upload(c, c->Kq, kq));
+    TRY(upload(c, c->d_src_chan, c->src_chan));
+    TRY(upload(c, c->d_spec_work, sw));
+    TRY(upload(c, c->d_klist_spec, kls));
+    TRY(upload(c, c->d_tgt_work, tw));
+    TRY(upload(c, c->d_klist_tgt, klt));
+    TRY(upload(c, c->d_gp, gp));
+    TRY(make_twiddles(c, g.W));
+    TRY(make_twiddles(c, g.H));
+    if (is3d) TRY(make_twiddles(c, g.D));
+    // zero state, affinity (channels no kernel targets keep U = 0) and flags
+    if (hipMemset(c->A[0], 0, CN * 4) != hipSuccess || hipMemset(c->A[1], 0, CN * 4) != hipSuccess ||
+        hipMemset(c->U, 0, CN * 4) != hipSuccess || hipMemset(c->d_flags, 0, 16) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess) {
+        release(c);
+        g_global_error = "hipMemset failed";
+        return HP_ERR_HIP;
+    }
+#undef TRY
+    *out = c;
+    return HP_OK;
+}
+
+template <typename F>
+hp_status guarded(hp_ctx* c, F&& f) {
+    if (!c) return HP_ERR_NULL;
+    try {
+        if (hipSetDevice(c->device) != hipSuccess) return c->fail(HP_ERR_HIP, "hipSetDevice failed");
+        return f();
+    } catch (const std::exception& e) {
+        return c->fail(HP_ERR_HIP, e.what());
+    } catch (...) {
+        return c->fail(HP_ERR_HIP, "unknown exception");
+    }
+}
+
+// temporary device buffers for the stage entry points
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+hp_status stage_device(int device) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || device < 0 || device >= ndev) {
+        g_global_error = "no HIP device";
+        return HP_ERR_NO_DEVICE;
+    }
+    if (hipSetDevice(device) != hipSuccess) return HP_ERR_HIP;
+    return HP_OK;
+}
+
+#define STAGE_TRY(expr)                                           \
+    do {                                                          \
+        hipError_t e_ = (expr);                                   \
+        if (e_ != hipSuccess) {                                   \
+            g_global_error = std::string(#expr) + ": " + hipGetErrorString(e_); \
+            return HP_ERR_HIP;                                    \
+        }                                                         \
+    } while (0)
+
+hp_status to_dev(DevBuf& b, const float* host, size_t n) {
+    STAGE_TRY(hipMalloc(&b.p, std::max<size_t>(1, n) * sizeof(float)));
+    if (host) STAGE_TRY(hipMemcpy(b.p, host, n * sizeof(float), hipMemcpyHostToDevice));
+    return HP_OK;
+}
+
+hp_status advect_stage(int device, const float* cur, const float* fx, const float* fy, const float* fz, float* next,
+                       uint32_t W, uint32_t H, uint32_t D, float dt, float s) {
+    if (!cur || !fx || !fy || !next || (D > 1 && !fz)) return HP_ERR_NULL;
+    if (W == 0 || H == 0 || D == 0) return HP_ERR_INVALID_DIMENSIONS;
+    hp_status st = stage_device(device);
+    if (st != HP_OK) return st;
+    hp::Geo g{};
+    g.W = (int)W;
+    g.H = (int)H;
+    g.D = (int)D;
+    g.N = (long long)W * H * D;
+    const size_t n = (size_t)g.N;
+    DevBuf a, bx, by, bz, o, lo, hi, fl;
+    if ((st = to_dev(a, cur, n)) != HP_OK || (st = to_dev(bx, fx, n)) != HP_OK || (st = to_dev(by, fy, n)) != HP_OK ||
+        (st = to_dev(o, nullptr, n)) != HP_OK)
+        return st;
+    if (D > 1 && (st = to_dev(bz, fz, n)) != HP_OK) return st;
+    STAGE_TRY(hipMalloc(&lo.p, n * sizeof(float4)));
+    STAGE_TRY(hipMalloc(&hi.p, n * sizeof(float4)));
+    STAGE_TRY(hipMalloc(&fl.p, 4 * sizeof(int)));
+    STAGE_TRY(hipMemset(fl.p, 0, 4 * sizeof(int)));
+    hp::launch_boxes_from_flow(nullptr, (const float*)a.p, (const float*)bx.p, (const float*)by.p,
+                               D > 1 ? (const float*)bz.p : nullptr, g, dt, s, (float4*)lo.p, (float4*)hi.p,
+                               (int*)fl.p);
+    int h[2];
+    STAGE_TRY(hipMemcpy(h, fl.p, sizeof(h), hipMemcpyDeviceToHost));
+    if (h[1]) {
+        g_global_error = "flow or mass holds NaN/Inf";
+        return HP_ERR_NONFINITE;
+    }
+    const int R = std::max(h[0], 1);
+    auto win = [](int R_, int n_) { return (double)std::min(2 * R_ + 1, n_); };
+    if (win(R, g.W) * win(R, g.H) * (D > 1 ? win(R, g.D) : 1.0) * (double)g.N > 2e10) {
+        g_global_error = "flow displacement beyond the general gather's work bound";
+        return HP_ERR_UNSUPPORTED_SIZE;
+    }
+    hp::launch_gather_general(nullptr, (const float4*)lo.p, (const float4*)hi.p, (float*)o.p, 1, g, s, R);
+    STAGE_TRY(hipGetLastError());
+    STAGE_TRY(hipMemcpy(next, o.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    return HP_OK;
+}
+
+}  // namespace
+
+// ================================== C ABI ===========================================
+extern "C" {
+
+hp_status hp_validate_config(const hp_config* cfg, uint32_t* bad_kernel, uint32_t* bad_channel) {
+    return hp::validate_config(cfg, bad_kernel, bad_channel);
+}
+
+hp_status hp_seed_generate(const hp_seed* seed, uint32_t width, uint32_t height, uint32_t depth, uint32_t channels,
+                           float* const* out_channels) {
+    try {
+        return hp::seed_generate(seed, width, height, depth, channels, out_channels);
+    } catch (...) {
+        return HP_ERR_SHAPE;
+    }
+}
+
+hp_status hp_create(const hp_config* cfg, int device, hp_ctx** out) {
+    if (!out) return HP_ERR_NULL;
+    *out = nullptr;
+    if (!cfg) return HP_ERR_NULL;
+    try {
+        return create(cfg, device, out);
+    } catch (const std::exception& e) {
+        g_global_error = e.what();
+        return HP_ERR_HIP;
+    } catch (...) {
+        g_global_error = "unknown exception";
+        return HP_ERR_HIP;
+    }
+}
+
+void hp_destroy(hp_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    release(ctx);
+}
+
+const char* hp_last_error(const hp_ctx* ctx) { return ctx ? ctx->err.c_str() : "NULL context"; }
+const char* hp_last_global_error(void) { return g_global_error.c_str(); }
+
+hp_status hp_get_config(const hp_ctx* ctx, hp_config* out) {
+    if (!ctx || !out) return HP_ERR_NULL;
+    *out = ctx->cfg;
+    return HP_OK;
+}
+
+hp_status hp_set_state(hp_ctx* ctx, const float* const* channels, float time, uint64_t step) {
+    return guarded(ctx, [&]() -> hp_status {
+        hp_status st = check_host_state(ctx, channels);
+        if (st != HP_OK) return st;
+        st = upload_state(ctx, channels, ctx->A[ctx->cur]);
+        if (st != HP_OK) return st;
+        ctx->time = time;
+        ctx->step = step;
+        return HP_OK;
+    });
+}
+
+hp_status hp_get_state(hp_ctx* ctx, float* const* channels, float* time, uint64_t* step) {
+    return guarded(ctx, [&]() -> hp_status {
+        hp_status st = check_host_state(ctx, channels);
+        if (st != HP_OK) return st;
+        st = download(ctx, ctx->A[ctx->cur], channels);
+        if (st != HP_OK) return st;
+        if (time) *time = ctx->time;
+        if (step) *step = ctx->step;
+        return HP_OK;
+    });
+}
+
+hp_status hp_step(hp_ctx* ctx, float* const* channels, float* time, uint64_t* step) {
+    return guarded(ctx, [&]() -> hp_status {
+        hp_status st = check_host_state(ctx, channels);
+        if (st != HP_OK) return st;
+        st = upload_state(ctx, channels, ctx->A[ctx->cur]);
+        if (st != HP_OK) return st;
+        ctx->time = time ? *time : 0.0f;
+        ctx->step = step ? *step : 0;
+        st = safe_step(ctx);
+        if (st != HP_OK) return st;
+        if (ctx->profiling) collect_profile(ctx);
+        st = download(ctx, ctx->A[ctx->cur], channels);
+        if (st != HP_OK) return st;
+        if (time) *time = ctx->time;
+        if (step) *step = ctx->step;
+        return HP_OK;
+    });
+}
+
+hp_status hp_run(hp_ctx* ctx, float* const* channels, float* time, uint64_t* step, uint64_t steps) {
+    return guarded(ctx, [&]() -> hp_status {
+        hp_status st = check_host_state(ctx, channels);
+        if (st != HP_OK) return st;
+        st = upload_state(ctx, channels, ctx->A[ctx->cur]);
+        if (st != HP_OK) return st;
+        ctx->time = time ? *time : 0.0f;
+        ctx->step = step ? *step : 0;
+        st = advance(ctx, steps);
+        if (st != HP_OK) return st;
+        st = download(ctx, ctx->A[ctx->cur], channels);
+        if (st != HP_OK) return st;
+        if (time) *time = ctx->time;
+        if (step) *step = ctx->step;
+        return HP_OK;
+    });
+}
+
+hp_status hp_advance(hp_ctx* ctx, uint64_t steps) {
+    return guarded(ctx, [&]() -> hp_status { return advance(ctx, steps); });
+}
+
+hp_status hp_mass_f64(hp_ctx* ctx, double* per_channel, double* total) {
+    return guarded(ctx, [&]() -> hp_status {
+        hp::launch_mass_f64(ctx->stream, ctx->A[ctx->cur], ctx->C, ctx->g.N, ctx->d_partials, ctx->mass_parts,
+                            ctx->d_mass);
+        std::vector<double> h(ctx->C + 1);
+        HIP_TRY(ctx, hipMemcpyAsync(h.data(), ctx->d_mass, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        if (per_channel) std::copy(h.begin(), h.begin() + ctx->C, per_channel);
+        if (total) *total = h[ctx->C];
+        return HP_OK;
+    });
+}
+
+hp_status hp_stats(const hp_ctx* ctx, uint32_t* max_gather_radius, uint64_t* general_path_steps) {
+    if (!ctx) return HP_ERR_NULL;
+    if (max_gather_radius) *max_gather_radius = ctx->max_radius;
+    if (general_path_steps) *general_path_steps = ctx->general_steps;
+    return HP_OK;
+}
+
+hp_status hp_profile_enable(hp_ctx* ctx, int enable) {
+    return guarded(ctx, [&]() -> hp_status {
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        collect_profile(ctx);
+        ctx->profiling = enable != 0;
+        for (int i = 0; i < HP_K_NUM_CLASSES; i++) {
+            ctx->prof_ms[i] = 0.0;
+            ctx->prof_n[i] = 0;
+        }
+        return HP_OK;
+    });
+}
+
+hp_status hp_profile_read(hp_ctx* ctx, double* ms, uint64_t* launches) {
+    return guarded(ctx, [&]() -> hp_status {
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        collect_profile(ctx);
+        for (int i = 0; i < HP_K_NUM_CLASSES; i++) {
+            if (ms) ms[i] = ctx->prof_ms[i];
+            if (launches) launches[i] = ctx->prof_n[i];
+        }
+        return HP_OK;
+    });
+}
+
+hp_status hp_kernel_bytes(const hp_ctx* ctx, double* bytes) {
+    if (!ctx || !bytes) return HP_ERR_NULL;
+    const hp::Geo& g = ctx->g;
+    const double N = (double)g.N;
+    const double S = 8.0 * g.Wh * (double)g.H * g.D;  // one half spectrum, complex64
+    const double Kq = 4.0 * g.Wh * (double)g.Hq * g.Dq;
+    const double nsrc = (double)ctx->src_chan.size(), K = ctx->K, C = ctx->C, nt = (double)ctx->tgt_chan.size();
+    for (int i = 0; i < HP_K_NUM_CLASSES; i++) bytes[i] = 0.0;
+    if (ctx->K > 0) {
+        bytes[HP_K_ROWS_FWD] = nsrc * (4.0 * N + S);
+        if (ctx->is3d) bytes[HP_K_COLS_FWD] = nsrc * 2.0 * S;
+        bytes[HP_K_SPECTRAL] = K * (S + Kq + S);
+        if (ctx->is3d) bytes[HP_K_COLS_INV] = K * 2.0 * S;
+        bytes[HP_K_ROWS_INV] = K * S + nt * 4.0 * N;
+    }
+    bytes[HP_K_STENCIL] = 3.0 * C * 4.0 * N;
+    bytes[HP_K_GENERAL] = 3.0 * C * 4.0 * N;
+    return HP_OK;
+}
+
+hp_status hp_advect_mass_2d(int device, const float* current, const float* flow_x, const float* flow_y, float* next,
+                            uint32_t width, uint32_t height, float dt, float distribution_size) {
+    try {
+        return advect_stage(device, current, flow_x, flow_y, nullptr, next, width, height, 1, dt, distribution_size);
+    } catch (...) {
+        return HP_ERR_HIP;
+    }
+}
+
+hp_status hp_advect_mass_3d(int device, const float* current, const float* flow_x, const float* flow_y,
+                            const float* flow_z, float* next, uint32_t width, uint32_t height, uint32_t depth,
+                            float dt, float distribution_size) {
+    try {
+        return advect_stage(device, current, flow_x, flow_y, flow_z, next, width, height, depth, dt,
+                            distribution_size);
+    } catch (...) {
+        return HP_ERR_HIP;
+    }
+}
+
+hp_status hp_sobel_gradient_2d(int device, const float* grid, float* grad_x, float* grad_y, uint32_t width,
+                               uint32_t height) {
+    if (!grid || !grad_x || !grad_y) return HP_ERR_NULL;
+    if (width == 0 || height == 0) return HP_ERR_INVALID_DIMENSIONS;
+    hp_status st = stage_device(device);
+    if (st != HP_OK) return st;
+    const size_t n = (size_t)width * height;
+    DevBuf g, gx, gy;
+    if ((st = to_dev(g, grid, n)) != HP_OK || (st = to_dev(gx, nullptr, n)) != HP_OK ||
+        (st = to_dev(gy, nullptr, n)) != HP_OK)
+        return st;
+    hp::launch_sobel2d(nullptr, (const float*)g.p, (float*)gx.p, (float*)gy.p, (int)width, (int)height);
+    STAGE_TRY(hipGetLastError());
+    STAGE_TRY(hipMemcpy(grad_x, gx.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    STAGE_TRY(hipMemcpy(grad_y, gy.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    return HP_OK;
+}
+
+hp_status hp_compute_flow_2d(int device, const float* gux, const float* guy, const float* gax, const float* gay,
+                             const float* mass_sum, uint32_t n_cells, hp_flow flow, float* flow_x, float* flow_y) {
+    if (!gux || !guy || !gax || !gay || !mass_sum || !flow_x || !flow_y) return HP_ERR_NULL;
+    hp_status st = stage_device(device);
+    if (st != HP_OK) return st;
+    const size_t n = n_cells;
+    DevBuf a, b, c2, d, m, fx, fy;
+    if ((st = to_dev(a, gux, n)) != HP_OK || (st = to_dev(b, guy, n)) != HP_OK || (st = to_dev(c2, gax, n)) != HP_OK ||
+        (st = to_dev(d, gay, n)) != HP_OK || (st = to_dev(m, mass_sum, n)) != HP_OK ||
+        (st = to_dev(fx, nullptr, n)) != HP_OK || (st = to_dev(fy, nullptr, n)) != HP_OK)
+        return st;
+    hp::launch_flow2d(nullptr, (const float*)a.p, (const float*)b.p, (const float*)c2.p, (const float*)d.p,
+                      (const float*)m.p, (long long)n, flow.beta_a, flow.n, (float*)fx.p, (float*)fy.p);
+    STAGE_TRY(hipGetLastError());
+    STAGE_TRY(hipMemcpy(flow_x, fx.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    STAGE_TRY(hipMemcpy(flow_y, fy.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    return HP_OK;
+}
+
+hp_status hp_growth_accumulate(int device, const float* conv, float* target, uint32_t n_cells, float weight, float mu,
+                               float sigma) {
+    if (!conv || !target) return HP_ERR_NULL;
+    hp_status st = stage_device(device);
+    if (st != HP_OK) return st;
+    DevBuf a, t;
+    if ((st = to_dev(a, conv, n_cells)) != HP_OK || (st = to_dev(t, target, n_cells)) != HP_OK) return st;
+    hp::launch_growth_acc(nullptr, (const float*)a.p, (float*)t.p, (long long)n_cells, weight, mu, sigma);
+    STAGE_TRY(hipGetLastError());
+    STAGE_TRY(hipMemcpy(target, t.p, (size_t)n_cells * sizeof(float), hipMemcpyDeviceToHost));
+    return HP_OK;
+}
+
+hp_status hp_stencil_with_affinity(hp_ctx* ctx, const float* const* state, const float* const* affinity,
+                                   float* const* next) {
+    return guarded(ctx, [&]() -> hp_status {
+        hp_status st = check_host_state(ctx, state);
+        if (st == HP_OK) st = check_host_state(ctx, affinity);
+        if (st == HP_OK) st = check_host_state(ctx, next);
+        if (st != HP_OK) return st;
+        // stage the inputs into the non-current state buffer and U; output into the current one's
+        // snapshot buffer, so the device-resident state is left untouched
+        float* A = ctx->A[1 - ctx->cur];
+        if ((st = upload_state(ctx, state, A)) != HP_OK) return st;
+        if ((st = upload_state(ctx, affinity, ctx->U)) != HP_OK) return st;
+        bool general = !fast_stencil_ok(ctx);
+        if (!general) {
+            hp::launch_clear_ints(ctx->stream, ctx->d_flags, 1);
+            if ((st = enqueue_fast_stencil(ctx, A, ctx->U, ctx->snapshot)) != HP_OK) return st;
+            int f = 0;
+            HIP_TRY(ctx, hipMemcpyAsync(&f, ctx->d_flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            if (f & 2) return ctx->fail(HP_ERR_NONFINITE, "flow holds NaN/Inf");
+            general = (f & 1) != 0;
+        }
+        if (general && (st = general_stencil(ctx, A, ctx->U, ctx->snapshot)) != HP_OK) return st;
+        if (ctx->profiling) collect_profile(ctx);
+        return download(ctx, ctx->snapshot, next);
+    });
+}
+
+hp_status hp_affinity(hp_ctx* ctx, const float* const* state, float* const* affinity) {
+    return guarded(ctx, [&]() -> hp_status {
+        hp_status st = check_host_state(ctx, state);
+        if (st == HP_OK) st = check_host_state(ctx, affinity);
+        if (st != HP_OK) return st;
+        float* A = ctx->A[1 - ctx->cur];
+        if ((st = upload_state(ctx, state, A)) != HP_OK) return st;
+        if ((st = enqueue_affinity(ctx, A)) != HP_OK) return st;
+        if (ctx->profiling) {
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            collect_profile(ctx);
+        }
+        return download(ctx, ctx->U, affinity);
+    });
+}
+
+}  // extern "C"

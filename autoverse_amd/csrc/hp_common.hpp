@@ -1,0 +1,86 @@
+// hp_common.hpp -- shared device/host declarations of the Flow Lenia HIP library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace hp {
+
+// Work item of the spectral pass: one source spectrum and a run of kernels that
+// read it (kernel indices klist[k_first .. k_first + k_count)).
+struct SpecWork {
+    int src_slot;
+    int k_first;
+    int k_count;
+    int pad;
+};
+
+// Work item of the inverse row pass: one target channel and, in kernel-index
+// order, the kernels that accumulate into it.
+struct TgtWork {
+    int target;
+    int k_first;
+    int k_count;
+    int pad;
+};
+
+// Growth parameters of one kernel: weight, mu, 1/(2 sigma^2) (growth.rs:51-52).
+struct GrowthParams {
+    float weight;
+    float mu;
+    float inv;
+    float pad;
+};
+
+// Shapes of one 2D/3D grid and its half spectrum.
+struct Geo {
+    int W, H, D;      // grid
+    int Wh;           // W/2 + 1 complex columns of the half spectrum
+    int Wp;           // padded row pitch (complex elements) of a spectrum row
+    int Hq, Dq;       // H/2 + 1, D/2 + 1 (folded kernel-spectrum extents)
+    long long N;      // W*H*D
+    long long spec;   // complex elements of one spectrum: D*H*Wp
+};
+
+// ---- launchers implemented in fft_kernels.hip -------------------------------------
+// Each returns false if the length is not one of the compiled FFT sizes.
+bool fft_supported(int L);
+bool launch_rows_fwd(int L, hipStream_t st, const float* A, float2* X, const int* src_chan, int nsrc,
+                     const Geo& g, const float2* tw);
+bool launch_cols(int L, hipStream_t st, int dir, float2* X, int nspec, const Geo& g, const float2* tw);
+bool launch_spectral(int L, int dim, hipStream_t st, const float2* X, float2* Y, const float* Kq,
+                     const SpecWork* work, int nwork, const int* klist, const Geo& g, const float2* tw);
+bool launch_rows_inv(int L, hipStream_t st, const float2* Y, float* U, const TgtWork* work, int nwork,
+                     const int* klist, const GrowthParams* gp, const Geo& g, const float2* tw);
+
+// ---- launchers implemented in stencil_kernels.hip ----------------------------------
+// Fused channel sum + Sobel + flow + gather reintegration with a fixed gather
+// radius; sets *flag when any active source's box leaves that radius (bit 0) or
+// is not finite (bit 1).  Returns false if the configuration is not covered.
+bool launch_stencil2d(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g,
+                      float dt, float beta_a, float nexp, float s, int* flag);
+bool launch_stencil3d(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g,
+                      float dt, float beta_a, float nexp, float s, int* flag);
+// General path: per-source destination boxes (+ needed radius via atomicMax)...
+void launch_boxes(hipStream_t st, const float* A, const float* U, int C, const Geo& g, float dt, float beta_a,
+                  float nexp, float s, float4* box_lo, float4* box_hi, int* need_radius);
+// ...then a gather over any radius with full periodic-image handling.
+void launch_gather_general(hipStream_t st, const float4* box_lo, const float4* box_hi, float* Aout, int C,
+                           const Geo& g, float s, int radius);
+// Stage kernels exposed for the stage entry points.
+void launch_sobel2d(hipStream_t st, const float* grid, float* gx, float* gy, int W, int H);
+void launch_flow2d(hipStream_t st, const float* gux, const float* guy, const float* gax, const float* gay,
+                   const float* mass, long long n, float beta_a, float nexp, float* fx, float* fy);
+void launch_growth_acc(hipStream_t st, const float* conv, float* target, long long n, float weight, float mu,
+                       float sigma);
+// Boxes from given flow fields (for the advect stage entry point).
+void launch_boxes_from_flow(hipStream_t st, const float* A, const float* fx, const float* fy, const float* fz,
+                            const Geo& g, float dt, float s, float4* box_lo, float4* box_hi, int* need_radius);
+// FP64 mass: per-channel partial sums in a fixed order, then a fixed-order final sum.
+void launch_mass_f64(hipStream_t st, const float* A, int C, long long n, double* partials, int nparts,
+                     double* out);
+// Clear a small int buffer with a kernel (graph-capture safe, no memset node).
+void launch_clear_ints(hipStream_t st, int* p, int n);
+
+}  // namespace hp

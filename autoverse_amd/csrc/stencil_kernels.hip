@@ -1,0 +1,594 @@
+// stencil_kernels.hip -- stages 2-5 of the Flow Lenia step on gfx950, and the FP64 mass.
+//
+// Compiled with -ffp-contract=off: every product and sum rounds on its own, as in
+// the reference's Rust f32 code, so that given the same affinity U these kernels
+// reproduce CpuPropagator's next state bit for bit.
+//
+// Reference stages restated (paths relative to the reference repository):
+//   channel sum            src/compute/propagator.rs:304-310  (0 + A_0 + A_1 + ...)
+//   Sobel 2D / 3D          src/compute/gradient.rs:68-108, src/compute/gradient3d.rs:41-143
+//   alpha, flow            src/compute/flow.rs:14-32, 68-87; src/compute/flow3d.rs:50-73
+//   reintegration          src/compute/reintegration.rs:60-174, src/compute/reintegration3d.rs:56-198
+//
+// The reference reintegrates by scattering each source's mass over the cells its
+// box [dest - s, dest + s] overlaps, adding into next[] in source raster order.
+// Here every destination gathers instead: it visits the sources that can reach it
+// in ascending (wrapped) raster order and adds exactly the same products, so each
+// destination's sum is formed in the same order with the same roundings --
+// deterministic and atomic-free.  The fused kernel assumes every active source
+// lands within RG cells; a source that does not raises a flag and the host reruns
+// the step through the general-radius path (boxes + gather_general), which walks
+// every periodic image the scatter could produce.
+#include <math.h>
+
+#include <algorithm>
+
+#include "hp_common.hpp"
+
+namespace hp {
+
+__device__ __forceinline__ int wrapi(int c, int n) {
+    int r = c % n;
+    return r < 0 ? r + n : r;
+}
+
+// Rust `f as i32`: saturating, NaN -> 0.
+__device__ __forceinline__ int sat_i32(float v) {
+    if (v != v) return 0;
+    if (v >= 2147483647.0f) return 2147483647;
+    if (v <= -2147483648.0f) return (int)0x80000000;
+    return (int)v;
+}
+
+// compute_alpha (flow.rs:14-32): integer fast paths for n = 1..4, else powf; clamp keeps NaN.
+__device__ __forceinline__ float alpha_of(float mass, float beta_a, float n) {
+    const float ratio = mass / beta_a;
+    float p;
+    if (n == 1.0f) p = ratio;
+    else if (n == 2.0f) p = ratio * ratio;
+    else if (n == 3.0f) p = ratio * ratio * ratio;
+    else if (n == 4.0f) {
+        const float r2 = ratio * ratio;
+        p = r2 * r2;
+    } else p = powf(ratio, n);
+    if (p < 0.0f) p = 0.0f;
+    if (p > 1.0f) p = 1.0f;
+    return p;
+}
+
+// gradient.rs:99-102
+__device__ __forceinline__ void sobel9(float tl, float tc, float tr, float ml, float mr, float bl, float bc,
+                                       float br, float& gx, float& gy) {
+    gx = (-tl + tr - 2.0f * ml + 2.0f * mr - bl + br) * 0.125f;
+    gy = (-tl - 2.0f * tc - tr + bl + 2.0f * bc + br) * 0.125f;
+}
+
+// Overlap of cell [c0, c0 + 1) (c1 = c0 + 1 as f32) with [lo, hi] (reintegration.rs:137-144):
+// max(min(c1, hi) - max(c0, lo), 0) with Rust max/min (a NaN operand is ignored).
+__device__ __forceinline__ float overlap1(float c0, float c1, float lo, float hi) {
+    return fmaxf(fminf(c1, hi) - fmaxf(c0, lo), 0.0f);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fused 2D stencil.  One workgroup owns a TX x TY tile of destinations.  LDS holds the channel
+// sum over the tile + (RG+1) halo, the mass gradient and alpha over the tile + RG halo, and per
+// channel the affinity halo tile and every source's box [xm, xM] x [ym, yM] and mass.
+// ---------------------------------------------------------------------------------------------
+template <int RG, bool POW2, int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) void k_stencil2d(const float* __restrict__ A, const float* __restrict__ U,
+                                                  float* __restrict__ Aout, int C, int W, int H, long long N,
+                                                  float dt, float beta_a, float nexp, float s, float ta,
+                                                  float inv_ta, int* __restrict__ flag) {
+    constexpr int LW = TX + 2 * RG + 2, LH = TY + 2 * RG + 2;
+    constexpr int SW = TX + 2 * RG, SH = TY + 2 * RG;
+    constexpr int WIN = 2 * RG + 1;
+    __shared__ float sSum[LH * LW];
+    __shared__ float sU[LH * LW];
+    __shared__ float sAl[SH * SW], sGx[SH * SW], sGy[SH * SW];
+    __shared__ float sXm[SH * SW], sXM[SH * SW], sYm[SH * SW], sYM[SH * SW], sM[SH * SW];
+
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const int lx0 = x0 - RG - 1, ly0 = y0 - RG - 1;
+    const int tid = threadIdx.x;
+
+    // channel sum over the load region (propagator.rs:304-310)
+    for (int i = tid; i < LH * LW; i += NT) {
+        const int ly = i / LW, lx = i - ly * LW;
+        const long long g = (long long)wrapi(ly0 + ly, H) * W + wrapi(lx0 + lx, W);
+        float sum = 0.0f;
+        for (int c = 0; c < C; ++c) sum += A[(long long)c * N + g];
+        sSum[i] = sum;
+    }
+    __syncthreads();
+    // mass gradient and alpha over the source region
+    for (int i = tid; i < SH * SW; i += NT) {
+        const int sy = i / SW, sx = i - sy * SW;
+        const float* r0 = sSum + sy * LW + sx;
+        const float* r1 = r0 + LW;
+        const float* r2 = r1 + LW;
+        float gx, gy;
+        sobel9(r0[0], r0[1], r0[2], r1[0], r1[2], r2[0], r2[1], r2[2], gx, gy);
+        sGx[i] = gx;
+        sGy[i] = gy;
+        sAl[i] = alpha_of(r1[1], beta_a, nexp);
+    }
+    int local_flag = 0;
+    for (int c = 0; c < C; ++c) {
+        __syncthreads();  // sU and the box arrays are free
+        const float* Uc = U + (long long)c * N;
+        for (int i = tid; i < LH * LW; i += NT) {
+            const int ly = i / LW, lx = i - ly * LW;
+            sU[i] = Uc[(long long)wrapi(ly0 + ly, H) * W + wrapi(lx0 + lx, W)];
+        }
+        __syncthreads();
+        const float* Ac = A + (long long)c * N;
+        for (int i = tid; i < SH * SW; i += NT) {
+            const int sy = i / SW, sx = i - sy * SW;
+            const int ux = x0 - RG + sx, uy = y0 - RG + sy;  // unwrapped source coords
+            const int gxs = wrapi(ux, W), gys = wrapi(uy, H);
+            const float m = Ac[(long long)gys * W + gxs];
+            if (fabsf(m) < 1e-10f) {  // reintegration.rs:76: skipped source
+                sXm[i] = INFINITY;
+                sXM[i] = -INFINITY;
+                sYm[i] = INFINITY;
+                sYM[i] = -INFINITY;
+                sM[i] = 0.0f;
+                continue;
+            }
+            const float* r0 = sU + sy * LW + sx;
+            const float* r1 = r0 + LW;
+            const float* r2 = r1 + LW;
+            float gux, guy;
+            sobel9(r0[0], r0[1], r0[2], r1[0], r1[2], r2[0], r2[1], r2[2], gux, guy);
+            const float al = sAl[i];
+            const float oma = 1.0f - al;
+            const float fx = oma * gux - al * sGx[i];
+            const float fy = oma * guy - al * sGy[i];
+            const float dxp = (float)gxs + dt * fx;  // reintegration.rs:81-82
+            const float dyp = (float)gys + dt * fy;
+            const float xm = dxp - s, xM = dxp + s, ym = dyp - s, yM = dyp + s;
+            sXm[i] = xm;
+            sXM[i] = xM;
+            sYm[i] = ym;
+            sYM[i] = yM;
+            sM[i] = m;
+            // each source is checked by the tile that owns it
+            if (ux >= x0 && ux < x0 + TX && uy >= y0 && uy < y0 + TY && ux < W && uy < H) {
+                if (!(isfinite(xm) && isfinite(xM) && isfinite(ym) && isfinite(yM) && isfinite(m))) local_flag |= 2;
+                else if (floorf(xm) < (float)(gxs - RG) || ceilf(xM) > (float)(gxs + RG + 1) ||
+                         floorf(ym) < (float)(gys - RG) || ceilf(yM) > (float)(gys + RG + 1))
+                    local_flag |= 1;
+            }
+        }
+        __syncthreads();
+        float* Oc = Aout + (long long)c * N;
+        for (int d = tid; d < TX * TY; d += NT) {
+            const int tx = d % TX, ty = d / TX;
+            const int gdx = x0 + tx, gdy = y0 + ty;
+            if (gdx >= W || gdy >= H) continue;
+            // visit window rows / columns in ascending wrapped order (source raster order)
+            const int rotx = gdx < RG ? RG - gdx : (gdx + RG >= W ? W - gdx + RG : 0);
+            const int roty = gdy < RG ? RG - gdy : (gdy + RG >= H ? H - gdy + RG : 0);
+            float acc = 0.0f;
+#pragma unroll
+            for (int iy = 0; iy < WIN; ++iy) {
+                int jy = iy + roty;
+                if (jy >= WIN) jy -= WIN;
+                const int oy = jy - RG;
+                const int syy = gdy + oy;
+                const int cyi = gdy + (syy < 0 ? H : (syy >= H ? -H : 0));  // dest row in source frame
+                const float cy0 = (float)cyi, cy1 = (float)(cyi + 1);
+                const int srow = (ty + RG + oy) * SW + tx + RG;
+#pragma unroll
+                for (int ix = 0; ix < WIN; ++ix) {
+                    int jx = ix + rotx;
+                    if (jx >= WIN) jx -= WIN;
+                    const int ox = jx - RG;
+                    const int sxx = gdx + ox;
+                    const int cxi = gdx + (sxx < 0 ? W : (sxx >= W ? -W : 0));
+                    const float cx0 = (float)cxi, cx1 = (float)(cxi + 1);
+                    const int si = srow + ox;
+                    const float ow = overlap1(cx0, cx1, sXm[si], sXM[si]);
+                    const float oh = overlap1(cy0, cy1, sYm[si], sYM[si]);
+                    const float area = ow * oh;
+                    if (area > 0.0f) {
+                        const float frac = POW2 ? area * inv_ta : area / ta;
+                        acc += sM[si] * frac;
+                    }
+                }
+            }
+            Oc[(long long)gdy * W + gdx] = acc;
+        }
+    }
+    if (local_flag) atomicOr(flag, local_flag);
+}
+
+bool launch_stencil2d(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g, float dt,
+                      float beta_a, float nexp, float s, int* flag) {
+    constexpr int RG = 2, TX = 64, TY = 16, NT = 256;
+    if (g.D != 1 || g.W < 2 * RG + 1 || g.H < 2 * RG + 1) return false;
+    const float ta = (2.0f * s) * (2.0f * s);
+    if (!(ta >= 1e-10f)) return false;  // point-distribution fallback lives in the general path
+    int e;
+    const float fr = frexpf(ta, &e);
+    const bool pow2 = fr == 0.5f && ta >= 1.17549435e-38f && ta < INFINITY;
+    const float inv_ta = 1.0f / ta;
+    dim3 grid((g.W + TX - 1) / TX, (g.H + TY - 1) / TY);
+    if (pow2)
+        hipLaunchKernelGGL((k_stencil2d<RG, true, TX, TY, NT>), grid, dim3(NT), 0, st, A, U, Aout, C, g.W, g.H, g.N,
+                           dt, beta_a, nexp, s, ta, inv_ta, flag);
+    else
+        hipLaunchKernelGGL((k_stencil2d<RG, false, TX, TY, NT>), grid, dim3(NT), 0, st, A, U, Aout, C, g.W, g.H,
+                           g.N, dt, beta_a, nexp, s, ta, inv_ta, flag);
+    return true;
+}
+
+bool launch_stencil3d(hipStream_t, const float*, const float*, float*, int, const Geo&, float, float, float, float,
+                      int*) {
+    return false;  // 3D runs through the general path
+}
+
+// ---------------------------------------------------------------------------------------------
+// General path.  (1) boxes: per (channel, source) the box [lo, hi] in the source's wrapped
+// frame and the mass (inactive sources get an empty box), plus the radius every active source
+// needs, via atomicMax.  (2) gather_general: per (channel, destination), every source in the
+// (2R+1)^d window in ascending raster order, every periodic image of the destination inside
+// the source's clamped index range (iz, iy, ix ascending, as the scatter loops).
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ void source_box(int dim, float m, int gxs, int gys, int gzs, float fx, float fy, float fz,
+                                           float dt, float s, int W, int H, int D, float4& lo, float4& hi,
+                                           int& need) {
+    need = 0;
+    if (fabsf(m) < 1e-10f) {
+        lo = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+        hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+        return;
+    }
+    const float dxp = (float)gxs + dt * fx;
+    const float dyp = (float)gys + dt * fy;
+    const float dzp = dim == 3 ? (float)gzs + dt * fz : 0.0f;
+    const float ta = dim == 3 ? (2.0f * s) * (2.0f * s) * (2.0f * s) : (2.0f * s) * (2.0f * s);
+    if (ta < 1e-10f) {  // point fallback (reintegration.rs:117-124): all mass to round(dest)
+        lo = make_float4(dxp, dyp, dzp, m);
+        hi = make_float4(dxp, dyp, dzp, 1.0f);  // w = 1 marks point mode
+        int rx = abs(sat_i32(roundf(dxp)) - gxs), ry = abs(sat_i32(roundf(dyp)) - gys);
+        need = max(rx, ry);
+        if (dim == 3) need = max(need, abs(sat_i32(roundf(dzp)) - gzs));
+        return;
+    }
+    lo = make_float4(dxp - s, dyp - s, dzp - s, m);
+    hi = make_float4(dxp + s, dyp + s, dzp + s, 0.0f);
+    const int ixl = max(sat_i32(floorf(lo.x)), -W), ixh = min(sat_i32(ceilf(hi.x)), 2 * W);
+    const int iyl = max(sat_i32(floorf(lo.y)), -H), iyh = min(sat_i32(ceilf(hi.y)), 2 * H);
+    need = max(max(gxs - ixl, ixh - gxs), max(gys - iyl, iyh - gys));
+    if (dim == 3) {
+        const int izl = max(sat_i32(floorf(lo.z)), -D), izh = min(sat_i32(ceilf(hi.z)), 2 * D);
+        need = max(need, max(gzs - izl, izh - gzs));
+    }
+    if (!(isfinite(lo.x) && isfinite(hi.x) && isfinite(lo.y) && isfinite(hi.y) && isfinite(lo.z) &&
+          isfinite(hi.z) && isfinite(m)))
+        need = -1;  // non-finite
+}
+
+__device__ __forceinline__ float asum_at(const float* A, int C, long long N, long long idx) {
+    float s = 0.0f;
+    for (int c = 0; c < C; ++c) s += A[(long long)c * N + idx];
+    return s;
+}
+
+__device__ __forceinline__ void sobel27(const float* g, const float* A, int C, long long N, int W, int H, int D,
+                                        int x, int y, int z, bool sum_channels, float& vx, float& vy, float& vz) {
+    // gradient3d.rs:75-138 with the same grouping and the final / 32
+    const int xp = (x + W - 1) % W, xn = (x + 1) % W, yp = (y + H - 1) % H, yn = (y + 1) % H;
+    const int zp = (z + D - 1) % D, zn = (z + 1) % D;
+    auto P = [&](int zz, int yy, int xx) -> float {
+        const long long i = ((long long)zz * H + yy) * W + xx;
+        return sum_channels ? asum_at(A, C, N, i) : g[i];
+    };
+    const float p_mmm = P(zp, yp, xp), p_mm0 = P(zp, yp, x), p_mmp = P(zp, yp, xn);
+    const float p_m0m = P(zp, y, xp), p_m00 = P(zp, y, x), p_m0p = P(zp, y, xn);
+    const float p_mpm = P(zp, yn, xp), p_mp0 = P(zp, yn, x), p_mpp = P(zp, yn, xn);
+    const float p_0mm = P(z, yp, xp), p_0m0 = P(z, yp, x), p_0mp = P(z, yp, xn);
+    const float p_00m = P(z, y, xp), p_00p = P(z, y, xn);
+    const float p_0pm = P(z, yn, xp), p_0p0 = P(z, yn, x), p_0pp = P(z, yn, xn);
+    const float p_pmm = P(zn, yp, xp), p_pm0 = P(zn, yp, x), p_pmp = P(zn, yp, xn);
+    const float p_p0m = P(zn, y, xp), p_p00 = P(zn, y, x), p_p0p = P(zn, y, xn);
+    const float p_ppm = P(zn, yn, xp), p_pp0 = P(zn, yn, x), p_ppp = P(zn, yn, xn);
+    const float gx_zm = -1.0f * (1.0f * p_mmm + 2.0f * p_m0m + 1.0f * p_mpm) + 1.0f * (1.0f * p_mmp + 2.0f * p_m0p + 1.0f * p_mpp);
+    const float gx_z0 = -1.0f * (1.0f * p_0mm + 2.0f * p_00m + 1.0f * p_0pm) + 1.0f * (1.0f * p_0mp + 2.0f * p_00p + 1.0f * p_0pp);
+    const float gx_zp = -1.0f * (1.0f * p_pmm + 2.0f * p_p0m + 1.0f * p_ppm) + 1.0f * (1.0f * p_pmp + 2.0f * p_p0p + 1.0f * p_ppp);
+    vx = (1.0f * gx_zm + 2.0f * gx_z0 + 1.0f * gx_zp) / 32.0f;
+    const float gy_zm = -1.0f * (1.0f * p_mmm + 2.0f * p_mm0 + 1.0f * p_mmp) + 1.0f * (1.0f * p_mpm + 2.0f * p_mp0 + 1.0f * p_mpp);
+    const float gy_z0 = -1.0f * (1.0f * p_0mm + 2.0f * p_0m0 + 1.0f * p_0mp) + 1.0f * (1.0f * p_0pm + 2.0f * p_0p0 + 1.0f * p_0pp);
+    const float gy_zp = -1.0f * (1.0f * p_pmm + 2.0f * p_pm0 + 1.0f * p_pmp) + 1.0f * (1.0f * p_ppm + 2.0f * p_pp0 + 1.0f * p_ppp);
+    vy = (1.0f * gy_zm + 2.0f * gy_z0 + 1.0f * gy_zp) / 32.0f;
+    const float gz_xm = -1.0f * (1.0f * p_mmm + 2.0f * p_m0m + 1.0f * p_mpm) + 1.0f * (1.0f * p_pmm + 2.0f * p_p0m + 1.0f * p_ppm);
+    const float gz_x0 = -1.0f * (1.0f * p_mm0 + 2.0f * p_m00 + 1.0f * p_mp0) + 1.0f * (1.0f * p_pm0 + 2.0f * p_p00 + 1.0f * p_pp0);
+    const float gz_xp = -1.0f * (1.0f * p_mmp + 2.0f * p_m0p + 1.0f * p_mpp) + 1.0f * (1.0f * p_pmp + 2.0f * p_p0p + 1.0f * p_ppp);
+    vz = (1.0f * gz_xm + 2.0f * gz_x0 + 1.0f * gz_xp) / 32.0f;
+}
+
+// grid-stride over C*N (channel-major)
+__global__ void k_boxes(const float* __restrict__ A, const float* __restrict__ U, int C, int W, int H, int D,
+                        long long N, float dt, float beta_a, float nexp, float s, float4* __restrict__ box_lo,
+                        float4* __restrict__ box_hi, int* __restrict__ need_radius) {
+    const long long total = (long long)C * N;
+    int local = 0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int c = (int)(t / N);
+        const long long i = t - (long long)c * N;
+        const int x = (int)(i % W);
+        const int y = (int)((i / W) % H);
+        const int z = (int)(i / ((long long)W * H));
+        const float m = A[t];
+        float fx, fy, fz = 0.0f;
+        const float al = alpha_of(asum_at(A, C, N, i), beta_a, nexp);
+        const float oma = 1.0f - al;
+        if (D == 1) {
+            const int xp = (x + W - 1) % W, xn = (x + 1) % W, yp = (y + H - 1) % H, yn = (y + 1) % H;
+            const long long rp = (long long)yp * W, rc = (long long)y * W, rn = (long long)yn * W;
+            float gax, gay, gux, guy;
+            sobel9(asum_at(A, C, N, rp + xp), asum_at(A, C, N, rp + x), asum_at(A, C, N, rp + xn),
+                   asum_at(A, C, N, rc + xp), asum_at(A, C, N, rc + xn), asum_at(A, C, N, rn + xp),
+                   asum_at(A, C, N, rn + x), asum_at(A, C, N, rn + xn), gax, gay);
+            const float* Uc = U + (long long)c * N;
+            sobel9(Uc[rp + xp], Uc[rp + x], Uc[rp + xn], Uc[rc + xp], Uc[rc + xn], Uc[rn + xp], Uc[rn + x],
+                   Uc[rn + xn], gux, guy);
+            fx = oma * gux - al * gax;
+            fy = oma * guy - al * gay;
+        } else {
+            float gax, gay, gaz, gux, guy, guz;
+            sobel27(nullptr, A, C, N, W, H, D, x, y, z, true, gax, gay, gaz);
+            sobel27(U + (long long)c * N, A, C, N, W, H, D, x, y, z, false, gux, guy, guz);
+            fx = oma * gux - al * gax;
+            fy = oma * guy - al * gay;
+            fz = oma * guz - al * gaz;
+        }
+        float4 lo, hi;
+        int need;
+        source_box(D == 1 ? 2 : 3, m, x, y, z, fx, fy, fz, dt, s, W, H, D, lo, hi, need);
+        box_lo[t] = lo;
+        box_hi[t] = hi;
+        local = (need < 0 || local < 0) ? -1 : max(local, need);
+    }
+    if (local < 0) atomicOr(need_radius + 1, 2);
+    else atomicMax(need_radius, local);
+}
+
+__global__ void k_boxes_from_flow(const float* __restrict__ A, const float* __restrict__ fxv,
+                                  const float* __restrict__ fyv, const float* __restrict__ fzv, int W, int H, int D,
+                                  long long N, float dt, float s, float4* __restrict__ box_lo,
+                                  float4* __restrict__ box_hi, int* __restrict__ need_radius) {
+    int local = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+        const int x = (int)(i % W);
+        const int y = (int)((i / W) % H);
+        const int z = (int)(i / ((long long)W * H));
+        float4 lo, hi;
+        int need;
+        source_box(D == 1 ? 2 : 3, A[i], x, y, z, fxv[i], fyv[i], fzv ? fzv[i] : 0.0f, dt, s, W, H, D, lo, hi, need);
+        box_lo[i] = lo;
+        box_hi[i] = hi;
+        local = (need < 0 || local < 0) ? -1 : max(local, need);
+    }
+    if (local < 0) atomicOr(need_radius + 1, 2);
+    else atomicMax(need_radius, local);
+}
+
+// Ascending wrapped coordinates of the window [d - R, d + R] on an axis of n cells
+// (all of [0, n) when the window covers the axis).
+__device__ __forceinline__ int win_count(int R, int n) { return 2 * R + 1 >= n ? n : 2 * R + 1; }
+__device__ __forceinline__ int win_at(int i, int d, int R, int n) {
+    if (2 * R + 1 >= n) return i;
+    const int lo = d - R, hi = d + R;
+    if (lo >= 0 && hi < n) return lo + i;
+    if (lo < 0) return i <= hi ? i : lo + n + (i - hi - 1);  // [0, hi] then [lo + n, n)
+    const int a = hi - n + 1;                                 // [0, hi - n] then [lo, n)
+    return i < a ? i : lo + (i - a);
+}
+
+__global__ void k_gather_general(const float4* __restrict__ box_lo, const float4* __restrict__ box_hi,
+                                 float* __restrict__ Aout, int C, int W, int H, int D, long long N, float s, int R) {
+    const long long total = (long long)C * N;
+    const bool is3 = D > 1;
+    const float ta = is3 ? (2.0f * s) * (2.0f * s) * (2.0f * s) : (2.0f * s) * (2.0f * s);
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int c = (int)(t / N);
+        const long long i = t - (long long)c * N;
+        const int dx = (int)(i % W);
+        const int dy = (int)((i / W) % H);
+        const int dz = (int)(i / ((long long)W * H));
+        const float4* lo_c = box_lo + (long long)c * N;
+        const float4* hi_c = box_hi + (long long)c * N;
+        const int nzw = is3 ? win_count(R, D) : 1, nyw = win_count(R, H), nxw = win_count(R, W);
+        float acc = 0.0f;
+        for (int a = 0; a < nzw; ++a) {
+            const int sz = is3 ? win_at(a, dz, R, D) : 0;
+            for (int b = 0; b < nyw; ++b) {
+                const int sy = win_at(b, dy, R, H);
+                for (int e = 0; e < nxw; ++e) {
+                    const int sx = win_at(e, dx, R, W);
+                    const long long si = ((long long)sz * H + sy) * W + sx;
+                    const float4 lo = lo_c[si];
+                    const float4 hi = hi_c[si];
+                    if (!(lo.x <= hi.x)) continue;  // inactive source (empty box)
+                    const float m = lo.w;
+                    if (hi.w != 0.0f) {  // point mode: lo holds dest
+                        const int nx = wrapi(sat_i32(roundf(lo.x)), W);
+                        const int ny = wrapi(sat_i32(roundf(lo.y)), H);
+                        const int nz = is3 ? wrapi(sat_i32(roundf(lo.z)), D) : 0;
+                        if (nx == dx && ny == dy && nz == dz) acc += m;
+                        continue;
+                    }
+                    const int ixl = max(sat_i32(floorf(lo.x)), -W), ixh = min(sat_i32(ceilf(hi.x)), 2 * W);
+                    const int iyl = max(sat_i32(floorf(lo.y)), -H), iyh = min(sat_i32(ceilf(hi.y)), 2 * H);
+                    int izl = 0, izh = 0;
+                    if (is3) {
+                        izl = max(sat_i32(floorf(lo.z)), -D);
+                        izh = min(sat_i32(ceilf(hi.z)), 2 * D);
+                    }
+                    for (int kz = (is3 ? -1 : 0); kz <= (is3 ? 2 : 0); ++kz) {
+                        const int iz = dz + kz * D;
+                        if (is3 && (iz < izl || iz > izh)) continue;
+                        float od = 1.0f;
+                        if (is3) od = overlap1((float)iz, (float)(iz + 1), lo.z, hi.z);
+                        for (int ky = -1; ky <= 2; ++ky) {
+                            const int iy = dy + ky * H;
+                            if (iy < iyl || iy > iyh) continue;
+                            const float oh = overlap1((float)iy, (float)(iy + 1), lo.y, hi.y);
+                            for (int kx = -1; kx <= 2; ++kx) {
+                                const int ix = dx + kx * W;
+                                if (ix < ixl || ix > ixh) continue;
+                                const float ow = overlap1((float)ix, (float)(ix + 1), lo.x, hi.x);
+                                const float vol = is3 ? ow * oh * od : ow * oh;
+                                if (vol > 0.0f) {
+                                    const float frac = vol / ta;
+                                    acc += m * frac;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        Aout[t] = acc;
+    }
+}
+
+void launch_boxes(hipStream_t st, const float* A, const float* U, int C, const Geo& g, float dt, float beta_a,
+                  float nexp, float s, float4* box_lo, float4* box_hi, int* need_radius) {
+    const long long total = (long long)C * g.N;
+    const int blocks = (int)std::min<long long>((total + 255) / 256, 65535);
+    hipLaunchKernelGGL(k_boxes, dim3(blocks), dim3(256), 0, st, A, U, C, g.W, g.H, g.D, g.N, dt, beta_a, nexp, s,
+                       box_lo, box_hi, need_radius);
+}
+
+void launch_boxes_from_flow(hipStream_t st, const float* A, const float* fx, const float* fy, const float* fz,
+                            const Geo& g, float dt, float s, float4* box_lo, float4* box_hi, int* need_radius) {
+    const int blocks = (int)std::min<long long>((g.N + 255) / 256, 65535);
+    hipLaunchKernelGGL(k_boxes_from_flow, dim3(blocks), dim3(256), 0, st, A, fx, fy, fz, g.W, g.H, g.D, g.N, dt, s,
+                       box_lo, box_hi, need_radius);
+}
+
+void launch_gather_general(hipStream_t st, const float4* box_lo, const float4* box_hi, float* Aout, int C,
+                           const Geo& g, float s, int radius) {
+    const long long total = (long long)C * g.N;
+    const int blocks = (int)std::min<long long>((total + 255) / 256, 65535);
+    hipLaunchKernelGGL(k_gather_general, dim3(blocks), dim3(256), 0, st, box_lo, box_hi, Aout, C, g.W, g.H, g.D,
+                       g.N, s, radius);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Stage kernels behind the stage entry points.
+// ---------------------------------------------------------------------------------------------
+__global__ void k_sobel2d(const float* __restrict__ g, float* __restrict__ gx, float* __restrict__ gy, int W, int H) {
+    const long long n = (long long)W * H;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int x = (int)(i % W), y = (int)(i / W);
+        const int xp = (x + W - 1) % W, xn = (x + 1) % W, yp = (y + H - 1) % H, yn = (y + 1) % H;
+        const long long rp = (long long)yp * W, rc = (long long)y * W, rn = (long long)yn * W;
+        float vx, vy;
+        sobel9(g[rp + xp], g[rp + x], g[rp + xn], g[rc + xp], g[rc + xn], g[rn + xp], g[rn + x], g[rn + xn], vx, vy);
+        gx[i] = vx;
+        gy[i] = vy;
+    }
+}
+
+__global__ void k_flow2d(const float* __restrict__ gux, const float* __restrict__ guy, const float* __restrict__ gax,
+                         const float* __restrict__ gay, const float* __restrict__ mass, long long n, float beta_a,
+                         float nexp, float* __restrict__ fx, float* __restrict__ fy) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const float al = alpha_of(mass[i], beta_a, nexp);
+        const float oma = 1.0f - al;
+        fx[i] = oma * gux[i] - al * gax[i];
+        fy[i] = oma * guy[i] - al * gay[i];
+    }
+}
+
+__global__ void k_growth_acc(const float* __restrict__ conv, float* __restrict__ target, long long n, float weight,
+                             float mu, float inv) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const float d = conv[i] - mu;
+        const float g = 2.0f * expf(-d * d * inv) - 1.0f;
+        target[i] += weight * g;
+    }
+}
+
+static int grid_for(long long n) { return (int)std::max<long long>(1, std::min<long long>((n + 255) / 256, 65535)); }
+
+void launch_sobel2d(hipStream_t st, const float* grid, float* gx, float* gy, int W, int H) {
+    hipLaunchKernelGGL(k_sobel2d, dim3(grid_for((long long)W * H)), dim3(256), 0, st, grid, gx, gy, W, H);
+}
+
+void launch_flow2d(hipStream_t st, const float* gux, const float* guy, const float* gax, const float* gay,
+                   const float* mass, long long n, float beta_a, float nexp, float* fx, float* fy) {
+    hipLaunchKernelGGL(k_flow2d, dim3(grid_for(n)), dim3(256), 0, st, gux, guy, gax, gay, mass, n, beta_a, nexp, fx, fy);
+}
+
+void launch_growth_acc(hipStream_t st, const float* conv, float* target, long long n, float weight, float mu,
+                       float sigma) {
+    const float sigma_sq_2 = 2.0f * sigma * sigma;  // growth.rs:51-52
+    const float inv = 1.0f / sigma_sq_2;
+    hipLaunchKernelGGL(k_growth_acc, dim3(grid_for(n)), dim3(256), 0, st, conv, target, n, weight, mu, inv);
+}
+
+// ---------------------------------------------------------------------------------------------
+// FP64 mass: block b of channel c sums the fixed chunk [b*chunk, (b+1)*chunk) with a fixed
+// per-thread stride and a fixed LDS tree; the second kernel adds the partials of each channel
+// in a fixed tree and the channels in order.  Same bits on every run.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mass_partial(const float* __restrict__ A, long long n, int nparts,
+                                                      double* __restrict__ partials) {
+    __shared__ double red[256];
+    const int c = blockIdx.y, b = blockIdx.x;
+    const long long chunk = (n + nparts - 1) / nparts;
+    const long long beg = (long long)b * chunk;
+    const long long end = beg + chunk < n ? beg + chunk : n;
+    const float* a = A + (long long)c * n;
+    double s = 0.0;
+    for (long long i = beg + threadIdx.x; i < end; i += 256) s += (double)a[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[(long long)c * nparts + b] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_mass_final(const double* __restrict__ partials, int C, int nparts,
+                                                    double* __restrict__ out) {
+    __shared__ double red[256];
+    double total = 0.0;
+    for (int c = 0; c < C; ++c) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < nparts; i += 256) s += partials[(long long)c * nparts + i];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            out[c] = red[0];
+            total += red[0];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[C] = total;
+}
+
+void launch_mass_f64(hipStream_t st, const float* A, int C, long long n, double* partials, int nparts, double* out) {
+    hipLaunchKernelGGL(k_mass_partial, dim3(nparts, C), dim3(256), 0, st, A, n, nparts, partials);
+    hipLaunchKernelGGL(k_mass_final, dim3(1), dim3(256), 0, st, partials, C, nparts, out);
+}
+
+__global__ void k_clear_ints(int* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+void launch_clear_ints(hipStream_t st, int* p, int n) { hipLaunchKernelGGL(k_clear_ints, dim3(1), dim3(64), 0, st, p, n); }
+
+}  // namespace hp

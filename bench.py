@@ -1,0 +1,152 @@
+#!/usr/bin/env python3
+"""Benchmark of the HIP Flow Lenia step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Workload: the 2D multi-species configuration, 1024 x 1024, 3 channels, 9 kernels
+(every source->target pair, 1-3 rings each), kernel_radius 15, dt 0.2, synthetic
+parameters and initial blobs (autoverse_amd/workloads.py).  A step is one full
+Flow Lenia time step over the device-resident state (FFT convolution + growth,
+Sobel, flow, reintegration); `value` is cell-steps/s = W*H*steps / seconds.
+
+Prints ONE JSON line.  `roofline` is computed from per-kernel HIP events recorded
+on the library's own stream during a second pass of K steps (the timed pass runs
+from hipGraphs without events); `cpu_baseline` times the CPU oracle (oracle/, a
+C restatement of the reference CPU propagator with its thread decomposition) on
+a bounded number of steps of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline sample length")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the step is single-GPU by design",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    import autoverse_amd as av
+    from autoverse_amd import workloads
+
+    cfg, seed = workloads.multispecies()
+    N = cfg.width * cfg.height
+    state = av.SimulationState.from_seed(seed, cfg)
+    prop = av.HipPropagator(cfg)
+    prop.set_state(state)
+    prop.advance(args.warmup)
+
+    # timed pass: K steps replayed from hipGraphs, bracketed by device synchronisation
+    t0 = time.perf_counter()
+    prop.advance(args.steps)  # returns after the stream has drained
+    t1 = time.perf_counter()
+    secs = t1 - t0
+    value = N * args.steps / secs
+    radius, general = prop.stats()
+
+    # per-kernel pass: the same K steps with an event pair around every launch
+    prop.profile_enable(True)
+    prop.advance(args.steps)
+    prof = prop.profile_read()
+    prop.profile_enable(False)
+    kbytes = prop.kernel_bytes()
+    per_kernel = {}
+    for k, (ms, n) in prof.items():
+        if n:
+            avg_us = ms * 1e3 / n
+            per_kernel[k] = {"launches": n, "avg_us": round(avg_us, 3),
+                             "bytes_per_launch": kbytes[k],
+                             "GBs": round(kbytes[k] / (avg_us * 1e-6) / 1e9, 1) if avg_us > 0 else None}
+    dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_us"] * per_kernel[k]["launches"])
+    d = per_kernel[dom]
+    achieved = d["GBs"]
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        try:
+            with open(tfile) as f:
+                traffic = json.load(f).get(dom)
+        except Exception:
+            traffic = None
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
+                "per_kernel": per_kernel}
+
+    # FP64 mass of the device state (conservation check, reported)
+    _, mass = prop.mass_f64()
+    prop.close()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(cfg, seed, args.cpu_seconds)
+
+    out = {
+        "metric": "cell-steps/s at 1024^2x3ch (plain HipPropagator forward step)",
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": secs * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "multispecies 1024x1024x3ch, 9 kernels, R=15, dt=0.2",
+                   "width": cfg.width, "height": cfg.height, "channels": cfg.channels,
+                   "kernels": len(cfg.kernels), "kernel_radius": cfg.kernel_radius,
+                   "max_gather_radius": radius, "general_path_steps": general,
+                   "mass_f64_after": mass},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def cpu_baseline(cfg, seed, target_s: float):
+    """CPU oracle (C restatement of CpuPropagator, reference thread decomposition: one thread
+    per kernel for the convolutions, one per channel for the stencil) on a bounded sample."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    ncpu = os.cpu_count() or 1
+    threads = max(1, min(ncpu, 16, len(cfg.kernels)))
+    st = O.seed_generate(seed, cfg.width, cfg.height, 1, cfg.channels)
+    p = O.Propagator(cfg, np.float32, threads=threads)
+    t0 = time.perf_counter()
+    st, _, _ = p.run(st, 1)  # first step also measures the per-step cost
+    one = time.perf_counter() - t0
+    n = max(1, min(200, int(target_s / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    p.run(st, n)
+    secs = time.perf_counter() - t0
+    p.close()
+    return {"value": cfg.width * cfg.height * n / secs, "unit": "cell-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n} steps of the same 1024x1024x3ch workload, f32, {threads} threads "
+                      f"(host has {ncpu} logical CPUs), {secs * 1e3 / n:.1f} ms/step"}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

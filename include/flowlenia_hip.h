@@ -1,0 +1,254 @@
+/*
+ * flowlenia_hip.h -- C ABI of the MI355X-native Flow Lenia time step.
+ *
+ * The library is a drop-in for the reference crate's CPU propagator: same
+ * configuration, same seed patterns, same state layout, same step semantics.
+ * Every entry point is extern "C", takes plain pointers and sizes, never throws
+ * and returns an hp_status.  Host buffers are owned by the caller and are never
+ * retained; a context owns all device memory it allocates.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   hp_config / hp_kernel / hp_ring / hp_flow  <- SimulationConfig, KernelConfig,
+ *       RingConfig, FlowConfig           (src/schema/config.rs:14-120)
+ *   hp_validate_config                      <- SimulationConfig::validate
+ *                                              (src/schema/config.rs:136-164)
+ *   hp_seed / hp_seed_generate              <- Seed / Pattern / Seed::generate
+ *                                              (src/schema/seed.rs:13-304)
+ *   hp_create                               <- CpuPropagator::new
+ *                                              (src/compute/propagator.rs:157-236)
+ *                                              and CpuPropagator3D::new
+ *                                              (src/compute/propagator3d.rs:49-117)
+ *   hp_step                                 <- CpuPropagator::step
+ *                                              (src/compute/propagator.rs:239-420)
+ *   hp_run                                  <- CpuPropagator::run (propagator.rs:423-427)
+ *   hp_get_config                           <- CpuPropagator::config (propagator.rs:430-432)
+ *   hp_mass_f64                             <- total_mass_all_channels
+ *                                              (src/compute/reintegration.rs:221-228),
+ *                                              computed in FP64 instead of f32
+ *   hp_advect_mass_2d / hp_sobel_gradient_2d / hp_compute_flow_2d /
+ *   hp_growth_accumulate / hp_convolve_2d   <- the stage functions the crate
+ *       re-exports from src/compute/mod.rs:5-41 (advect_mass_into
+ *       reintegration.rs:60-88, sobel_gradient_into gradient.rs:68-108,
+ *       compute_flow_field_into flow.rs:68-87, growth_accumulate growth.rs:44-59,
+ *       CachedConvolver::convolve_with_kernel_scratch fft.rs:288-324)
+ */
+#ifndef FLOWLENIA_HIP_H
+#define FLOWLENIA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The first five mirror ConfigError (src/schema/config.rs:169-180). */
+typedef enum hp_status {
+    HP_OK = 0,
+    HP_ERR_INVALID_DIMENSIONS = 1,    /* width, height or depth is zero            */
+    HP_ERR_INVALID_CHANNELS = 2,      /* channels is zero                          */
+    HP_ERR_INVALID_TIME_STEP = 3,     /* dt <= 0                                   */
+    HP_ERR_INVALID_KERNEL_RADIUS = 4, /* kernel_radius is zero                     */
+    HP_ERR_INVALID_CHANNEL_INDEX = 5, /* a kernel names a channel >= channels      */
+    HP_ERR_NO_DEVICE = 6,             /* no HIP device, or the ordinal is invalid  */
+    HP_ERR_HIP = 7,                   /* a HIP runtime call failed                 */
+    HP_ERR_SHAPE = 8,                 /* buffer count / size mismatch              */
+    HP_ERR_NULL = 9,                  /* a required pointer is NULL                */
+    HP_ERR_NONFINITE = 10,            /* the state holds NaN or Inf                */
+    HP_ERR_UNSUPPORTED_SIZE = 11      /* a size outside what the kernels support   */
+} hp_status;
+
+/* RingConfig (config.rs:92-99). */
+typedef struct hp_ring {
+    float amplitude;
+    float distance;
+    float width;
+} hp_ring;
+
+/* KernelConfig (config.rs:55-70).  The rings of kernel i are
+ * rings[ring_offset .. ring_offset + ring_count) of hp_config.rings. */
+typedef struct hp_kernel {
+    float radius;
+    float weight;
+    float mu;
+    float sigma;
+    uint32_t source_channel;
+    uint32_t target_channel;
+    uint32_t ring_offset;
+    uint32_t ring_count;
+} hp_kernel;
+
+/* FlowConfig (config.rs:103-110). */
+typedef struct hp_flow {
+    float beta_a;
+    float n;
+    float distribution_size;
+} hp_flow;
+
+/* SimulationConfig (config.rs:14-35).  depth == 1 selects the 2D step. */
+typedef struct hp_config {
+    uint32_t width;
+    uint32_t height;
+    uint32_t depth;
+    uint32_t channels;
+    float dt;
+    uint32_t kernel_radius;
+    hp_flow flow;
+    uint32_t num_kernels;
+    const hp_kernel* kernels;
+    uint32_t num_rings;
+    const hp_ring* rings;
+} hp_config;
+
+/* SimulationConfig::validate (config.rs:136-164).  On HP_ERR_INVALID_CHANNEL_INDEX,
+ * *bad_kernel / *bad_channel (if non-NULL) receive the offending pair. */
+hp_status hp_validate_config(const hp_config* cfg, uint32_t* bad_kernel, uint32_t* bad_channel);
+
+/* ---- Seeds (src/schema/seed.rs) ------------------------------------------ */
+typedef enum hp_pattern_kind {
+    HP_PATTERN_GAUSSIAN_BLOB = 0,   /* seed.rs:40-49   */
+    HP_PATTERN_MULTI_BLOB = 1,      /* seed.rs:51-54   */
+    HP_PATTERN_RING = 2,            /* seed.rs:56-68   */
+    HP_PATTERN_GAUSSIAN_SPHERE = 3, /* seed.rs:71-80   */
+    HP_PATTERN_MULTI_SPHERE = 4,    /* seed.rs:82-85   */
+    HP_PATTERN_SHELL = 5,           /* seed.rs:87-98   */
+    HP_PATTERN_TORUS3D = 6,         /* seed.rs:100-111 */
+    HP_PATTERN_NOISE = 7,           /* seed.rs:115-122 */
+    HP_PATTERN_CUSTOM = 8,          /* seed.rs:125-128 */
+    HP_PATTERN_CUSTOM3D = 9         /* seed.rs:130-134 */
+} hp_pattern_kind;
+
+/* One blob (BlobSpec, seed.rs:138-143) or sphere (SphereSpec, seed.rs:146-152).
+ * For blobs center[2] is ignored. */
+typedef struct hp_blob {
+    float center[3];
+    float radius;
+    float amplitude;
+    uint32_t channel;
+} hp_blob;
+
+/* One Custom / Custom3D entry; z is ignored for Custom. */
+typedef struct hp_custom_value {
+    uint32_t x, y, z, channel;
+    float value;
+} hp_custom_value;
+
+typedef struct hp_seed {
+    int32_t kind;                 /* hp_pattern_kind */
+    float center[3];              /* GaussianBlob/Ring use center[0..1] */
+    float radius;                 /* GaussianBlob, GaussianSphere */
+    float inner_radius;           /* Ring, Shell */
+    float outer_radius;           /* Ring, Shell */
+    float major_radius;           /* Torus3D */
+    float minor_radius;           /* Torus3D */
+    float amplitude;
+    int32_t channel;              /* Noise: -1 means every channel */
+    uint64_t noise_seed;          /* Noise */
+    uint32_t num_blobs;           /* MultiBlob / MultiSphere */
+    const hp_blob* blobs;
+    uint32_t num_values;          /* Custom / Custom3D */
+    const hp_custom_value* values;
+} hp_seed;
+
+/* Seed::generate (seed.rs:160-304): rasterise into channels[c][z*H*W + y*W + x].
+ * The caller passes `channels` host buffers of width*height*depth floats; they
+ * are overwritten (zeroed first).  Pure host code, no device needed. */
+hp_status hp_seed_generate(const hp_seed* seed, uint32_t width, uint32_t height, uint32_t depth,
+                           uint32_t channels, float* const* out_channels);
+
+/* ---- Propagator ----------------------------------------------------------- */
+typedef struct hp_ctx hp_ctx;
+
+/* Build the propagator: validates the config, builds the ring kernels
+ * (kernel.rs:32-86 / kernel3d.rs:32-97), their spectra, and every device buffer.
+ * `device` is a HIP device ordinal.  Returns HP_ERR_NO_DEVICE when there is no
+ * GPU; *out is NULL on any error. */
+hp_status hp_create(const hp_config* cfg, int device, hp_ctx** out);
+void hp_destroy(hp_ctx* ctx);
+
+/* Message for the last error on this context (valid until the next call). */
+const char* hp_last_error(const hp_ctx* ctx);
+/* Message for the last error of a call that had no context (hp_create). */
+const char* hp_last_global_error(void);
+
+/* Copy of the config the context was built from; kernels/rings point into the
+ * context and stay valid until hp_destroy. */
+hp_status hp_get_config(const hp_ctx* ctx, hp_config* out);
+
+/* Upload a state: `channels` host pointers, width*height*depth floats each. */
+hp_status hp_set_state(hp_ctx* ctx, const float* const* channels, float time, uint64_t step);
+/* Download the device-resident state. */
+hp_status hp_get_state(hp_ctx* ctx, float* const* channels, float* time, uint64_t* step);
+
+/* CpuPropagator::step semantics on a host state: upload, one step, download,
+ * time += dt (f32), step += 1. */
+hp_status hp_step(hp_ctx* ctx, float* const* channels, float* time, uint64_t* step);
+/* CpuPropagator::run semantics: one upload, `steps` device-resident steps, one download. */
+hp_status hp_run(hp_ctx* ctx, float* const* channels, float* time, uint64_t* step, uint64_t steps);
+/* Advance the device-resident state by `steps` steps with no host transfer
+ * (the state set by hp_set_state or left by earlier calls).  Steps are replayed
+ * from a hipGraph; synchronises before returning. */
+hp_status hp_advance(hp_ctx* ctx, uint64_t steps);
+
+/* FP64 per-channel and total mass of the device-resident state, reduced in a
+ * fixed order (bitwise reproducible run to run).  per_channel may be NULL. */
+hp_status hp_mass_f64(hp_ctx* ctx, double* per_channel, double* total);
+
+/* Largest gather radius any step has needed so far, and how many steps had to
+ * take the general reintegration path (a flow too large for the fused stencil). */
+hp_status hp_stats(const hp_ctx* ctx, uint32_t* max_gather_radius, uint64_t* general_path_steps);
+
+/* ---- Per-kernel timing (HIP events on the context's stream) ------------------ */
+/* Kernel classes of the step, in launch order. */
+typedef enum hp_kernel_class {
+    HP_K_ROWS_FWD = 0,   /* R2C along x (2D rows / 3D x-lines)                 */
+    HP_K_COLS_FWD = 1,   /* forward FFT along y (3D only)                        */
+    HP_K_SPECTRAL = 2,   /* last forward axis, x kernel spectrum, inverse axis */
+    HP_K_COLS_INV = 3,   /* inverse FFT along y (3D only)                        */
+    HP_K_ROWS_INV = 4,   /* C2R along x + growth + weighted accumulate           */
+    HP_K_STENCIL = 5,    /* Sobel + flow + gather reintegration                  */
+    HP_K_GENERAL = 6,    /* general-radius reintegration (rare path)             */
+    HP_K_NUM_CLASSES = 7
+} hp_kernel_class;
+
+/* Turn per-kernel event timing on or off (off by default; on adds one event pair
+ * per launch and disables graph replay).  Enabling resets the counters. */
+hp_status hp_profile_enable(hp_ctx* ctx, int enable);
+/* Total milliseconds and launch count per class since enabling. */
+hp_status hp_profile_read(hp_ctx* ctx, double* ms /*HP_K_NUM_CLASSES*/, uint64_t* launches /*HP_K_NUM_CLASSES*/);
+/* Algorithmic HBM bytes one launch of each class moves for this config. */
+hp_status hp_kernel_bytes(const hp_ctx* ctx, double* bytes /*HP_K_NUM_CLASSES*/);
+
+/* ---- Stage functions (host buffers in and out, run on `device`) -------------- */
+/* advect_mass_into (reintegration.rs:60-88): next = reintegrate(current, flow);
+ * `next` is overwritten (the reference adds into a zeroed buffer). */
+hp_status hp_advect_mass_2d(int device, const float* current, const float* flow_x, const float* flow_y,
+                            float* next, uint32_t width, uint32_t height, float dt, float distribution_size);
+/* advect_mass_3d_into (reintegration3d.rs:56-101). */
+hp_status hp_advect_mass_3d(int device, const float* current, const float* flow_x, const float* flow_y,
+                            const float* flow_z, float* next, uint32_t width, uint32_t height,
+                            uint32_t depth, float dt, float distribution_size);
+/* sobel_gradient_into (gradient.rs:68-108). */
+hp_status hp_sobel_gradient_2d(int device, const float* grid, float* grad_x, float* grad_y,
+                               uint32_t width, uint32_t height);
+/* compute_flow_field_into (flow.rs:68-87). */
+hp_status hp_compute_flow_2d(int device, const float* grad_u_x, const float* grad_u_y,
+                             const float* grad_a_x, const float* grad_a_y, const float* mass_sum,
+                             uint32_t n_cells, hp_flow flow, float* flow_x, float* flow_y);
+/* growth_accumulate (growth.rs:44-59): target[i] += weight * G(conv[i]). */
+hp_status hp_growth_accumulate(int device, const float* conv, float* target, uint32_t n_cells,
+                               float weight, float mu, float sigma);
+/* Flow-field reintegration of a context's step fed with a given affinity:
+ * next = stencil(state, affinity) for every channel, i.e. stages 2-5 of
+ * CpuPropagator::step (propagator.rs:303-414) with U supplied by the caller. */
+hp_status hp_stencil_with_affinity(hp_ctx* ctx, const float* const* state, const float* const* affinity,
+                                   float* const* next);
+/* Affinity U of the context's step for a given state: stage 1 of
+ * CpuPropagator::step (propagator.rs:244-280): FFT convolution + growth. */
+hp_status hp_affinity(hp_ctx* ctx, const float* const* state, float* const* affinity);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLOWLENIA_HIP_H */

@@ -1,0 +1,226 @@
+"""GPU parity of the HIP Flow Lenia step against the CPU oracle (oracle/), through the C ABI.
+
+Bars: bit-exact where the arithmetic is the reference's own f32 sequence (Sobel, alpha,
+flow, reintegration, seeds); within a stated tolerance where the FFT is involved
+(the FFT's rounding is not the reference's rustfft's; see oracle/flowlenia_oracle.h).
+"""
+import numpy as np
+import pytest
+
+import autoverse_amd as av
+from autoverse_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _evolve(O, cfg, seed, steps):
+    st = O.seed_generate(seed, cfg.width, cfg.height, cfg.depth, cfg.channels)
+    p = O.Propagator(cfg, np.float32)
+    ch, _, _ = p.run(st, steps)
+    return p, ch
+
+
+def _small_multispecies(size=128, channels=3, seed=7):
+    cfg, sd = workloads.multispecies(size=size, channels=channels, seed=seed)
+    return cfg, sd
+
+
+# ---------------------------------------------------------------------------------------------
+# stencil (stages 2-5) given the oracle's affinity: bit-exact
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("case", ["plain256", "multi128x3", "multi96x2"])
+def test_stencil_bitwise(oracle, case):
+    _say(f"[stencil_bitwise] {case}")
+    if case == "plain256":
+        cfg, sd = workloads.plain_256()
+        warm = 20
+    elif case == "multi128x3":
+        cfg, sd = _small_multispecies(128, 3)
+        warm = 10
+    else:
+        cfg, sd = _small_multispecies(96, 2, seed=11)
+        warm = 5
+    p, ch = _evolve(oracle, cfg, sd, warm)
+    aff = p.affinity(ch)
+    want = p.stencil(ch, aff)
+    with av.HipPropagator(cfg) as hp:
+        got = hp.stencil(ch, aff)
+        r, g = hp.stats()
+    for c in range(cfg.channels):
+        np.testing.assert_array_equal(got[c], want[c])
+    assert g == 0, "the fused path should have covered this flow"
+
+
+def test_stencil_general_path_bitwise(oracle):
+    """Large dt pushes sources beyond the fused radius: the general path must still be bit-exact."""
+    _say("[stencil_general] dt=4")
+    cfg, sd = _small_multispecies(64, 2, seed=3)
+    p, ch = _evolve(oracle, cfg, sd, 5)
+    cfg.dt = 4.0
+    p2 = oracle.Propagator(cfg, np.float32)
+    aff = p2.affinity(ch)
+    want = p2.stencil(ch, aff)
+    with av.HipPropagator(cfg) as hp:
+        got = hp.stencil(ch, aff)
+        r, g = hp.stats()
+    for c in range(cfg.channels):
+        np.testing.assert_array_equal(got[c], want[c])
+    assert g >= 1 and r > 2
+
+
+# ---------------------------------------------------------------------------------------------
+# affinity (stage 1): FFT convolution + growth, tolerance
+# ---------------------------------------------------------------------------------------------
+AFF_TOL = 2e-5  # max-abs on U; G amplifies conv error by up to ~1.2/sigma
+
+
+@pytest.mark.parametrize("size,channels", [(256, 1), (128, 3), (192, 2), (96, 1), (48, 2), (1024, 3)])
+def test_affinity_vs_oracle(oracle, size, channels):
+    _say(f"[affinity] {size}x{size}x{channels}")
+    if channels == 1 and size == 256:
+        cfg, sd = workloads.plain_256()
+    else:
+        cfg, sd = _small_multispecies(size, channels, seed=size + channels)
+    st = oracle.seed_generate(sd, cfg.width, cfg.height, 1, cfg.channels)
+    p32 = oracle.Propagator(cfg, np.float32)
+    p64 = oracle.Propagator(cfg, np.float64)
+    want32 = p32.affinity(st)
+    want64 = p64.affinity([s.astype(np.float64) for s in st])
+    with av.HipPropagator(cfg) as hp:
+        got = hp.affinity(st)
+    e32 = max(float(np.abs(got[c] - want32[c]).max()) for c in range(cfg.channels))
+    e64 = max(float(np.abs(got[c] - want64[c]).max()) for c in range(cfg.channels))
+    o64 = max(float(np.abs(want32[c] - want64[c]).max()) for c in range(cfg.channels))
+    _say(f"  max|gpu-f32 oracle|={e32:.3e} max|gpu-f64|={e64:.3e} max|f32 oracle-f64|={o64:.3e}")
+    assert e32 <= AFF_TOL
+    assert e64 <= AFF_TOL
+
+
+# ---------------------------------------------------------------------------------------------
+# full steps
+# ---------------------------------------------------------------------------------------------
+def test_plain256_100_steps(oracle):
+    """The plain configuration, 100 steps: max-abs <= 1e-4 against the f32 oracle, and the error
+    against the FP64 oracle is stated."""
+    _say("[plain256_100]")
+    cfg, sd = workloads.plain_256()
+    st0 = oracle.seed_generate(sd, 256, 256, 1, 1)
+    want32, t32, s32 = oracle.Propagator(cfg, np.float32).run(st0, 100)
+    want64, _, _ = oracle.Propagator(cfg, np.float64).run([st0[0].astype(np.float64)], 100)
+    state = av.SimulationState([st0[0].copy()], 256, 256)
+    with av.HipPropagator(cfg) as hp:
+        hp.run(state, 100)
+        _, mass = hp.mass_f64()
+    e32 = float(np.abs(state.channels[0] - want32[0]).max())
+    e64 = float(np.abs(state.channels[0] - want64[0]).max())
+    m0 = float(st0[0].astype(np.float64).sum())
+    _say(f"  max|gpu-f32|={e32:.3e} max|gpu-f64|={e64:.3e} mass drift={(mass - m0) / m0:.3e}")
+    assert e32 <= 1e-4
+    assert state.step == s32 == 100
+    assert state.time == pytest.approx(t32, abs=0)
+    assert abs(mass - float(want64[0].sum())) / m0 < 1e-6
+
+
+def test_step_equals_run_and_is_deterministic(oracle):
+    _say("[step_vs_run]")
+    cfg, sd = _small_multispecies(128, 2, seed=5)
+    st0 = oracle.seed_generate(sd, 128, 128, 1, 2)
+    a = av.SimulationState([c.copy() for c in st0], 128, 128)
+    b = av.SimulationState([c.copy() for c in st0], 128, 128)
+    c = av.SimulationState([x.copy() for x in st0], 128, 128)
+    with av.HipPropagator(cfg) as hp:
+        for _ in range(12):
+            hp.step(a)
+        hp.run(b, 12)
+        hp.run(c, 12)
+    for i in range(2):
+        np.testing.assert_array_equal(a.channels[i], b.channels[i])
+        np.testing.assert_array_equal(b.channels[i], c.channels[i])
+    assert a.step == b.step == 12 and a.time == b.time
+
+
+def test_multispecies_1024_steps(oracle):
+    """The 1024^2 x 3 workload for 10 steps against the f32 oracle."""
+    _say("[multispecies_1024x3 10 steps]")
+    cfg, sd = workloads.multispecies()
+    st0 = oracle.seed_generate(sd, 1024, 1024, 1, 3)
+    want, _, _ = oracle.Propagator(cfg, np.float32, threads=8).run(st0, 10)
+    state = av.SimulationState([c.copy() for c in st0], 1024, 1024)
+    with av.HipPropagator(cfg) as hp:
+        hp.run(state, 10)
+    e = max(float(np.abs(state.channels[c] - want[c]).max()) for c in range(3))
+    _say(f"  max|gpu-f32|={e:.3e}")
+    assert e <= 1e-4
+
+
+# ---------------------------------------------------------------------------------------------
+# stage entry points
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("fx,fy,s,dt", [(0.0, 0.0, 1.0, 0.2), (5.0, 2.0, 1.0, 0.2), (8.0, 8.0, 1.0, 0.2),
+                                        (100.0, 50.0, 1.0, 0.2), (1.0, 0.0, 0.01, 0.2), (2.0, 1.0, 0.5, 0.2),
+                                        (0.3, -0.7, 1e-6, 1.0)])
+def test_advect_stage_bitwise(oracle, fx, fy, s, dt):
+    """advect_mass on the reference's own cases (reintegration.rs:235-470) and random fields."""
+    _say(f"[advect] flow=({fx},{fy}) s={s}")
+    rng = np.random.default_rng(1)
+    for (w, h) in [(16, 16), (32, 32), (40, 24)]:
+        cur = rng.uniform(0, 1, (h, w)).astype(np.float32)
+        cur[rng.uniform(size=(h, w)) < 0.2] = 0.0
+        flx = (np.full((h, w), fx) + rng.normal(0, 0.3, (h, w))).astype(np.float32)
+        fly = (np.full((h, w), fy) + rng.normal(0, 0.3, (h, w))).astype(np.float32)
+        want = oracle.advect2d(cur, flx, fly, dt, s)
+        got = av.advect_mass(cur, flx, fly, w, h, dt, s).reshape(h, w)
+        np.testing.assert_array_equal(got, want)
+
+
+def test_sobel_flow_growth_stages(oracle):
+    _say("[stages]")
+    rng = np.random.default_rng(2)
+    g = rng.normal(size=(37, 53)).astype(np.float32)
+    gx, gy = av.sobel_gradient(g, 53, 37)
+    wx, wy = oracle.sobel2d(g)
+    np.testing.assert_array_equal(gx.reshape(37, 53), wx)
+    np.testing.assert_array_equal(gy.reshape(37, 53), wy)
+    n = 4096
+    a = [rng.normal(size=n).astype(np.float32) for _ in range(4)]
+    m = rng.uniform(0, 3, n).astype(np.float32)
+    for (beta, nexp) in [(1.0, 2.0), (2.0, 4.0), (1.5, 1.0), (1.0, 3.0)]:
+        fx, fy = av.compute_flow_field(a[0], a[1], a[2], a[3], m, av.FlowConfig(beta, nexp, 1.0))
+        wx, wy = oracle.flow2d(a[0], a[1], a[2], a[3], m, beta, nexp)
+        np.testing.assert_array_equal(fx, wx)
+        np.testing.assert_array_equal(fy, wy)
+    conv = rng.uniform(0, 0.4, n).astype(np.float32)
+    t = rng.normal(size=n).astype(np.float32)
+    want = t.copy()
+    oracle.growth_accumulate(conv, want, 0.7, 0.15, 0.015)
+    av.growth_accumulate(conv, t, 0.7, 0.15, 0.015)
+    assert float(np.abs(t - want).max()) <= 4e-7  # device expf is not glibc expf (<= 2 ulp)
+
+
+# ---------------------------------------------------------------------------------------------
+# 3D
+# ---------------------------------------------------------------------------------------------
+def test_3d_small_vs_oracle(oracle):
+    _say("[3d 32^3]")
+    cfg, sd = workloads.sphere_3d(32)
+    cfg.kernel_radius = 6
+    st0 = oracle.seed_generate(sd, 32, 32, 32, 1)
+    p = oracle.Propagator(cfg, np.float32)
+    want, _, _ = p.run(st0, 10)
+    aff = p.affinity(want)
+    nxt = p.stencil(want, aff)
+    state = av.SimulationState([st0[0].copy()], 32, 32, 32)
+    with av.HipPropagator(cfg) as hp:
+        got_aff = hp.affinity(want)
+        got_nxt = hp.stencil(want, aff)
+        hp.run(state, 10)
+    ea = float(np.abs(got_aff[0] - aff[0]).max())
+    e = float(np.abs(state.channels[0] - want[0]).max())
+    _say(f"  affinity max err {ea:.3e}; 10-step max err {e:.3e}")
+    np.testing.assert_array_equal(got_nxt[0], nxt[0])
+    assert ea <= AFF_TOL
+    assert e <= 1e-4
