@@ -36,7 +36,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline sample length")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain device copy of 256 MiB ten times (calibrates rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE against a kernel of known traffic)")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the step is single-GPU by design",
               file=sys.stderr)
@@ -120,6 +125,19 @@ def main() -> int:
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def copy_probe() -> int:
+    """A plain copy kernel of known size: 64 Mi float32 (256 MiB) read and written, ten times."""
+    import torch
+
+    a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(10):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_each_way": a.numel() * 4, "launches": 10}), flush=True)
     return 0
 
 
