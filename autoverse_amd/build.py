@@ -5,7 +5,7 @@
 Per-file floating-point flags:
   * stencil_kernels.hip, host*.cpp: -ffp-contract=off -- these reproduce the
     reference's f32 arithmetic operation for operation (Rust never fuses a*b+c);
-  * fft_kernels.hip: -ffp-contract=fast -- the FFT is not bitwise anyway, FMAs make
+  * fft_kernels.hip, wave_fft.hip: -ffp-contract=fast -- the FFT is not bitwise anyway, FMAs make
     it faster and more accurate; its growth epilogue turns contraction back off.
 All device code keeps f32 denormals, correctly rounded f32 division/sqrt, and is
 built without packed-FP32 instructions.
@@ -31,6 +31,7 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math
 
 SOURCES = [
     ("fft_kernels.hip", DEVICE_FLAGS + ["-ffp-contract=fast"]),
+    ("wave_fft.hip", DEVICE_FLAGS + ["-ffp-contract=fast"]),
     ("stencil_kernels.hip", DEVICE_FLAGS + ["-ffp-contract=off"]),
     ("host.cpp", HOST_FLAGS),
     ("host_seed.cpp", HOST_FLAGS),

@@ -12,6 +12,7 @@
 // stencil; steps replay from two captured hipGraphs (A0->A1 and A1->A0).
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -45,7 +46,9 @@ struct hp_ctx {
     hp::Geo g{};
     int C = 0, K = 0;
     bool is3d = false;
+    bool wave = false;  // 2D sides in {256..2048}: column-major spectra, wave-level FFTs
     std::vector<int> src_chan;  // source slot -> channel
+    std::vector<hp::SpecWork> spec_work;  // host copy of the spectral work list
     std::vector<int> tgt_chan;  // target work item -> channel
     float* A[2] = {nullptr, nullptr};
     float* U = nullptr;
@@ -174,6 +177,24 @@ hp_status make_twiddles(hp_ctx* c, int L) {
 hp_status enqueue_affinity(hp_ctx* c, const float* A) {
     if (c->K == 0) return HP_OK;  // U stays zero (propagator.rs:245-247)
     const hp::Geo& g = c->g;
+    if (c->wave) {
+        {
+            Timed t(c, HP_K_ROWS_FWD);
+            hp::launch_rows_fwd_t(c->stream, A, c->X, c->d_src_chan, (int)c->src_chan.size(), g, c->tw[g.W]);
+        }
+        {
+            Timed t(c, HP_K_SPECTRAL);
+            hp::launch_spectral_t(c->stream, c->X, c->Y, c->Kq, c->d_spec_work, c->n_spec_work, c->d_klist_spec, g,
+                                  c->tw[g.H]);
+        }
+        {
+            Timed t(c, HP_K_ROWS_INV);
+            hp::launch_rows_inv_t(c->stream, c->Y, c->U, c->d_tgt_work, c->n_tgt_work, c->d_klist_tgt, c->d_gp, g,
+                                  c->tw[g.W]);
+        }
+        HIP_TRY(c, hipGetLastError());
+        return HP_OK;
+    }
     {
         Timed t(c, HP_K_ROWS_FWD);
         hp::launch_rows_fwd(g.W, c->stream, A, c->X, c->d_src_chan, (int)c->src_chan.size(), g, c->tw[g.W]);
@@ -435,6 +456,7 @@ hp_status create(const hp_config* cfg, int device, hp_ctx** out) {
     g.Dq = is3d ? (int)(D / 2 + 1) : 1;
     g.N = (long long)W * H * D;
     g.spec = (long long)D * H * g.Wp;
+    c->wave = !is3d && hp::wave_fft_supported(g.W, g.H);
     // kernel spectra (host FP64) and work lists
     std::vector<float> kq((size_t)std::max(1, c->K) * g.Dq * g.Hq * g.Wh, 0.0f);
     std::vector<int> slot_of(c->C, -1);
@@ -449,6 +471,12 @@ hp_status create(const hp_config* cfg, int device, hp_ctx** out) {
             return HP_ERR_UNSUPPORTED_SIZE;
         }
         std::vector<float> f = hp::folded_spectrum(data, size, is3d ? 3 : 2, W, H, D);
+        if (c->wave) {  // column-major [kx][kyf] for the y pass
+            std::vector<float> t(f.size());
+            for (int fy = 0; fy < g.Hq; fy++)
+                for (int fx = 0; fx < g.Wh; fx++) t[(size_t)fx * g.Hq + fy] = f[(size_t)fy * g.Wh + fx];
+            f.swap(t);
+        }
         std::copy(f.begin(), f.end(), kq.begin() + (size_t)k * f.size());
         if (slot_of[kc.source_channel] < 0) {
             slot_of[kc.source_channel] = (int)c->src_chan.size();
@@ -457,12 +485,28 @@ hp_status create(const hp_config* cfg, int device, hp_ctx** out) {
         const float sigma_sq_2 = 2.0f * kc.sigma * kc.sigma;  // growth.rs:51-52
         gp[k] = {kc.weight, kc.mu, 1.0f / sigma_sq_2, 0.0f};
     }
-    // spectral work: one item per kernel (reads its source spectrum)
+    // spectral work: either one item per source (its forward transform is shared by all its
+    // kernels) or one item per kernel (more workgroups, the forward transform is redone)
     std::vector<hp::SpecWork> sw;
     std::vector<int> kls;
-    for (int k = 0; k < c->K; k++) {
-        sw.push_back({slot_of[c->kernels[k].source_channel], (int)kls.size(), 1, 0});
-        kls.push_back(k);
+    const char* grp = getenv("HP_SPECTRAL_GROUP");
+    const int blocks_per_source = c->wave ? (g.Wh + 3) / 4 : 1;
+    const bool group = grp ? atoi(grp) != 0 : (!c->wave || blocks_per_source * (long long)c->src_chan.size() >= 256);
+    if (group && c->wave) {
+        for (size_t sl = 0; sl < c->src_chan.size(); sl++) {
+            int first = (int)kls.size(), cnt = 0;
+            for (int k = 0; k < c->K; k++)
+                if (slot_of[c->kernels[k].source_channel] == (int)sl) {
+                    kls.push_back(k);
+                    cnt++;
+                }
+            sw.push_back({(int)sl, first, cnt, 0});
+        }
+    } else {
+        for (int k = 0; k < c->K; k++) {
+            sw.push_back({slot_of[c->kernels[k].source_channel], (int)kls.size(), 1, 0});
+            kls.push_back(k);
+        }
     }
     // inverse-row work: one item per target with its kernels in index order
     std::vector<hp::TgtWork> tw;
@@ -480,6 +524,7 @@ hp_status create(const hp_config* cfg, int device, hp_ctx** out) {
         }
     }
     c->n_spec_work = (int)sw.size();
+    c->spec_work = sw;
     c->n_tgt_work = (int)tw.size();
     c->mass_parts = 256;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
@@ -798,7 +843,11 @@ hp_status hp_kernel_bytes(const hp_ctx* ctx, double* bytes) {
     if (ctx->K > 0) {
         bytes[HP_K_ROWS_FWD] = nsrc * (4.0 * N + S);
         if (ctx->is3d) bytes[HP_K_COLS_FWD] = nsrc * 2.0 * S;
-        bytes[HP_K_SPECTRAL] = K * (S + Kq + S);
+        // every work item reads its source spectrum once, then per kernel the folded
+        // kernel spectrum and one filtered spectrum written
+        double sp = 0.0;
+        for (const hp::SpecWork& w : ctx->spec_work) sp += S + w.k_count * (Kq + S);
+        bytes[HP_K_SPECTRAL] = sp;
         if (ctx->is3d) bytes[HP_K_COLS_INV] = K * 2.0 * S;
         bytes[HP_K_ROWS_INV] = K * S + nt * 4.0 * N;
     }

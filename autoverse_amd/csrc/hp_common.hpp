@@ -54,6 +54,15 @@ bool launch_spectral(int L, int dim, hipStream_t st, const float2* X, float2* Y,
 bool launch_rows_inv(int L, hipStream_t st, const float2* Y, float* U, const TgtWork* work, int nwork,
                      const int* klist, const GrowthParams* gp, const Geo& g, const float2* tw);
 
+// ---- launchers implemented in wave_fft.hip (2D, sides 256..2048, column-major spectra) ----
+bool wave_fft_supported(int W, int H);
+bool launch_rows_fwd_t(hipStream_t st, const float* A, float2* XT, const int* src_chan, int nsrc, const Geo& g,
+                       const float2* tw);
+bool launch_spectral_t(hipStream_t st, const float2* XT, float2* YT, const float* KqT, const SpecWork* work,
+                       int nwork, const int* klist, const Geo& g, const float2* tw);
+bool launch_rows_inv_t(hipStream_t st, const float2* YT, float* U, const TgtWork* work, int nwork, const int* klist,
+                       const GrowthParams* gp, const Geo& g, const float2* tw);
+
 // ---- launchers implemented in stencil_kernels.hip ----------------------------------
 // Fused channel sum + Sobel + flow + gather reintegration with a fixed gather
 // radius; sets *flag when any active source's box leaves that radius (bit 0) or

@@ -32,6 +32,13 @@ __device__ __forceinline__ int wrapi(int c, int n) {
     return r < 0 ? r + n : r;
 }
 
+// Wrap for coordinates a few grid lengths out at most (no integer division).
+__device__ __forceinline__ int wrap_near(int c, int n) {
+    while (c < 0) c += n;
+    while (c >= n) c -= n;
+    return c;
+}
+
 // Rust `f as i32`: saturating, NaN -> 0.
 __device__ __forceinline__ int sat_i32(float v) {
     if (v != v) return 0;
@@ -70,38 +77,77 @@ __device__ __forceinline__ float overlap1(float c0, float c1, float lo, float hi
 }
 
 // ---------------------------------------------------------------------------------------------
-// Fused 2D stencil.  One workgroup owns a TX x TY tile of destinations.  LDS holds the channel
-// sum over the tile + (RG+1) halo, the mass gradient and alpha over the tile + RG halo, and per
-// channel the affinity halo tile and every source's box [xm, xM] x [ym, yM] and mass.
+// Fused 2D stencil.
+//
+// Window.  A source at x whose box [x + d - s, x + d + s] has |d| = |dt F| < 1 and s <= 1
+// overlaps only cells x-2 .. x+1 (monotone f32 rounding keeps that true for the computed box),
+// so a destination receives from sources at offsets o = source - dest in {-1, 0, 1, 2} per
+// axis: 16 sources.  Any active source whose box reaches further sets flag bit 0 and the host
+// reruns the step through the general path.
+//
+// Work split.  One workgroup owns a TX x TY destination tile.  Per channel, every source of the
+// (TX+3) x (TY+3) source region computes its flow, its box, and its four x and four y overlaps
+// with the destinations it can reach (in its own wrapped frame, exactly as the scatter does),
+// into LDS.  Each thread then gathers a vertical strip of SR destinations: it walks the source
+// rows in ascending order and, within a row, the four source columns in ascending wrapped
+// order, so every destination adds its contributions in the scatter's source raster order.
+// Tiles whose source rows wrap around the grid take a per-destination loop instead.
 // ---------------------------------------------------------------------------------------------
-template <int RG, bool POW2, int TX, int TY, int NT>
-__global__ __launch_bounds__(NT) void k_stencil2d(const float* __restrict__ A, const float* __restrict__ U,
-                                                  float* __restrict__ Aout, int C, int W, int H, long long N,
-                                                  float dt, float beta_a, float nexp, float s, float ta,
-                                                  float inv_ta, int* __restrict__ flag) {
-    constexpr int LW = TX + 2 * RG + 2, LH = TY + 2 * RG + 2;
-    constexpr int SW = TX + 2 * RG, SH = TY + 2 * RG;
-    constexpr int WIN = 2 * RG + 1;
+namespace st2 {
+constexpr int TX = 64, TY = 16, NT = 256, SR = 4;
+constexpr int SW = TX + 3, SH = TY + 3;  // source region: offsets -1 .. +2 around the tile
+constexpr int LW = SW + 2, LH = SH + 2;  // + Sobel halo
+constexpr int NSRC = SW * SH;
+static_assert(TX * TY == NT * SR, "one strip per thread");
+}  // namespace st2
+
+template <bool POW2>
+__global__ __launch_bounds__(st2::NT, 2) void k_stencil2d(const float* __restrict__ A, const float* __restrict__ U,
+                                                       float* __restrict__ Aout, int C, int W, int H, long long N,
+                                                       int tiles_x, int ntiles, float dt, float beta_a, float nexp,
+                                                       float s, float ta, float inv_ta, int* __restrict__ flag) {
+    using namespace st2;
     __shared__ float sSum[LH * LW];
     __shared__ float sU[LH * LW];
-    __shared__ float sAl[SH * SW], sGx[SH * SW], sGy[SH * SW];
-    __shared__ float sXm[SH * SW], sXM[SH * SW], sYm[SH * SW], sYM[SH * SW], sM[SH * SW];
+    __shared__ float sAl[NSRC], sGx[NSRC], sGy[NSRC];
+    __shared__ float sM[NSRC];
+    __shared__ float sOX[4][NSRC];
+    __shared__ float4 sOY[NSRC];
 
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
-    const int lx0 = x0 - RG - 1, ly0 = y0 - RG - 1;
+    // XCD-aware tile order: workgroup b runs on XCD b % 8; give each XCD a contiguous run of
+    // tiles so that neighbouring tiles (which share halo rows) meet in the same L2.
+    const int b = blockIdx.x;
+    const int per = (ntiles + 7) / 8;
+    const int tile = (b % 8) * per + b / 8;
+    if (tile >= ntiles) return;
+    const int x0 = (tile % tiles_x) * TX, y0 = (tile / tiles_x) * TY;
+    const int lx0 = x0 - 2, ly0 = y0 - 2;  // load-region origin (unwrapped)
     const int tid = threadIdx.x;
 
-    // channel sum over the load region (propagator.rs:304-310)
-    for (int i = tid; i < LH * LW; i += NT) {
+    // per-thread global offsets of the load region (wrapped once, reused for every channel)
+    constexpr int LPT = (LH * LW + NT - 1) / NT;
+    int goff[LPT];
+#pragma unroll
+    for (int q = 0; q < LPT; ++q) {
+        const int i = tid + q * NT;
         const int ly = i / LW, lx = i - ly * LW;
-        const long long g = (long long)wrapi(ly0 + ly, H) * W + wrapi(lx0 + lx, W);
-        float sum = 0.0f;
-        for (int c = 0; c < C; ++c) sum += A[(long long)c * N + g];
-        sSum[i] = sum;
+        goff[q] = i < LH * LW ? wrap_near(ly0 + ly, H) * W + wrap_near(lx0 + lx, W) : 0;
     }
+#pragma unroll
+    for (int q = 0; q < LPT; ++q) {  // channel sum (propagator.rs:304-310)
+        const int i = tid + q * NT;
+        if (i < LH * LW) {
+            float sum = 0.0f;
+            for (int c = 0; c < C; ++c) sum += A[(long long)c * N + goff[q]];
+            sSum[i] = sum;
+        }
+    }
+    float upre[LPT];  // affinity of the next channel, loaded ahead of its use
+#pragma unroll
+    for (int q = 0; q < LPT; ++q) upre[q] = tid + q * NT < LH * LW ? U[goff[q]] : 0.0f;
     __syncthreads();
-    // mass gradient and alpha over the source region
-    for (int i = tid; i < SH * SW; i += NT) {
+#pragma unroll 1
+    for (int i = tid; i < NSRC; i += NT) {  // mass gradient and alpha over the source region
         const int sy = i / SW, sx = i - sy * SW;
         const float* r0 = sSum + sy * LW + sx;
         const float* r1 = r0 + LW;
@@ -112,27 +158,32 @@ __global__ __launch_bounds__(NT) void k_stencil2d(const float* __restrict__ A, c
         sGy[i] = gy;
         sAl[i] = alpha_of(r1[1], beta_a, nexp);
     }
+
+    // gather geometry of this thread's strip
+    const int tx = tid % TX, ty0 = (tid / TX) * SR;
+    const int gdx = x0 + tx, gdy0 = y0 + ty0;
+    // source columns gdx-1 .. gdx+2 in ascending wrapped order: start after the ones that wrap
+    const int rotx = gdx < 1 ? 1 - gdx : (gdx + 2 >= W ? 4 - (gdx + 2 - (W - 1)) : 0);
+    const bool rows_inside = (y0 >= 1) && (y0 + TY + 2 <= H);
+
     int local_flag = 0;
     for (int c = 0; c < C; ++c) {
-        __syncthreads();  // sU and the box arrays are free
-        const float* Uc = U + (long long)c * N;
-        for (int i = tid; i < LH * LW; i += NT) {
-            const int ly = i / LW, lx = i - ly * LW;
-            sU[i] = Uc[(long long)wrapi(ly0 + ly, H) * W + wrapi(lx0 + lx, W)];
-        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < LPT; ++q)
+            if (tid + q * NT < LH * LW) sU[tid + q * NT] = upre[q];
         __syncthreads();
         const float* Ac = A + (long long)c * N;
-        for (int i = tid; i < SH * SW; i += NT) {
+#pragma unroll 1
+        for (int i = tid; i < NSRC; i += NT) {
             const int sy = i / SW, sx = i - sy * SW;
-            const int ux = x0 - RG + sx, uy = y0 - RG + sy;  // unwrapped source coords
-            const int gxs = wrapi(ux, W), gys = wrapi(uy, H);
+            const int ux = x0 - 1 + sx, uy = y0 - 1 + sy;  // unwrapped source coordinates
+            const int gxs = wrap_near(ux, W), gys = wrap_near(uy, H);
             const float m = Ac[(long long)gys * W + gxs];
-            if (fabsf(m) < 1e-10f) {  // reintegration.rs:76: skipped source
-                sXm[i] = INFINITY;
-                sXM[i] = -INFINITY;
-                sYm[i] = INFINITY;
-                sYM[i] = -INFINITY;
+            if (fabsf(m) < 1e-10f) {  // reintegration.rs:76: a skipped source adds nothing
                 sM[i] = 0.0f;
+                sOX[0][i] = sOX[1][i] = sOX[2][i] = sOX[3][i] = 0.0f;
+                sOY[i] = make_float4(0.f, 0.f, 0.f, 0.f);
                 continue;
             }
             const float* r0 = sU + sy * LW + sx;
@@ -147,57 +198,90 @@ __global__ __launch_bounds__(NT) void k_stencil2d(const float* __restrict__ A, c
             const float dxp = (float)gxs + dt * fx;  // reintegration.rs:81-82
             const float dyp = (float)gys + dt * fy;
             const float xm = dxp - s, xM = dxp + s, ym = dyp - s, yM = dyp + s;
-            sXm[i] = xm;
-            sXM[i] = xM;
-            sYm[i] = ym;
-            sYM[i] = yM;
+            // overlaps with destination cells gxs - o, o = j - 1 (the cell index in the source's frame)
+            float ox[4], oyv[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int ix = gxs + 1 - j, iy = gys + 1 - j;
+                ox[j] = overlap1((float)ix, (float)(ix + 1), xm, xM);
+                oyv[j] = overlap1((float)iy, (float)(iy + 1), ym, yM);
+            }
             sM[i] = m;
-            // each source is checked by the tile that owns it
+            sOX[0][i] = ox[0];
+            sOX[1][i] = ox[1];
+            sOX[2][i] = ox[2];
+            sOX[3][i] = ox[3];
+            sOY[i] = make_float4(oyv[0], oyv[1], oyv[2], oyv[3]);
             if (ux >= x0 && ux < x0 + TX && uy >= y0 && uy < y0 + TY && ux < W && uy < H) {
                 if (!(isfinite(xm) && isfinite(xM) && isfinite(ym) && isfinite(yM) && isfinite(m))) local_flag |= 2;
-                else if (floorf(xm) < (float)(gxs - RG) || ceilf(xM) > (float)(gxs + RG + 1) ||
-                         floorf(ym) < (float)(gys - RG) || ceilf(yM) > (float)(gys + RG + 1))
+                else if (floorf(xm) < (float)(gxs - 2) || ceilf(xM) > (float)(gxs + 2) ||
+                         floorf(ym) < (float)(gys - 2) || ceilf(yM) > (float)(gys + 2))
                     local_flag |= 1;
             }
         }
         __syncthreads();
+        if (c + 1 < C) {
+            const float* Un = U + (long long)(c + 1) * N;
+#pragma unroll
+            for (int q = 0; q < LPT; ++q) upre[q] = tid + q * NT < LH * LW ? Un[goff[q]] : 0.0f;
+        }
+        if (gdx >= W) continue;
         float* Oc = Aout + (long long)c * N;
-        for (int d = tid; d < TX * TY; d += NT) {
-            const int tx = d % TX, ty = d / TX;
-            const int gdx = x0 + tx, gdy = y0 + ty;
-            if (gdx >= W || gdy >= H) continue;
-            // visit window rows / columns in ascending wrapped order (source raster order)
-            const int rotx = gdx < RG ? RG - gdx : (gdx + RG >= W ? W - gdx + RG : 0);
-            const int roty = gdy < RG ? RG - gdy : (gdy + RG >= H ? H - gdy + RG : 0);
-            float acc = 0.0f;
+        if (rows_inside) {
+            float acc[SR];
 #pragma unroll
-            for (int iy = 0; iy < WIN; ++iy) {
-                int jy = iy + roty;
-                if (jy >= WIN) jy -= WIN;
-                const int oy = jy - RG;
-                const int syy = gdy + oy;
-                const int cyi = gdy + (syy < 0 ? H : (syy >= H ? -H : 0));  // dest row in source frame
-                const float cy0 = (float)cyi, cy1 = (float)(cyi + 1);
-                const int srow = (ty + RG + oy) * SW + tx + RG;
+            for (int r = 0; r < SR; ++r) acc[r] = 0.0f;
+#pragma unroll 1
+            for (int a = 0; a < SR + 3; ++a) {  // source rows gdy0 - 1 + a, ascending
+                const int srow = (ty0 + a) * SW + tx + 1;
 #pragma unroll
-                for (int ix = 0; ix < WIN; ++ix) {
-                    int jx = ix + rotx;
-                    if (jx >= WIN) jx -= WIN;
-                    const int ox = jx - RG;
-                    const int sxx = gdx + ox;
-                    const int cxi = gdx + (sxx < 0 ? W : (sxx >= W ? -W : 0));
-                    const float cx0 = (float)cxi, cx1 = (float)(cxi + 1);
-                    const int si = srow + ox;
-                    const float ow = overlap1(cx0, cx1, sXm[si], sXM[si]);
-                    const float oh = overlap1(cy0, cy1, sYm[si], sYM[si]);
-                    const float area = ow * oh;
-                    if (area > 0.0f) {
-                        const float frac = POW2 ? area * inv_ta : area / ta;
-                        acc += sM[si] * frac;
+                for (int i = 0; i < 4; ++i) {
+                    const int jx = (i + rotx) & 3;  // plane j = o + 1, o = source - dest
+                    const int si = srow + jx - 1;
+                    const float m = sM[si];
+                    const float ow = sOX[jx][si];
+                    const float4 oh4 = sOY[si];
+#pragma unroll
+                    for (int r = 0; r < SR; ++r) {
+                        const int jy = a - r;  // o_y + 1
+                        if (jy >= 0 && jy <= 3) {
+                            const float oh = jy == 0 ? oh4.x : jy == 1 ? oh4.y : jy == 2 ? oh4.z : oh4.w;
+                            const float area = ow * oh;
+                            if (area > 0.0f) {
+                                const float frac = POW2 ? area * inv_ta : area / ta;
+                                acc[r] += m * frac;
+                            }
+                        }
                     }
                 }
             }
-            Oc[(long long)gdy * W + gdx] = acc;
+#pragma unroll
+            for (int r = 0; r < SR; ++r)
+                if (gdy0 + r < H) Oc[(long long)(gdy0 + r) * W + gdx] = acc[r];
+        } else {
+#pragma unroll 1
+            for (int r = 0; r < SR; ++r) {
+                const int gdy = gdy0 + r;
+                if (gdy >= H) break;
+                const int roty = gdy < 1 ? 1 - gdy : (gdy + 2 >= H ? 4 - (gdy + 2 - (H - 1)) : 0);
+                float acc = 0.0f;
+                for (int b2 = 0; b2 < 4; ++b2) {
+                    const int jy = (b2 + roty) & 3;
+                    const int srow = (ty0 + r + jy) * SW + tx + 1;  // local row of source gdy + jy - 1
+                    for (int i = 0; i < 4; ++i) {
+                        const int jx = (i + rotx) & 3;
+                        const int si = srow + jx - 1;
+                        const float ow = sOX[jx][si];
+                        const float oh = reinterpret_cast<const float*>(sOY)[si * 4 + jy];
+                        const float area = ow * oh;
+                        if (area > 0.0f) {
+                            const float frac = POW2 ? area * inv_ta : area / ta;
+                            acc += sM[si] * frac;
+                        }
+                    }
+                }
+                Oc[(long long)gdy * W + gdx] = acc;
+            }
         }
     }
     if (local_flag) atomicOr(flag, local_flag);
@@ -205,21 +289,24 @@ __global__ __launch_bounds__(NT) void k_stencil2d(const float* __restrict__ A, c
 
 bool launch_stencil2d(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g, float dt,
                       float beta_a, float nexp, float s, int* flag) {
-    constexpr int RG = 2, TX = 64, TY = 16, NT = 256;
-    if (g.D != 1 || g.W < 2 * RG + 1 || g.H < 2 * RG + 1) return false;
+    using namespace st2;
+    if (g.D != 1 || g.W < 4 || g.H < 4) return false;
+    if (!(s > 0.0f && s <= 1.0f)) return false;  // the 16-source window assumes s <= 1
     const float ta = (2.0f * s) * (2.0f * s);
     if (!(ta >= 1e-10f)) return false;  // point-distribution fallback lives in the general path
     int e;
     const float fr = frexpf(ta, &e);
     const bool pow2 = fr == 0.5f && ta >= 1.17549435e-38f && ta < INFINITY;
     const float inv_ta = 1.0f / ta;
-    dim3 grid((g.W + TX - 1) / TX, (g.H + TY - 1) / TY);
+    const int tiles_x = (g.W + TX - 1) / TX, tiles_y = (g.H + TY - 1) / TY;
+    const int ntiles = tiles_x * tiles_y;
+    const int blocks = (ntiles + 7) / 8 * 8;
     if (pow2)
-        hipLaunchKernelGGL((k_stencil2d<RG, true, TX, TY, NT>), grid, dim3(NT), 0, st, A, U, Aout, C, g.W, g.H, g.N,
-                           dt, beta_a, nexp, s, ta, inv_ta, flag);
+        hipLaunchKernelGGL(k_stencil2d<true>, dim3(blocks), dim3(NT), 0, st, A, U, Aout, C, g.W, g.H, g.N, tiles_x,
+                           ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag);
     else
-        hipLaunchKernelGGL((k_stencil2d<RG, false, TX, TY, NT>), grid, dim3(NT), 0, st, A, U, Aout, C, g.W, g.H,
-                           g.N, dt, beta_a, nexp, s, ta, inv_ta, flag);
+        hipLaunchKernelGGL(k_stencil2d<false>, dim3(blocks), dim3(NT), 0, st, A, U, Aout, C, g.W, g.H, g.N, tiles_x,
+                           ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag);
     return true;
 }
 

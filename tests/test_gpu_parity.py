@@ -78,13 +78,17 @@ def test_stencil_general_path_bitwise(oracle):
 AFF_TOL = 2e-5  # max-abs on U; G amplifies conv error by up to ~1.2/sigma
 
 
-@pytest.mark.parametrize("size,channels", [(256, 1), (128, 3), (192, 2), (96, 1), (48, 2), (1024, 3)])
+@pytest.mark.parametrize("size,channels", [(256, 1), (128, 3), (192, 2), (96, 1), (48, 2), (1024, 3), (512, 2),
+                                           ((256, 512), 1), ((512, 256), 2), (2048, 1)])
 def test_affinity_vs_oracle(oracle, size, channels):
-    _say(f"[affinity] {size}x{size}x{channels}")
+    """Sides 256..2048 run the column-major wave-FFT path, the others the generic passes."""
+    _say(f"[affinity] {size}x{channels}")
     if channels == 1 and size == 256:
         cfg, sd = workloads.plain_256()
     else:
-        cfg, sd = _small_multispecies(size, channels, seed=size + channels)
+        w, h = (size, size) if isinstance(size, int) else size
+        cfg, sd = _small_multispecies(w, channels, seed=w + h + channels)
+        cfg.width, cfg.height = w, h
     st = oracle.seed_generate(sd, cfg.width, cfg.height, 1, cfg.channels)
     p32 = oracle.Propagator(cfg, np.float32)
     p64 = oracle.Propagator(cfg, np.float64)
@@ -97,7 +101,22 @@ def test_affinity_vs_oracle(oracle, size, channels):
     o64 = max(float(np.abs(want32[c] - want64[c]).max()) for c in range(cfg.channels))
     _say(f"  max|gpu-f32 oracle|={e32:.3e} max|gpu-f64|={e64:.3e} max|f32 oracle-f64|={o64:.3e}")
     assert e32 <= AFF_TOL
-    assert e64 <= AFF_TOL
+    # against FP64 the bar is the f32 oracle's own error (f32 arithmetic, growth-amplified)
+    assert e64 <= max(AFF_TOL, 1.5 * o64)
+
+
+def test_spectral_grouping_is_bitwise_neutral(oracle, monkeypatch):
+    """Grouping the spectral pass per source or per kernel runs the same transforms."""
+    _say("[spectral grouping]")
+    cfg, sd = _small_multispecies(512, 3, seed=9)
+    st = oracle.seed_generate(sd, 512, 512, 1, 3)
+    outs = []
+    for grp in ("0", "1"):
+        monkeypatch.setenv("HP_SPECTRAL_GROUP", grp)
+        with av.HipPropagator(cfg) as hp:
+            outs.append(hp.affinity(st))
+    for c in range(3):
+        np.testing.assert_array_equal(outs[0][c], outs[1][c])
 
 
 # ---------------------------------------------------------------------------------------------
