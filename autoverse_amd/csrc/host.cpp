@@ -223,10 +223,7 @@ hp_status enqueue_affinity(hp_ctx* c, const float* A) {
 }
 
 bool fast_stencil_ok(const hp_ctx* c) {
-    if (c->is3d) return false;
-    const float s = c->cfg.flow.distribution_size;
-    const float ta = (2.0f * s) * (2.0f * s);
-    return ta >= 1e-10f && c->g.W >= 5 && c->g.H >= 5;
+    return !c->is3d && hp::stencil2d_supported(c->g, c->cfg.flow.distribution_size);
 }
 
 hp_status enqueue_fast_stencil(hp_ctx* c, const float* A, const float* U, float* out) {

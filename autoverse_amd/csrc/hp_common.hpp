@@ -67,6 +67,7 @@ bool launch_rows_inv_t(hipStream_t st, const float2* YT, float* U, const TgtWork
 // Fused channel sum + Sobel + flow + gather reintegration with a fixed gather
 // radius; sets *flag when any active source's box leaves that radius (bit 0) or
 // is not finite (bit 1).  Returns false if the configuration is not covered.
+bool stencil2d_supported(const Geo& g, float s);
 bool launch_stencil2d(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g,
                       float dt, float beta_a, float nexp, float s, int* flag);
 bool launch_stencil3d(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g,

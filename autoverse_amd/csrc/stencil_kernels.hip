@@ -85,66 +85,102 @@ __device__ __forceinline__ float overlap1(float c0, float c1, float lo, float hi
 // axis: 16 sources.  Any active source whose box reaches further sets flag bit 0 and the host
 // reruns the step through the general path.
 //
-// Work split.  One workgroup owns a TX x TY destination tile.  Per channel, every source of the
-// (TX+3) x (TY+3) source region computes its flow, its box, and its four x and four y overlaps
-// with the destinations it can reach (in its own wrapped frame, exactly as the scatter does),
-// into LDS.  Each thread then gathers a vertical strip of SR destinations: it walks the source
-// rows in ascending order and, within a row, the four source columns in ascending wrapped
-// order, so every destination adds its contributions in the scatter's source raster order.
-// Tiles whose source rows wrap around the grid take a per-destination loop instead.
+// Work split.  One workgroup owns a TX x TY destination tile.  All global reads of the tile
+// (the state over the load region, the masses of the workgroup's sources for every channel,
+// the first channel's affinity) are issued together up front; later channels' affinity is
+// loaded into registers while the previous channel is gathered.  Per channel, every source of
+// the (TX+3) x (TY+3) source region computes its flow, its box and its four x and four y
+// overlaps with the destinations it can reach (in its own wrapped frame, exactly as the
+// scatter does) into LDS.  Each thread then gathers a vertical strip of SR destinations: it
+// walks the strip's source rows and, within a row, the four source columns, both in ascending
+// wrapped order, so every destination adds its contributions in the scatter's source raster
+// order -- also at the periodic edges, where the visiting order is rotated.
 // ---------------------------------------------------------------------------------------------
+#ifndef HP_ST_TX
+#define HP_ST_TX 64  // destination tile width
+#endif
+#ifndef HP_ST_TY
+#define HP_ST_TY 16  // destination tile height
+#endif
+#ifndef HP_ST_NT
+#define HP_ST_NT 512  // threads per workgroup
+#endif
 namespace st2 {
-constexpr int TX = 64, TY = 16, NT = 256, SR = 4;
+constexpr int TX = HP_ST_TX, TY = HP_ST_TY, NT = HP_ST_NT, SR = TX * TY / NT;
 constexpr int SW = TX + 3, SH = TY + 3;  // source region: offsets -1 .. +2 around the tile
 constexpr int LW = SW + 2, LH = SH + 2;  // + Sobel halo
 constexpr int NSRC = SW * SH;
-static_assert(TX * TY == NT * SR, "one strip per thread");
+constexpr int NLOAD = LW * LH;
+constexpr int LPT = (NLOAD + NT - 1) / NT;  // load-region cells per thread
+constexpr int SPT = (NSRC + NT - 1) / NT;   // sources per thread
+constexpr int SROWS = SR + 3;               // source rows of one strip
+static_assert(TX * TY == NT * SR && NT % TX == 0, "one strip per thread");
 }  // namespace st2
 
-template <bool POW2>
-__global__ __launch_bounds__(st2::NT, 2) void k_stencil2d(const float* __restrict__ A, const float* __restrict__ U,
-                                                       float* __restrict__ Aout, int C, int W, int H, long long N,
+// CC: channel count known at compile time (1..4), or 0 for any count (masses re-read per channel).
+template <int CC, bool POW2>
+__global__ __launch_bounds__(st2::NT) void k_stencil2d(const float* __restrict__ A, const float* __restrict__ U,
+                                                       float* __restrict__ Aout, int Cdyn, int W, int H, long long N,
                                                        int tiles_x, int ntiles, float dt, float beta_a, float nexp,
                                                        float s, float ta, float inv_ta, int* __restrict__ flag) {
     using namespace st2;
-    __shared__ float sSum[LH * LW];
-    __shared__ float sU[LH * LW];
+    const int C = CC > 0 ? CC : Cdyn;
+    __shared__ float sSum[NLOAD];
+    __shared__ float sU[2][NLOAD];
     __shared__ float sAl[NSRC], sGx[NSRC], sGy[NSRC];
     __shared__ float sM[NSRC];
     __shared__ float sOX[4][NSRC];
     __shared__ float4 sOY[NSRC];
 
-    // XCD-aware tile order: workgroup b runs on XCD b % 8; give each XCD a contiguous run of
+    // XCD-aware tile order: workgroup b runs on XCD b % 8; each XCD gets a contiguous run of
     // tiles so that neighbouring tiles (which share halo rows) meet in the same L2.
     const int b = blockIdx.x;
     const int per = (ntiles + 7) / 8;
     const int tile = (b % 8) * per + b / 8;
     if (tile >= ntiles) return;
     const int x0 = (tile % tiles_x) * TX, y0 = (tile / tiles_x) * TY;
-    const int lx0 = x0 - 2, ly0 = y0 - 2;  // load-region origin (unwrapped)
     const int tid = threadIdx.x;
 
-    // per-thread global offsets of the load region (wrapped once, reused for every channel)
-    constexpr int LPT = (LH * LW + NT - 1) / NT;
+    // ---- every global read of the tile, issued together ----
     int goff[LPT];
 #pragma unroll
     for (int q = 0; q < LPT; ++q) {
         const int i = tid + q * NT;
         const int ly = i / LW, lx = i - ly * LW;
-        goff[q] = i < LH * LW ? wrap_near(ly0 + ly, H) * W + wrap_near(lx0 + lx, W) : 0;
+        goff[q] = i < NLOAD ? wrap_near(y0 - 2 + ly, H) * W + wrap_near(x0 - 2 + lx, W) : 0;
     }
+    int soff[SPT];
 #pragma unroll
-    for (int q = 0; q < LPT; ++q) {  // channel sum (propagator.rs:304-310)
+    for (int q = 0; q < SPT; ++q) {
         const int i = tid + q * NT;
-        if (i < LH * LW) {
-            float sum = 0.0f;
+        const int sy = i / SW, sx = i - sy * SW;
+        soff[q] = i < NSRC ? wrap_near(y0 - 1 + sy, H) * W + wrap_near(x0 - 1 + sx, W) : 0;
+    }
+    float msrc[CC > 0 ? CC : 1][SPT];
+    if (CC > 0) {
+#pragma unroll
+        for (int c = 0; c < (CC > 0 ? CC : 1); ++c)
+#pragma unroll
+            for (int q = 0; q < SPT; ++q) msrc[c][q] = A[(long long)c * N + soff[q]];
+    }
+    float upre[LPT];
+#pragma unroll
+    for (int q = 0; q < LPT; ++q) upre[q] = U[goff[q]];
+#pragma unroll
+    for (int q = 0; q < LPT; ++q) {  // channel sum (propagator.rs:304-310): 0 + A_0 + A_1 + ...
+        float sum = 0.0f;
+        if (CC > 0) {
+#pragma unroll
+            for (int c = 0; c < (CC > 0 ? CC : 1); ++c) sum += A[(long long)c * N + goff[q]];
+        } else {
             for (int c = 0; c < C; ++c) sum += A[(long long)c * N + goff[q]];
+        }
+        const int i = tid + q * NT;
+        if (i < NLOAD) {
             sSum[i] = sum;
+            sU[0][i] = upre[q];
         }
     }
-    float upre[LPT];  // affinity of the next channel, loaded ahead of its use
-#pragma unroll
-    for (int q = 0; q < LPT; ++q) upre[q] = tid + q * NT < LH * LW ? U[goff[q]] : 0.0f;
     __syncthreads();
 #pragma unroll 1
     for (int i = tid; i < NSRC; i += NT) {  // mass gradient and alpha over the source region
@@ -158,35 +194,42 @@ __global__ __launch_bounds__(st2::NT, 2) void k_stencil2d(const float* __restric
         sGy[i] = gy;
         sAl[i] = alpha_of(r1[1], beta_a, nexp);
     }
+    __syncthreads();
 
-    // gather geometry of this thread's strip
+    // strip geometry: destination column gdx, rows gdy0 .. gdy0 + SR - 1
     const int tx = tid % TX, ty0 = (tid / TX) * SR;
     const int gdx = x0 + tx, gdy0 = y0 + ty0;
-    // source columns gdx-1 .. gdx+2 in ascending wrapped order: start after the ones that wrap
+    // ascending wrapped order: start with the first source column / row that does not wrap low
     const int rotx = gdx < 1 ? 1 - gdx : (gdx + 2 >= W ? 4 - (gdx + 2 - (W - 1)) : 0);
-    const bool rows_inside = (y0 >= 1) && (y0 + TY + 2 <= H);
+    const int roty = gdy0 < 1 ? 1 - gdy0 : (gdy0 + SR + 1 >= H ? SROWS - (gdy0 + SR + 1 - (H - 1)) : 0);
 
     int local_flag = 0;
     for (int c = 0; c < C; ++c) {
-        __syncthreads();
+        const float* su = sU[c & 1];
+        // ---- sources: flow, box, overlaps ----
 #pragma unroll
-        for (int q = 0; q < LPT; ++q)
-            if (tid + q * NT < LH * LW) sU[tid + q * NT] = upre[q];
-        __syncthreads();
-        const float* Ac = A + (long long)c * N;
-#pragma unroll 1
-        for (int i = tid; i < NSRC; i += NT) {
+        for (int q = 0; q < SPT; ++q) {
+            const int i = tid + q * NT;
+            if (i >= NSRC) break;
             const int sy = i / SW, sx = i - sy * SW;
             const int ux = x0 - 1 + sx, uy = y0 - 1 + sy;  // unwrapped source coordinates
             const int gxs = wrap_near(ux, W), gys = wrap_near(uy, H);
-            const float m = Ac[(long long)gys * W + gxs];
+            float m;
+            if (CC > 0) {
+                m = 0.0f;
+#pragma unroll
+                for (int cc = 0; cc < (CC > 0 ? CC : 1); ++cc)
+                    if (cc == c) m = msrc[cc][q];
+            } else {
+                m = A[(long long)c * N + soff[q]];
+            }
             if (fabsf(m) < 1e-10f) {  // reintegration.rs:76: a skipped source adds nothing
                 sM[i] = 0.0f;
                 sOX[0][i] = sOX[1][i] = sOX[2][i] = sOX[3][i] = 0.0f;
                 sOY[i] = make_float4(0.f, 0.f, 0.f, 0.f);
                 continue;
             }
-            const float* r0 = sU + sy * LW + sx;
+            const float* r0 = su + sy * LW + sx;
             const float* r1 = r0 + LW;
             const float* r2 = r1 + LW;
             float gux, guy;
@@ -198,7 +241,7 @@ __global__ __launch_bounds__(st2::NT, 2) void k_stencil2d(const float* __restric
             const float dxp = (float)gxs + dt * fx;  // reintegration.rs:81-82
             const float dyp = (float)gys + dt * fy;
             const float xm = dxp - s, xM = dxp + s, ym = dyp - s, yM = dyp + s;
-            // overlaps with destination cells gxs - o, o = j - 1 (the cell index in the source's frame)
+            // overlaps with destination cells gxs - o, o = j - 1 (cell index in the source's frame)
             float ox[4], oyv[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
@@ -219,20 +262,27 @@ __global__ __launch_bounds__(st2::NT, 2) void k_stencil2d(const float* __restric
                     local_flag |= 1;
             }
         }
-        __syncthreads();
+        // next channel's affinity: into registers now, into the other LDS buffer after the barrier
         if (c + 1 < C) {
             const float* Un = U + (long long)(c + 1) * N;
 #pragma unroll
-            for (int q = 0; q < LPT; ++q) upre[q] = tid + q * NT < LH * LW ? Un[goff[q]] : 0.0f;
+            for (int q = 0; q < LPT; ++q) upre[q] = Un[goff[q]];
         }
-        if (gdx >= W) continue;
-        float* Oc = Aout + (long long)c * N;
-        if (rows_inside) {
+        __syncthreads();
+        if (c + 1 < C) {
+#pragma unroll
+            for (int q = 0; q < LPT; ++q)
+                if (tid + q * NT < NLOAD) sU[(c + 1) & 1][tid + q * NT] = upre[q];
+        }
+        // ---- gather: strip of SR destinations ----
+        if (gdx < W) {
             float acc[SR];
 #pragma unroll
             for (int r = 0; r < SR; ++r) acc[r] = 0.0f;
-#pragma unroll 1
-            for (int a = 0; a < SR + 3; ++a) {  // source rows gdy0 - 1 + a, ascending
+#pragma unroll
+            for (int it = 0; it < SROWS; ++it) {
+                int a = it + roty;  // strip source row a <-> grid row gdy0 - 1 + a
+                if (a >= SROWS) a -= SROWS;
                 const int srow = (ty0 + a) * SW + tx + 1;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
@@ -255,58 +305,55 @@ __global__ __launch_bounds__(st2::NT, 2) void k_stencil2d(const float* __restric
                     }
                 }
             }
+            float* Oc = Aout + (long long)c * N;
 #pragma unroll
             for (int r = 0; r < SR; ++r)
                 if (gdy0 + r < H) Oc[(long long)(gdy0 + r) * W + gdx] = acc[r];
-        } else {
-#pragma unroll 1
-            for (int r = 0; r < SR; ++r) {
-                const int gdy = gdy0 + r;
-                if (gdy >= H) break;
-                const int roty = gdy < 1 ? 1 - gdy : (gdy + 2 >= H ? 4 - (gdy + 2 - (H - 1)) : 0);
-                float acc = 0.0f;
-                for (int b2 = 0; b2 < 4; ++b2) {
-                    const int jy = (b2 + roty) & 3;
-                    const int srow = (ty0 + r + jy) * SW + tx + 1;  // local row of source gdy + jy - 1
-                    for (int i = 0; i < 4; ++i) {
-                        const int jx = (i + rotx) & 3;
-                        const int si = srow + jx - 1;
-                        const float ow = sOX[jx][si];
-                        const float oh = reinterpret_cast<const float*>(sOY)[si * 4 + jy];
-                        const float area = ow * oh;
-                        if (area > 0.0f) {
-                            const float frac = POW2 ? area * inv_ta : area / ta;
-                            acc += sM[si] * frac;
-                        }
-                    }
-                }
-                Oc[(long long)gdy * W + gdx] = acc;
-            }
         }
+        __syncthreads();  // the next channel's source phase overwrites the overlaps
     }
     if (local_flag) atomicOr(flag, local_flag);
+}
+
+template <int CC>
+static void launch_st2(hipStream_t st, bool pow2, dim3 grid, const float* A, const float* U, float* Aout, int C,
+                       const Geo& g, int tiles_x, int ntiles, float dt, float beta_a, float nexp, float s, float ta,
+                       float inv_ta, int* flag) {
+    if (pow2)
+        hipLaunchKernelGGL((k_stencil2d<CC, true>), grid, dim3(st2::NT), 0, st, A, U, Aout, C, g.W, g.H, g.N, tiles_x,
+                           ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag);
+    else
+        hipLaunchKernelGGL((k_stencil2d<CC, false>), grid, dim3(st2::NT), 0, st, A, U, Aout, C, g.W, g.H, g.N, tiles_x,
+                           ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag);
+}
+
+bool stencil2d_supported(const Geo& g, float s) {
+    // the rotated visiting orders need every strip's source rows / columns to be distinct cells;
+    // the 16-source window assumes s <= 1; the point-distribution fallback (total area < 1e-10)
+    // lives in the general path
+    const float ta = (2.0f * s) * (2.0f * s);
+    return g.D == 1 && g.W >= 4 && g.H >= st2::SROWS && s > 0.0f && s <= 1.0f && ta >= 1e-10f;
 }
 
 bool launch_stencil2d(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g, float dt,
                       float beta_a, float nexp, float s, int* flag) {
     using namespace st2;
-    if (g.D != 1 || g.W < 4 || g.H < 4) return false;
-    if (!(s > 0.0f && s <= 1.0f)) return false;  // the 16-source window assumes s <= 1
+    if (!stencil2d_supported(g, s)) return false;
     const float ta = (2.0f * s) * (2.0f * s);
-    if (!(ta >= 1e-10f)) return false;  // point-distribution fallback lives in the general path
     int e;
     const float fr = frexpf(ta, &e);
     const bool pow2 = fr == 0.5f && ta >= 1.17549435e-38f && ta < INFINITY;
     const float inv_ta = 1.0f / ta;
     const int tiles_x = (g.W + TX - 1) / TX, tiles_y = (g.H + TY - 1) / TY;
     const int ntiles = tiles_x * tiles_y;
-    const int blocks = (ntiles + 7) / 8 * 8;
-    if (pow2)
-        hipLaunchKernelGGL(k_stencil2d<true>, dim3(blocks), dim3(NT), 0, st, A, U, Aout, C, g.W, g.H, g.N, tiles_x,
-                           ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag);
-    else
-        hipLaunchKernelGGL(k_stencil2d<false>, dim3(blocks), dim3(NT), 0, st, A, U, Aout, C, g.W, g.H, g.N, tiles_x,
-                           ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag);
+    const dim3 grid((ntiles + 7) / 8 * 8);
+    switch (C) {
+    case 1: launch_st2<1>(st, pow2, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
+    case 2: launch_st2<2>(st, pow2, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
+    case 3: launch_st2<3>(st, pow2, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
+    case 4: launch_st2<4>(st, pow2, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
+    default: launch_st2<0>(st, pow2, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag);
+    }
     return true;
 }
 
