@@ -105,6 +105,7 @@ __device__ __forceinline__ float overlap1(float c0, float c1, float lo, float hi
 #ifndef HP_ST_NT
 #define HP_ST_NT 512  // threads per workgroup
 #endif
+
 namespace st2 {
 constexpr int TX = HP_ST_TX, TY = HP_ST_TY, NT = HP_ST_NT, SR = TX * TY / NT;
 constexpr int SW = TX + 3, SH = TY + 3;  // source region: offsets -1 .. +2 around the tile
@@ -121,8 +122,9 @@ static_assert(TX * TY == NT * SR && NT % TX == 0, "one strip per thread");
 template <int CC, bool POW2>
 __global__ __launch_bounds__(st2::NT) void k_stencil2d(const float* __restrict__ A, const float* __restrict__ U,
                                                        float* __restrict__ Aout, int Cdyn, int W, int H, long long N,
-                                                       int tiles_x, int ntiles, float dt, float beta_a, float nexp,
-                                                       float s, float ta, float inv_ta, int* __restrict__ flag) {
+                                                       int tiles_x, int ntiles, float dt,
+                                                       float beta_a, float nexp, float s, float ta, float inv_ta,
+                                                       int* __restrict__ flag) {
     using namespace st2;
     const int C = CC > 0 ? CC : Cdyn;
     __shared__ float sSum[NLOAD];
@@ -275,29 +277,53 @@ __global__ __launch_bounds__(st2::NT) void k_stencil2d(const float* __restrict__
                 if (tid + q * NT < NLOAD) sU[(c + 1) & 1][tid + q * NT] = upre[q];
         }
         // ---- gather: strip of SR destinations ----
+        // A zero-area pair contributes m * 0 = +-0; added to an accumulator that started at +0 it
+        // changes no bit, so every pair is added without the reference's `area > 0` test.
         if (gdx < W) {
             float acc[SR];
 #pragma unroll
             for (int r = 0; r < SR; ++r) acc[r] = 0.0f;
+            if (roty == 0) {
 #pragma unroll
-            for (int it = 0; it < SROWS; ++it) {
-                int a = it + roty;  // strip source row a <-> grid row gdy0 - 1 + a
-                if (a >= SROWS) a -= SROWS;
-                const int srow = (ty0 + a) * SW + tx + 1;
+                for (int a = 0; a < SROWS; ++a) {  // strip source row a <-> grid row gdy0 - 1 + a
+                    const int srow = (ty0 + a) * SW + tx + 1;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int jx = (i + rotx) & 3;  // plane j = o + 1, o = source - dest
-                    const int si = srow + jx - 1;
-                    const float m = sM[si];
-                    const float ow = sOX[jx][si];
-                    const float4 oh4 = sOY[si];
+                    for (int i = 0; i < 4; ++i) {
+                        const int jx = (i + rotx) & 3;  // plane j = o + 1, o = source - dest
+                        const int si = srow + jx - 1;
+                        const float m = sM[si];
+                        const float ow = sOX[jx][si];
+                        const float4 oh4 = sOY[si];
+                        const float ohv[4] = {oh4.x, oh4.y, oh4.z, oh4.w};
 #pragma unroll
-                    for (int r = 0; r < SR; ++r) {
-                        const int jy = a - r;  // o_y + 1
-                        if (jy >= 0 && jy <= 3) {
-                            const float oh = jy == 0 ? oh4.x : jy == 1 ? oh4.y : jy == 2 ? oh4.z : oh4.w;
-                            const float area = ow * oh;
-                            if (area > 0.0f) {
+                        for (int r = 0; r < SR; ++r) {
+                            if (a - r >= 0 && a - r <= 3) {
+                                const float area = ow * ohv[a - r];
+                                const float frac = POW2 ? area * inv_ta : area / ta;
+                                acc[r] += m * frac;
+                            }
+                        }
+                    }
+                }
+            } else {
+#pragma unroll 1
+                for (int it = 0; it < SROWS; ++it) {
+                    int a = it + roty;
+                    if (a >= SROWS) a -= SROWS;
+                    const int srow = (ty0 + a) * SW + tx + 1;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int jx = (i + rotx) & 3;
+                        const int si = srow + jx - 1;
+                        const float m = sM[si];
+                        const float ow = sOX[jx][si];
+                        const float4 oh4 = sOY[si];
+#pragma unroll
+                        for (int r = 0; r < SR; ++r) {
+                            const int jy = a - r;  // o_y + 1
+                            if (jy >= 0 && jy <= 3) {
+                                const float oh = jy == 0 ? oh4.x : jy == 1 ? oh4.y : jy == 2 ? oh4.z : oh4.w;
+                                const float area = ow * oh;
                                 const float frac = POW2 ? area * inv_ta : area / ta;
                                 acc[r] += m * frac;
                             }
@@ -317,8 +343,8 @@ __global__ __launch_bounds__(st2::NT) void k_stencil2d(const float* __restrict__
 
 template <int CC>
 static void launch_st2(hipStream_t st, bool pow2, dim3 grid, const float* A, const float* U, float* Aout, int C,
-                       const Geo& g, int tiles_x, int ntiles, float dt, float beta_a, float nexp, float s, float ta,
-                       float inv_ta, int* flag) {
+                       const Geo& g, int tiles_x, int ntiles, float dt, float beta_a, float nexp, float s,
+                       float ta, float inv_ta, int* flag) {
     if (pow2)
         hipLaunchKernelGGL((k_stencil2d<CC, true>), grid, dim3(st2::NT), 0, st, A, U, Aout, C, g.W, g.H, g.N, tiles_x,
                            ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag);
