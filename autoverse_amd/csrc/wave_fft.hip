@@ -166,6 +166,19 @@ __device__ __forceinline__ void wave_fft(float2 (&e)[L / WAVE], float2* line, co
     if constexpr (stages(L) > 3) wave_stage<L, 3, DIR>(e, line, tw);
 }
 
+// XCD-aware block order for the row passes: workgroup b runs on XCD b % 8, so give every XCD a
+// contiguous run of row blocks -- adjacent blocks read the two halves of the same 128-byte
+// lines of a column-major spectrum, and then meet in the same L2.
+__device__ __forceinline__ bool xcd_block(int nbx, int total, int& bx, int& by) {
+    const int b = blockIdx.x;
+    const int per = (total + 7) / 8;
+    const int lin = (b % 8) * per + b / 8;
+    if (lin >= total) return false;
+    bx = lin % nbx;
+    by = lin / nbx;
+    return true;
+}
+
 constexpr int WAVES = 4;  // waves per workgroup
 constexpr int NT = WAVES * WAVE;
 constexpr int ROWS = 2 * WAVES;  // rows per workgroup in the row passes (one packed pair per wave)
@@ -175,14 +188,15 @@ constexpr int ROWS = 2 * WAVES;  // rows per workgroup in the row passes (one pa
 template <int L>
 __global__ __launch_bounds__(NT) void k_rows_fwd_t(const float* __restrict__ A, float2* __restrict__ XT,
                                                    const int* __restrict__ src_chan, int H, long long N,
-                                                   long long spec, const float2* __restrict__ twg) {
+                                                   long long spec, int nblocks, const float2* __restrict__ twg) {
     constexpr int E = L / WAVE, LH = L / 2 + 1;
     constexpr int LINE = padded(L);
     __shared__ float2 tw[L];
     __shared__ float2 lines[WAVES * LINE > LH * ROWS ? WAVES * LINE : LH * ROWS];  // FFT lines, then the tile
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const int s = blockIdx.y;
-    const int ybase = blockIdx.x * ROWS;
+    int bx, s;
+    if (!xcd_block(H / ROWS, nblocks, bx, s)) return;
+    const int ybase = bx * ROWS;
     for (int i = threadIdx.x; i < L; i += NT) tw[i] = twg[i];
     const float* a0 = A + (long long)src_chan[s] * N + (long long)(ybase + 2 * wave) * L;
     const float* a1 = a0 + L;
@@ -270,19 +284,24 @@ template <int L>
 __global__ __launch_bounds__(NT) void k_rows_inv_t(const float2* __restrict__ YT, float* __restrict__ U,
                                                    const TgtWork* __restrict__ work, const int* __restrict__ klist,
                                                    const GrowthParams* __restrict__ gp, int H, long long N,
-                                                   long long spec, float scale, const float2* __restrict__ twg) {
+                                                   long long spec, float scale, int nblocks,
+                                                   const float2* __restrict__ twg) {
     constexpr int E = L / WAVE, LINE = padded(L), LH = L / 2 + 1;
     __shared__ float2 tw[L];
-    __shared__ float2 lines[WAVES * LINE];
-    __shared__ float2 tile[LH * ROWS];
+    // the staging tile and the per-wave FFT lines share memory: the tile is read completely
+    // into registers (and a barrier passed) before any wave starts transforming
+    __shared__ float2 buf[WAVES * LINE > LH * ROWS ? WAVES * LINE : LH * ROWS];
+    float2* tile = buf;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const TgtWork wk = work[blockIdx.y];
-    const int ybase = blockIdx.x * ROWS;
+    int bx, wi;
+    if (!xcd_block(H / ROWS, nblocks, bx, wi)) return;
+    const TgtWork wk = work[wi];
+    const int ybase = bx * ROWS;
     for (int i = threadIdx.x; i < L; i += NT) tw[i] = twg[i];
     float acc0[E], acc1[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) acc0[i] = acc1[i] = 0.0f;
-    float2* line = lines + wave * LINE;
+    float2* line = buf + wave * LINE;
     const int r0 = 2 * wave;
     // the next kernel's tile is loaded into registers while the current one is transformed
     constexpr int TPT = (LH * ROWS + NT - 1) / NT;
@@ -319,6 +338,7 @@ __global__ __launch_bounds__(NT) void k_rows_inv_t(const float2* __restrict__ YT
                 e[i] = make_float2(a.x + b.y, b.x - a.y);
             }
         }
+        __syncthreads();  // every wave holds its rows: the tile memory becomes FFT lines
         wave_fft<L, +1>(e, line, tw);
         const GrowthParams p = gp[k];
 #pragma unroll
@@ -344,7 +364,9 @@ __global__ __launch_bounds__(NT) void k_rows_inv_t(const float2* __restrict__ YT
 template <int L>
 void rows_fwd_t(hipStream_t st, const float* A, float2* XT, const int* src_chan, int nsrc, const Geo& g,
                 const float2* tw) {
-    hipLaunchKernelGGL(k_rows_fwd_t<L>, dim3(g.H / ROWS, nsrc), dim3(NT), 0, st, A, XT, src_chan, g.H, g.N, g.spec, tw);
+    const int nblocks = g.H / ROWS * nsrc;
+    hipLaunchKernelGGL(k_rows_fwd_t<L>, dim3((nblocks + 7) / 8 * 8), dim3(NT), 0, st, A, XT, src_chan, g.H, g.N, g.spec,
+                       nblocks, tw);
 }
 
 template <int L>
@@ -359,8 +381,9 @@ template <int L>
 void rows_inv_t(hipStream_t st, const float2* YT, float* U, const TgtWork* work, int nwork, const int* klist,
                 const GrowthParams* gp, const Geo& g, const float2* tw) {
     const float scale = 1.0f / (float)(g.N);  // fft.rs:389: 1 / (W*H) as f32
-    hipLaunchKernelGGL(k_rows_inv_t<L>, dim3(g.H / ROWS, nwork), dim3(NT), 0, st, YT, U, work, klist, gp, g.H, g.N,
-                       g.spec, scale, tw);
+    const int nblocks = g.H / ROWS * nwork;
+    hipLaunchKernelGGL(k_rows_inv_t<L>, dim3((nblocks + 7) / 8 * 8), dim3(NT), 0, st, YT, U, work, klist, gp, g.H, g.N,
+                       g.spec, scale, nblocks, tw);
 }
 
 }  // namespace wv
