@@ -112,6 +112,15 @@ EXPORTS = [
     ("hp_growth_accumulate", c_int, [c_int, FP, FP, c_uint32, c_float, c_float, c_float]),
     ("hp_stencil_with_affinity", c_int, [c_void_p, FloatPP, FloatPP, FloatPP]),
     ("hp_affinity", c_int, [c_void_p, FloatPP, FloatPP]),
+    ("hp_batch_create", c_int, [POINTER(HpConfig), c_uint32, c_int, POINTER(c_void_p)]),
+    ("hp_batch_destroy", None, [c_void_p]),
+    ("hp_batch_last_error", c_char_p, [c_void_p]),
+    ("hp_batch_set_state", c_int, [c_void_p, c_uint32, FloatPP]),
+    ("hp_batch_get_state", c_int, [c_void_p, c_uint32, FloatPP, POINTER(c_float), POINTER(c_uint64)]),
+    ("hp_batch_advance", c_int, [c_void_p, c_uint64]),
+    ("hp_batch_centroids", c_int, [c_void_p, POINTER(c_double), POINTER(c_double)]),
+    ("hp_batch_mass_f64", c_int, [c_void_p, POINTER(c_double)]),
+    ("hp_batch_stats", c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint64)]),
 ]
 
 _LIB = None

@@ -27,7 +27,7 @@ DEVICE_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
                 "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                 "-fno-gpu-flush-denormals-to-zero", "-fhip-fp32-correctly-rounded-divide-sqrt",
                 "-Wno-unused-command-line-argument"]
-HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math"]
+HOST_FLAGS = [f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math"]
 
 SOURCES = [
     ("fft_kernels.hip", DEVICE_FLAGS + ["-ffp-contract=fast"]),
@@ -35,6 +35,7 @@ SOURCES = [
     ("stencil_kernels.hip", DEVICE_FLAGS + ["-ffp-contract=off"]),
     ("host.cpp", HOST_FLAGS),
     ("host_seed.cpp", HOST_FLAGS),
+    ("host_batch.cpp", HOST_FLAGS),
 ]
 
 

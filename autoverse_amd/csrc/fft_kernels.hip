@@ -174,13 +174,12 @@ __device__ __forceinline__ void stage_twiddles(float2* tw, const float2* __restr
 // ---- pass 1: R2C along x on pairs of real rows ----------------------------------------
 // grid (ceil(rows/2), nsrc); rows = H*D; src_chan[s] = channel of source slot s.
 template <int L, int NT>
-__global__ __launch_bounds__(NT) void k_rows_fwd(const float* __restrict__ A, float2* __restrict__ X,
-                                                 const int* __restrict__ src_chan, int rows, int Wp,
-                                                 long long N, long long spec, const float2* __restrict__ twg) {
+__device__ __forceinline__ void rows_fwd_body(const float* __restrict__ A, float2* __restrict__ X,
+                                              const int* __restrict__ src_chan, int rows, int Wp, long long N,
+                                              long long spec, const float2* __restrict__ twg, int pair, int s) {
     __shared__ float2 buf[L];
     __shared__ float2 tw[L];
-    const int s = blockIdx.y;
-    const int y0 = 2 * blockIdx.x;
+    const int y0 = 2 * pair;
     const bool has1 = (y0 + 1) < rows;
     const float* a = A + (long long)src_chan[s] * N + (long long)y0 * L;
     stage_twiddles<NT>(tw, twg, L);
@@ -195,6 +194,22 @@ __global__ __launch_bounds__(NT) void k_rows_fwd(const float* __restrict__ A, fl
         x0[k] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
         if (has1) x1[k] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
     }
+}
+
+template <int L, int NT>
+__global__ __launch_bounds__(NT) void k_rows_fwd(const float* __restrict__ A, float2* __restrict__ X,
+                                                 const int* __restrict__ src_chan, int rows, int Wp,
+                                                 long long N, long long spec, const float2* __restrict__ twg) {
+    rows_fwd_body<L, NT>(A, X, src_chan, rows, Wp, N, spec, twg, blockIdx.x, blockIdx.y);
+}
+
+// batched: item = (member, row pair, source slot)
+template <int L, int NT>
+__global__ __launch_bounds__(NT) void kb_rows_fwd(const MemberDesc* __restrict__ mem, const int4* __restrict__ items,
+                                                  int cur) {
+    const int4 it = items[blockIdx.x];
+    const MemberDesc& m = mem[it.x];
+    rows_fwd_body<L, NT>(m.A[cur], m.X, m.src_chan, m.g.H * m.g.D, m.g.Wp, m.g.N, m.g.spec, m.tw_w, it.y, it.z);
 }
 
 // ---- 3D only: FFT along y, in place, forward or inverse ------------------------------------
@@ -227,20 +242,20 @@ __global__ __launch_bounds__(NT) void k_cols(float2* __restrict__ X, int Wh, int
 // DIM 3: lines are ky*Wp + kx (element stride H*Wp, L = D).
 // grid (ceil(lines/CB), nwork); work[w] names the source slot and its kernels.
 template <int L, int CB, int NT, int DIM>
-__global__ __launch_bounds__(NT) void k_spectral(const float2* __restrict__ X, float2* __restrict__ Y,
-                                                 const float* __restrict__ Kq, const SpecWork* __restrict__ work,
-                                                 const int* __restrict__ klist, int Wh, int Wp, int H, int Hq,
-                                                 long long spec, long long kq_stride,
-                                                 const float2* __restrict__ twg) {
+__device__ __forceinline__ void spectral_body(const float2* __restrict__ X, float2* __restrict__ Y,
+                                              const float* __restrict__ Kq, const SpecWork* __restrict__ work,
+                                              const int* __restrict__ klist, int Wh, int Wp, int H, int Hq,
+                                              long long spec, long long kq_stride, const float2* __restrict__ twg,
+                                              int blk, int wi) {
     constexpr int LP = L + 1;
     constexpr int PER = CB * L / NT;
     static_assert((CB * L) % NT == 0, "tile must split evenly over the threads");
     __shared__ float2 buf[CB * LP];
     __shared__ float2 tw[L];
-    const SpecWork wk = work[blockIdx.y];
+    const SpecWork wk = work[wi];
     const long long nlines = DIM == 2 ? (long long)Wh : (long long)H * Wp;
     const long long es = DIM == 2 ? (long long)Wp : (long long)H * Wp;
-    const long long line0 = (long long)blockIdx.x * CB;
+    const long long line0 = (long long)blk * CB;
     const float2* xs = X + (long long)wk.src_slot * spec;
     stage_twiddles<NT>(tw, twg, L);
     float2 reg[PER];
@@ -299,19 +314,37 @@ __global__ __launch_bounds__(NT) void k_spectral(const float2* __restrict__ X, f
     }
 }
 
+template <int L, int CB, int NT, int DIM>
+__global__ __launch_bounds__(NT) void k_spectral(const float2* __restrict__ X, float2* __restrict__ Y,
+                                                 const float* __restrict__ Kq, const SpecWork* __restrict__ work,
+                                                 const int* __restrict__ klist, int Wh, int Wp, int H, int Hq,
+                                                 long long spec, long long kq_stride,
+                                                 const float2* __restrict__ twg) {
+    spectral_body<L, CB, NT, DIM>(X, Y, Kq, work, klist, Wh, Wp, H, Hq, spec, kq_stride, twg, blockIdx.x, blockIdx.y);
+}
+
+// batched (2D): item = (member, column block, work item)
+template <int L, int CB, int NT>
+__global__ __launch_bounds__(NT) void kb_spectral(const MemberDesc* __restrict__ mem, const int4* __restrict__ items) {
+    const int4 it = items[blockIdx.x];
+    const MemberDesc& m = mem[it.x];
+    spectral_body<L, CB, NT, 2>(m.X, m.Y, m.Kq, m.spec_work, m.klist_spec, m.g.Wh, m.g.Wp, m.g.H, m.g.Hq, m.g.spec,
+                                (long long)m.g.Dq * m.g.Hq * m.g.Wh, m.tw_h, it.y, it.z);
+}
+
 // ---- pass 3: C2R along x on pairs of rows + growth + weighted accumulate -----------------
 // grid (ceil(rows/2), nwork); work[w] is a target channel and its kernels in index order.
 template <int L, int NT>
-__global__ __launch_bounds__(NT) void k_rows_inv(const float2* __restrict__ Y, float* __restrict__ U,
-                                                 const TgtWork* __restrict__ work, const int* __restrict__ klist,
-                                                 const GrowthParams* __restrict__ gp, int rows, int Wp,
-                                                 long long N, long long spec, float scale,
-                                                 const float2* __restrict__ twg) {
+__device__ __forceinline__ void rows_inv_body(const float2* __restrict__ Y, float* __restrict__ U,
+                                              const TgtWork* __restrict__ work, const int* __restrict__ klist,
+                                              const GrowthParams* __restrict__ gp, int rows, int Wp, long long N,
+                                              long long spec, float scale, const float2* __restrict__ twg, int pair,
+                                              int wi) {
     constexpr int PER = (L + NT - 1) / NT;
     __shared__ float2 buf[L];
     __shared__ float2 tw[L];
-    const TgtWork wk = work[blockIdx.y];
-    const int y0 = 2 * blockIdx.x;
+    const TgtWork wk = work[wi];
+    const int y0 = 2 * pair;
     const bool has1 = (y0 + 1) < rows;
     stage_twiddles<NT>(tw, twg, L);
     float acc0[PER], acc1[PER];
@@ -366,6 +399,24 @@ __global__ __launch_bounds__(NT) void k_rows_inv(const float2* __restrict__ Y, f
             if (has1) u0[L + m] = acc1[q];
         }
     }
+}
+
+template <int L, int NT>
+__global__ __launch_bounds__(NT) void k_rows_inv(const float2* __restrict__ Y, float* __restrict__ U,
+                                                 const TgtWork* __restrict__ work, const int* __restrict__ klist,
+                                                 const GrowthParams* __restrict__ gp, int rows, int Wp,
+                                                 long long N, long long spec, float scale,
+                                                 const float2* __restrict__ twg) {
+    rows_inv_body<L, NT>(Y, U, work, klist, gp, rows, Wp, N, spec, scale, twg, blockIdx.x, blockIdx.y);
+}
+
+// batched: item = (member, row pair, target work item)
+template <int L, int NT>
+__global__ __launch_bounds__(NT) void kb_rows_inv(const MemberDesc* __restrict__ mem, const int4* __restrict__ items) {
+    const int4 it = items[blockIdx.x];
+    const MemberDesc& m = mem[it.x];
+    rows_inv_body<L, NT>(m.Y, m.U, m.tgt_work, m.klist_tgt, m.gp, m.g.H * m.g.D, m.g.Wp, m.g.N, m.g.spec,
+                         1.0f / (float)m.g.N, m.tw_w, it.y, it.z);
 }
 
 // ---- host-side dispatch ------------------------------------------------------------------
@@ -463,6 +514,35 @@ bool launch_rows_inv(int L, hipStream_t st, const float2* Y, float* U, const Tgt
     HP_FFT_SIZES(X_)
 #undef X_
     return false;
+}
+
+// ---- batched launchers (2D generic path): one launch per pass for a size class ----------
+bool launch_batch_rows_fwd(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cur) {
+#define X_(n) if (L == n) { hipLaunchKernelGGL((kb_rows_fwd<n, Cfg<n>::ROW_NT>), dim3(nitems), dim3(Cfg<n>::ROW_NT), 0, st, mem, items, cur); return true; }
+    HP_FFT_SIZES(X_)
+#undef X_
+    return false;
+}
+
+bool launch_batch_spectral(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems) {
+#define X_(n) if (L == n) { hipLaunchKernelGGL((kb_spectral<n, Cfg<n>::CB, Cfg<n>::COL_NT>), dim3(nitems), dim3(Cfg<n>::COL_NT), 0, st, mem, items); return true; }
+    HP_FFT_SIZES(X_)
+#undef X_
+    return false;
+}
+
+bool launch_batch_rows_inv(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems) {
+#define X_(n) if (L == n) { hipLaunchKernelGGL((kb_rows_inv<n, Cfg<n>::ROW_NT>), dim3(nitems), dim3(Cfg<n>::ROW_NT), 0, st, mem, items); return true; }
+    HP_FFT_SIZES(X_)
+#undef X_
+    return false;
+}
+
+int spectral_block_lines(int L) {
+#define X_(n) if (L == n) return Cfg<n>::CB;
+    HP_FFT_SIZES(X_)
+#undef X_
+    return 0;
 }
 
 }  // namespace hp

@@ -427,11 +427,17 @@ hp_status create(const hp_config* cfg, int device, hp_ctx** out) {
         g_global_error = "device ordinal out of range";
         return HP_ERR_NO_DEVICE;
     }
-    const bool is3d = cfg->depth > 1;
-    const uint32_t W = cfg->width, H = cfg->height, D = cfg->depth;
-    if (!hp::fft_supported((int)W) || !hp::fft_supported((int)H) || (is3d && !hp::fft_supported((int)D))) {
-        g_global_error = "grid sides must be 2^a * 3^b between 8 and 2048";
-        return HP_ERR_UNSUPPORTED_SIZE;
+    // plan overrides: HP_SPECTRAL_GROUP (1 per source, 0 per kernel), HP_FFT_PATH=generic
+    // (row-major LDS FFTs even where the wave-level column-major path applies)
+    const char* grp = getenv("HP_SPECTRAL_GROUP");
+    const char* path = getenv("HP_FFT_PATH");
+    const bool allow_wave = !(path && strcmp(path, "generic") == 0);
+    hp::MemberPlan plan;
+    std::string perr;
+    st = hp::plan_member(cfg, allow_wave, grp ? atoi(grp) : -1, &plan, &perr);
+    if (st != HP_OK) {
+        g_global_error = perr;
+        return st;
     }
     hp_ctx* c = new hp_ctx();
     c->device = device;
@@ -440,86 +446,21 @@ hp_status create(const hp_config* cfg, int device, hp_ctx** out) {
     c->rings.assign(cfg->rings, cfg->rings + cfg->num_rings);
     c->cfg.kernels = c->kernels.data();
     c->cfg.rings = c->rings.data();
-    c->C = (int)cfg->channels;
-    c->K = (int)cfg->num_kernels;
-    c->is3d = is3d;
-    hp::Geo& g = c->g;
-    g.W = (int)W;
-    g.H = (int)H;
-    g.D = (int)D;
-    g.Wh = (int)(W / 2 + 1);
-    g.Wp = (g.Wh + 15) / 16 * 16;
-    g.Hq = (int)(H / 2 + 1);
-    g.Dq = is3d ? (int)(D / 2 + 1) : 1;
-    g.N = (long long)W * H * D;
-    g.spec = (long long)D * H * g.Wp;
-    c->wave = !is3d && hp::wave_fft_supported(g.W, g.H);
-    // kernel spectra (host FP64) and work lists
-    std::vector<float> kq((size_t)std::max(1, c->K) * g.Dq * g.Hq * g.Wh, 0.0f);
-    std::vector<int> slot_of(c->C, -1);
-    std::vector<hp::GrowthParams> gp(std::max(1, c->K));
-    for (int k = 0; k < c->K; k++) {
-        const hp_kernel& kc = c->kernels[k];
-        uint32_t size = 0;
-        std::vector<float> data = hp::build_ring_kernel(kc, c->rings.data(), cfg->kernel_radius, is3d ? 3 : 2, &size);
-        if (size > W || size > H || (is3d && size > D)) {
-            release(c);
-            g_global_error = "kernel diameter 2R+1 exceeds a grid side";
-            return HP_ERR_UNSUPPORTED_SIZE;
-        }
-        std::vector<float> f = hp::folded_spectrum(data, size, is3d ? 3 : 2, W, H, D);
-        if (c->wave) {  // column-major [kx][kyf] for the y pass
-            std::vector<float> t(f.size());
-            for (int fy = 0; fy < g.Hq; fy++)
-                for (int fx = 0; fx < g.Wh; fx++) t[(size_t)fx * g.Hq + fy] = f[(size_t)fy * g.Wh + fx];
-            f.swap(t);
-        }
-        std::copy(f.begin(), f.end(), kq.begin() + (size_t)k * f.size());
-        if (slot_of[kc.source_channel] < 0) {
-            slot_of[kc.source_channel] = (int)c->src_chan.size();
-            c->src_chan.push_back((int)kc.source_channel);
-        }
-        const float sigma_sq_2 = 2.0f * kc.sigma * kc.sigma;  // growth.rs:51-52
-        gp[k] = {kc.weight, kc.mu, 1.0f / sigma_sq_2, 0.0f};
-    }
-    // spectral work: either one item per source (its forward transform is shared by all its
-    // kernels) or one item per kernel (more workgroups, the forward transform is redone)
-    std::vector<hp::SpecWork> sw;
-    std::vector<int> kls;
-    const char* grp = getenv("HP_SPECTRAL_GROUP");
-    const int blocks_per_source = c->wave ? (g.Wh + 3) / 4 : 1;
-    const bool group = grp ? atoi(grp) != 0 : (!c->wave || blocks_per_source * (long long)c->src_chan.size() >= 256);
-    if (group && c->wave) {
-        for (size_t sl = 0; sl < c->src_chan.size(); sl++) {
-            int first = (int)kls.size(), cnt = 0;
-            for (int k = 0; k < c->K; k++)
-                if (slot_of[c->kernels[k].source_channel] == (int)sl) {
-                    kls.push_back(k);
-                    cnt++;
-                }
-            sw.push_back({(int)sl, first, cnt, 0});
-        }
-    } else {
-        for (int k = 0; k < c->K; k++) {
-            sw.push_back({slot_of[c->kernels[k].source_channel], (int)kls.size(), 1, 0});
-            kls.push_back(k);
-        }
-    }
-    // inverse-row work: one item per target with its kernels in index order
-    std::vector<hp::TgtWork> tw;
-    std::vector<int> klt;
-    for (int t = 0; t < c->C; t++) {
-        int first = (int)klt.size(), cnt = 0;
-        for (int k = 0; k < c->K; k++)
-            if ((int)c->kernels[k].target_channel == t) {
-                klt.push_back(k);
-                cnt++;
-            }
-        if (cnt) {
-            tw.push_back({t, first, cnt, 0});
-            c->tgt_chan.push_back(t);
-        }
-    }
+    c->C = plan.C;
+    c->K = plan.K;
+    c->is3d = plan.is3d;
+    c->wave = plan.wave;
+    c->g = plan.g;
+    const hp::Geo& g = c->g;
+    const bool is3d = c->is3d;
+    c->src_chan = plan.src_chan;
+    c->tgt_chan = plan.tgt_chan;
+    const std::vector<float>& kq = plan.kq;
+    const std::vector<hp::SpecWork>& sw = plan.spec_work;
+    const std::vector<int>& kls = plan.klist_spec;
+    const std::vector<hp::TgtWork>& tw = plan.tgt_work;
+    const std::vector<int>& klt = plan.klist_tgt;
+    const std::vector<hp::GrowthParams>& gp = plan.gp;
     c->n_spec_work = (int)sw.size();
     c->spec_work = sw;
     c->n_tgt_work = (int)tw.size();

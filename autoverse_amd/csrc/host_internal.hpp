@@ -3,11 +3,34 @@
 
 #include <stdint.h>
 
+#include <string>
 #include <vector>
 
 #include "../../include/flowlenia_hip.h"
+#include "hp_common.hpp"
 
 namespace hp {
+
+// Everything a propagator needs on the device, computed on the host from a config:
+// geometry, folded kernel spectra, source slots and the work lists of the passes.
+struct MemberPlan {
+    Geo g{};
+    int C = 0, K = 0;
+    bool is3d = false;
+    bool wave = false;  // 2D sides 256..2048: column-major spectra, wave-level FFTs
+    std::vector<float> kq;
+    std::vector<int> src_chan;  // source slot -> channel
+    std::vector<SpecWork> spec_work;
+    std::vector<int> klist_spec;
+    std::vector<TgtWork> tgt_work;
+    std::vector<int> klist_tgt;
+    std::vector<int> tgt_chan;
+    std::vector<GrowthParams> gp;
+};
+
+// group: 1 = one spectral item per source, 0 = one per kernel, -1 = choose.
+// allow_wave: use the column-major wave path when the sizes allow it.
+hp_status plan_member(const hp_config* cfg, bool allow_wave, int group, MemberPlan* out, std::string* err);
 
 hp_status validate_config(const hp_config* cfg, uint32_t* bad_kernel, uint32_t* bad_channel);
 uint32_t actual_radius(float radius, uint32_t kernel_radius);

@@ -7,6 +7,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "../../include/flowlenia_hip.h"
@@ -144,6 +145,91 @@ std::vector<float> folded_spectrum(const std::vector<float>& k, uint32_t size, i
         }
     for (size_t i = 0; i < res.size(); i++) res[i] = (float)p3[i];
     return res;
+}
+
+hp_status plan_member(const hp_config* cfg, bool allow_wave, int group, MemberPlan* out, std::string* err) {
+    MemberPlan& p = *out;
+    const bool is3d = cfg->depth > 1;
+    const uint32_t W = cfg->width, H = cfg->height, D = cfg->depth;
+    if (!fft_supported((int)W) || !fft_supported((int)H) || (is3d && !fft_supported((int)D))) {
+        *err = "grid sides must be 2^a * 3^b between 8 and 2048";
+        return HP_ERR_UNSUPPORTED_SIZE;
+    }
+    p.C = (int)cfg->channels;
+    p.K = (int)cfg->num_kernels;
+    p.is3d = is3d;
+    Geo& g = p.g;
+    g.W = (int)W;
+    g.H = (int)H;
+    g.D = (int)D;
+    g.Wh = (int)(W / 2 + 1);
+    g.Wp = (g.Wh + 15) / 16 * 16;
+    g.Hq = (int)(H / 2 + 1);
+    g.Dq = is3d ? (int)(D / 2 + 1) : 1;
+    g.N = (long long)W * H * D;
+    g.spec = (long long)D * H * g.Wp;
+    p.wave = allow_wave && !is3d && wave_fft_supported(g.W, g.H);
+    // kernel spectra (host FP64)
+    p.kq.assign((size_t)std::max(1, p.K) * g.Dq * g.Hq * g.Wh, 0.0f);
+    std::vector<int> slot_of(p.C, -1);
+    p.gp.assign(std::max(1, p.K), GrowthParams{0, 0, 0, 0});
+    for (int k = 0; k < p.K; k++) {
+        const hp_kernel& kc = cfg->kernels[k];
+        uint32_t size = 0;
+        std::vector<float> data = build_ring_kernel(kc, cfg->rings, cfg->kernel_radius, is3d ? 3 : 2, &size);
+        if (size > W || size > H || (is3d && size > D)) {
+            *err = "kernel diameter 2R+1 exceeds a grid side";
+            return HP_ERR_UNSUPPORTED_SIZE;
+        }
+        std::vector<float> f = folded_spectrum(data, size, is3d ? 3 : 2, W, H, D);
+        if (p.wave) {  // column-major [kx][kyf] for the y pass
+            std::vector<float> t(f.size());
+            for (int fy = 0; fy < g.Hq; fy++)
+                for (int fx = 0; fx < g.Wh; fx++) t[(size_t)fx * g.Hq + fy] = f[(size_t)fy * g.Wh + fx];
+            f.swap(t);
+        }
+        std::copy(f.begin(), f.end(), p.kq.begin() + (size_t)k * f.size());
+        if (slot_of[kc.source_channel] < 0) {
+            slot_of[kc.source_channel] = (int)p.src_chan.size();
+            p.src_chan.push_back((int)kc.source_channel);
+        }
+        const float sigma_sq_2 = 2.0f * kc.sigma * kc.sigma;  // growth.rs:51-52
+        p.gp[k] = {kc.weight, kc.mu, 1.0f / sigma_sq_2, 0.0f};
+    }
+    // spectral work: one item per source (its forward transform is shared by all its kernels)
+    // or one item per kernel (more workgroups; the forward transform is redone)
+    const int blocks_per_source = p.wave ? (g.Wh + 3) / 4 : 1;
+    const bool grouped = group >= 0 ? group != 0 : (!p.wave || blocks_per_source * (long long)p.src_chan.size() >= 256);
+    if (grouped) {
+        for (size_t sl = 0; sl < p.src_chan.size(); sl++) {
+            int first = (int)p.klist_spec.size(), cnt = 0;
+            for (int k = 0; k < p.K; k++)
+                if (slot_of[cfg->kernels[k].source_channel] == (int)sl) {
+                    p.klist_spec.push_back(k);
+                    cnt++;
+                }
+            p.spec_work.push_back({(int)sl, first, cnt, 0});
+        }
+    } else {
+        for (int k = 0; k < p.K; k++) {
+            p.spec_work.push_back({slot_of[cfg->kernels[k].source_channel], (int)p.klist_spec.size(), 1, 0});
+            p.klist_spec.push_back(k);
+        }
+    }
+    // inverse-row work: one item per target with its kernels in index order (propagator.rs:270-279)
+    for (int t = 0; t < p.C; t++) {
+        int first = (int)p.klist_tgt.size(), cnt = 0;
+        for (int k = 0; k < p.K; k++)
+            if ((int)cfg->kernels[k].target_channel == t) {
+                p.klist_tgt.push_back(k);
+                cnt++;
+            }
+        if (cnt) {
+            p.tgt_work.push_back({t, first, cnt, 0});
+            p.tgt_chan.push_back(t);
+        }
+    }
+    return HP_OK;
 }
 
 // ---- seeds (seed.rs:160-524) ------------------------------------------------------------

@@ -43,6 +43,26 @@ struct Geo {
     long long spec;   // complex elements of one spectrum: D*H*Wp
 };
 
+// One member of a batched population: its buffers and geometry, in device memory.
+struct MemberDesc {
+    float* A[2];                 // state ping-pong
+    float* U;                    // affinity
+    float2* X;                   // source half spectra (row-major, pitch Wp)
+    float2* Y;                   // per-kernel filtered spectra
+    const float* Kq;             // folded kernel spectra
+    const int* src_chan;         // source slot -> channel
+    const SpecWork* spec_work;   // spectral work items
+    const int* klist_spec;
+    const TgtWork* tgt_work;     // inverse-row work items
+    const int* klist_tgt;
+    const GrowthParams* gp;
+    const float2* tw_w;          // twiddles of length W
+    const float2* tw_h;          // twiddles of length H
+    Geo g;
+    int C, K, nsrc, ntgt, nspec;
+    float dt, beta_a, nexp, s;
+};
+
 // ---- launchers implemented in fft_kernels.hip -------------------------------------
 // Each returns false if the length is not one of the compiled FFT sizes.
 bool fft_supported(int L);
@@ -53,6 +73,12 @@ bool launch_spectral(int L, int dim, hipStream_t st, const float2* X, float2* Y,
                      const SpecWork* work, int nwork, const int* klist, const Geo& g, const float2* tw);
 bool launch_rows_inv(int L, hipStream_t st, const float2* Y, float* U, const TgtWork* work, int nwork,
                      const int* klist, const GrowthParams* gp, const Geo& g, const float2* tw);
+
+// Batched 2D passes over a size class: items[i] = (member, block, slot/work, -).
+bool launch_batch_rows_fwd(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cur);
+bool launch_batch_spectral(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems);
+bool launch_batch_rows_inv(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems);
+int spectral_block_lines(int L);  // columns per spectral workgroup for length-L columns
 
 // ---- launchers implemented in wave_fft.hip (2D, sides 256..2048, column-major spectra) ----
 bool wave_fft_supported(int W, int H);
@@ -68,6 +94,14 @@ bool launch_rows_inv_t(hipStream_t st, const float2* YT, float* U, const TgtWork
 // radius; sets *flag when any active source's box leaves that radius (bit 0) or
 // is not finite (bit 1).  Returns false if the configuration is not covered.
 bool stencil2d_supported(const Geo& g, float s);
+// Batched fused stencil: items[i] = (member, tile); members of one launch share C and the
+// power-of-two-area property.
+bool launch_batch_stencil2d(hipStream_t st, int C, bool pow2, const MemberDesc* mem, const int4* items, int nitems,
+                            int cur, int* flag);
+int stencil2d_tile_w();
+// Per member (sum m, sum x m, sum y m) of the channel sum over cells with m > 0, FP64.
+void launch_batch_moments(hipStream_t st, const MemberDesc* mem, int nmem, int cur, double* out);
+int stencil2d_tile_h();
 bool launch_stencil2d(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g,
                       float dt, float beta_a, float nexp, float s, int* flag);
 bool launch_stencil3d(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g,

@@ -120,11 +120,10 @@ static_assert(TX * TY == NT * SR && NT % TX == 0, "one strip per thread");
 
 // CC: channel count known at compile time (1..4), or 0 for any count (masses re-read per channel).
 template <int CC, bool POW2>
-__global__ __launch_bounds__(st2::NT) void k_stencil2d(const float* __restrict__ A, const float* __restrict__ U,
-                                                       float* __restrict__ Aout, int Cdyn, int W, int H, long long N,
-                                                       int tiles_x, int ntiles, float dt,
-                                                       float beta_a, float nexp, float s, float ta, float inv_ta,
-                                                       int* __restrict__ flag) {
+__device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, const float* __restrict__ U,
+                                               float* __restrict__ Aout, int Cdyn, int W, int H, long long N,
+                                               int tiles_x, int tile, float dt, float beta_a, float nexp, float s,
+                                               float ta, float inv_ta, int* __restrict__ flag) {
     using namespace st2;
     const int C = CC > 0 ? CC : Cdyn;
     __shared__ float sSum[NLOAD];
@@ -134,12 +133,6 @@ __global__ __launch_bounds__(st2::NT) void k_stencil2d(const float* __restrict__
     __shared__ float sOX[4][NSRC];
     __shared__ float4 sOY[NSRC];
 
-    // XCD-aware tile order: workgroup b runs on XCD b % 8; each XCD gets a contiguous run of
-    // tiles so that neighbouring tiles (which share halo rows) meet in the same L2.
-    const int b = blockIdx.x;
-    const int per = (ntiles + 7) / 8;
-    const int tile = (b % 8) * per + b / 8;
-    if (tile >= ntiles) return;
     const int x0 = (tile % tiles_x) * TX, y0 = (tile / tiles_x) * TY;
     const int tid = threadIdx.x;
 
@@ -340,6 +333,54 @@ __global__ __launch_bounds__(st2::NT) void k_stencil2d(const float* __restrict__
     }
     if (local_flag) atomicOr(flag, local_flag);
 }
+
+template <int CC, bool POW2>
+__global__ __launch_bounds__(st2::NT) void k_stencil2d(const float* __restrict__ A, const float* __restrict__ U,
+                                                       float* __restrict__ Aout, int Cdyn, int W, int H, long long N,
+                                                       int tiles_x, int ntiles, float dt, float beta_a, float nexp,
+                                                       float s, float ta, float inv_ta, int* __restrict__ flag) {
+    // XCD-aware tile order: workgroup b runs on XCD b % 8; each XCD gets a contiguous run of
+    // tiles so that neighbouring tiles (which share halo rows) meet in the same L2.
+    const int b = blockIdx.x;
+    const int per = (ntiles + 7) / 8;
+    const int tile = (b % 8) * per + b / 8;
+    if (tile >= ntiles) return;
+    stencil2d_body<CC, POW2>(A, U, Aout, Cdyn, W, H, N, tiles_x, tile, dt, beta_a, nexp, s, ta, inv_ta, flag);
+}
+
+// batched: item = (member, tile); every member of one launch has the same channel count class
+template <int CC, bool POW2>
+__global__ __launch_bounds__(st2::NT) void kb_stencil2d(const MemberDesc* __restrict__ mem,
+                                                        const int4* __restrict__ items, int cur,
+                                                        int* __restrict__ flag) {
+    const int4 it = items[blockIdx.x];
+    const MemberDesc& m = mem[it.x];
+    const float ta = (2.0f * m.s) * (2.0f * m.s);
+    const int tiles_x = (m.g.W + st2::TX - 1) / st2::TX;
+    stencil2d_body<CC, POW2>(m.A[cur], m.U, m.A[1 - cur], m.C, m.g.W, m.g.H, m.g.N, tiles_x, it.y, m.dt, m.beta_a,
+                             m.nexp, m.s, ta, 1.0f / ta, flag);
+}
+
+bool launch_batch_stencil2d(hipStream_t st, int C, bool pow2, const MemberDesc* mem, const int4* items, int nitems,
+                            int cur, int* flag) {
+#define B_(cc)                                                                                                  \
+    if (pow2)                                                                                                   \
+        hipLaunchKernelGGL((kb_stencil2d<cc, true>), dim3(nitems), dim3(st2::NT), 0, st, mem, items, cur, flag); \
+    else                                                                                                        \
+        hipLaunchKernelGGL((kb_stencil2d<cc, false>), dim3(nitems), dim3(st2::NT), 0, st, mem, items, cur, flag);
+    switch (C) {
+    case 1: B_(1) break;
+    case 2: B_(2) break;
+    case 3: B_(3) break;
+    case 4: B_(4) break;
+    default: B_(0)
+    }
+#undef B_
+    return true;
+}
+
+int stencil2d_tile_w() { return st2::TX; }
+int stencil2d_tile_h() { return st2::TY; }
 
 template <int CC>
 static void launch_st2(hipStream_t st, bool pow2, dim3 grid, const float* A, const float* U, float* Aout, int C,
@@ -750,5 +791,43 @@ __global__ void k_clear_ints(int* p, int n) {
 }
 
 void launch_clear_ints(hipStream_t st, int* p, int n) { hipLaunchKernelGGL(k_clear_ints, dim3(1), dim3(64), 0, st, p, n); }
+
+// ---------------------------------------------------------------------------------------------
+// Population fitness: centre of mass of the channel sum of every member (fitness.rs:150-152,
+// 196-218: cells with m > 0 contribute m, x*m, y*m), reduced in FP64 in a fixed order.  One
+// workgroup per member; out[3*member + {0,1,2}] = (sum m, sum x*m, sum y*m).
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_batch_moments(const MemberDesc* __restrict__ mem, int cur,
+                                                       double* __restrict__ out) {
+    __shared__ double red[3][256];
+    const MemberDesc& m = mem[blockIdx.x];
+    const float* A = m.A[cur];
+    const long long n = (long long)m.g.W * m.g.H;  // 2D members
+    double s0 = 0.0, sx = 0.0, sy = 0.0;
+    for (long long i = threadIdx.x; i < n; i += 256) {
+        float v = 0.0f;
+        for (int c = 0; c < m.C; ++c) v += A[(long long)c * m.g.N + i];  // channel_sum: 0 + A_0 + ...
+        if (v > 0.0f) {
+            s0 += (double)v;
+            sx += (double)(i % m.g.W) * (double)v;
+            sy += (double)(i / m.g.W) * (double)v;
+        }
+    }
+    red[0][threadIdx.x] = s0;
+    red[1][threadIdx.x] = sx;
+    red[2][threadIdx.x] = sy;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) out[3 * blockIdx.x + k] = red[k][0];
+}
+
+void launch_batch_moments(hipStream_t st, const MemberDesc* mem, int nmem, int cur, double* out) {
+    hipLaunchKernelGGL(k_batch_moments, dim3(nmem), dim3(256), 0, st, mem, cur, out);
+}
 
 }  // namespace hp

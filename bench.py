@@ -2,6 +2,7 @@
 """Benchmark of the HIP Flow Lenia step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py --population [--pop-steps K]
 
 Workload: the 2D multi-species configuration, 1024 x 1024, 3 channels, 9 kernels
 (every source->target pair, 1-3 rings each), kernel_radius 15, dt 0.2, synthetic
@@ -36,6 +37,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline sample length")
+    ap.add_argument("--population", action="store_true",
+                    help="population mode: 256 uneven members stepped as one batch vs one lone "
+                         "propagator per member (prints its own JSON line)")
+    ap.add_argument("--pop-steps", type=int, default=200)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of 256 MiB ten times (calibrates rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE against a kernel of known traffic)")
@@ -46,6 +51,8 @@ def main() -> int:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the step is single-GPU by design",
               file=sys.stderr)
         return 2
+    if args.population:
+        return population_bench(args)
 
     import numpy as np
 
@@ -123,6 +130,80 @@ def main() -> int:
                    "mass_f64_after": mass},
         "roofline": roofline,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def population_bench(args) -> int:
+    """256 members, sides U{64,128,192,256}, 1-3 channels (autoverse_amd/workloads.population):
+    the batched step (one launch per pass per size class, hipGraph replay) against the plain
+    per-instance path (one HipPropagator per member, advanced one after the other)."""
+    import numpy as np
+
+    import autoverse_amd as av
+    from autoverse_amd import workloads
+    from autoverse_amd.population import HipPopulation
+
+    pop_def = workloads.population(256)
+    cfgs = [c for c, _ in pop_def]
+    seeds = [s for _, s in pop_def]
+    cells = sum(c.width * c.height for c in cfgs)
+    K = args.pop_steps
+    print(f"[population] {len(cfgs)} members, {cells} cells, {K} steps", file=sys.stderr, flush=True)
+
+    pop = HipPopulation(cfgs)
+    pop.set_seeds(seeds)
+    pop.advance(args.warmup)
+    m0 = pop.mass_f64()
+    c0, _ = pop.centroids()
+    t0 = time.perf_counter()
+    pop.advance(K)
+    secs_b = time.perf_counter() - t0
+    c1, _ = pop.centroids()
+    m1 = pop.mass_f64()
+    radius, general = pop.stats()
+    pop.close()
+    disp = np.hypot(c1[:, 0] - c0[:, 0], c1[:, 1] - c0[:, 1])
+    drift = float(np.max(np.abs(m1 - m0) / np.maximum(m0, 1e-30)))
+
+    props = []
+    for c, s in pop_def:
+        p = av.HipPropagator(c)
+        p.set_state(av.SimulationState.from_seed(s, c))
+        p.advance(args.warmup)
+        props.append(p)
+    print("[population] per-instance pass", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for p in props:
+        p.advance(K)
+    secs_i = time.perf_counter() - t0
+    for p in props:
+        p.close()
+
+    value = cells * K / secs_b
+    per_inst = cells * K / secs_i
+    out = {
+        "metric": "cell-steps/s, 256-member population (batched forward step)",
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": secs_b * 1e3 / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "population of 256: side U{64,128,192,256}, 1-3 channels, every "
+                               "source->target kernel, R=15, dt=0.2", "members": len(cfgs), "cells": cells,
+                   "max_gather_radius": radius, "general_path_steps": general},
+        "per_instance": {"value": per_inst, "unit": "cell-steps/s", "ms_per_step": secs_i * 1e3 / K,
+                         "path": "one HipPropagator per member, advanced in turn"},
+        "speedup_vs_per_instance": value / per_inst,
+        "fitness": {"displacement_mean": float(disp.mean()), "displacement_max": float(disp.max())},
+        "mass_drift_max_rel": drift,
     }
     print(json.dumps(out), flush=True)
     return 0

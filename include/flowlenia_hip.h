@@ -247,6 +247,30 @@ hp_status hp_stencil_with_affinity(hp_ctx* ctx, const float* const* state, const
  * CpuPropagator::step (propagator.rs:244-280): FFT convolution + growth. */
 hp_status hp_affinity(hp_ctx* ctx, const float* const* state, float* const* affinity);
 
+/* ---- Populations ------------------------------------------------------------- */
+/* A population of independent simulations, possibly of different sizes and channel
+ * counts, stepped together: every pass of a step is one launch per size class (2D),
+ * all passes of a step replay from one hipGraph.  Replaces the per-candidate
+ * `CpuPropagator::new` + `step` loop of FitnessEvaluator::evaluate
+ * (src/compute/evolution/fitness.rs:26-54) driven by EvolutionEngine::evaluate_population
+ * (src/compute/evolution/search.rs:153-172).  2D members only. */
+typedef struct hp_batch hp_batch;
+hp_status hp_batch_create(const hp_config* members, uint32_t count, int device, hp_batch** out);
+void hp_batch_destroy(hp_batch* b);
+const char* hp_batch_last_error(const hp_batch* b);
+/* Upload / download one member's state (its `channels` host buffers, W*H floats each). */
+hp_status hp_batch_set_state(hp_batch* b, uint32_t member, const float* const* channels);
+hp_status hp_batch_get_state(hp_batch* b, uint32_t member, float* const* channels, float* time, uint64_t* step);
+/* Advance every member by `steps` steps, device-resident; synchronises before returning. */
+hp_status hp_batch_advance(hp_batch* b, uint64_t steps);
+/* Centre of mass of every member's channel sum (compute_center_of_mass, fitness.rs:196-218,
+ * accumulated in FP64): out[2*i], out[2*i+1] = (cx, cy); the grid centre when the mass is
+ * <= 1e-6.  mass (may be NULL) receives the FP64 positive mass. */
+hp_status hp_batch_centroids(hp_batch* b, double* out_xy, double* mass);
+/* FP64 total mass of every member (fixed reduction order). */
+hp_status hp_batch_mass_f64(hp_batch* b, double* per_member);
+hp_status hp_batch_stats(const hp_batch* b, uint32_t* max_gather_radius, uint64_t* general_path_steps);
+
 #ifdef __cplusplus
 }
 #endif
