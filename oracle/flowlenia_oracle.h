@@ -23,6 +23,7 @@
  *   seeds                  src/schema/seed.rs:160-524
  *   config validation      src/schema/config.rs:136-164
  *   centre of mass         src/compute/evolution/fitness.rs:196-218, 309-326
+ *   adjoint (oracle_grad.inc)  no reference counterpart: the transpose of the steps above
  *
  * Parity pins (which check covers which function) are listed in the tests; the
  * only stage with no bit-level pin is the FFT (rustfft's exact rounding cannot
@@ -79,7 +80,11 @@ int32_t orc_sat_i32_f64(double v);
                               T* const* next);                                                 \
     void orc_prop_step_##S(orc_prop_##S* p, T* const* channels, float* time, uint64_t* step);   \
     void orc_prop_run_##S(orc_prop_##S* p, T* const* channels, float* time, uint64_t* step,     \
-                          uint64_t steps);
+                          uint64_t steps);                                                     \
+    orc_prop_##S* orc_prop_create_theta_##S(const hp_config* cfg, int threads, const double* theta); \
+    uint32_t orc_theta_count_##S(const orc_prop_##S* p);                                       \
+    int orc_grad_##S(orc_prop_##S* p, T* const* A0, uint64_t steps, T* const* gAT, T* const* AT, \
+                     T* const* gA0, double* dtheta);
 
 ORC_DECLARE(float, f32)
 ORC_DECLARE(double, f64)

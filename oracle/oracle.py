@@ -68,6 +68,9 @@ def lib() -> ctypes.CDLL:
                 f"orc_prop_stencil_{S}": (None, [c_void_p, PP, PP, PP]),
                 f"orc_prop_step_{S}": (None, [c_void_p, PP, POINTER(c_float), POINTER(c_uint64)]),
                 f"orc_prop_run_{S}": (None, [c_void_p, PP, POINTER(c_float), POINTER(c_uint64), c_uint64]),
+                f"orc_prop_create_theta_{S}": (c_void_p, [POINTER(_abi.HpConfig), c_int, FPd]),
+                f"orc_theta_count_{S}": (c_uint32, [c_void_p]),
+                f"orc_grad_{S}": (c_int, [c_void_p, PP, c_uint64, PP, PP, PP, FPd]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, name)
@@ -251,15 +254,33 @@ def locomotion(x0, y0, x1, y1, w, h) -> float:
     return lib().orc_locomotion(x0, y0, x1, y1, w, h)
 
 
-class Propagator:
-    """CpuPropagator / CpuPropagator3D restated (propagator.rs:124-427), f32 or f64."""
+def theta_of(cfg) -> np.ndarray:
+    """Differentiable parameters of a config, float64: per kernel (weight, mu, sigma), then
+    per ring in config order (amplitude, distance, width)."""
+    out = []
+    for k in cfg.kernels:
+        out += [k.weight, k.mu, k.sigma]
+    for k in cfg.kernels:
+        for r in k.rings:
+            out += [r.amplitude, r.distance, r.width]
+    return np.array(out, dtype=np.float64)
 
-    def __init__(self, cfg, dtype=np.float32, threads: int = 1):
+
+class Propagator:
+    """CpuPropagator / CpuPropagator3D restated (propagator.rs:124-427), f32 or f64.
+    `theta` (float64, layout of theta_of) overrides the config's kernel and ring values."""
+
+    def __init__(self, cfg, dtype=np.float32, threads: int = 1, theta=None):
         self.cfg = cfg
         self.dtype = np.dtype(dtype)
         self.S = _sfx(dtype)
         self._c = cfg.to_c()
-        self.ptr = getattr(lib(), f"orc_prop_create_{self.S}")(ctypes.byref(self._c.cfg), threads)
+        if theta is None:
+            self.ptr = getattr(lib(), f"orc_prop_create_{self.S}")(ctypes.byref(self._c.cfg), threads)
+        else:
+            self._theta = np.ascontiguousarray(theta, dtype=np.float64)
+            self.ptr = getattr(lib(), f"orc_prop_create_theta_{self.S}")(
+                ctypes.byref(self._c.cfg), threads, self._theta.ctypes.data_as(POINTER(c_double)))
         if not self.ptr:
             raise ValueError("invalid configuration")
         self.n = cfg.width * cfg.height * cfg.depth
@@ -305,3 +326,27 @@ class Propagator:
         t, s = c_float(time), c_uint64(step)
         getattr(lib(), f"orc_prop_run_{self.S}")(self.ptr, _pp(ch), ctypes.byref(t), ctypes.byref(s), steps)
         return ch, t.value, s.value
+
+    def grad(self, channels, steps: int, seed):
+        """Reverse-mode derivative of L = sum_c <seed_c, A_T,c> over `steps` steps (2D).
+        Returns (A_T, dL/dA_0, dL/dtheta) with theta laid out as theta_of()."""
+        ch = self._chans(channels)
+        sd = self._chans(seed)
+        at = [np.zeros(self.n, dtype=self.dtype) for _ in ch]
+        g0 = [np.zeros(self.n, dtype=self.dtype) for _ in ch]
+        nth = getattr(lib(), f"orc_theta_count_{self.S}")(self.ptr)
+        dth = np.zeros(max(1, nth), dtype=np.float64)
+        st = getattr(lib(), f"orc_grad_{self.S}")(self.ptr, _pp(ch), steps, _pp(sd), _pp(at), _pp(g0),
+                                                 dth.ctypes.data_as(POINTER(c_double)))
+        if st != 0:
+            raise ValueError(f"orc_grad failed ({st}): 3D, or a kernel wider than the grid")
+        return at, g0, dth[:nth]
+
+
+def loss(cfg, channels, steps: int, seed, theta=None, dtype=np.float64) -> float:
+    """L = sum_c <seed_c, A_T,c> after `steps` oracle steps with parameters theta."""
+    p = Propagator(cfg, dtype, theta=theta)
+    at, _, _ = p.run(channels, steps)
+    p.close()
+    return float(sum(np.dot(np.asarray(a, np.float64), np.asarray(s, np.float64).reshape(-1))
+                     for a, s in zip(at, seed)))
