@@ -33,6 +33,8 @@ SOURCES = [
     ("fft_kernels.hip", DEVICE_FLAGS + ["-ffp-contract=fast"]),
     ("wave_fft.hip", DEVICE_FLAGS + ["-ffp-contract=fast"]),
     ("stencil_kernels.hip", DEVICE_FLAGS + ["-ffp-contract=off"]),
+    ("adjoint_stencil.hip", DEVICE_FLAGS + ["-ffp-contract=off"]),
+    ("adjoint_fft.hip", DEVICE_FLAGS + ["-ffp-contract=fast"]),
     ("host.cpp", HOST_FLAGS),
     ("host_seed.cpp", HOST_FLAGS),
     ("host_batch.cpp", HOST_FLAGS),

@@ -1,0 +1,330 @@
+// adjoint_fft.hip -- reverse-mode stage 1 (convolution + growth) of the 2D step on gfx950.
+//
+// Forward (propagator.rs:244-280): conv_k = K_k (*) A_src(k) (circular, via FFT, scaled 1/N),
+// U_t = sum_{k -> t} w_k G(conv_k; mu_k, sigma_k) with G = 2 exp(-(u - mu)^2 / (2 sigma^2)) - 1.
+// Transpose, given dL/dU (from the divergence pass):
+//   g_k = dL/dU_tgt(k) * w_k * G'(conv_k),  G'(u) = -4 (u - mu) inv e,  inv = 1/(2 sigma^2)
+//   dL/dw_k  = sum dL/dU * (2e - 1)
+//   dL/dmu_k = sum dL/dU * w_k * 4 (u - mu) inv e
+//   dL/dsigma_k = sum dL/dU * w_k * 2 e (u - mu)^2 / sigma^3
+//   dL/dA_src += K_k^T g_k = K_k (*) g_k (the ring kernel is point-symmetric, so its transpose
+//                is itself and its spectrum is real: the same folded spectrum multiplies)
+//   dL/dK_k[j] = sum_i g_k[i] A_src[i - j], accumulated over steps in the spectral domain as
+//                P_k += FFT(g_k) * conj(FFT(A_src)) and brought back once per backward for
+//                the (2R+1)^2 taps only (kb_window_dft).
+// Three passes as in the forward: R2C rows of g_k (growth derivative fused in, parameter sums
+// reduced per row pair in FP64), a column pass per source that sums K_k * FFT(g_k) over the
+// source's kernels, accumulates P_k and inverts, and C2R rows that add into dL/dA.
+#include "fft_lds.hpp"
+
+namespace hp {
+
+// item = (member, row pair, kernel).  Reads conv_k and dL/dU; writes FFT_x(g_k) into Y.
+template <int L, int NT>
+__global__ __launch_bounds__(NT) void kb_adj_rows_fwd(const MemberDesc* __restrict__ mem,
+                                                      const int4* __restrict__ items) {
+    __shared__ float2 buf[L];
+    __shared__ float2 tw[L];
+    __shared__ double red[3][NT];
+    const int4 it = items[blockIdx.x];
+    const MemberDesc& m = mem[it.x];
+    const int pair = it.y, k = it.z;
+    const int rows = m.g.H, pairs = (rows + 1) / 2;
+    const long long N = m.g.N;
+    const int y0 = 2 * pair;
+    const bool has1 = (y0 + 1) < rows;
+    const GrowthParams p = m.gp[k];
+    const int2 km = m.kmap[k];
+    const float* cv = m.conv + (long long)k * N + (long long)y0 * L;
+    const float* lu = m.lamU + (long long)km.y * N + (long long)y0 * L;
+    const float inv_s3 = 1.0f / (p.sigma * p.sigma * p.sigma);
+    stage_twiddles<NT>(tw, m.tw_w, L);
+    double sw = 0.0, smu = 0.0, ssig = 0.0;
+    for (int i = threadIdx.x; i < L; i += NT) {
+        float g[2] = {0.0f, 0.0f};
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            if (r == 1 && !has1) break;
+            const float u = cv[r * L + i], l = lu[r * L + i];
+            const float d = u - p.mu;
+            const float e = expf(-d * d * p.inv);
+            g[r] = l * p.weight * (-4.0f * d * p.inv * e);
+            sw += (double)(l * (2.0f * e - 1.0f));
+            smu += (double)(l * p.weight * 4.0f * d * p.inv * e);
+            ssig += (double)(l * p.weight * 2.0f * e * d * d * inv_s3);
+        }
+        buf[i] = make_float2(g[0], g[1]);
+    }
+    red[0][threadIdx.x] = sw;
+    red[1][threadIdx.x] = smu;
+    red[2][threadIdx.x] = ssig;
+    __syncthreads();
+    lds_fft<L, -1, 1, NT, L>(buf, tw);
+    float2* x0 = m.Y + (long long)k * m.g.spec + (long long)y0 * m.g.Wp;
+    float2* x1 = x0 + m.g.Wp;
+    for (int kx = threadIdx.x; kx <= L / 2; kx += NT) {
+        const float2 zk = buf[kx];
+        const float2 zm = buf[kx == 0 ? 0 : L - kx];
+        x0[kx] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+        if (has1) x1[kx] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+    }
+    if (threadIdx.x < 3) {  // fixed-order block sum, then into this row pair's own slot
+        double t = 0.0;
+        for (int j = 0; j < NT; ++j) t += red[threadIdx.x][j];
+        m.pacc[((long long)k * pairs + pair) * 3 + threadIdx.x] += t;
+    }
+}
+
+// Launch shape of the adjoint column pass: a quarter of the forward's columns per
+// workgroup, so that the source column, the accumulator and the offsets (8 values each per
+// thread) stay in registers.
+template <int L>
+struct AdjCfg {
+    static constexpr int CB = Cfg<L>::CB / 4 < 1 ? 1 : Cfg<L>::CB / 4;
+    static constexpr int NT = (CB * L) / 8 < 64 ? 64 : ((CB * L) / 8 > 256 ? 256 : (CB * L) / 8);
+    static_assert((CB * L) % NT == 0, "adjoint column tile must split evenly");
+};
+
+// item = (member, column block, source work item).  X holds FFT_x(A_src) of the recomputed
+// step and is overwritten with FFT_x of sum_k K_k (*) g_k; P_k accumulates.
+template <int L, int CB, int NT>
+__global__ __launch_bounds__(NT) void kb_adj_spectral(const MemberDesc* __restrict__ mem,
+                                                      const int4* __restrict__ items) {
+    constexpr int LP = L + 1;
+    constexpr int PER = CB * L / NT;
+    static_assert((CB * L) % NT == 0, "tile must split evenly over the threads");
+    __shared__ float2 buf[CB * LP];
+    __shared__ float2 tw[L];
+    const int4 it = items[blockIdx.x];
+    const MemberDesc& m = mem[it.x];
+    const SpecWork wk = m.spec_work[it.z];
+    const int Wh = m.g.Wh, Wp = m.g.Wp;
+    const long long spec = m.g.spec, kq_stride = (long long)m.g.Hq * Wh;
+    const long long line0 = (long long)it.y * CB;
+    float2* xs = m.X + (long long)wk.src_slot * spec;
+    stage_twiddles<NT>(tw, m.tw_h, L);
+    float2 xr[PER], acc[PER];
+    int koff[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int i = threadIdx.x + q * NT;
+        const int c = i % CB, e = i / CB;
+        const long long kx = line0 + c;
+        const bool valid = kx < Wh;
+        buf[c * LP + e] = valid ? xs[(long long)e * Wp + kx] : make_float2(0.f, 0.f);
+        const int kyf = e <= L / 2 ? e : L - e;
+        koff[q] = valid ? kyf * Wh + (int)kx : -1;
+        acc[q] = make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    lds_fft<L, -1, CB, NT, LP>(buf, tw);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int i = threadIdx.x + q * NT;
+        xr[q] = buf[(i % CB) * LP + i / CB];
+    }
+    for (int j = 0; j < wk.k_count; ++j) {
+        const int k = m.klist_spec[wk.k_first + j];
+        const float* kq = m.Kq + (long long)k * kq_stride;
+        const float2* gs = m.Y + (long long)k * spec;
+        float2* pk = m.P + (long long)k * spec;
+        __syncthreads();  // previous reads of buf are done
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int i = threadIdx.x + q * NT;
+            const int c = i % CB, e = i / CB;
+            buf[c * LP + e] = koff[q] >= 0 ? gs[(long long)e * Wp + line0 + c] : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+        lds_fft<L, -1, CB, NT, LP>(buf, tw);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int i = threadIdx.x + q * NT;
+            const int c = i % CB, e = i / CB;
+            if (koff[q] < 0) continue;
+            const float2 gq = buf[c * LP + e];
+            const float kv = kq[koff[q]];
+            float2* pp = pk + (long long)e * Wp + line0 + c;
+            const float2 pv = *pp;
+            // g * conj(x)
+            *pp = make_float2(pv.x + (gq.x * xr[q].x + gq.y * xr[q].y), pv.y + (gq.y * xr[q].x - gq.x * xr[q].y));
+            acc[q].x += kv * gq.x;
+            acc[q].y += kv * gq.y;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int i = threadIdx.x + q * NT;
+        buf[(i % CB) * LP + i / CB] = acc[q];
+    }
+    __syncthreads();
+    lds_fft<L, +1, CB, NT, LP>(buf, tw);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int i = threadIdx.x + q * NT;
+        const int c = i % CB, e = i / CB;
+        if (koff[q] >= 0) xs[(long long)e * Wp + line0 + c] = buf[c * LP + e];
+    }
+}
+
+// item = (member, row pair, source slot).  C2R of the slot's summed spectrum, added into
+// lam[1 - li] of the slot's channel with the forward's 1/N.
+template <int L, int NT>
+__global__ __launch_bounds__(NT) void kb_adj_rows_inv(const MemberDesc* __restrict__ mem,
+                                                      const int4* __restrict__ items, int li) {
+    constexpr int PER = (L + NT - 1) / NT;
+    __shared__ float2 buf[L];
+    __shared__ float2 tw[L];
+    const int4 it = items[blockIdx.x];
+    const MemberDesc& m = mem[it.x];
+    const int rows = m.g.H;
+    const int y0 = 2 * it.y;
+    const bool has1 = (y0 + 1) < rows;
+    const long long N = m.g.N;
+    const float scale = 1.0f / (float)N;
+    stage_twiddles<NT>(tw, m.tw_w, L);
+    const float2* r0 = m.X + (long long)it.z * m.g.spec + (long long)y0 * m.g.Wp;
+    const float2* r1 = r0 + m.g.Wp;
+    for (int n = threadIdx.x; n < L; n += NT) {
+        float2 a, b;
+        if (n <= L / 2) {
+            a = r0[n];
+            b = has1 ? r1[n] : make_float2(0.f, 0.f);
+            buf[n] = make_float2(a.x - b.y, a.y + b.x);
+        } else {
+            a = r0[L - n];
+            b = has1 ? r1[L - n] : make_float2(0.f, 0.f);
+            buf[n] = make_float2(a.x + b.y, b.x - a.y);
+        }
+    }
+    __syncthreads();
+    lds_fft<L, +1, 1, NT, L>(buf, tw);
+    float* out = m.lam[1 - li] + (long long)m.src_chan[it.z] * N + (long long)y0 * L;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int n = threadIdx.x + q * NT;
+        if (L % NT == 0 || n < L) {
+            const float2 z = buf[n];
+            out[n] += z.x * scale;
+            if (has1) out[L + n] += z.y * scale;
+        }
+    }
+}
+
+// item = (member, kernel).  dkwin[off + (jy+R)(2R+1) + (jx+R)] = sum over the full spectrum
+// of P_k e^{+2 pi i (kx jx / W + ky jy / H)} (unscaled; times 1/N on the host), evaluated from
+// the half spectrum as sum_ky sum_{kx <= W/2} m_kx Re(...), m_kx = 1 at kx = 0, W/2 else 2;
+// separable: a chunk of 8 spectrum rows is reduced over kx into LDS first.  FP64 throughout.
+constexpr int WDFT_NT = 256, WDFT_ROWS = 8, WDFT_MAXS = 129, WDFT_MAXL = 1024;
+__global__ __launch_bounds__(WDFT_NT) void kb_window_dft(const MemberDesc* __restrict__ mem,
+                                                         const int4* __restrict__ items) {
+    __shared__ double2 twW[WDFT_MAXL];
+    __shared__ double2 twH[WDFT_MAXL];
+    __shared__ double2 Q[WDFT_ROWS][WDFT_MAXS];
+    const int4 it = items[blockIdx.x];
+    const MemberDesc& m = mem[it.x];
+    const int k = it.y;
+    const int W = m.g.W, H = m.g.H, Wh = m.g.Wh, Wp = m.g.Wp;
+    const int R = m.dkw[k].y, S = 2 * R + 1;
+    double* out = m.dkwin + m.dkw[k].x;
+    const float2* P = m.P + (long long)k * m.g.spec;
+    for (int i = threadIdx.x; i < W; i += WDFT_NT) {
+        double sn, cs;
+        sincospi(2.0 * (double)i / (double)W, &sn, &cs);
+        twW[i] = make_double2(cs, sn);
+    }
+    for (int i = threadIdx.x; i < H; i += WDFT_NT) {
+        double sn, cs;
+        sincospi(2.0 * (double)i / (double)H, &sn, &cs);
+        twH[i] = make_double2(cs, sn);
+    }
+    __syncthreads();
+    for (int ky0 = 0; ky0 < H; ky0 += WDFT_ROWS) {
+        for (int t = threadIdx.x; t < WDFT_ROWS * S; t += WDFT_NT) {
+            const int r = t / S, jxi = t - r * S;
+            const int ky = ky0 + r;
+            double re = 0.0, im = 0.0;
+            if (ky < H) {
+                const int jx = jxi - R;
+                const int step = ((jx % W) + W) % W;
+                int ph = 0;
+                const float2* row = P + (long long)ky * Wp;
+                for (int kx = 0; kx < Wh; ++kx) {
+                    const double2 w = twW[ph];
+                    const float2 p = row[kx];
+                    const double mk = (kx == 0 || 2 * kx == W) ? 1.0 : 2.0;
+                    re += mk * ((double)p.x * w.x - (double)p.y * w.y);
+                    im += mk * ((double)p.x * w.y + (double)p.y * w.x);
+                    ph += step;
+                    if (ph >= W) ph -= W;
+                }
+            }
+            Q[r][jxi] = make_double2(re, im);
+        }
+        __syncthreads();
+        for (int o = threadIdx.x; o < S * S; o += WDFT_NT) {
+            const int jyi = o / S, jxi = o - jyi * S;
+            const int jy = jyi - R;
+            const int step = ((jy % H) + H) % H;
+            int ph = (int)(((long long)ky0 * step) % H);
+            double a = 0.0;
+            for (int r = 0; r < WDFT_ROWS && ky0 + r < H; ++r) {
+                const double2 w = twH[ph];
+                a += Q[r][jxi].x * w.x - Q[r][jxi].y * w.y;
+                ph += step;
+                if (ph >= H) ph -= H;
+            }
+            out[o] += a;
+        }
+        __syncthreads();
+    }
+}
+
+// ---- launchers -------------------------------------------------------------------------------
+#define HP_ADJ_SIZES(X_) X_(8) X_(16) X_(32) X_(64) X_(128) X_(256) X_(512) X_(1024) \
+    X_(24) X_(48) X_(96) X_(192) X_(384) X_(768)
+
+bool launch_adj_rows_fwd(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems) {
+#define X_(n) if (L == n) { hipLaunchKernelGGL((kb_adj_rows_fwd<n, Cfg<n>::ROW_NT>), dim3(nitems), dim3(Cfg<n>::ROW_NT), 0, st, mem, items); return true; }
+    HP_ADJ_SIZES(X_)
+#undef X_
+    return false;
+}
+
+bool launch_adj_spectral(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems) {
+#define X_(n) if (L == n) { hipLaunchKernelGGL((kb_adj_spectral<n, AdjCfg<n>::CB, AdjCfg<n>::NT>), dim3(nitems), dim3(AdjCfg<n>::NT), 0, st, mem, items); return true; }
+    HP_ADJ_SIZES(X_)
+#undef X_
+    return false;
+}
+
+bool launch_adj_rows_inv(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int li) {
+#define X_(n) if (L == n) { hipLaunchKernelGGL((kb_adj_rows_inv<n, Cfg<n>::ROW_NT>), dim3(nitems), dim3(Cfg<n>::ROW_NT), 0, st, mem, items, li); return true; }
+    HP_ADJ_SIZES(X_)
+#undef X_
+    return false;
+}
+
+int adj_spectral_block_lines(int L) {
+#define X_(n) if (L == n) return AdjCfg<n>::CB;
+    HP_ADJ_SIZES(X_)
+#undef X_
+    return 0;
+}
+
+bool adj_fft_supported(int L) {
+#define X_(n) if (L == n) return true;
+    HP_ADJ_SIZES(X_)
+#undef X_
+    return false;
+}
+
+void launch_window_dft(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems) {
+    hipLaunchKernelGGL(kb_window_dft, dim3(nitems), dim3(WDFT_NT), 0, st, mem, items);
+}
+
+int window_dft_max_side() { return WDFT_MAXL; }
+int window_dft_max_taps() { return WDFT_MAXS; }
+
+}  // namespace hp

@@ -1,0 +1,250 @@
+// adjoint_stencil.hip -- reverse-mode stages 2-5 of the 2D Flow Lenia step on gfx950.
+//
+// The forward step (propagator.rs:303-371) maps the state A and affinity U to the next
+// state through the channel sum, Sobel gradients, alpha, the flow F and reintegration.
+// Its transpose, for one step with lam' = dL/dA_{t+1}:
+//
+//   flow pass  (one thread per source cell, per channel):
+//     recompute F_c exactly as the forward (same f32 operations, -ffp-contract=off), walk the
+//     source's box exactly as the scatter does and read lam'_c at every destination:
+//       dL/dm      = sum lam' * area / (2s)^2                     -> lam_t,c (first term)
+//       dL/d(dest) = sum lam' * m * d(area)/d(dest) / (2s)^2,  dL/dF = dt * dL/d(dest)
+//     then through F = (1 - alpha) grad U - alpha grad A_sum:
+//       dL/d(grad U_c) = (1 - alpha) dL/dF_c,  dL/d(grad A_sum) = -alpha sum_c dL/dF_c,
+//       dL/dalpha = -sum_c dL/dF_c . (grad U_c + grad A_sum),  dL/dA_sum += dL/dalpha * alpha'
+//   divergence pass (one thread per cell):
+//     the Sobel operator is antisymmetric along its derivative axis, so its transpose is its
+//     negative: dL/dA_sum = -Sobel . dL/d(grad A_sum) + (alpha part), added to every channel;
+//     dL/dU_c = -Sobel . dL/d(grad U_c).
+//
+// No atomics: the adjoint of the scatter is a gather over the source's own box, so every
+// output cell has exactly one writer.  Subgradient conventions at the kinks:
+//   d overlap / d dest = [hi < c+1] - [lo > c] on cells with positive area;
+//   alpha' = 0 where the clamp is active ((m/beta)^n outside (0, 1));
+//   a source below the forward's |m| < 1e-10 guard keeps its mass derivative (the step
+//   without the guard) and has no flow derivative (it is m * (...) = O(1e-10));
+//   the point-mass fallback ((2s)^2 < 1e-10) passes lam' of the rounded destination.
+#include "stencil_common.hpp"
+
+namespace hp {
+
+namespace adj {
+constexpr int TX = 64, TY = 4, NT = TX * TY;
+constexpr int HW = TX + 2, HH = TY + 2;
+}  // namespace adj
+
+// alpha'(m) on the open interval where clamp((m/beta)^n, 0, 1) is inactive.
+__device__ __forceinline__ float dalpha_of(float mass, float beta_a, float n) {
+    const float r = mass / beta_a;
+    float p, d;
+    if (n == 1.0f) {
+        p = r;
+        d = 1.0f;
+    } else if (n == 2.0f) {
+        p = r * r;
+        d = 2.0f * r;
+    } else if (n == 3.0f) {
+        p = r * r * r;
+        d = 3.0f * r * r;
+    } else if (n == 4.0f) {
+        p = (r * r) * (r * r);
+        d = 4.0f * r * r * r;
+    } else {
+        p = powf(r, n);
+        d = n * powf(r, n - 1.0f);
+    }
+    if (!(p > 0.0f && p < 1.0f)) return 0.0f;
+    return d / beta_a;
+}
+
+// items[b] = (member, tile); tiles of TX x TY cells, row-major over ceil(W/TX) columns.
+// li: lam[li] holds dL/dA_{t+1}; lam[1 - li] receives the first term of dL/dA_t.
+__global__ __launch_bounds__(adj::NT) void kb_adj_flow(const MemberDesc* __restrict__ mem,
+                                                      const int4* __restrict__ items, int cur, int li) {
+    using namespace adj;
+    __shared__ float sA[HH][HW];
+    __shared__ float sU[HH][HW];
+    const int4 it = items[blockIdx.x];
+    const MemberDesc& m = mem[it.x];
+    const int W = m.g.W, H = m.g.H, C = m.C;
+    const long long N = m.g.N;
+    const int tiles_x = (W + TX - 1) / TX;
+    const int tx0 = (it.y % tiles_x) * TX, ty0 = (it.y / tiles_x) * TY;
+    const float* A = m.A[cur];
+    const float* lamn = m.lam[li];
+    float* lamo = m.lam[1 - li];
+    const float dt = m.dt, s = m.s, beta_a = m.beta_a, nexp = m.nexp;
+    const float ta = (2.0f * s) * (2.0f * s);
+    const float inv_ta = 1.0f / ta;
+    for (int t = threadIdx.x; t < HH * HW; t += NT) {
+        const int r = t / HW, c = t - r * HW;
+        const long long idx = (long long)wrap_near(ty0 + r - 1, H) * W + wrap_near(tx0 + c - 1, W);
+        float v = 0.0f;
+        for (int ch = 0; ch < C; ++ch) v += A[(long long)ch * N + idx];  // 0 + A_0 + A_1 + ...
+        sA[r][c] = v;
+    }
+    __syncthreads();
+    const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
+    const int x = tx0 + lx, y = ty0 + ly;
+    const bool inside = x < W && y < H;
+    const long long i = (long long)y * W + x;
+    float gax = 0.0f, gay = 0.0f, asum = 0.0f, al = 0.0f;
+    if (inside) {
+        asum = sA[ly + 1][lx + 1];
+        sobel9(sA[ly][lx], sA[ly][lx + 1], sA[ly][lx + 2], sA[ly + 1][lx], sA[ly + 1][lx + 2], sA[ly + 2][lx],
+               sA[ly + 2][lx + 1], sA[ly + 2][lx + 2], gax, gay);
+        al = alpha_of(asum, beta_a, nexp);
+    }
+    const float oma = 1.0f - al;
+    float gGAx = 0.0f, gGAy = 0.0f, gal = 0.0f;
+    for (int c = 0; c < C; ++c) {
+        const float* Uc = m.U + (long long)c * N;
+        __syncthreads();  // previous channel's sU reads are done
+        for (int t = threadIdx.x; t < HH * HW; t += NT) {
+            const int r = t / HW, cc = t - r * HW;
+            sU[r][cc] = Uc[(long long)wrap_near(ty0 + r - 1, H) * W + wrap_near(tx0 + cc - 1, W)];
+        }
+        __syncthreads();
+        if (!inside) continue;
+        float gux, guy;
+        sobel9(sU[ly][lx], sU[ly][lx + 1], sU[ly][lx + 2], sU[ly + 1][lx], sU[ly + 1][lx + 2], sU[ly + 2][lx],
+               sU[ly + 2][lx + 1], sU[ly + 2][lx + 2], gux, guy);
+        const float fx = oma * gux - al * gax;
+        const float fy = oma * guy - al * gay;
+        const float mass = A[(long long)c * N + i];
+        const bool active = !(fabsf(mass) < 1e-10f);
+        const float dxp = (float)x + dt * fx;  // reintegration.rs:81-82
+        const float dyp = (float)y + dt * fy;
+        const float* ln = lamn + (long long)c * N;
+        float dm = 0.0f, gx = 0.0f, gy = 0.0f;
+        if (ta < 1e-10f) {  // point fallback (reintegration.rs:117-124)
+            dm = ln[(long long)wrapi(sat_i32(roundf(dyp)), H) * W + wrapi(sat_i32(roundf(dxp)), W)];
+        } else {
+            const float xm = dxp - s, xM = dxp + s, ym = dyp - s, yM = dyp + s;
+            const int ixl = max(sat_i32(floorf(xm)), -W), ixh = min(sat_i32(ceilf(xM)), 2 * W);
+            const int iyl = max(sat_i32(floorf(ym)), -H), iyh = min(sat_i32(ceilf(yM)), 2 * H);
+            for (int iy = iyl; iy <= iyh; ++iy) {
+                const float cy0 = (float)iy, cy1 = (float)(iy + 1);
+                const float oh = overlap1(cy0, cy1, ym, yM);
+                if (!(oh > 0.0f)) continue;
+                const float doh = (float)(yM < cy1) - (float)(ym > cy0);
+                const float* lrow = ln + (long long)wrap_near(iy, H) * W;
+                for (int ix = ixl; ix <= ixh; ++ix) {
+                    const float cx0 = (float)ix, cx1 = (float)(ix + 1);
+                    const float ow = overlap1(cx0, cx1, xm, xM);
+                    if (!(ow * oh > 0.0f)) continue;
+                    const float dow = (float)(xM < cx1) - (float)(xm > cx0);
+                    const float l = lrow[wrap_near(ix, W)];
+                    dm += l * (ow * oh);
+                    gx += l * (dow * oh);
+                    gy += l * (ow * doh);
+                }
+            }
+            dm *= inv_ta;
+            const float k = active ? mass * inv_ta * dt : 0.0f;
+            gx *= k;
+            gy *= k;
+        }
+        lamo[(long long)c * N + i] = dm;
+        m.gscr[(long long)(2 * c) * N + i] = oma * gx;
+        m.gscr[(long long)(2 * c + 1) * N + i] = oma * gy;
+        gGAx -= al * gx;
+        gGAy -= al * gy;
+        gal -= gx * (gux + gax) + gy * (guy + gay);
+    }
+    if (inside) {
+        m.gscr[(long long)(2 * C) * N + i] = gGAx;
+        m.gscr[(long long)(2 * C + 1) * N + i] = gGAy;
+        m.gscr[(long long)(2 * C + 2) * N + i] = gal * dalpha_of(asum, beta_a, nexp);
+    }
+}
+
+// Sobel transpose at one cell: -(Sobel_x(ax) + Sobel_y(ay)) from the 3x3 neighbourhood.
+__device__ __forceinline__ float sobel_adj(const float* ax, const float* ay, long long rp, long long rc,
+                                           long long rn, int xp, int x, int xn) {
+    float sx, sy, dummy;
+    sobel9(ax[rp + xp], ax[rp + x], ax[rp + xn], ax[rc + xp], ax[rc + xn], ax[rn + xp], ax[rn + x], ax[rn + xn], sx,
+           dummy);
+    sobel9(ay[rp + xp], ay[rp + x], ay[rp + xn], ay[rc + xp], ay[rc + xn], ay[rn + xp], ay[rn + x], ay[rn + xn],
+           dummy, sy);
+    return -(sx + sy);
+}
+
+// items as kb_adj_flow.  lam[1 - li] += dL/dA_sum (every channel); lamU_c = dL/dU_c.
+__global__ __launch_bounds__(adj::NT) void kb_adj_div(const MemberDesc* __restrict__ mem,
+                                                     const int4* __restrict__ items, int li) {
+    using namespace adj;
+    const int4 it = items[blockIdx.x];
+    const MemberDesc& m = mem[it.x];
+    const int W = m.g.W, H = m.g.H, C = m.C;
+    const long long N = m.g.N;
+    const int tiles_x = (W + TX - 1) / TX;
+    const int x = (it.y % tiles_x) * TX + (int)threadIdx.x % TX;
+    const int y = (it.y / tiles_x) * TY + (int)threadIdx.x / TX;
+    if (x >= W || y >= H) return;
+    const int xp = x == 0 ? W - 1 : x - 1, xn = x == W - 1 ? 0 : x + 1;
+    const int yp = y == 0 ? H - 1 : y - 1, yn = y == H - 1 ? 0 : y + 1;
+    const long long rp = (long long)yp * W, rc = (long long)y * W, rn = (long long)yn * W;
+    const long long i = rc + x;
+    const float* g = m.gscr;
+    const float lsum = sobel_adj(g + (long long)(2 * C) * N, g + (long long)(2 * C + 1) * N, rp, rc, rn, xp, x, xn) +
+                       g[(long long)(2 * C + 2) * N + i];
+    float* lamo = m.lam[1 - li];
+    for (int c = 0; c < C; ++c) {
+        lamo[(long long)c * N + i] += lsum;
+        m.lamU[(long long)c * N + i] =
+            sobel_adj(g + (long long)(2 * c) * N, g + (long long)(2 * c + 1) * N, rp, rc, rn, xp, x, xn);
+    }
+}
+
+void launch_adj_flow(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cur, int li) {
+    hipLaunchKernelGGL(kb_adj_flow, dim3(nitems), dim3(adj::NT), 0, st, mem, items, cur, li);
+}
+
+void launch_adj_div(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int li) {
+    hipLaunchKernelGGL(kb_adj_div, dim3(nitems), dim3(adj::NT), 0, st, mem, items, li);
+}
+
+int adj_tile_w() { return adj::TX; }
+int adj_tile_h() { return adj::TY; }
+
+// ---- tape copies and clears -----------------------------------------------------------------
+// items[b] = (member, part, nparts): block b strides over the member's C*N floats.
+// to_tape: tape[slot] <- A[cur]; else A[cur] <- tape[slot].
+__global__ __launch_bounds__(256) void kb_tape_copy(const MemberDesc* __restrict__ mem, const int4* __restrict__ items,
+                                                   int cur, int slot, int to_tape) {
+    const int4 it = items[blockIdx.x];
+    const MemberDesc& m = mem[it.x];
+    const long long n = (long long)m.C * m.g.N;
+    float* a = m.A[cur];
+    float* t = slot < 0 ? m.snap : m.tape + (long long)slot * n;
+    const float* src = to_tape ? a : t;
+    float* dst = to_tape ? t : a;
+    for (long long j = (long long)it.y * 256 + threadIdx.x; j < n; j += (long long)it.z * 256) dst[j] = src[j];
+}
+
+// Clear the accumulators of a backward pass: P, pacc and the kernel-tap windows.
+__global__ __launch_bounds__(256) void kb_tape_clear(const MemberDesc* __restrict__ mem,
+                                                    const int4* __restrict__ items) {
+    const int4 it = items[blockIdx.x];
+    const MemberDesc& m = mem[it.x];
+    const long long stride = (long long)it.z * 256, j0 = (long long)it.y * 256 + threadIdx.x;
+    const long long np = (long long)m.K * m.g.spec;
+    for (long long j = j0; j < np; j += stride) m.P[j] = make_float2(0.0f, 0.0f);
+    const long long na = (long long)m.K * ((m.g.H + 1) / 2) * 3;
+    for (long long j = j0; j < na; j += stride) m.pacc[j] = 0.0;
+    long long nw = 0;
+    for (int k = 0; k < m.K; ++k) nw += (long long)(2 * m.dkw[k].y + 1) * (2 * m.dkw[k].y + 1);
+    for (long long j = j0; j < nw; j += stride) m.dkwin[j] = 0.0;
+}
+
+void launch_tape_copy(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cur, int slot,
+                      bool to_tape) {
+    hipLaunchKernelGGL(kb_tape_copy, dim3(nitems), dim3(256), 0, st, mem, items, cur, slot, to_tape ? 1 : 0);
+}
+
+void launch_tape_clear(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems) {
+    hipLaunchKernelGGL(kb_tape_clear, dim3(nitems), dim3(256), 0, st, mem, items);
+}
+
+}  // namespace hp

@@ -1,0 +1,167 @@
+// fft_lds.hpp -- the LDS Stockham FFT core shared by the forward and adjoint passes.
+//
+// In-place Stockham autosort transforms in LDS: each stage loads its radix-R
+// butterfly inputs into registers, synchronises, and stores the outputs; radices
+// 8/4/2/3 cover lengths 2^a * 3^b.  Twiddles come from a table exp(-2 pi i m / L)
+// (host FP64, rounded to f32) staged in LDS.
+#pragma once
+
+#include "hp_common.hpp"
+
+namespace hp {
+
+// ---- compile-time radix plan ------------------------------------------------------
+__host__ __device__ constexpr int pick_radix(int m) {
+    return (m % 8 == 0 && m != 16) ? 8 : (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2 : (m % 3 == 0) ? 3 : 1;
+}
+__host__ __device__ constexpr int num_stages(int L) {
+    int n = 0;
+    while (L > 1) {
+        L /= pick_radix(L);
+        ++n;
+    }
+    return n;
+}
+__host__ __device__ constexpr int radix_at(int L, int s) {
+    for (int i = 0; i < s; ++i) L /= pick_radix(L);
+    return pick_radix(L);
+}
+__host__ __device__ constexpr int ns_at(int L, int s) {  // product of the radices before stage s
+    int ns = 1;
+    for (int i = 0; i < s; ++i) {
+        int r = pick_radix(L);
+        ns *= r;
+        L /= r;
+    }
+    return ns;
+}
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// multiply by -i (DIR = -1, forward) or +i (DIR = +1, inverse)
+template <int DIR>
+__device__ __forceinline__ float2 mul_mi(float2 a) {
+    return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+// In-register DFT of size R with exponent sign DIR.
+template <int R, int DIR>
+__device__ __forceinline__ void dft(float2* v) {
+    if constexpr (R == 2) {
+        float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 3) {
+        const float c1 = -0.5f, s1 = 0.86602540378443864676f;
+        float2 t = cadd(v[1], v[2]);
+        float2 d = csub(v[1], v[2]);
+        float2 m = make_float2(v[0].x + c1 * t.x, v[0].y + c1 * t.y);
+        float2 j = mul_mi<DIR>(make_float2(s1 * d.x, s1 * d.y));  // -i*s*d (fwd)
+        v[0] = cadd(v[0], t);
+        v[1] = cadd(m, j);
+        v[2] = csub(m, j);
+    } else if constexpr (R == 4) {
+        float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+        float2 b0 = cadd(v[1], v[3]), b1 = mul_mi<DIR>(csub(v[1], v[3]));
+        v[0] = cadd(a0, b0);
+        v[2] = csub(a0, b0);
+        v[1] = cadd(a1, b1);
+        v[3] = csub(a1, b1);
+    } else if constexpr (R == 8) {
+        float2 e[4] = {v[0], v[2], v[4], v[6]};
+        float2 o[4] = {v[1], v[3], v[5], v[7]};
+        dft<4, DIR>(e);
+        dft<4, DIR>(o);
+        const float h = 0.70710678118654752440f;
+        // W8^1, W8^2, W8^3 with exponent sign DIR
+        float2 o1 = cmul(o[1], make_float2(h, DIR * h));
+        float2 o2 = mul_mi<DIR>(o[2]);
+        float2 o3 = cmul(o[3], make_float2(-h, DIR * h));
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);
+        v[7] = csub(e[3], o3);
+    }
+}
+
+// One Stockham stage over NFFT transforms of length L at buf[f*LP + i], run by NT threads.
+template <int L, int STAGE, int DIR, int NFFT, int NT, int LP>
+__device__ __forceinline__ void fft_stage(float2* buf, const float2* tw) {
+    constexpr int R = radix_at(L, STAGE);
+    constexpr int NS = ns_at(L, STAGE);
+    constexpr int NB = L / R;
+    constexpr int TOT = NFFT * NB;
+    constexpr int Q = (TOT + NT - 1) / NT;
+    constexpr int TSTEP = L / (NS * R);
+    float2 v[Q][R];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int b = threadIdx.x + q * NT;
+        if (TOT % NT == 0 || b < TOT) {
+            const int f = b / NB, j = b - f * NB;
+            const float2* src = buf + f * LP + j;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = src[r * NB];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int b = threadIdx.x + q * NT;
+        if (TOT % NT == 0 || b < TOT) {
+            const int f = b / NB, j = b - f * NB;
+            const int k = j % NS;
+            if (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    float2 w = tw[r * k * TSTEP];
+                    if (DIR > 0) w.y = -w.y;
+                    v[q][r] = cmul(v[q][r], w);
+                }
+            }
+            dft<R, DIR>(v[q]);
+            float2* dst = buf + f * LP + (j / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) dst[r * NS] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int L, int STAGE, int DIR, int NFFT, int NT, int LP>
+__device__ __forceinline__ void fft_stages(float2* buf, const float2* tw) {
+    if constexpr (STAGE < num_stages(L)) {
+        fft_stage<L, STAGE, DIR, NFFT, NT, LP>(buf, tw);
+        fft_stages<L, STAGE + 1, DIR, NFFT, NT, LP>(buf, tw);
+    }
+}
+
+// NFFT in-place transforms of length L (unnormalised), DIR = -1 forward, +1 inverse.
+// Caller synchronises before (inputs written); returns after a barrier.
+template <int L, int DIR, int NFFT, int NT, int LP>
+__device__ __forceinline__ void lds_fft(float2* buf, const float2* tw) {
+    fft_stages<L, 0, DIR, NFFT, NT, LP>(buf, tw);
+}
+
+template <int NT>
+__device__ __forceinline__ void stage_twiddles(float2* tw, const float2* __restrict__ g, int L) {
+    for (int i = threadIdx.x; i < L; i += NT) tw[i] = g[i];
+}
+
+// Launch shapes per length: threads of a row pass, columns per spectral workgroup and
+// threads of a spectral workgroup.
+template <int L>
+struct Cfg {
+    static constexpr int ROW_NT = L / 8 < 64 ? 64 : (L / 8 > 256 ? 256 : L / 8);
+    static constexpr int CB = 8192 / L < 1 ? 1 : (8192 / L > 64 ? 64 : 8192 / L);
+    static constexpr int COL_NT = (CB * L) / 32 < 64 ? 64 : ((CB * L) / 32 > 256 ? 256 : (CB * L) / 32);
+};
+
+}  // namespace hp

@@ -1,4 +1,5 @@
-// host_batch.cpp -- populations: many independent 2D simulations stepped together.
+// host_batch.cpp -- populations: many independent 2D simulations stepped together, and
+// their gradient tape.
 //
 // The reference evaluates a population by building one CpuPropagator per candidate and
 // stepping each on its own thread (FitnessEvaluator::evaluate, fitness.rs:26-54, under
@@ -8,6 +9,14 @@
 // is one launch per channel count.  Both step parities are captured once into hipGraphs.
 // A member whose flow leaves the fused stencil's radius is handled like hp_advance does:
 // the whole batch is replayed from a snapshot with per-step checks and the general path.
+//
+// Gradient tape (no reference counterpart): tape_forward runs T steps keeping the state
+// every k steps (and the final state); tape_backward walks the segments from the last,
+// recomputes each segment's states from its checkpoint, and runs the adjoint of every step
+// in reverse -- the forward's three FFT passes (keeping the convolutions), the flow and
+// divergence adjoints, and the three adjoint FFT passes -- accumulating dL/dA, the growth
+// parameter sums and the kernel-tap spectrum.  The ring parameters' derivatives follow on
+// the host in FP64 from the kernel-tap derivatives.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -28,7 +37,7 @@ struct HostMember {
     hp_config cfg{};
     std::vector<hp_kernel> kernels;
     std::vector<hp_ring> rings;
-    float* snapshot = nullptr;
+    std::vector<int> radius;  // actual kernel radius per kernel
     float time = 0.0f;
     uint64_t step = 0;
 };
@@ -40,7 +49,7 @@ struct ItemList {
 
 struct SizeClass {
     int W, H;
-    ItemList rows_fwd, spectral, rows_inv;
+    ItemList rows_fwd, spectral, rows_inv, adj_rows_fwd, adj_spectral;
 };
 
 struct StencilGroup {
@@ -62,8 +71,12 @@ struct hp_batch {
     hp::MemberDesc* d_desc = nullptr;
     std::vector<SizeClass> classes;
     std::vector<StencilGroup> stencil_groups;
+    ItemList copy_items;  // (member, part, nparts) over each member's C*N floats
+    ItemList adj_tiles;   // (member, tile) of the adjoint stencil passes
+    ItemList wdft_items;  // (member, kernel)
     std::map<int, float2*> tw;
     std::vector<void*> allocs;
+    std::vector<void*> tape_allocs;
     int* d_flags = nullptr;  // [0] stencil flags, [1] needed radius, [2] non-finite
     float4* box_lo = nullptr;
     float4* box_hi = nullptr;
@@ -74,6 +87,12 @@ struct hp_batch {
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     uint32_t max_radius = 0;
     uint64_t general_steps = 0;
+    // tape
+    uint64_t tape_cap_slots = 0;  // states each member's tape holds
+    uint64_t tape_T = 0;
+    uint32_t tape_k = 0;
+    bool tape_recorded = false;
+    int tape_cur = 0;  // parity of the final state after tape_forward
     std::string err;
 
     hp_status fail(hp_status st, const std::string& m) {
@@ -89,20 +108,26 @@ struct hp_batch {
             return (b)->fail(HP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+#define STRY(expr)                      \
+    do {                                \
+        hp_status s_ = (expr);          \
+        if (s_ != HP_OK) return s_;     \
+    } while (0)
+
 namespace {
 
 template <typename T>
-hp_status balloc(hp_batch* b, T** p, size_t count) {
+hp_status balloc(hp_batch* b, T** p, size_t count, std::vector<void*>* list = nullptr) {
     void* v = nullptr;
     BTRY(b, hipMalloc(&v, std::max<size_t>(1, count) * sizeof(T)));
-    b->allocs.push_back(v);
+    (list ? list : &b->allocs)->push_back(v);
     *p = (T*)v;
     return HP_OK;
 }
 
 template <typename T>
-hp_status bupload(hp_batch* b, T** p, const std::vector<T>& v) {
-    hp_status st = balloc(b, p, v.size());
+hp_status bupload(hp_batch* b, T** p, const std::vector<T>& v, std::vector<void*>* list = nullptr) {
+    hp_status st = balloc(b, p, v.size(), list);
     if (st != HP_OK) return st;
     if (!v.empty()) BTRY(b, hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
     return HP_OK;
@@ -124,12 +149,17 @@ hp_status twiddles(hp_batch* b, int L, const float2** out) {
     return HP_OK;
 }
 
-hp_status enqueue_step(hp_batch* b, int cur) {
+void enqueue_fft_passes(hp_batch* b, int cur, bool conv) {
     for (const SizeClass& sc : b->classes) {
         if (sc.rows_fwd.n) hp::launch_batch_rows_fwd(sc.W, b->stream, b->d_desc, sc.rows_fwd.d, sc.rows_fwd.n, cur);
         if (sc.spectral.n) hp::launch_batch_spectral(sc.H, b->stream, b->d_desc, sc.spectral.d, sc.spectral.n);
-        if (sc.rows_inv.n) hp::launch_batch_rows_inv(sc.W, b->stream, b->d_desc, sc.rows_inv.d, sc.rows_inv.n);
+        if (sc.rows_inv.n)
+            hp::launch_batch_rows_inv(sc.W, b->stream, b->d_desc, sc.rows_inv.d, sc.rows_inv.n, conv);
     }
+}
+
+hp_status enqueue_step(hp_batch* b, int cur) {
+    enqueue_fft_passes(b, cur, false);
     for (const StencilGroup& gr : b->stencil_groups)
         hp::launch_batch_stencil2d(b->stream, gr.C, gr.pow2, b->d_desc, gr.items.d, gr.items.n, cur, b->d_flags);
     BTRY(b, hipGetLastError());
@@ -189,54 +219,296 @@ hp_status member_stencil(hp_batch* b, int i, int cur) {
     return HP_OK;
 }
 
+// One step with every member's stencil checked (A[cur] -> A[1 - cur]).
+hp_status step_checked(hp_batch* b, int cur) {
+    enqueue_fft_passes(b, cur, false);
+    for (size_t i = 0; i < b->members.size(); i++) STRY(member_stencil(b, (int)i, cur));
+    return HP_OK;
+}
+
+void copy_states(hp_batch* b, int cur, int slot, bool to_tape) {
+    hp::launch_tape_copy(b->stream, b->d_desc, b->copy_items.d, b->copy_items.n, cur, slot, to_tape);
+}
+
+hp_status read_flag(hp_batch* b, int* f) {
+    BTRY(b, hipMemcpyAsync(f, b->d_flags, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    BTRY(b, hipStreamSynchronize(b->stream));
+    return HP_OK;
+}
+
 hp_status advance(hp_batch* b, uint64_t steps) {
     if (steps == 0) return HP_OK;
     const int start = b->cur;
-    for (size_t i = 0; i < b->members.size(); i++) {
-        const hp::MemberDesc& m = b->desc[i];
-        BTRY(b, hipMemcpyAsync(b->members[i].snapshot, m.A[start], (size_t)m.C * m.g.N * sizeof(float),
-                               hipMemcpyDeviceToDevice, b->stream));
-    }
+    copy_states(b, start, -1, true);  // snapshot
     hp::launch_clear_ints(b->stream, b->d_flags, 1);
-    hp_status st = build_graphs(b);
-    if (st != HP_OK) return st;
+    STRY(build_graphs(b));
     for (uint64_t k = 0; k < steps; k++) {
         BTRY(b, hipGraphLaunch(b->graph[b->cur], b->stream));
         b->cur = 1 - b->cur;
     }
     int f = 0;
-    BTRY(b, hipMemcpyAsync(&f, b->d_flags, sizeof(int), hipMemcpyDeviceToHost, b->stream));
-    BTRY(b, hipStreamSynchronize(b->stream));
-    if (f == 0) {
+    STRY(read_flag(b, &f));
+    if (f != 0) {  // replay from the snapshot, one checked step at a time
+        b->cur = start;
+        copy_states(b, start, -1, false);
+        for (uint64_t k = 0; k < steps; k++) {
+            STRY(step_checked(b, b->cur));
+            b->cur = 1 - b->cur;
+        }
+    } else {
         b->max_radius = std::max<uint32_t>(b->max_radius, 2);
-        for (HostMember& hm : b->members)
-            for (uint64_t k = 0; k < steps; k++) {
-                hm.time += hm.cfg.dt;
-                hm.step += 1;
-            }
-        return HP_OK;
     }
-    // replay from the snapshots, one checked step at a time
-    b->cur = start;
-    for (size_t i = 0; i < b->members.size(); i++) {
-        const hp::MemberDesc& m = b->desc[i];
-        BTRY(b, hipMemcpyAsync(m.A[start], b->members[i].snapshot, (size_t)m.C * m.g.N * sizeof(float),
-                               hipMemcpyDeviceToDevice, b->stream));
-    }
-    for (uint64_t k = 0; k < steps; k++) {
-        for (const SizeClass& sc : b->classes) {
-            if (sc.rows_fwd.n) hp::launch_batch_rows_fwd(sc.W, b->stream, b->d_desc, sc.rows_fwd.d, sc.rows_fwd.n, b->cur);
-            if (sc.spectral.n) hp::launch_batch_spectral(sc.H, b->stream, b->d_desc, sc.spectral.d, sc.spectral.n);
-            if (sc.rows_inv.n) hp::launch_batch_rows_inv(sc.W, b->stream, b->d_desc, sc.rows_inv.d, sc.rows_inv.n);
-        }
-        for (size_t i = 0; i < b->members.size(); i++) {
-            st = member_stencil(b, (int)i, b->cur);
-            if (st != HP_OK) return st;
-        }
-        b->cur = 1 - b->cur;
-        for (HostMember& hm : b->members) {
+    for (HostMember& hm : b->members)
+        for (uint64_t k = 0; k < steps; k++) {
             hm.time += hm.cfg.dt;
             hm.step += 1;
+        }
+    return HP_OK;
+}
+
+// ---- gradient tape --------------------------------------------------------------------------
+// Tape slots per member: [0, nck) checkpoints at t = j*k, nck the final state, then k
+// segment states.
+uint64_t num_checkpoints(uint64_t T, uint32_t k) { return (T + k - 1) / k; }
+
+hp_status tape_check_member(hp_batch* b, size_t i) {
+    const HostMember& hm = b->members[i];
+    const hp::MemberPlan& p = hm.plan;
+    const std::string who = "member " + std::to_string(i) + ": ";
+    if (!hp::adj_fft_supported(p.g.W) || !hp::adj_fft_supported(p.g.H) || p.g.W > hp::window_dft_max_side() ||
+        p.g.H > hp::window_dft_max_side())
+        return b->fail(HP_ERR_UNSUPPORTED_SIZE, who + "gradient tapes need sides 2^a * 3^b up to " +
+                                                   std::to_string(hp::window_dft_max_side()));
+    if (p.spec_work.size() != p.src_chan.size())
+        return b->fail(HP_ERR_UNSUPPORTED_SIZE, who + "gradient tapes need one spectral item per source");
+    for (int r : hm.radius)
+        if (r < 1 || 2 * r + 1 > std::min(p.g.W, p.g.H) || 2 * r + 1 > hp::window_dft_max_taps())
+            return b->fail(HP_ERR_UNSUPPORTED_SIZE,
+                           who + "gradient tapes need every kernel's support (2R+1) within the grid and <= " +
+                               std::to_string(hp::window_dft_max_taps()));
+    return HP_OK;
+}
+
+hp_status ensure_tape(hp_batch* b, uint64_t T, uint32_t k) {
+    const uint64_t slots = num_checkpoints(T, k) + 1 + k;
+    if (slots <= b->tape_cap_slots) return HP_OK;
+    for (size_t i = 0; i < b->members.size(); i++) STRY(tape_check_member(b, i));
+    BTRY(b, hipStreamSynchronize(b->stream));
+    for (void* p : b->tape_allocs) (void)hipFree(p);
+    b->tape_allocs.clear();
+    b->tape_cap_slots = 0;
+    for (size_t i = 0; i < b->members.size(); i++) {
+        const HostMember& hm = b->members[i];
+        const hp::MemberPlan& p = hm.plan;
+        hp::MemberDesc& d = b->desc[i];
+        const size_t N = (size_t)p.g.N, CN = (size_t)p.C * N, K = (size_t)std::max(p.K, 0);
+        std::vector<void*>* L = &b->tape_allocs;
+        STRY(balloc(b, &d.lam[0], CN, L));
+        STRY(balloc(b, &d.lam[1], CN, L));
+        STRY(balloc(b, &d.lamU, CN, L));
+        STRY(balloc(b, &d.conv, K * N, L));
+        STRY(balloc(b, &d.gscr, (2 * (size_t)p.C + 3) * N, L));
+        STRY(balloc(b, &d.P, K * (size_t)p.g.spec, L));
+        STRY(balloc(b, &d.pacc, K * (size_t)((p.g.H + 1) / 2) * 3, L));
+        STRY(balloc(b, &d.tape, slots * CN, L));
+        std::vector<int2> km(std::max<size_t>(K, 1)), dkw(std::max<size_t>(K, 1));
+        size_t off = 0;
+        for (size_t kk = 0; kk < K; kk++) {
+            km[kk] = make_int2((int)hm.kernels[kk].source_channel, (int)hm.kernels[kk].target_channel);
+            const int R = hm.radius[kk];
+            dkw[kk] = make_int2((int)off, R);
+            off += (size_t)(2 * R + 1) * (2 * R + 1);
+        }
+        STRY(balloc(b, &d.dkwin, off, L));
+        int2* dkm = nullptr;
+        int2* ddk = nullptr;
+        STRY(bupload(b, &dkm, km, L));
+        STRY(bupload(b, &ddk, dkw, L));
+        d.kmap = dkm;
+        d.dkw = ddk;
+    }
+    BTRY(b, hipMemcpy(b->d_desc, b->desc.data(), b->desc.size() * sizeof(hp::MemberDesc), hipMemcpyHostToDevice));
+    b->tape_cap_slots = slots;
+    return HP_OK;
+}
+
+hp_status tape_forward(hp_batch* b, uint64_t T, uint32_t k) {
+    if (k == 0) return b->fail(HP_ERR_SHAPE, "checkpoint interval must be at least 1");
+    b->tape_recorded = false;
+    STRY(ensure_tape(b, T, k));
+    const uint64_t nck = num_checkpoints(T, k);
+    for (uint64_t j = 0; j < nck; j++) {
+        copy_states(b, b->cur, (int)j, true);
+        STRY(advance(b, std::min<uint64_t>(k, T - j * k)));
+    }
+    copy_states(b, b->cur, (int)nck, true);
+    BTRY(b, hipStreamSynchronize(b->stream));
+    b->tape_T = T;
+    b->tape_k = k;
+    b->tape_cur = b->cur;
+    b->tape_recorded = true;
+    return HP_OK;
+}
+
+// States t0 .. t1-1 into the segment slots, from checkpoint j (t0 = j*k); A[0]/A[1] are
+// scratch here.
+hp_status recompute_segment(hp_batch* b, uint64_t j, uint64_t t0, uint64_t t1) {
+    const int seg = (int)(num_checkpoints(b->tape_T, b->tape_k) + 1);
+    STRY(build_graphs(b));
+    for (int pass = 0; pass < 2; pass++) {
+        int cur = 0;
+        copy_states(b, 0, (int)j, false);
+        hp::launch_clear_ints(b->stream, b->d_flags, 1);
+        for (uint64_t t = t0; t < t1; t++) {
+            copy_states(b, cur, seg + (int)(t - t0), true);
+            if (t + 1 < t1) {
+                if (pass == 0) BTRY(b, hipGraphLaunch(b->graph[cur], b->stream));
+                else STRY(step_checked(b, cur));
+                cur = 1 - cur;
+            }
+        }
+        int f = 0;
+        STRY(read_flag(b, &f));
+        if (pass == 0 && f == 0) break;  // the fused stencil covered every step
+        if (pass == 1) break;
+    }
+    return HP_OK;
+}
+
+// Adjoint of one step whose input state is in A[0]; lam[li] holds dL/dA_{t+1}, lam[1-li]
+// receives dL/dA_t.
+hp_status backward_step(hp_batch* b, int li) {
+    enqueue_fft_passes(b, 0, true);
+    hp::launch_adj_flow(b->stream, b->d_desc, b->adj_tiles.d, b->adj_tiles.n, 0, li);
+    hp::launch_adj_div(b->stream, b->d_desc, b->adj_tiles.d, b->adj_tiles.n, li);
+    for (const SizeClass& sc : b->classes) {
+        if (sc.adj_rows_fwd.n) hp::launch_adj_rows_fwd(sc.W, b->stream, b->d_desc, sc.adj_rows_fwd.d, sc.adj_rows_fwd.n);
+        if (sc.adj_spectral.n)
+            hp::launch_adj_spectral(sc.H, b->stream, b->d_desc, sc.adj_spectral.d, sc.adj_spectral.n);
+        if (sc.rows_fwd.n) hp::launch_adj_rows_inv(sc.W, b->stream, b->d_desc, sc.rows_fwd.d, sc.rows_fwd.n, li);
+    }
+    BTRY(b, hipGetLastError());
+    return HP_OK;
+}
+
+// theta = per kernel (weight, mu, sigma), then per ring (amplitude, distance, width).
+size_t theta_count(const HostMember& hm) { return 3 * (size_t)hm.cfg.num_kernels + 3 * (size_t)hm.cfg.num_rings; }
+
+// dL/dtheta of one member from the device sums: growth sums reduced over row pairs; ring
+// parameters through K = v / sum(v), v = sum_r a_r exp(-(rho - d_r)^2 / (2 w_r^2)) on
+// rho = |offset| / R <= 1 (kernel.rs:40-72), in FP64.
+void member_theta(const HostMember& hm, const std::vector<double>& pacc, const std::vector<double>& dkwin,
+                  double* th) {
+    const int K = (int)hm.cfg.num_kernels;
+    const int pairs = (hm.plan.g.H + 1) / 2;
+    std::fill(th, th + theta_count(hm), 0.0);
+    for (int k = 0; k < K; k++)
+        for (int j = 0; j < 3; j++) {
+            double s = 0.0;
+            for (int p = 0; p < pairs; p++) s += pacc[((size_t)k * pairs + p) * 3 + j];
+            th[3 * k + j] = s;
+        }
+    const double invN = 1.0 / (double)hm.plan.g.N;
+    size_t off = 0;
+    for (int k = 0; k < K; k++) {
+        const hp_kernel& kc = hm.kernels[k];
+        const int R = hm.radius[k], S = 2 * R + 1;
+        const double* dk = dkwin.data() + off;
+        off += (size_t)S * S;
+        std::vector<double> v((size_t)S * S, 0.0);
+        std::vector<char> in((size_t)S * S, 0);
+        double sum = 0.0, dot = 0.0;
+        for (int y = 0; y < S; y++)
+            for (int x = 0; x < S; x++) {
+                const double dx = x - R, dy = y - R;
+                const double rho = sqrt(dx * dx + dy * dy) / R;
+                if (rho > 1.0) continue;
+                double val = 0.0;
+                for (uint32_t r = 0; r < kc.ring_count; r++) {
+                    const hp_ring& rg = hm.rings[kc.ring_offset + r];
+                    const double rd = rho - rg.distance, w = rg.width;
+                    val += rg.amplitude * exp(-rd * rd / (2.0 * w * w));
+                }
+                v[(size_t)y * S + x] = val;
+                in[(size_t)y * S + x] = 1;
+                sum += val;
+                dot += dk[(size_t)y * S + x] * invN * val;
+            }
+        for (int y = 0; y < S; y++)
+            for (int x = 0; x < S; x++) {
+                if (!in[(size_t)y * S + x]) continue;
+                const double g = dk[(size_t)y * S + x] * invN;
+                const double dv = sum > 0.0 ? g / sum - dot / (sum * sum) : g;
+                const double dx = x - R, dy = y - R;
+                const double rho = sqrt(dx * dx + dy * dy) / R;
+                for (uint32_t r = 0; r < kc.ring_count; r++) {
+                    const hp_ring& rg = hm.rings[kc.ring_offset + r];
+                    const double a = rg.amplitude, rd = rho - rg.distance, w = rg.width;
+                    const double e = exp(-rd * rd / (2.0 * w * w));
+                    double* tr = th + 3 * (size_t)K + 3 * (size_t)(kc.ring_offset + r);
+                    tr[0] += dv * e;
+                    tr[1] += dv * a * e * rd / (w * w);
+                    tr[2] += dv * a * e * rd * rd / (w * w * w);
+                }
+            }
+    }
+}
+
+hp_status tape_backward(hp_batch* b, const float* const* seed, float* const* gA0, double* dtheta) {
+    if (!b->tape_recorded) return b->fail(HP_ERR_SHAPE, "no recorded tape: call hp_batch_tape_forward first");
+    const uint64_t T = b->tape_T, k = b->tape_k, nck = num_checkpoints(T, b->tape_k);
+    size_t ci = 0;
+    for (size_t i = 0; i < b->members.size(); i++) {
+        const hp::MemberDesc& m = b->desc[i];
+        for (int c = 0; c < m.C; c++, ci++) {
+            if (!seed[ci]) return b->fail(HP_ERR_NULL, "dL/dA_T pointer is NULL");
+            BTRY(b, hipMemcpyAsync(m.lam[0] + (size_t)c * m.g.N, seed[ci], (size_t)m.g.N * sizeof(float),
+                                   hipMemcpyHostToDevice, b->stream));
+        }
+    }
+    hp::launch_tape_clear(b->stream, b->d_desc, b->copy_items.d, b->copy_items.n);
+    int li = 0;
+    for (uint64_t j = nck; j-- > 0;) {
+        const uint64_t t0 = j * k, t1 = std::min<uint64_t>(T, t0 + k);
+        STRY(recompute_segment(b, j, t0, t1));
+        const int seg = (int)(nck + 1);
+        for (uint64_t t = t1; t-- > t0;) {
+            copy_states(b, 0, seg + (int)(t - t0), false);
+            STRY(backward_step(b, li));
+            li = 1 - li;
+        }
+    }
+    hp::launch_window_dft(b->stream, b->d_desc, b->wdft_items.d, b->wdft_items.n);
+    BTRY(b, hipGetLastError());
+    // results
+    ci = 0;
+    std::vector<std::vector<double>> pacc(b->members.size()), dkwin(b->members.size());
+    for (size_t i = 0; i < b->members.size(); i++) {
+        const hp::MemberDesc& m = b->desc[i];
+        const HostMember& hm = b->members[i];
+        for (int c = 0; c < m.C; c++, ci++)
+            if (gA0 && gA0[ci])
+                BTRY(b, hipMemcpyAsync(gA0[ci], m.lam[li] + (size_t)c * m.g.N, (size_t)m.g.N * sizeof(float),
+                                       hipMemcpyDeviceToHost, b->stream));
+        const int K = (int)hm.cfg.num_kernels;
+        pacc[i].resize((size_t)K * ((m.g.H + 1) / 2) * 3);
+        size_t nw = 0;
+        for (int r : hm.radius) nw += (size_t)(2 * r + 1) * (2 * r + 1);
+        dkwin[i].resize(nw);
+        if (K > 0) {
+            BTRY(b, hipMemcpyAsync(pacc[i].data(), m.pacc, pacc[i].size() * sizeof(double), hipMemcpyDeviceToHost,
+                                   b->stream));
+            BTRY(b, hipMemcpyAsync(dkwin[i].data(), m.dkwin, nw * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+        }
+    }
+    copy_states(b, b->tape_cur, (int)nck, false);  // the population holds A_T again
+    b->cur = b->tape_cur;
+    BTRY(b, hipStreamSynchronize(b->stream));
+    if (dtheta) {
+        size_t off = 0;
+        for (size_t i = 0; i < b->members.size(); i++) {
+            member_theta(b->members[i], pacc[i], dkwin[i], dtheta + off);
+            off += theta_count(b->members[i]);
         }
     }
     return HP_OK;
@@ -246,6 +518,7 @@ void release(hp_batch* b) {
     if (!b) return;
     for (int p = 0; p < 2; p++)
         if (b->graph[p]) (void)hipGraphExecDestroy(b->graph[p]);
+    for (void* p : b->tape_allocs) (void)hipFree(p);
     for (void* p : b->allocs) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -278,6 +551,7 @@ hp_status create(const hp_config* cfgs, uint32_t count, int device, hp_batch** o
         hm.rings.assign(cfg->rings, cfg->rings + cfg->num_rings);
         hm.cfg.kernels = hm.kernels.data();
         hm.cfg.rings = hm.rings.data();
+        for (const hp_kernel& kc : hm.kernels) hm.radius.push_back((int)hp::actual_radius(kc.radius, cfg->kernel_radius));
         max_cn = std::max(max_cn, (size_t)hm.plan.C * (size_t)hm.plan.g.N);
         max_c = std::max(max_c, (size_t)hm.plan.C);
     }
@@ -298,6 +572,8 @@ hp_status create(const hp_config* cfgs, uint32_t count, int device, hp_batch** o
     b->desc.resize(count);
     std::map<std::pair<int, int>, std::vector<uint32_t>> by_size;
     std::map<std::pair<int, bool>, std::vector<uint32_t>> by_stencil;
+    std::vector<int4> copy_it, tiles, wdft;
+    const int ATX = hp::adj_tile_w(), ATY = hp::adj_tile_h();
     for (uint32_t i = 0; i < count; i++) {
         HostMember& hm = b->members[i];
         const hp::MemberPlan& p = hm.plan;
@@ -307,7 +583,7 @@ hp_status create(const hp_config* cfgs, uint32_t count, int device, hp_batch** o
         TRY(balloc(b, &d.A[0], CN));
         TRY(balloc(b, &d.A[1], CN));
         TRY(balloc(b, &d.U, CN));
-        TRY(balloc(b, &hm.snapshot, CN));
+        TRY(balloc(b, &d.snap, CN));
         TRY(balloc(b, &d.X, std::max<size_t>(1, p.src_chan.size()) * p.g.spec));
         TRY(balloc(b, &d.Y, (size_t)std::max(1, p.K) * p.g.spec));
         const float* kq = nullptr;
@@ -351,13 +627,25 @@ hp_status create(const hp_config* cfgs, uint32_t count, int device, hp_batch** o
         int e;
         const bool pow2 = frexpf(ta, &e) == 0.5f && ta >= 1.17549435e-38f && ta < INFINITY;
         if (hp::stencil2d_supported(p.g, s)) by_stencil[{p.C, pow2}].push_back(i);
+        const int nparts = (int)std::max<size_t>(1, std::min<size_t>(256, CN / 4096));
+        for (int q = 0; q < nparts; q++) copy_it.push_back(make_int4((int)i, q, nparts, 0));
+        const int nt = ((p.g.W + ATX - 1) / ATX) * ((p.g.H + ATY - 1) / ATY);
+        for (int t = 0; t < nt; t++) tiles.push_back(make_int4((int)i, t, 0, 0));
+        for (int kk = 0; kk < p.K; kk++) wdft.push_back(make_int4((int)i, kk, 0, 0));
     }
     TRY(bupload(b, &b->d_desc, b->desc));
+    TRY(bupload(b, &b->copy_items.d, copy_it));
+    b->copy_items.n = (int)copy_it.size();
+    TRY(bupload(b, &b->adj_tiles.d, tiles));
+    b->adj_tiles.n = (int)tiles.size();
+    TRY(bupload(b, &b->wdft_items.d, wdft));
+    b->wdft_items.n = (int)wdft.size();
     // work items per size class
     for (auto& kv : by_size) {
-        SizeClass scl{kv.first.first, kv.first.second, {}, {}, {}};
-        std::vector<int4> rf, sp, ri;
+        SizeClass scl{kv.first.first, kv.first.second, {}, {}, {}, {}, {}};
+        std::vector<int4> rf, sp, ri, arf, asp;
         const int CB = hp::spectral_block_lines(scl.H);
+        const int ACB = std::max(1, hp::adj_spectral_block_lines(scl.H));
         for (uint32_t i : kv.second) {
             const hp::MemberPlan& p = b->members[i].plan;
             const int pairs = (p.g.H + 1) / 2;
@@ -368,6 +656,11 @@ hp_status create(const hp_config* cfgs, uint32_t count, int device, hp_batch** o
                     for (int blk = 0; blk < (p.g.Wh + CB - 1) / CB; blk++) sp.push_back(make_int4((int)i, blk, wi, 0));
                 for (int wi = 0; wi < (int)p.tgt_work.size(); wi++)
                     for (int pr = 0; pr < pairs; pr++) ri.push_back(make_int4((int)i, pr, wi, 0));
+                for (int kk = 0; kk < p.K; kk++)
+                    for (int pr = 0; pr < pairs; pr++) arf.push_back(make_int4((int)i, pr, kk, 0));
+                for (int wi = 0; wi < (int)p.spec_work.size(); wi++)
+                    for (int blk = 0; blk < (p.g.Wh + ACB - 1) / ACB; blk++)
+                        asp.push_back(make_int4((int)i, blk, wi, 0));
             }
         }
         TRY(bupload(b, &scl.rows_fwd.d, rf));
@@ -376,6 +669,10 @@ hp_status create(const hp_config* cfgs, uint32_t count, int device, hp_batch** o
         scl.spectral.n = (int)sp.size();
         TRY(bupload(b, &scl.rows_inv.d, ri));
         scl.rows_inv.n = (int)ri.size();
+        TRY(bupload(b, &scl.adj_rows_fwd.d, arf));
+        scl.adj_rows_fwd.n = (int)arf.size();
+        TRY(bupload(b, &scl.adj_spectral.d, asp));
+        scl.adj_spectral.n = (int)asp.size();
         b->classes.push_back(scl);
     }
     const int TX = hp::stencil2d_tile_w(), TY = hp::stencil2d_tile_h();
@@ -391,7 +688,6 @@ hp_status create(const hp_config* cfgs, uint32_t count, int device, hp_batch** o
         gr.items.n = (int)it.size();
         b->stencil_groups.push_back(gr);
     }
-    // members the fused stencil cannot take run the checked path on every step
     size_t fused = 0;
     for (auto& kv : by_stencil) fused += kv.second.size();
     if (fused != count) {
@@ -465,6 +761,7 @@ hp_status hp_batch_set_state(hp_batch* b, uint32_t member, const float* const* c
         BTRY(b, hipStreamSynchronize(b->stream));
         b->members[member].time = 0.0f;
         b->members[member].step = 0;
+        b->tape_recorded = false;
         return HP_OK;
     });
 }
@@ -487,7 +784,10 @@ hp_status hp_batch_get_state(hp_batch* b, uint32_t member, float* const* channel
 }
 
 hp_status hp_batch_advance(hp_batch* b, uint64_t steps) {
-    return guarded(b, [&]() -> hp_status { return advance(b, steps); });
+    return guarded(b, [&]() -> hp_status {
+        b->tape_recorded = false;
+        return advance(b, steps);
+    });
 }
 
 hp_status hp_batch_centroids(hp_batch* b, double* out_xy, double* mass) {
@@ -535,6 +835,25 @@ hp_status hp_batch_stats(const hp_batch* b, uint32_t* max_gather_radius, uint64_
     if (max_gather_radius) *max_gather_radius = b->max_radius;
     if (general_path_steps) *general_path_steps = b->general_steps;
     return HP_OK;
+}
+
+hp_status hp_batch_theta_count(const hp_batch* b, uint32_t member, uint32_t* count) {
+    if (!b || !count) return HP_ERR_NULL;
+    if (member >= b->members.size()) return HP_ERR_SHAPE;
+    *count = (uint32_t)theta_count(b->members[member]);
+    return HP_OK;
+}
+
+hp_status hp_batch_tape_forward(hp_batch* b, uint64_t steps, uint32_t checkpoint_every) {
+    return guarded(b, [&]() -> hp_status { return tape_forward(b, steps, checkpoint_every); });
+}
+
+hp_status hp_batch_tape_backward(hp_batch* b, const float* const* dl_dfinal, float* const* dl_dinitial,
+                                 double* dtheta) {
+    return guarded(b, [&]() -> hp_status {
+        if (!dl_dfinal) return b->fail(HP_ERR_NULL, "dl_dfinal is NULL");
+        return tape_backward(b, dl_dfinal, dl_dinitial, dtheta);
+    });
 }
 
 }  // extern "C"

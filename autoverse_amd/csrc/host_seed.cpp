@@ -194,7 +194,7 @@ hp_status plan_member(const hp_config* cfg, bool allow_wave, int group, MemberPl
             p.src_chan.push_back((int)kc.source_channel);
         }
         const float sigma_sq_2 = 2.0f * kc.sigma * kc.sigma;  // growth.rs:51-52
-        p.gp[k] = {kc.weight, kc.mu, 1.0f / sigma_sq_2, 0.0f};
+        p.gp[k] = {kc.weight, kc.mu, 1.0f / sigma_sq_2, kc.sigma};
     }
     // spectral work: one item per source (its forward transform is shared by all its kernels)
     // or one item per kernel (more workgroups; the forward transform is redone)

@@ -25,12 +25,13 @@ struct TgtWork {
     int pad;
 };
 
-// Growth parameters of one kernel: weight, mu, 1/(2 sigma^2) (growth.rs:51-52).
+// Growth parameters of one kernel: weight, mu, 1/(2 sigma^2) (growth.rs:51-52), and sigma
+// itself for the adjoint's d/dsigma.
 struct GrowthParams {
     float weight;
     float mu;
     float inv;
-    float pad;
+    float sigma;
 };
 
 // Shapes of one 2D/3D grid and its half spectrum.
@@ -61,6 +62,19 @@ struct MemberDesc {
     Geo g;
     int C, K, nsrc, ntgt, nspec;
     float dt, beta_a, nexp, s;
+    // gradient tape (null unless the member carries one)
+    const int2* kmap;            // per kernel (source channel, target channel)
+    float* lam[2];               // dL/dA ping-pong, C*N each
+    float* conv;                 // K*N convolution outputs of the recomputed step
+    float* gscr;                 // (2C+3)*N: dL/d(grad U_c) x, y per channel; dL/d(grad A_sum) x, y;
+                                 //   dL/dA_sum through alpha
+    float* lamU;                 // C*N: dL/dU
+    float2* P;                   // K*spec: sum over steps of FFT(g_k) * conj(FFT(A_src)), pitch Wp
+    double* pacc;                // K*pairs*3: FP64 sums of dL/d(weight, mu, sigma) per row pair
+    float* tape;                 // checkpoint and segment states, C*N each
+    double* dkwin;               // per kernel (2R+1)^2 FP64 derivatives of the kernel taps
+    const int2* dkw;             // per kernel (offset of its window in dkwin, radius R)
+    float* snap;                 // C*N snapshot taken before a multi-step advance
 };
 
 // ---- launchers implemented in fft_kernels.hip -------------------------------------
@@ -77,7 +91,9 @@ bool launch_rows_inv(int L, hipStream_t st, const float2* Y, float* U, const Tgt
 // Batched 2D passes over a size class: items[i] = (member, block, slot/work, -).
 bool launch_batch_rows_fwd(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cur);
 bool launch_batch_spectral(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems);
-bool launch_batch_rows_inv(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems);
+// conv: also store each kernel's convolution (MemberDesc::conv), for the adjoint.
+bool launch_batch_rows_inv(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems,
+                           bool conv = false);
 int spectral_block_lines(int L);  // columns per spectral workgroup for length-L columns
 
 // ---- launchers implemented in wave_fft.hip (2D, sides 256..2048, column-major spectra) ----
@@ -126,5 +142,30 @@ void launch_mass_f64(hipStream_t st, const float* A, int C, long long n, double*
                      double* out);
 // Clear a small int buffer with a kernel (graph-capture safe, no memset node).
 void launch_clear_ints(hipStream_t st, int* p, int n);
+
+// ---- gradient tape: adjoint_stencil.hip ----------------------------------------------------
+// items = (member, tile) over adj_tile_w() x adj_tile_h() tiles.
+void launch_adj_flow(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cur, int li);
+void launch_adj_div(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int li);
+int adj_tile_w();
+int adj_tile_h();
+// items = (member, part, nparts).  to_tape: tape[slot] <- A[cur], else A[cur] <- tape[slot];
+// slot < 0 addresses MemberDesc::snap instead of the tape.
+void launch_tape_copy(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cur, int slot,
+                      bool to_tape);
+void launch_tape_clear(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems);
+// ---- gradient tape: adjoint_fft.hip ---------------------------------------------------------
+bool adj_fft_supported(int L);
+// items = (member, row pair, kernel)
+bool launch_adj_rows_fwd(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems);
+// items = (member, column block of adj_spectral_block_lines(L) columns, source work item)
+bool launch_adj_spectral(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems);
+int adj_spectral_block_lines(int L);
+// items as the forward row pass (member, row pair, source slot)
+bool launch_adj_rows_inv(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int li);
+// items = (member, kernel)
+void launch_window_dft(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems);
+int window_dft_max_side();
+int window_dft_max_taps();
 
 }  // namespace hp

@@ -1,0 +1,61 @@
+// stencil_common.hpp -- the reference's f32 stencil arithmetic shared by the forward
+// stencil kernels and their adjoints (both compiled with -ffp-contract=off, so that the
+// adjoint recomputes exactly the forward's flow and destinations).
+#pragma once
+
+#include <math.h>
+
+#include "hp_common.hpp"
+
+namespace hp {
+
+__device__ __forceinline__ int wrapi(int c, int n) {
+    int r = c % n;
+    return r < 0 ? r + n : r;
+}
+
+// Wrap for coordinates a few grid lengths out at most (no integer division).
+__device__ __forceinline__ int wrap_near(int c, int n) {
+    while (c < 0) c += n;
+    while (c >= n) c -= n;
+    return c;
+}
+
+// Rust `f as i32`: saturating, NaN -> 0.
+__device__ __forceinline__ int sat_i32(float v) {
+    if (v != v) return 0;
+    if (v >= 2147483647.0f) return 2147483647;
+    if (v <= -2147483648.0f) return (int)0x80000000;
+    return (int)v;
+}
+
+// compute_alpha (flow.rs:14-32): integer fast paths for n = 1..4, else powf; clamp keeps NaN.
+__device__ __forceinline__ float alpha_of(float mass, float beta_a, float n) {
+    const float ratio = mass / beta_a;
+    float p;
+    if (n == 1.0f) p = ratio;
+    else if (n == 2.0f) p = ratio * ratio;
+    else if (n == 3.0f) p = ratio * ratio * ratio;
+    else if (n == 4.0f) {
+        const float r2 = ratio * ratio;
+        p = r2 * r2;
+    } else p = powf(ratio, n);
+    if (p < 0.0f) p = 0.0f;
+    if (p > 1.0f) p = 1.0f;
+    return p;
+}
+
+// gradient.rs:99-102
+__device__ __forceinline__ void sobel9(float tl, float tc, float tr, float ml, float mr, float bl, float bc,
+                                       float br, float& gx, float& gy) {
+    gx = (-tl + tr - 2.0f * ml + 2.0f * mr - bl + br) * 0.125f;
+    gy = (-tl - 2.0f * tc - tr + bl + 2.0f * bc + br) * 0.125f;
+}
+
+// Overlap of cell [c0, c0 + 1) (c1 = c0 + 1 as f32) with [lo, hi] (reintegration.rs:137-144):
+// max(min(c1, hi) - max(c0, lo), 0) with Rust max/min (a NaN operand is ignored).
+__device__ __forceinline__ float overlap1(float c0, float c1, float lo, float hi) {
+    return fmaxf(fminf(c1, hi) - fmaxf(c0, lo), 0.0f);
+}
+
+}  // namespace hp

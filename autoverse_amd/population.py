@@ -115,6 +115,47 @@ class HipPopulation:
         self._check(self._lib.hp_batch_stats(self._ptr, ctypes.byref(r), ctypes.byref(g)), "hp_batch_stats")
         return r.value, g.value
 
+    # -- gradient tape ---------------------------------------------------------------------
+    def theta_count(self, member: int) -> int:
+        n = ctypes.c_uint32(0)
+        self._check(self._lib.hp_batch_theta_count(self._ptr, member, ctypes.byref(n)), "hp_batch_theta_count")
+        return n.value
+
+    def tape_forward(self, steps: int, checkpoint_every: int) -> None:
+        """Advance every member `steps` steps, keeping a state every `checkpoint_every` steps
+        for tape_backward."""
+        self._check(self._lib.hp_batch_tape_forward(self._ptr, int(steps), int(checkpoint_every)),
+                    "hp_batch_tape_forward")
+
+    def tape_backward(self, dl_dfinal: Sequence[Sequence[np.ndarray]]):
+        """For L = sum over members and channels of <dl_dfinal, A_T>: per member, the list of
+        dL/dA_0 channels and dL/dtheta (per kernel weight, mu, sigma; per ring amplitude,
+        distance, width), from the last tape_forward."""
+        seeds, outs = [], []
+        for i, cfg in enumerate(self._configs):
+            n = cfg.width * cfg.height
+            if len(dl_dfinal[i]) != cfg.channels:
+                raise ValueError(f"member {i}: expected {cfg.channels} channels of dL/dA_T")
+            for c in range(cfg.channels):
+                a = np.ascontiguousarray(dl_dfinal[i][c], dtype=np.float32).reshape(-1)
+                if a.size != n:
+                    raise ValueError(f"member {i}: dL/dA_T channel has {a.size} cells, expected {n}")
+                seeds.append(a)
+                outs.append(np.empty(n, dtype=np.float32))
+        counts = [self.theta_count(i) for i in range(len(self))]
+        dth = np.zeros(max(1, sum(counts)), dtype=np.float64)
+        self._check(self._lib.hp_batch_tape_backward(self._ptr, _abi.float_ptr_array(seeds),
+                                                     _abi.float_ptr_array(outs),
+                                                     dth.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                    "hp_batch_tape_backward")
+        g0, th, ci, ti = [], [], 0, 0
+        for i, cfg in enumerate(self._configs):
+            g0.append(outs[ci:ci + cfg.channels])
+            ci += cfg.channels
+            th.append(dth[ti:ti + counts[i]].copy())
+            ti += counts[i]
+        return g0, th
+
     def evaluate_displacement(self, seeds: Optional[Sequence[Seed]], warmup: int, steps: int) -> np.ndarray:
         """Centroid displacement of every member over `steps` steps after `warmup` steps
         (FitnessEvaluator::evaluate with the trajectory's total_displacement)."""

@@ -3,6 +3,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
     python bench.py --population [--pop-steps K]
+    python bench.py --grad [--grad-steps T] [--grad-every k]
 
 Workload: the 2D multi-species configuration, 1024 x 1024, 3 channels, 9 kernels
 (every source->target pair, 1-3 rings each), kernel_radius 15, dt 0.2, synthetic
@@ -41,6 +42,11 @@ def main() -> int:
                     help="population mode: 256 uneven members stepped as one batch vs one lone "
                          "propagator per member (prints its own JSON line)")
     ap.add_argument("--pop-steps", type=int, default=200)
+    ap.add_argument("--grad", action="store_true",
+                    help="differentiation mode: 32 members of 512^2 x 2 channels, forward tape of "
+                         "--grad-steps steps with a checkpoint every --grad-every, then the backward pass")
+    ap.add_argument("--grad-steps", type=int, default=64)
+    ap.add_argument("--grad-every", type=int, default=8)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of 256 MiB ten times (calibrates rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE against a kernel of known traffic)")
@@ -53,6 +59,8 @@ def main() -> int:
         return 2
     if args.population:
         return population_bench(args)
+    if args.grad:
+        return grad_bench(args)
 
     import numpy as np
 
@@ -181,8 +189,27 @@ def population_bench(args) -> int:
     for p in props:
         p.close()
 
+    # batched forward + adjoint over the same population: tape of K steps (checkpoint every
+    # ~sqrt(K)), then the backward pass for L = sum <seed, A_T> with a fixed random seed
+    every = max(1, int(round(K ** 0.5)))
+    pop = HipPopulation(cfgs)
+    pop.set_seeds(seeds)
+    pop.advance(args.warmup)
+    rng = np.random.default_rng(7)
+    dl = [[rng.standard_normal(c.width * c.height).astype(np.float32) for _ in range(c.channels)] for c in cfgs]
+    print("[population] forward + adjoint pass", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    pop.tape_forward(K, every)
+    t1 = time.perf_counter()
+    _, dth = pop.tape_backward(dl)
+    secs_fa = time.perf_counter() - t0
+    secs_f = t1 - t0
+    pop.close()
+    finite = all(np.all(np.isfinite(t)) for t in dth)
+
     value = cells * K / secs_b
     per_inst = cells * K / secs_i
+    fwd_adj = cells * K / secs_fa
     out = {
         "metric": "cell-steps/s, 256-member population (batched forward step)",
         "value": value,
@@ -202,8 +229,62 @@ def population_bench(args) -> int:
         "per_instance": {"value": per_inst, "unit": "cell-steps/s", "ms_per_step": secs_i * 1e3 / K,
                          "path": "one HipPropagator per member, advanced in turn"},
         "speedup_vs_per_instance": value / per_inst,
+        "forward_adjoint": {"value": fwd_adj, "unit": "cell-steps/s", "steps": K, "checkpoint_every": every,
+                            "ms_per_step": secs_fa * 1e3 / K, "tape_forward_ms_per_step": secs_f * 1e3 / K,
+                            "speedup_vs_per_instance_forward": fwd_adj / per_inst, "gradients_finite": finite},
         "fitness": {"displacement_mean": float(disp.mean()), "displacement_max": float(disp.max())},
         "mass_drift_max_rel": drift,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def grad_bench(args) -> int:
+    """Differentiation workload: 32 members of 512^2 x 2 channels, 4 kernels each
+    (autoverse_amd/workloads.differentiation with seeds 0x5EED4 + j), forward tape of T
+    steps with a checkpoint every k, backward pass for L = sum <seed, A_T>."""
+    import numpy as np
+
+    from autoverse_amd import workloads
+    from autoverse_amd.population import HipPopulation
+
+    members = [workloads.differentiation(seed=0x5EED4 + j) for j in range(32)]
+    cfgs = [c for c, _ in members]
+    cells = sum(c.width * c.height for c in cfgs)
+    T, k = args.grad_steps, args.grad_every
+    print(f"[grad] {len(cfgs)} members, {cells} cells, T={T}, k={k}", file=sys.stderr, flush=True)
+    pop = HipPopulation(cfgs)
+    pop.set_seeds([s for _, s in members])
+    rng = np.random.default_rng(11)
+    dl = [[rng.standard_normal(c.width * c.height).astype(np.float32) for _ in range(c.channels)] for c in cfgs]
+    pop.tape_forward(T, k)  # warm: allocations, graphs, first touches
+    pop.tape_backward(dl)
+    pop.set_seeds([s for _, s in members])
+    t0 = time.perf_counter()
+    pop.advance(T)
+    secs_plain = time.perf_counter() - t0
+    pop.set_seeds([s for _, s in members])
+    t0 = time.perf_counter()
+    pop.tape_forward(T, k)
+    t1 = time.perf_counter()
+    _, dth = pop.tape_backward(dl)
+    t2 = time.perf_counter()
+    pop.close()
+    out = {
+        "metric": "cell-steps/s, forward + adjoint (gradient tape) over 32 x 512^2 x 2ch",
+        "value": cells * T / (t2 - t0),
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": T,
+        "higher_is_better": True,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "differentiation: 32 members of 512^2, 2 channels, 4 kernels, R=15, dt=0.2",
+                   "members": len(cfgs), "checkpoint_every": k},
+        "ms_per_step": {"forward_plain": secs_plain * 1e3 / T, "tape_forward": (t1 - t0) * 1e3 / T,
+                        "backward": (t2 - t1) * 1e3 / T, "forward_plus_adjoint": (t2 - t0) * 1e3 / T},
+        "backward_over_forward": (t2 - t1) / secs_plain,
+        "gradients_finite": all(bool(np.all(np.isfinite(t))) for t in dth),
     }
     print(json.dumps(out), flush=True)
     return 0

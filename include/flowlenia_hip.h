@@ -271,6 +271,21 @@ hp_status hp_batch_centroids(hp_batch* b, double* out_xy, double* mass);
 hp_status hp_batch_mass_f64(hp_batch* b, double* per_member);
 hp_status hp_batch_stats(const hp_batch* b, uint32_t* max_gather_radius, uint64_t* general_path_steps);
 
+/* ---- gradient tape (2D members) ---------------------------------------------------------
+ * No counterpart in the reference, which has no derivative.  A tape records T steps of the
+ * whole population from its current state (the state advances to A_T, as hp_batch_advance),
+ * keeping one state every `checkpoint_every` steps; a backward pass then gives, for the loss
+ * L = sum_c <dl_dfinal_c, A_T,c> of every member, dL/dA_0 and dL/dtheta, recomputing each
+ * segment of steps from its checkpoint.  theta of a member is, per kernel in config order,
+ * (weight, mu, sigma), then per ring in config order (amplitude, distance, width).
+ * Pointer arrays are member-major: member 0's channels, then member 1's, ...; dtheta holds
+ * the members' theta blocks back to back.  The tape is dropped by set_state / advance.
+ * Limits: sides 2^a * 3^b up to 1024; every kernel's support 2R+1 within the grid. */
+hp_status hp_batch_theta_count(const hp_batch* b, uint32_t member, uint32_t* count);
+hp_status hp_batch_tape_forward(hp_batch* b, uint64_t steps, uint32_t checkpoint_every);
+hp_status hp_batch_tape_backward(hp_batch* b, const float* const* dl_dfinal, float* const* dl_dinitial,
+                                 double* dtheta);
+
 #ifdef __cplusplus
 }
 #endif
