@@ -122,7 +122,7 @@ EXPORTS = [
     ("hp_batch_mass_f64", c_int, [c_void_p, POINTER(c_double)]),
     ("hp_batch_stats", c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint64)]),
     ("hp_batch_theta_count", c_int, [c_void_p, c_uint32, POINTER(c_uint32)]),
-    ("hp_batch_tape_forward", c_int, [c_void_p, c_uint64, c_uint32]),
+    ("hp_batch_tape_forward", c_int, [c_void_p, c_uint64, c_uint32, POINTER(c_uint32)]),
     ("hp_batch_tape_backward", c_int, [c_void_p, FloatPP, FloatPP, POINTER(c_double)]),
 ]
 

@@ -22,7 +22,7 @@ namespace hp {
 // item = (member, row pair, kernel).  Reads conv_k and dL/dU; writes FFT_x(g_k) into Y.
 template <int L, int NT>
 __global__ __launch_bounds__(NT) void kb_adj_rows_fwd(const MemberDesc* __restrict__ mem,
-                                                      const int4* __restrict__ items) {
+                                                      const int4* __restrict__ items, int cslot) {
     __shared__ float2 buf[L];
     __shared__ float2 tw[L];
     __shared__ double red[3][NT];
@@ -35,7 +35,7 @@ __global__ __launch_bounds__(NT) void kb_adj_rows_fwd(const MemberDesc* __restri
     const bool has1 = (y0 + 1) < rows;
     const GrowthParams p = m.gp[k];
     const int2 km = m.kmap[k];
-    const float* cv = m.conv + (long long)k * N + (long long)y0 * L;
+    const float* cv = m.conv + ((long long)cslot * m.K + k) * N + (long long)y0 * L;
     const float* lu = m.lamU + (long long)km.y * N + (long long)y0 * L;
     const float inv_s3 = 1.0f / (p.sigma * p.sigma * p.sigma);
     stage_twiddles<NT>(tw, m.tw_w, L);
@@ -285,8 +285,8 @@ __global__ __launch_bounds__(WDFT_NT) void kb_window_dft(const MemberDesc* __res
 #define HP_ADJ_SIZES(X_) X_(8) X_(16) X_(32) X_(64) X_(128) X_(256) X_(512) X_(1024) \
     X_(24) X_(48) X_(96) X_(192) X_(384) X_(768)
 
-bool launch_adj_rows_fwd(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems) {
-#define X_(n) if (L == n) { hipLaunchKernelGGL((kb_adj_rows_fwd<n, Cfg<n>::ROW_NT>), dim3(nitems), dim3(Cfg<n>::ROW_NT), 0, st, mem, items); return true; }
+bool launch_adj_rows_fwd(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cslot) {
+#define X_(n) if (L == n) { hipLaunchKernelGGL((kb_adj_rows_fwd<n, Cfg<n>::ROW_NT>), dim3(nitems), dim3(Cfg<n>::ROW_NT), 0, st, mem, items, cslot); return true; }
     HP_ADJ_SIZES(X_)
 #undef X_
     return false;

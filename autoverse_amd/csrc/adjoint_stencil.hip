@@ -247,4 +247,25 @@ void launch_tape_clear(hipStream_t st, const MemberDesc* mem, const int4* items,
     hipLaunchKernelGGL(kb_tape_clear, dim3(nitems), dim3(256), 0, st, mem, items);
 }
 
+// Every member's C*N adjoint values to / from one contiguous staging array, so that a host
+// transfer of the whole population is one copy.
+__global__ __launch_bounds__(256) void kb_stage_lam(const MemberDesc* __restrict__ mem, const int4* __restrict__ items,
+                                                   float* __restrict__ stage, int li, int to_stage) {
+    const int4 it = items[blockIdx.x];
+    const MemberDesc& m = mem[it.x];
+    const long long n = (long long)m.C * m.g.N;
+    float* l = m.lam[li];
+    float* s = stage + m.stage_off;
+    for (long long j = (long long)it.y * 256 + threadIdx.x; j < n; j += (long long)it.z * 256) {
+        if (to_stage) s[j] = l[j];
+        else l[j] = s[j];
+    }
+}
+
+void launch_stage_lam(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, float* stage, int li,
+                      bool to_stage) {
+    hipLaunchKernelGGL(kb_stage_lam, dim3(nitems), dim3(256), 0, st, mem, items, stage, li, to_stage ? 1 : 0);
+}
+
+
 }  // namespace hp

@@ -189,6 +189,16 @@ __global__ __launch_bounds__(NT) void kb_spectral(const MemberDesc* __restrict__
 
 // ---- pass 3: C2R along x on pairs of rows + growth + weighted accumulate -----------------
 // grid (ceil(rows/2), nwork); work[w] is a target channel and its kernels in index order.
+// One kernel's growth term w * G(c) (growth_accumulate, growth.rs:51-57: g = 2 exp(-d*d*inv) - 1,
+// t += w*g).  The inverse row pass and the adjoint's rebuild of U both call this one function
+// in this one translation unit, so the two produce bit-identical U.
+__device__ __forceinline__ float growth_term(float c, const GrowthParams& p) {
+#pragma clang fp contract(off)
+    const float d = c - p.mu;
+    const float g = 2.0f * expf(-d * d * p.inv) - 1.0f;
+    return p.weight * g;
+}
+
 // CONV: also store every kernel's convolution rows (conv + k*N), for the adjoint's recompute.
 template <int L, int NT, bool CONV = false>
 __device__ __forceinline__ void rows_inv_body(const float2* __restrict__ Y, float* __restrict__ U,
@@ -242,11 +252,8 @@ __device__ __forceinline__ void rows_inv_body(const float2* __restrict__ Y, floa
                         cv[0] = c0;
                         if (has1) cv[L] = c1;
                     }
-                    const float d0 = c0 - p.mu, d1 = c1 - p.mu;
-                    const float g0 = 2.0f * expf(-d0 * d0 * p.inv) - 1.0f;
-                    const float g1 = 2.0f * expf(-d1 * d1 * p.inv) - 1.0f;
-                    acc0[q] += p.weight * g0;
-                    acc1[q] += p.weight * g1;
+                    acc0[q] += growth_term(c0, p);
+                    acc1[q] += growth_term(c1, p);
                 }
             }
         }
@@ -271,13 +278,41 @@ __global__ __launch_bounds__(NT) void k_rows_inv(const float2* __restrict__ Y, f
     rows_inv_body<L, NT>(Y, U, work, klist, gp, rows, Wp, N, spec, scale, twg, blockIdx.x, blockIdx.y);
 }
 
-// batched: item = (member, row pair, target work item)
+// batched: item = (member, row pair, target work item); CONV: convolutions into conv slot cslot
 template <int L, int NT, bool CONV>
-__global__ __launch_bounds__(NT) void kb_rows_inv(const MemberDesc* __restrict__ mem, const int4* __restrict__ items) {
+__global__ __launch_bounds__(NT) void kb_rows_inv(const MemberDesc* __restrict__ mem, const int4* __restrict__ items,
+                                                  int cslot) {
     const int4 it = items[blockIdx.x];
     const MemberDesc& m = mem[it.x];
     rows_inv_body<L, NT, CONV>(m.Y, m.U, m.tgt_work, m.klist_tgt, m.gp, m.g.H * m.g.D, m.g.Wp, m.g.N, m.g.spec,
-                               1.0f / (float)m.g.N, m.tw_w, it.y, it.z, m.conv);
+                               1.0f / (float)m.g.N, m.tw_w, it.y, it.z,
+                               CONV ? m.conv + (long long)cslot * m.K * m.g.N : nullptr);
+}
+
+// U of a taped step from its stored convolutions (conv slot cslot): per target channel the
+// kernels in index order, acc from 0 -- the inverse row pass's epilogue on the same values.
+// items = (member, part, nparts) striding over the member's N cells.
+__global__ __launch_bounds__(256) void kb_u_from_conv(const MemberDesc* __restrict__ mem,
+                                                     const int4* __restrict__ items, int cslot) {
+    const int4 it = items[blockIdx.x];
+    const MemberDesc& m = mem[it.x];
+    const long long N = m.g.N;
+    const float* conv = m.conv + (long long)cslot * m.K * N;
+    for (long long i = (long long)it.y * 256 + threadIdx.x; i < N; i += (long long)it.z * 256) {
+        for (int w = 0; w < m.ntgt; ++w) {
+            const TgtWork wk = m.tgt_work[w];
+            float acc = 0.0f;
+            for (int j = 0; j < wk.k_count; ++j) {
+                const int k = m.klist_tgt[wk.k_first + j];
+                acc += growth_term(conv[(long long)k * N + i], m.gp[k]);
+            }
+            m.U[(long long)wk.target * N + i] = acc;
+        }
+    }
+}
+
+void launch_u_from_conv(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cslot) {
+    hipLaunchKernelGGL(kb_u_from_conv, dim3(nitems), dim3(256), 0, st, mem, items, cslot);
 }
 
 // ---- host-side dispatch ------------------------------------------------------------------
@@ -385,9 +420,10 @@ bool launch_batch_spectral(int L, hipStream_t st, const MemberDesc* mem, const i
     return false;
 }
 
-bool launch_batch_rows_inv(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, bool conv) {
-#define X_(n) if (L == n) { if (conv) hipLaunchKernelGGL((kb_rows_inv<n, Cfg<n>::ROW_NT, true>), dim3(nitems), dim3(Cfg<n>::ROW_NT), 0, st, mem, items); \
-                           else hipLaunchKernelGGL((kb_rows_inv<n, Cfg<n>::ROW_NT, false>), dim3(nitems), dim3(Cfg<n>::ROW_NT), 0, st, mem, items); return true; }
+bool launch_batch_rows_inv(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, bool conv,
+                           int cslot) {
+#define X_(n) if (L == n) { if (conv) hipLaunchKernelGGL((kb_rows_inv<n, Cfg<n>::ROW_NT, true>), dim3(nitems), dim3(Cfg<n>::ROW_NT), 0, st, mem, items, cslot); \
+                           else hipLaunchKernelGGL((kb_rows_inv<n, Cfg<n>::ROW_NT, false>), dim3(nitems), dim3(Cfg<n>::ROW_NT), 0, st, mem, items, 0); return true; }
     HP_FFT_SIZES(X_)
 #undef X_
     return false;

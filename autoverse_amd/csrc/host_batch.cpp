@@ -88,7 +88,14 @@ struct hp_batch {
     uint32_t max_radius = 0;
     uint64_t general_steps = 0;
     // tape
+    float* d_stage = nullptr;     // contiguous device staging of every member's C*N values
+    float* h_stage = nullptr;     // pinned host twin of d_stage
+    double* d_pacc = nullptr;     // every member's growth-parameter sums, back to back
+    double* d_dkwin = nullptr;    // every member's kernel-tap windows, back to back
+    size_t stage_n = 0, pacc_n = 0, dkwin_n = 0;
     uint64_t tape_cap_slots = 0;  // states each member's tape holds
+    uint64_t conv_cap_slots = 0;  // convolution slots (K*N each) each member holds
+    bool tape_conv = false;       // the tape keeps every step's convolutions
     uint64_t tape_T = 0;
     uint32_t tape_k = 0;
     bool tape_recorded = false;
@@ -149,19 +156,26 @@ hp_status twiddles(hp_batch* b, int L, const float2** out) {
     return HP_OK;
 }
 
-void enqueue_fft_passes(hp_batch* b, int cur, bool conv) {
+// The three FFT passes of a step from A[cur]; conv: also store the convolutions in conv slot
+// cslot; rows_only: the forward row pass alone (the adjoint's source spectra).
+void enqueue_fft_passes(hp_batch* b, int cur, bool conv, int cslot = 0, bool rows_only = false) {
     for (const SizeClass& sc : b->classes) {
         if (sc.rows_fwd.n) hp::launch_batch_rows_fwd(sc.W, b->stream, b->d_desc, sc.rows_fwd.d, sc.rows_fwd.n, cur);
+        if (rows_only) continue;
         if (sc.spectral.n) hp::launch_batch_spectral(sc.H, b->stream, b->d_desc, sc.spectral.d, sc.spectral.n);
         if (sc.rows_inv.n)
-            hp::launch_batch_rows_inv(sc.W, b->stream, b->d_desc, sc.rows_inv.d, sc.rows_inv.n, conv);
+            hp::launch_batch_rows_inv(sc.W, b->stream, b->d_desc, sc.rows_inv.d, sc.rows_inv.n, conv, cslot);
     }
+}
+
+void enqueue_stencils(hp_batch* b, int cur) {
+    for (const StencilGroup& gr : b->stencil_groups)
+        hp::launch_batch_stencil2d(b->stream, gr.C, gr.pow2, b->d_desc, gr.items.d, gr.items.n, cur, b->d_flags);
 }
 
 hp_status enqueue_step(hp_batch* b, int cur) {
     enqueue_fft_passes(b, cur, false);
-    for (const StencilGroup& gr : b->stencil_groups)
-        hp::launch_batch_stencil2d(b->stream, gr.C, gr.pow2, b->d_desc, gr.items.d, gr.items.n, cur, b->d_flags);
+    enqueue_stencils(b, cur);
     BTRY(b, hipGetLastError());
     return HP_OK;
 }
@@ -220,8 +234,8 @@ hp_status member_stencil(hp_batch* b, int i, int cur) {
 }
 
 // One step with every member's stencil checked (A[cur] -> A[1 - cur]).
-hp_status step_checked(hp_batch* b, int cur) {
-    enqueue_fft_passes(b, cur, false);
+hp_status step_checked(hp_batch* b, int cur, bool conv = false, int cslot = 0) {
+    enqueue_fft_passes(b, cur, conv, cslot);
     for (size_t i = 0; i < b->members.size(); i++) STRY(member_stencil(b, (int)i, cur));
     return HP_OK;
 }
@@ -289,9 +303,10 @@ hp_status tape_check_member(hp_batch* b, size_t i) {
     return HP_OK;
 }
 
-hp_status ensure_tape(hp_batch* b, uint64_t T, uint32_t k) {
+hp_status ensure_tape(hp_batch* b, uint64_t T, uint32_t k, bool store_conv) {
     const uint64_t slots = num_checkpoints(T, k) + 1 + k;
-    if (slots <= b->tape_cap_slots) return HP_OK;
+    const uint64_t cslots = store_conv ? std::max<uint64_t>(T, 1) : 1;
+    if (slots <= b->tape_cap_slots && cslots <= b->conv_cap_slots) return HP_OK;
     for (size_t i = 0; i < b->members.size(); i++) STRY(tape_check_member(b, i));
     BTRY(b, hipStreamSynchronize(b->stream));
     for (void* p : b->tape_allocs) (void)hipFree(p);
@@ -306,10 +321,9 @@ hp_status ensure_tape(hp_batch* b, uint64_t T, uint32_t k) {
         STRY(balloc(b, &d.lam[0], CN, L));
         STRY(balloc(b, &d.lam[1], CN, L));
         STRY(balloc(b, &d.lamU, CN, L));
-        STRY(balloc(b, &d.conv, K * N, L));
+        STRY(balloc(b, &d.conv, cslots * K * N, L));
         STRY(balloc(b, &d.gscr, (2 * (size_t)p.C + 3) * N, L));
         STRY(balloc(b, &d.P, K * (size_t)p.g.spec, L));
-        STRY(balloc(b, &d.pacc, K * (size_t)((p.g.H + 1) / 2) * 3, L));
         STRY(balloc(b, &d.tape, slots * CN, L));
         std::vector<int2> km(std::max<size_t>(K, 1)), dkw(std::max<size_t>(K, 1));
         size_t off = 0;
@@ -319,7 +333,6 @@ hp_status ensure_tape(hp_batch* b, uint64_t T, uint32_t k) {
             dkw[kk] = make_int2((int)off, R);
             off += (size_t)(2 * R + 1) * (2 * R + 1);
         }
-        STRY(balloc(b, &d.dkwin, off, L));
         int2* dkm = nullptr;
         int2* ddk = nullptr;
         STRY(bupload(b, &dkm, km, L));
@@ -327,45 +340,131 @@ hp_status ensure_tape(hp_batch* b, uint64_t T, uint32_t k) {
         d.kmap = dkm;
         d.dkw = ddk;
     }
+    // contiguous accumulators and transfer staging, so that results come back in one copy each
+    size_t pacc_n = 0, dkwin_n = 0, stage_n = 0;
+    for (size_t i = 0; i < b->members.size(); i++) {
+        const HostMember& hm = b->members[i];
+        const hp::MemberPlan& p = hm.plan;
+        pacc_n += (size_t)std::max(p.K, 0) * ((p.g.H + 1) / 2) * 3;
+        for (int r : hm.radius) dkwin_n += (size_t)(2 * r + 1) * (2 * r + 1);
+        stage_n += (size_t)p.C * p.g.N;
+    }
+    STRY(balloc(b, &b->d_pacc, pacc_n, &b->tape_allocs));
+    STRY(balloc(b, &b->d_dkwin, dkwin_n, &b->tape_allocs));
+    STRY(balloc(b, &b->d_stage, stage_n, &b->tape_allocs));
+    if (b->h_stage) (void)hipHostFree(b->h_stage);
+    b->h_stage = nullptr;
+    BTRY(b, hipHostMalloc((void**)&b->h_stage, std::max<size_t>(1, stage_n) * sizeof(float), hipHostMallocDefault));
+    b->pacc_n = pacc_n;
+    b->dkwin_n = dkwin_n;
+    b->stage_n = stage_n;
+    pacc_n = dkwin_n = stage_n = 0;
+    for (size_t i = 0; i < b->members.size(); i++) {
+        const HostMember& hm = b->members[i];
+        const hp::MemberPlan& p = hm.plan;
+        hp::MemberDesc& d = b->desc[i];
+        d.pacc = b->d_pacc + pacc_n;
+        d.dkwin = b->d_dkwin + dkwin_n;
+        d.stage_off = (long long)stage_n;
+        pacc_n += (size_t)std::max(p.K, 0) * ((p.g.H + 1) / 2) * 3;
+        for (int r : hm.radius) dkwin_n += (size_t)(2 * r + 1) * (2 * r + 1);
+        stage_n += (size_t)p.C * p.g.N;
+    }
     BTRY(b, hipMemcpy(b->d_desc, b->desc.data(), b->desc.size() * sizeof(hp::MemberDesc), hipMemcpyHostToDevice));
     b->tape_cap_slots = slots;
+    b->conv_cap_slots = cslots;
     return HP_OK;
 }
 
-hp_status tape_forward(hp_batch* b, uint64_t T, uint32_t k) {
-    if (k == 0) return b->fail(HP_ERR_SHAPE, "checkpoint interval must be at least 1");
-    b->tape_recorded = false;
-    STRY(ensure_tape(b, T, k));
-    const uint64_t nck = num_checkpoints(T, k);
-    for (uint64_t j = 0; j < nck; j++) {
-        copy_states(b, b->cur, (int)j, true);
-        STRY(advance(b, std::min<uint64_t>(k, T - j * k)));
+// Tape layout for checkpoint_every = 0, from the free device memory (half of it is the
+// budget): every state and every step's convolutions if they fit (no recomputation at all,
+// and the backward skips the forward's spectral passes); else every state; else a
+// checkpoint every ~sqrt(T) steps (about 2 sqrt(T) states per member).
+void auto_layout(hp_batch* b, uint64_t T, uint32_t* k, bool* store_conv) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+    double set = 0.0, cset = 0.0;  // bytes of one state / one step's convolutions, all members
+    for (const HostMember& hm : b->members) {
+        set += 4.0 * hm.plan.C * (double)hm.plan.g.N;
+        cset += 4.0 * std::max(hm.plan.K, 0) * (double)hm.plan.g.N;
     }
-    copy_states(b, b->cur, (int)nck, true);
-    BTRY(b, hipStreamSynchronize(b->stream));
+    const double budget = 0.5 * (double)free_b;
+    *store_conv = false;
+    if ((double)(T + 3) * (set + cset) <= budget) {
+        *k = 1;
+        *store_conv = true;
+    } else if ((double)(T + 3) * set <= budget) {
+        *k = 1;
+    } else {
+        *k = (uint32_t)std::max<uint64_t>(1, (uint64_t)ceil(sqrt((double)std::max<uint64_t>(T, 1))));
+    }
+}
+
+// T steps from the current state, keeping a state every k steps (and the final one); with
+// store_conv every step's convolutions too.  The steps run back to back; if any member
+// left the fused stencil's radius the whole tape is redone with checked steps.
+hp_status tape_forward(hp_batch* b, uint64_t T, uint32_t k) {
+    bool store_conv = false;
+    if (k == 0) auto_layout(b, T, &k, &store_conv);
+    b->tape_recorded = false;
+    STRY(ensure_tape(b, T, k, store_conv));
+    const uint64_t nck = num_checkpoints(T, k);
+    const int start = b->cur;
+    if (!store_conv) STRY(build_graphs(b));
+    for (int pass = 0; pass < 2; pass++) {
+        int cur = start;
+        if (pass == 1) copy_states(b, start, 0, false);  // back to the first checkpoint
+        hp::launch_clear_ints(b->stream, b->d_flags, 1);
+        for (uint64_t t = 0; t < T; t++) {
+            if (t % k == 0) copy_states(b, cur, (int)(t / k), true);
+            if (pass == 1) {
+                STRY(step_checked(b, cur, store_conv, (int)t));
+            } else if (store_conv) {
+                enqueue_fft_passes(b, cur, true, (int)t);
+                enqueue_stencils(b, cur);
+            } else {
+                BTRY(b, hipGraphLaunch(b->graph[cur], b->stream));
+            }
+            cur = 1 - cur;
+        }
+        BTRY(b, hipGetLastError());
+        copy_states(b, cur, (int)nck, true);
+        b->cur = cur;
+        int f = 0;
+        STRY(read_flag(b, &f));
+        if (pass == 0 && f == 0) {
+            if (T) b->max_radius = std::max<uint32_t>(b->max_radius, 2);
+            break;
+        }
+    }
+    for (HostMember& hm : b->members)
+        for (uint64_t t = 0; t < T; t++) {
+            hm.time += hm.cfg.dt;
+            hm.step += 1;
+        }
     b->tape_T = T;
     b->tape_k = k;
+    b->tape_conv = store_conv;
     b->tape_cur = b->cur;
     b->tape_recorded = true;
     return HP_OK;
 }
 
-// States t0 .. t1-1 into the segment slots, from checkpoint j (t0 = j*k); A[0]/A[1] are
-// scratch here.
+// States t0+1 .. t1-1 into segment slots seg + (t - t0), from checkpoint j (the state at
+// t0 = j*k); A[0]/A[1] are scratch here.  Nothing to do for one-step segments.
 hp_status recompute_segment(hp_batch* b, uint64_t j, uint64_t t0, uint64_t t1) {
+    if (t1 - t0 <= 1) return HP_OK;
     const int seg = (int)(num_checkpoints(b->tape_T, b->tape_k) + 1);
     STRY(build_graphs(b));
     for (int pass = 0; pass < 2; pass++) {
         int cur = 0;
         copy_states(b, 0, (int)j, false);
         hp::launch_clear_ints(b->stream, b->d_flags, 1);
-        for (uint64_t t = t0; t < t1; t++) {
+        for (uint64_t t = t0 + 1; t < t1; t++) {
+            if (pass == 0) BTRY(b, hipGraphLaunch(b->graph[cur], b->stream));
+            else STRY(step_checked(b, cur));
+            cur = 1 - cur;
             copy_states(b, cur, seg + (int)(t - t0), true);
-            if (t + 1 < t1) {
-                if (pass == 0) BTRY(b, hipGraphLaunch(b->graph[cur], b->stream));
-                else STRY(step_checked(b, cur));
-                cur = 1 - cur;
-            }
         }
         int f = 0;
         STRY(read_flag(b, &f));
@@ -375,14 +474,23 @@ hp_status recompute_segment(hp_batch* b, uint64_t j, uint64_t t0, uint64_t t1) {
     return HP_OK;
 }
 
-// Adjoint of one step whose input state is in A[0]; lam[li] holds dL/dA_{t+1}, lam[1-li]
-// receives dL/dA_t.
-hp_status backward_step(hp_batch* b, int li) {
-    enqueue_fft_passes(b, 0, true);
+// Adjoint of step t whose input state is in A[0]; lam[li] holds dL/dA_{t+1}, lam[1-li]
+// receives dL/dA_t.  With stored convolutions only the forward row pass (the source spectra)
+// and U are rebuilt; otherwise the step's FFT passes run again, keeping the convolutions.
+hp_status backward_step(hp_batch* b, int li, uint64_t t) {
+    int cslot = 0;
+    if (b->tape_conv) {
+        cslot = (int)t;
+        enqueue_fft_passes(b, 0, false, 0, true);
+        hp::launch_u_from_conv(b->stream, b->d_desc, b->copy_items.d, b->copy_items.n, cslot);
+    } else {
+        enqueue_fft_passes(b, 0, true, 0);
+    }
     hp::launch_adj_flow(b->stream, b->d_desc, b->adj_tiles.d, b->adj_tiles.n, 0, li);
     hp::launch_adj_div(b->stream, b->d_desc, b->adj_tiles.d, b->adj_tiles.n, li);
     for (const SizeClass& sc : b->classes) {
-        if (sc.adj_rows_fwd.n) hp::launch_adj_rows_fwd(sc.W, b->stream, b->d_desc, sc.adj_rows_fwd.d, sc.adj_rows_fwd.n);
+        if (sc.adj_rows_fwd.n)
+            hp::launch_adj_rows_fwd(sc.W, b->stream, b->d_desc, sc.adj_rows_fwd.d, sc.adj_rows_fwd.n, cslot);
         if (sc.adj_spectral.n)
             hp::launch_adj_spectral(sc.H, b->stream, b->d_desc, sc.adj_spectral.d, sc.adj_spectral.n);
         if (sc.rows_fwd.n) hp::launch_adj_rows_inv(sc.W, b->stream, b->d_desc, sc.rows_fwd.d, sc.rows_fwd.n, li);
@@ -462,10 +570,11 @@ hp_status tape_backward(hp_batch* b, const float* const* seed, float* const* gA0
         const hp::MemberDesc& m = b->desc[i];
         for (int c = 0; c < m.C; c++, ci++) {
             if (!seed[ci]) return b->fail(HP_ERR_NULL, "dL/dA_T pointer is NULL");
-            BTRY(b, hipMemcpyAsync(m.lam[0] + (size_t)c * m.g.N, seed[ci], (size_t)m.g.N * sizeof(float),
-                                   hipMemcpyHostToDevice, b->stream));
+            memcpy(b->h_stage + m.stage_off + (size_t)c * m.g.N, seed[ci], (size_t)m.g.N * sizeof(float));
         }
     }
+    BTRY(b, hipMemcpyAsync(b->d_stage, b->h_stage, b->stage_n * sizeof(float), hipMemcpyHostToDevice, b->stream));
+    hp::launch_stage_lam(b->stream, b->d_desc, b->copy_items.d, b->copy_items.n, b->d_stage, 0, false);
     hp::launch_tape_clear(b->stream, b->d_desc, b->copy_items.d, b->copy_items.n);
     int li = 0;
     for (uint64_t j = nck; j-- > 0;) {
@@ -473,43 +582,45 @@ hp_status tape_backward(hp_batch* b, const float* const* seed, float* const* gA0
         STRY(recompute_segment(b, j, t0, t1));
         const int seg = (int)(nck + 1);
         for (uint64_t t = t1; t-- > t0;) {
-            copy_states(b, 0, seg + (int)(t - t0), false);
-            STRY(backward_step(b, li));
+            copy_states(b, 0, t == t0 ? (int)j : seg + (int)(t - t0), false);
+            STRY(backward_step(b, li, t));
             li = 1 - li;
         }
     }
     hp::launch_window_dft(b->stream, b->d_desc, b->wdft_items.d, b->wdft_items.n);
     BTRY(b, hipGetLastError());
-    // results
+    // results: one copy each for dL/dA_0, the growth sums and the kernel-tap windows
+    hp::launch_stage_lam(b->stream, b->d_desc, b->copy_items.d, b->copy_items.n, b->d_stage, li, true);
+    BTRY(b, hipGetLastError());
+    BTRY(b, hipMemcpyAsync(b->h_stage, b->d_stage, b->stage_n * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    std::vector<double> pacc(std::max<size_t>(1, b->pacc_n)), dkwin(std::max<size_t>(1, b->dkwin_n));
+    if (b->pacc_n)
+        BTRY(b, hipMemcpyAsync(pacc.data(), b->d_pacc, b->pacc_n * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    if (b->dkwin_n)
+        BTRY(b, hipMemcpyAsync(dkwin.data(), b->d_dkwin, b->dkwin_n * sizeof(double), hipMemcpyDeviceToHost,
+                               b->stream));
+    copy_states(b, b->tape_cur, (int)nck, false);  // the population holds A_T again
+    b->cur = b->tape_cur;
+    BTRY(b, hipStreamSynchronize(b->stream));
     ci = 0;
-    std::vector<std::vector<double>> pacc(b->members.size()), dkwin(b->members.size());
+    size_t off = 0, poff = 0, woff = 0;
     for (size_t i = 0; i < b->members.size(); i++) {
         const hp::MemberDesc& m = b->desc[i];
         const HostMember& hm = b->members[i];
         for (int c = 0; c < m.C; c++, ci++)
             if (gA0 && gA0[ci])
-                BTRY(b, hipMemcpyAsync(gA0[ci], m.lam[li] + (size_t)c * m.g.N, (size_t)m.g.N * sizeof(float),
-                                       hipMemcpyDeviceToHost, b->stream));
-        const int K = (int)hm.cfg.num_kernels;
-        pacc[i].resize((size_t)K * ((m.g.H + 1) / 2) * 3);
-        size_t nw = 0;
-        for (int r : hm.radius) nw += (size_t)(2 * r + 1) * (2 * r + 1);
-        dkwin[i].resize(nw);
-        if (K > 0) {
-            BTRY(b, hipMemcpyAsync(pacc[i].data(), m.pacc, pacc[i].size() * sizeof(double), hipMemcpyDeviceToHost,
-                                   b->stream));
-            BTRY(b, hipMemcpyAsync(dkwin[i].data(), m.dkwin, nw * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+                memcpy(gA0[ci], b->h_stage + m.stage_off + (size_t)c * m.g.N, (size_t)m.g.N * sizeof(float));
+        const size_t pn = (size_t)hm.cfg.num_kernels * ((m.g.H + 1) / 2) * 3;
+        size_t wn = 0;
+        for (int r : hm.radius) wn += (size_t)(2 * r + 1) * (2 * r + 1);
+        if (dtheta) {
+            std::vector<double> pm(pacc.begin() + poff, pacc.begin() + poff + pn);
+            std::vector<double> wm(dkwin.begin() + woff, dkwin.begin() + woff + wn);
+            member_theta(hm, pm, wm, dtheta + off);
         }
-    }
-    copy_states(b, b->tape_cur, (int)nck, false);  // the population holds A_T again
-    b->cur = b->tape_cur;
-    BTRY(b, hipStreamSynchronize(b->stream));
-    if (dtheta) {
-        size_t off = 0;
-        for (size_t i = 0; i < b->members.size(); i++) {
-            member_theta(b->members[i], pacc[i], dkwin[i], dtheta + off);
-            off += theta_count(b->members[i]);
-        }
+        off += theta_count(hm);
+        poff += pn;
+        woff += wn;
     }
     return HP_OK;
 }
@@ -520,6 +631,7 @@ void release(hp_batch* b) {
         if (b->graph[p]) (void)hipGraphExecDestroy(b->graph[p]);
     for (void* p : b->tape_allocs) (void)hipFree(p);
     for (void* p : b->allocs) (void)hipFree(p);
+    if (b->h_stage) (void)hipHostFree(b->h_stage);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -844,8 +956,12 @@ hp_status hp_batch_theta_count(const hp_batch* b, uint32_t member, uint32_t* cou
     return HP_OK;
 }
 
-hp_status hp_batch_tape_forward(hp_batch* b, uint64_t steps, uint32_t checkpoint_every) {
-    return guarded(b, [&]() -> hp_status { return tape_forward(b, steps, checkpoint_every); });
+hp_status hp_batch_tape_forward(hp_batch* b, uint64_t steps, uint32_t checkpoint_every, uint32_t* used_every) {
+    return guarded(b, [&]() -> hp_status {
+        hp_status st = tape_forward(b, steps, checkpoint_every);
+        if (st == HP_OK && used_every) *used_every = b->tape_k;
+        return st;
+    });
 }
 
 hp_status hp_batch_tape_backward(hp_batch* b, const float* const* dl_dfinal, float* const* dl_dinitial,

@@ -65,7 +65,7 @@ struct MemberDesc {
     // gradient tape (null unless the member carries one)
     const int2* kmap;            // per kernel (source channel, target channel)
     float* lam[2];               // dL/dA ping-pong, C*N each
-    float* conv;                 // K*N convolution outputs of the recomputed step
+    float* conv;                 // convolution outputs, K*N per slot (one slot, or one per taped step)
     float* gscr;                 // (2C+3)*N: dL/d(grad U_c) x, y per channel; dL/d(grad A_sum) x, y;
                                  //   dL/dA_sum through alpha
     float* lamU;                 // C*N: dL/dU
@@ -75,6 +75,7 @@ struct MemberDesc {
     double* dkwin;               // per kernel (2R+1)^2 FP64 derivatives of the kernel taps
     const int2* dkw;             // per kernel (offset of its window in dkwin, radius R)
     float* snap;                 // C*N snapshot taken before a multi-step advance
+    long long stage_off;         // this member's offset in the contiguous host-transfer staging
 };
 
 // ---- launchers implemented in fft_kernels.hip -------------------------------------
@@ -91,9 +92,10 @@ bool launch_rows_inv(int L, hipStream_t st, const float2* Y, float* U, const Tgt
 // Batched 2D passes over a size class: items[i] = (member, block, slot/work, -).
 bool launch_batch_rows_fwd(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cur);
 bool launch_batch_spectral(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems);
-// conv: also store each kernel's convolution (MemberDesc::conv), for the adjoint.
+// conv: also store each kernel's convolution into MemberDesc::conv slot cslot (K*N each),
+// for the adjoint.
 bool launch_batch_rows_inv(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems,
-                           bool conv = false);
+                           bool conv = false, int cslot = 0);
 int spectral_block_lines(int L);  // columns per spectral workgroup for length-L columns
 
 // ---- launchers implemented in wave_fft.hip (2D, sides 256..2048, column-major spectra) ----
@@ -154,10 +156,16 @@ int adj_tile_h();
 void launch_tape_copy(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cur, int slot,
                       bool to_tape);
 void launch_tape_clear(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems);
+// items as launch_tape_copy.  to_stage: stage[stage_off + j] <- lam[li][j], else the reverse.
+void launch_stage_lam(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, float* stage, int li,
+                      bool to_stage);
+// items as launch_tape_copy.  U of a step from its stored convolutions (conv slot cslot):
+// the forward's growth accumulate, same f32 operations in the same kernel order.
+void launch_u_from_conv(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cslot);
 // ---- gradient tape: adjoint_fft.hip ---------------------------------------------------------
 bool adj_fft_supported(int L);
-// items = (member, row pair, kernel)
-bool launch_adj_rows_fwd(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems);
+// items = (member, row pair, kernel); reads conv slot cslot
+bool launch_adj_rows_fwd(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cslot);
 // items = (member, column block of adj_spectral_block_lines(L) columns, source work item)
 bool launch_adj_spectral(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems);
 int adj_spectral_block_lines(int L);

@@ -121,11 +121,13 @@ class HipPopulation:
         self._check(self._lib.hp_batch_theta_count(self._ptr, member, ctypes.byref(n)), "hp_batch_theta_count")
         return n.value
 
-    def tape_forward(self, steps: int, checkpoint_every: int) -> None:
+    def tape_forward(self, steps: int, checkpoint_every: int = 0) -> int:
         """Advance every member `steps` steps, keeping a state every `checkpoint_every` steps
-        for tape_backward."""
-        self._check(self._lib.hp_batch_tape_forward(self._ptr, int(steps), int(checkpoint_every)),
-                    "hp_batch_tape_forward")
+        for tape_backward (0: chosen from free device memory).  Returns the interval used."""
+        used = ctypes.c_uint32(0)
+        self._check(self._lib.hp_batch_tape_forward(self._ptr, int(steps), int(checkpoint_every),
+                                                    ctypes.byref(used)), "hp_batch_tape_forward")
+        return used.value
 
     def tape_backward(self, dl_dfinal: Sequence[Sequence[np.ndarray]]):
         """For L = sum over members and channels of <dl_dfinal, A_T>: per member, the list of

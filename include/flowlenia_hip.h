@@ -280,9 +280,12 @@ hp_status hp_batch_stats(const hp_batch* b, uint32_t* max_gather_radius, uint64_
  * (weight, mu, sigma), then per ring in config order (amplitude, distance, width).
  * Pointer arrays are member-major: member 0's channels, then member 1's, ...; dtheta holds
  * the members' theta blocks back to back.  The tape is dropped by set_state / advance.
+ * checkpoint_every = 0 picks the interval from free device memory: every step (no
+ * recomputation) when the whole trajectory fits in half of it, else about sqrt(steps);
+ * *used_every (optional) receives the interval used.
  * Limits: sides 2^a * 3^b up to 1024; every kernel's support 2R+1 within the grid. */
 hp_status hp_batch_theta_count(const hp_batch* b, uint32_t member, uint32_t* count);
-hp_status hp_batch_tape_forward(hp_batch* b, uint64_t steps, uint32_t checkpoint_every);
+hp_status hp_batch_tape_forward(hp_batch* b, uint64_t steps, uint32_t checkpoint_every, uint32_t* used_every);
 hp_status hp_batch_tape_backward(hp_batch* b, const float* const* dl_dfinal, float* const* dl_dinitial,
                                  double* dtheta);
 
