@@ -74,6 +74,22 @@ class HpCustomValue(ctypes.Structure):
     _fields_ = [("x", c_uint32), ("y", c_uint32), ("z", c_uint32), ("channel", c_uint32), ("value", c_float)]
 
 
+class HpPlan(ctypes.Structure):
+    """hp_plan: how the step runs on the device (see include/flowlenia_hip.h)."""
+    _fields_ = [("fft_path", c_int32), ("spectral_group", c_int32), ("radix_order", c_int32),
+                ("column_split", c_int32), ("stencil_tile", c_int32)]
+
+    def as_dict(self) -> dict:
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+    @classmethod
+    def from_dict(cls, d: dict) -> "HpPlan":
+        p = cls(0, -1, 0, 1, 0)
+        for k, v in d.items():
+            setattr(p, k, int(v))
+        return p
+
+
 class HpSeed(ctypes.Structure):
     _fields_ = [("kind", c_int32), ("center", c_float * 3), ("radius", c_float),
                 ("inner_radius", c_float), ("outer_radius", c_float),
@@ -121,6 +137,14 @@ EXPORTS = [
     ("hp_batch_centroids", c_int, [c_void_p, POINTER(c_double), POINTER(c_double)]),
     ("hp_batch_mass_f64", c_int, [c_void_p, POINTER(c_double)]),
     ("hp_batch_stats", c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint64)]),
+    ("hp_get_plan", c_int, [c_void_p, POINTER(HpPlan)]),
+    ("hp_set_plan", c_int, [c_void_p, POINTER(HpPlan)]),
+    ("hp_autotune", c_int, [c_void_p, c_uint32, POINTER(HpPlan), c_uint32, POINTER(HpPlan), POINTER(c_double),
+                            POINTER(c_uint32)]),
+    ("hp_batch_get_plan", c_int, [c_void_p, POINTER(HpPlan)]),
+    ("hp_batch_set_plan", c_int, [c_void_p, POINTER(HpPlan)]),
+    ("hp_batch_autotune", c_int, [c_void_p, c_uint32, POINTER(HpPlan), c_uint32, POINTER(HpPlan),
+                                  POINTER(c_double), POINTER(c_uint32)]),
     ("hp_batch_theta_count", c_int, [c_void_p, c_uint32, POINTER(c_uint32)]),
     ("hp_batch_tape_forward", c_int, [c_void_p, c_uint64, c_uint32, POINTER(c_uint32)]),
     ("hp_batch_tape_backward", c_int, [c_void_p, FloatPP, FloatPP, POINTER(c_double)]),
@@ -166,3 +190,16 @@ def float_ptr_array(arrays) -> ctypes.Array:
         assert a.dtype.name == "float32" and a.flags["C_CONTIGUOUS"], "float32 C-contiguous required"
         arr[i] = a.ctypes.data_as(FP)
     return arr
+
+
+def autotune_call(fn, ptr, steps: int):
+    """Shared driver of hp_autotune / hp_batch_autotune: (status, best plan dict, [(plan dict,
+    microseconds per step or None if the plan disagreed with the first)])."""
+    cap = 64
+    best = HpPlan()
+    tried = (HpPlan * cap)()
+    us = (c_double * cap)()
+    n = c_uint32(0)
+    st = fn(ptr, int(steps), ctypes.byref(best), cap, tried, us, ctypes.byref(n))
+    table = [(tried[i].as_dict(), us[i] if us[i] >= 0 else None) for i in range(min(n.value, cap))]
+    return st, best.as_dict(), table

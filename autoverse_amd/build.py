@@ -31,6 +31,10 @@ HOST_FLAGS = [f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-fPIC", "-ffp-cont
 
 SOURCES = [
     ("fft_kernels.hip", DEVICE_FLAGS + ["-ffp-contract=fast"]),
+    ("fft_g0.hip", DEVICE_FLAGS + ["-ffp-contract=fast"]),
+    ("fft_g1.hip", DEVICE_FLAGS + ["-ffp-contract=fast"]),
+    ("fft_g2.hip", DEVICE_FLAGS + ["-ffp-contract=fast"]),
+    ("fft_g3.hip", DEVICE_FLAGS + ["-ffp-contract=fast"]),
     ("wave_fft.hip", DEVICE_FLAGS + ["-ffp-contract=fast"]),
     ("stencil_kernels.hip", DEVICE_FLAGS + ["-ffp-contract=off"]),
     ("adjoint_stencil.hip", DEVICE_FLAGS + ["-ffp-contract=off"]),

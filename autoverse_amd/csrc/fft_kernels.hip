@@ -1,293 +1,9 @@
-// fft_kernels.hip -- LDS-staged Stockham FFT passes of the Flow Lenia convolution on gfx950.
-//
-// The reference convolves every kernel with a full complex 2D/3D FFT of the
-// source channel (CachedConvolver::convolve_with_kernel_scratch, fft.rs:288-324;
-// fft3d.rs:308-332) and then applies growth_accumulate per kernel in kernel order
-// (propagator.rs:270-279, growth.rs:44-59).  Here the same mathematics runs as
-// three memory passes over a half spectrum (the input is real):
-//   rows_fwd  : two real rows packed as one complex line, FFT along x, untangled
-//               into two Hermitian half rows (R2C);
-//   spectral  : FFT along the last axis (y in 2D, z in 3D), product with the
-//               kernel spectrum -- real and even in every axis because the ring
-//               kernel is point-symmetric (kernel.rs:41-64), so stored folded as
-//               (D/2+1)(H/2+1)(W/2+1) floats -- and the inverse FFT along the same
-//               axis, written per kernel;
-//   rows_inv  : Hermitian completion of two half rows, inverse FFT along x,
-//               Re * 1/(W*H*D) (fft.rs:389-392), growth G(u) and the weighted
-//               accumulate of every kernel that targets the channel, in kernel
-//               index order, all in registers; one write of the affinity U.
-// 3D adds a forward y pass after rows_fwd and an inverse y pass before rows_inv.
-//
-// Every transform is an in-place Stockham autosort FFT in LDS: each stage loads
-// its radix-R butterfly inputs into registers, synchronises, and stores the
-// outputs; radices 8/4/2/3 cover lengths 2^a * 3^b.  Twiddles come from a table
-// exp(-2 pi i m / L) (host FP64, rounded to f32) staged in LDS.
-#include "fft_lds.hpp"
+// fft_kernels.hip -- public launchers of the LDS FFT passes: each length goes to the
+// translation unit (fft_g*.hip) that instantiates its kernels; and the adjoint's rebuild of
+// U from stored convolutions, which shares growth_term with the inverse row pass.
+#include "fft_passes.hpp"
 
 namespace hp {
-
-// ---- pass 1: R2C along x on pairs of real rows ----------------------------------------
-// grid (ceil(rows/2), nsrc); rows = H*D; src_chan[s] = channel of source slot s.
-template <int L, int NT>
-__device__ __forceinline__ void rows_fwd_body(const float* __restrict__ A, float2* __restrict__ X,
-                                              const int* __restrict__ src_chan, int rows, int Wp, long long N,
-                                              long long spec, const float2* __restrict__ twg, int pair, int s) {
-    __shared__ float2 buf[L];
-    __shared__ float2 tw[L];
-    const int y0 = 2 * pair;
-    const bool has1 = (y0 + 1) < rows;
-    const float* a = A + (long long)src_chan[s] * N + (long long)y0 * L;
-    stage_twiddles<NT>(tw, twg, L);
-    for (int i = threadIdx.x; i < L; i += NT) buf[i] = make_float2(a[i], has1 ? a[L + i] : 0.0f);
-    __syncthreads();
-    lds_fft<L, -1, 1, NT, L>(buf, tw);
-    float2* x0 = X + (long long)s * spec + (long long)y0 * Wp;
-    float2* x1 = x0 + Wp;
-    for (int k = threadIdx.x; k <= L / 2; k += NT) {
-        const float2 zk = buf[k];
-        const float2 zm = buf[k == 0 ? 0 : L - k];
-        x0[k] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-        if (has1) x1[k] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
-    }
-}
-
-template <int L, int NT>
-__global__ __launch_bounds__(NT) void k_rows_fwd(const float* __restrict__ A, float2* __restrict__ X,
-                                                 const int* __restrict__ src_chan, int rows, int Wp,
-                                                 long long N, long long spec, const float2* __restrict__ twg) {
-    rows_fwd_body<L, NT>(A, X, src_chan, rows, Wp, N, spec, twg, blockIdx.x, blockIdx.y);
-}
-
-// batched: item = (member, row pair, source slot)
-template <int L, int NT>
-__global__ __launch_bounds__(NT) void kb_rows_fwd(const MemberDesc* __restrict__ mem, const int4* __restrict__ items,
-                                                  int cur) {
-    const int4 it = items[blockIdx.x];
-    const MemberDesc& m = mem[it.x];
-    rows_fwd_body<L, NT>(m.A[cur], m.X, m.src_chan, m.g.H * m.g.D, m.g.Wp, m.g.N, m.g.spec, m.tw_w, it.y, it.z);
-}
-
-// ---- 3D only: FFT along y, in place, forward or inverse ------------------------------------
-// grid (ceil(Wh/CB), D, nspec).  Lines are the kx columns of one z plane.
-template <int L, int CB, int NT, int DIR>
-__global__ __launch_bounds__(NT) void k_cols(float2* __restrict__ X, int Wh, int Wp, long long spec,
-                                             const float2* __restrict__ twg) {
-    constexpr int LP = L + 1;
-    __shared__ float2 buf[CB * LP];
-    __shared__ float2 tw[L];
-    const int kx0 = blockIdx.x * CB;
-    float2* base = X + (long long)blockIdx.z * spec + (long long)blockIdx.y * L * Wp;
-    stage_twiddles<NT>(tw, twg, L);
-    for (int i = threadIdx.x; i < CB * L; i += NT) {
-        const int c = i % CB, e = i / CB;
-        const int kx = kx0 + c;
-        buf[c * LP + e] = kx < Wh ? base[(long long)e * Wp + kx] : make_float2(0.f, 0.f);
-    }
-    __syncthreads();
-    lds_fft<L, DIR, CB, NT, LP>(buf, tw);
-    for (int i = threadIdx.x; i < CB * L; i += NT) {
-        const int c = i % CB, e = i / CB;
-        const int kx = kx0 + c;
-        if (kx < Wh) base[(long long)e * Wp + kx] = buf[c * LP + e];
-    }
-}
-
-// ---- pass 2: FFT along the last axis, x kernel spectrum, inverse FFT ----------------------
-// DIM 2: lines are kx (element stride Wp, L = H).
-// DIM 3: lines are ky*Wp + kx (element stride H*Wp, L = D).
-// grid (ceil(lines/CB), nwork); work[w] names the source slot and its kernels.
-template <int L, int CB, int NT, int DIM>
-__device__ __forceinline__ void spectral_body(const float2* __restrict__ X, float2* __restrict__ Y,
-                                              const float* __restrict__ Kq, const SpecWork* __restrict__ work,
-                                              const int* __restrict__ klist, int Wh, int Wp, int H, int Hq,
-                                              long long spec, long long kq_stride, const float2* __restrict__ twg,
-                                              int blk, int wi) {
-    constexpr int LP = L + 1;
-    constexpr int PER = CB * L / NT;
-    static_assert((CB * L) % NT == 0, "tile must split evenly over the threads");
-    __shared__ float2 buf[CB * LP];
-    __shared__ float2 tw[L];
-    const SpecWork wk = work[wi];
-    const long long nlines = DIM == 2 ? (long long)Wh : (long long)H * Wp;
-    const long long es = DIM == 2 ? (long long)Wp : (long long)H * Wp;
-    const long long line0 = (long long)blk * CB;
-    const float2* xs = X + (long long)wk.src_slot * spec;
-    stage_twiddles<NT>(tw, twg, L);
-    float2 reg[PER];
-    int koff[PER];  // offset of this element's folded kernel-spectrum value, or -1 if the line is padding
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int i = threadIdx.x + q * NT;
-        const int c = i % CB, e = i / CB;
-        const long long line = line0 + c;
-        int kx, ky;
-        if (DIM == 2) {
-            kx = (int)line;
-            ky = e;
-        } else {
-            ky = (int)(line / Wp);
-            kx = (int)(line - (long long)ky * Wp);
-        }
-        const bool valid = line < nlines && kx < Wh;
-        const float2 v = valid ? xs[(long long)e * es + line] : make_float2(0.f, 0.f);
-        buf[c * LP + e] = v;
-        if (DIM == 2) {
-            const int kyf = ky <= L / 2 ? ky : L - ky;
-            koff[q] = valid ? kyf * Wh + kx : -1;
-        } else {
-            const int kyf = ky <= H / 2 ? ky : H - ky;
-            const int kzf = e <= L / 2 ? e : L - e;
-            koff[q] = valid ? (kzf * Hq + kyf) * Wh + kx : -1;
-        }
-    }
-    __syncthreads();
-    lds_fft<L, -1, CB, NT, LP>(buf, tw);
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int i = threadIdx.x + q * NT;
-        reg[q] = buf[(i % CB) * LP + i / CB];
-    }
-    for (int j = 0; j < wk.k_count; ++j) {
-        const int k = klist[wk.k_first + j];
-        const float* kq = Kq + (long long)k * kq_stride;
-        __syncthreads();  // previous kernel's outputs have been read
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int i = threadIdx.x + q * NT;
-            const float kv = koff[q] >= 0 ? kq[koff[q]] : 0.0f;
-            buf[(i % CB) * LP + i / CB] = make_float2(reg[q].x * kv, reg[q].y * kv);
-        }
-        __syncthreads();
-        lds_fft<L, +1, CB, NT, LP>(buf, tw);
-        float2* ys = Y + (long long)k * spec;
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int i = threadIdx.x + q * NT;
-            const int c = i % CB, e = i / CB;
-            if (koff[q] >= 0) ys[(long long)e * es + line0 + c] = buf[c * LP + e];
-        }
-    }
-}
-
-template <int L, int CB, int NT, int DIM>
-__global__ __launch_bounds__(NT) void k_spectral(const float2* __restrict__ X, float2* __restrict__ Y,
-                                                 const float* __restrict__ Kq, const SpecWork* __restrict__ work,
-                                                 const int* __restrict__ klist, int Wh, int Wp, int H, int Hq,
-                                                 long long spec, long long kq_stride,
-                                                 const float2* __restrict__ twg) {
-    spectral_body<L, CB, NT, DIM>(X, Y, Kq, work, klist, Wh, Wp, H, Hq, spec, kq_stride, twg, blockIdx.x, blockIdx.y);
-}
-
-// batched (2D): item = (member, column block, work item)
-template <int L, int CB, int NT>
-__global__ __launch_bounds__(NT) void kb_spectral(const MemberDesc* __restrict__ mem, const int4* __restrict__ items) {
-    const int4 it = items[blockIdx.x];
-    const MemberDesc& m = mem[it.x];
-    spectral_body<L, CB, NT, 2>(m.X, m.Y, m.Kq, m.spec_work, m.klist_spec, m.g.Wh, m.g.Wp, m.g.H, m.g.Hq, m.g.spec,
-                                (long long)m.g.Dq * m.g.Hq * m.g.Wh, m.tw_h, it.y, it.z);
-}
-
-// ---- pass 3: C2R along x on pairs of rows + growth + weighted accumulate -----------------
-// grid (ceil(rows/2), nwork); work[w] is a target channel and its kernels in index order.
-// One kernel's growth term w * G(c) (growth_accumulate, growth.rs:51-57: g = 2 exp(-d*d*inv) - 1,
-// t += w*g).  The inverse row pass and the adjoint's rebuild of U both call this one function
-// in this one translation unit, so the two produce bit-identical U.
-__device__ __forceinline__ float growth_term(float c, const GrowthParams& p) {
-#pragma clang fp contract(off)
-    const float d = c - p.mu;
-    const float g = 2.0f * expf(-d * d * p.inv) - 1.0f;
-    return p.weight * g;
-}
-
-// CONV: also store every kernel's convolution rows (conv + k*N), for the adjoint's recompute.
-template <int L, int NT, bool CONV = false>
-__device__ __forceinline__ void rows_inv_body(const float2* __restrict__ Y, float* __restrict__ U,
-                                              const TgtWork* __restrict__ work, const int* __restrict__ klist,
-                                              const GrowthParams* __restrict__ gp, int rows, int Wp, long long N,
-                                              long long spec, float scale, const float2* __restrict__ twg, int pair,
-                                              int wi, float* __restrict__ conv = nullptr) {
-    constexpr int PER = (L + NT - 1) / NT;
-    __shared__ float2 buf[L];
-    __shared__ float2 tw[L];
-    const TgtWork wk = work[wi];
-    const int y0 = 2 * pair;
-    const bool has1 = (y0 + 1) < rows;
-    stage_twiddles<NT>(tw, twg, L);
-    float acc0[PER], acc1[PER];
-#pragma unroll
-    for (int q = 0; q < PER; ++q) acc0[q] = acc1[q] = 0.0f;
-    for (int j = 0; j < wk.k_count; ++j) {
-        const int k = klist[wk.k_first + j];
-        const float2* r0 = Y + (long long)k * spec + (long long)y0 * Wp;
-        const float2* r1 = r0 + Wp;
-        __syncthreads();  // previous kernel's results have been read
-        for (int n = threadIdx.x; n < L; n += NT) {
-            // Hermitian completion: X0[n] = conj(X0[L-n]) for n > L/2, likewise X1;
-            // the packed line is Z = X0 + i X1, whose inverse is x0 + i x1.
-            float2 a, b;
-            if (n <= L / 2) {
-                a = r0[n];
-                b = has1 ? r1[n] : make_float2(0.f, 0.f);
-                buf[n] = make_float2(a.x - b.y, a.y + b.x);
-            } else {
-                a = r0[L - n];
-                b = has1 ? r1[L - n] : make_float2(0.f, 0.f);
-                buf[n] = make_float2(a.x + b.y, b.x - a.y);
-            }
-        }
-        __syncthreads();
-        lds_fft<L, +1, 1, NT, L>(buf, tw);
-        const GrowthParams p = gp[k];
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int m = threadIdx.x + q * NT;
-            if (L % NT == 0 || m < L) {
-                const float2 z = buf[m];
-                {
-#pragma clang fp contract(off)
-                    // growth_accumulate (growth.rs:51-57): g = 2 exp(-d*d*inv) - 1; t += w*g
-                    const float c0 = z.x * scale, c1 = z.y * scale;
-                    if (CONV) {
-                        float* cv = conv + (long long)k * N + (long long)y0 * L + m;
-                        cv[0] = c0;
-                        if (has1) cv[L] = c1;
-                    }
-                    acc0[q] += growth_term(c0, p);
-                    acc1[q] += growth_term(c1, p);
-                }
-            }
-        }
-    }
-    float* u0 = U + (long long)wk.target * N + (long long)y0 * L;
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int m = threadIdx.x + q * NT;
-        if (L % NT == 0 || m < L) {
-            u0[m] = acc0[q];
-            if (has1) u0[L + m] = acc1[q];
-        }
-    }
-}
-
-template <int L, int NT>
-__global__ __launch_bounds__(NT) void k_rows_inv(const float2* __restrict__ Y, float* __restrict__ U,
-                                                 const TgtWork* __restrict__ work, const int* __restrict__ klist,
-                                                 const GrowthParams* __restrict__ gp, int rows, int Wp,
-                                                 long long N, long long spec, float scale,
-                                                 const float2* __restrict__ twg) {
-    rows_inv_body<L, NT>(Y, U, work, klist, gp, rows, Wp, N, spec, scale, twg, blockIdx.x, blockIdx.y);
-}
-
-// batched: item = (member, row pair, target work item); CONV: convolutions into conv slot cslot
-template <int L, int NT, bool CONV>
-__global__ __launch_bounds__(NT) void kb_rows_inv(const MemberDesc* __restrict__ mem, const int4* __restrict__ items,
-                                                  int cslot) {
-    const int4 it = items[blockIdx.x];
-    const MemberDesc& m = mem[it.x];
-    rows_inv_body<L, NT, CONV>(m.Y, m.U, m.tgt_work, m.klist_tgt, m.gp, m.g.H * m.g.D, m.g.Wp, m.g.N, m.g.spec,
-                               1.0f / (float)m.g.N, m.tw_w, it.y, it.z,
-                               CONV ? m.conv + (long long)cslot * m.K * m.g.N : nullptr);
-}
 
 // U of a taped step from its stored convolutions (conv slot cslot): per target channel the
 // kernels in index order, acc from 0 -- the inverse row pass's epilogue on the same values.
@@ -315,125 +31,86 @@ void launch_u_from_conv(hipStream_t st, const MemberDesc* mem, const int4* items
     hipLaunchKernelGGL(kb_u_from_conv, dim3(nitems), dim3(256), 0, st, mem, items, cslot);
 }
 
-// ---- host-side dispatch ------------------------------------------------------------------
-namespace {
-template <int L>
-void rows_fwd(hipStream_t st, const float* A, float2* X, const int* src_chan, int nsrc, const Geo& g,
-              const float2* tw) {
-    const int rows = g.H * g.D;
-    dim3 grid((rows + 1) / 2, nsrc);
-    hipLaunchKernelGGL((k_rows_fwd<L, Cfg<L>::ROW_NT>), grid, dim3(Cfg<L>::ROW_NT), 0, st, A, X, src_chan, rows,
-                       g.Wp, g.N, g.spec, tw);
-}
-
-template <int L>
-void cols(hipStream_t st, int dir, float2* X, int nspec, const Geo& g, const float2* tw) {
-    constexpr int CB = Cfg<L>::CB, NT = Cfg<L>::COL_NT;
-    dim3 grid((g.Wh + CB - 1) / CB, g.D, nspec);
-    if (dir < 0)
-        hipLaunchKernelGGL((k_cols<L, CB, NT, -1>), grid, dim3(NT), 0, st, X, g.Wh, g.Wp, g.spec, tw);
-    else
-        hipLaunchKernelGGL((k_cols<L, CB, NT, 1>), grid, dim3(NT), 0, st, X, g.Wh, g.Wp, g.spec, tw);
-}
-
-template <int L>
-void spectral(int dim, hipStream_t st, const float2* X, float2* Y, const float* Kq, const SpecWork* work,
-              int nwork, const int* klist, const Geo& g, const float2* tw) {
-    constexpr int CB = Cfg<L>::CB, NT = Cfg<L>::COL_NT;
-    const long long kq_stride = (long long)g.Dq * g.Hq * g.Wh;
-    if (dim == 2) {
-        dim3 grid((g.Wh + CB - 1) / CB, nwork);
-        hipLaunchKernelGGL((k_spectral<L, CB, NT, 2>), grid, dim3(NT), 0, st, X, Y, Kq, work, klist, g.Wh, g.Wp,
-                           g.H, g.Hq, g.spec, kq_stride, tw);
-    } else {
-        const long long lines = (long long)g.H * g.Wp;
-        dim3 grid((unsigned)((lines + CB - 1) / CB), nwork);
-        hipLaunchKernelGGL((k_spectral<L, CB, NT, 3>), grid, dim3(NT), 0, st, X, Y, Kq, work, klist, g.Wh, g.Wp,
-                           g.H, g.Hq, g.spec, kq_stride, tw);
-    }
-}
-
-template <int L>
-void rows_inv(hipStream_t st, const float2* Y, float* U, const TgtWork* work, int nwork, const int* klist,
-              const GrowthParams* gp, const Geo& g, const float2* tw) {
-    const int rows = g.H * g.D;
-    dim3 grid((rows + 1) / 2, nwork);
-    const float scale = 1.0f / (float)(g.N);  // fft.rs:389, fft3d.rs:449: 1 / (W*H*D) as f32
-    hipLaunchKernelGGL((k_rows_inv<L, Cfg<L>::ROW_NT>), grid, dim3(Cfg<L>::ROW_NT), 0, st, Y, U, work, klist, gp,
-                       rows, g.Wp, g.N, g.spec, scale, tw);
-}
-}  // namespace
-
-#define HP_FFT_SIZES(X) \
-    X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(24) X(48) X(96) X(192) X(384) X(768) X(1536)
+#define HP_GROUPS(M) M(g0) M(g1) M(g2) M(g3)
+#define HP_DECL(G)                                                                                               \
+    bool launch_rows_fwd_##G(int, hipStream_t, const float*, float2*, const int*, int, const Geo&, const float2*,  \
+                             const FftVariant&);                                                                 \
+    bool launch_cols_##G(int, hipStream_t, int, float2*, int, const Geo&, const float2*, const FftVariant&);      \
+    bool launch_spectral_##G(int, int, hipStream_t, const float2*, float2*, const float*, const SpecWork*, int,   \
+                             const int*, const Geo&, const float2*, const FftVariant&);                          \
+    bool launch_rows_inv_##G(int, hipStream_t, const float2*, float*, const TgtWork*, int, const int*,            \
+                             const GrowthParams*, const Geo&, const float2*, const FftVariant&);                 \
+    bool launch_batch_rows_fwd_##G(int, hipStream_t, const MemberDesc*, const int4*, int, int, const FftVariant&); \
+    bool launch_batch_spectral_##G(int, hipStream_t, const MemberDesc*, const int4*, int, const FftVariant&);    \
+    bool launch_batch_rows_inv_##G(int, hipStream_t, const MemberDesc*, const int4*, int, bool, int,             \
+                                   const FftVariant&);
+HP_GROUPS(HP_DECL)
+#undef HP_DECL
 
 bool fft_supported(int L) {
 #define X_(n) if (L == n) return true;
-    HP_FFT_SIZES(X_)
+    HP_FFT_G0(X_) HP_FFT_G1(X_) HP_FFT_G2(X_) HP_FFT_G3(X_)
 #undef X_
     return false;
 }
+
+int spectral_block_lines(int L, const FftVariant& v) {
+#define X_(n) if (L == n) return v.cs == 2 ? Cfg<n, 2>::CB : Cfg<n, 1>::CB;
+    HP_FFT_G0(X_) HP_FFT_G1(X_) HP_FFT_G2(X_) HP_FFT_G3(X_)
+#undef X_
+    return 0;
+}
+
+#define HP_ANY(call) (call##_g0 ARGS || call##_g1 ARGS || call##_g2 ARGS || call##_g3 ARGS)
 
 bool launch_rows_fwd(int L, hipStream_t st, const float* A, float2* X, const int* src_chan, int nsrc, const Geo& g,
-                     const float2* tw) {
-#define X_(n) if (L == n) { rows_fwd<n>(st, A, X, src_chan, nsrc, g, tw); return true; }
-    HP_FFT_SIZES(X_)
-#undef X_
-    return false;
+                     const float2* tw, const FftVariant& v) {
+#define ARGS (L, st, A, X, src_chan, nsrc, g, tw, v)
+    return HP_ANY(launch_rows_fwd);
+#undef ARGS
 }
 
-bool launch_cols(int L, hipStream_t st, int dir, float2* X, int nspec, const Geo& g, const float2* tw) {
-#define X_(n) if (L == n) { cols<n>(st, dir, X, nspec, g, tw); return true; }
-    HP_FFT_SIZES(X_)
-#undef X_
-    return false;
+bool launch_cols(int L, hipStream_t st, int dir, float2* X, int nspec, const Geo& g, const float2* tw,
+                 const FftVariant& v) {
+#define ARGS (L, st, dir, X, nspec, g, tw, v)
+    return HP_ANY(launch_cols);
+#undef ARGS
 }
 
 bool launch_spectral(int L, int dim, hipStream_t st, const float2* X, float2* Y, const float* Kq,
-                     const SpecWork* work, int nwork, const int* klist, const Geo& g, const float2* tw) {
-#define X_(n) if (L == n) { spectral<n>(dim, st, X, Y, Kq, work, nwork, klist, g, tw); return true; }
-    HP_FFT_SIZES(X_)
-#undef X_
-    return false;
+                     const SpecWork* work, int nwork, const int* klist, const Geo& g, const float2* tw,
+                     const FftVariant& v) {
+#define ARGS (L, dim, st, X, Y, Kq, work, nwork, klist, g, tw, v)
+    return HP_ANY(launch_spectral);
+#undef ARGS
 }
 
 bool launch_rows_inv(int L, hipStream_t st, const float2* Y, float* U, const TgtWork* work, int nwork,
-                     const int* klist, const GrowthParams* gp, const Geo& g, const float2* tw) {
-#define X_(n) if (L == n) { rows_inv<n>(st, Y, U, work, nwork, klist, gp, g, tw); return true; }
-    HP_FFT_SIZES(X_)
-#undef X_
-    return false;
+                     const int* klist, const GrowthParams* gp, const Geo& g, const float2* tw, const FftVariant& v) {
+#define ARGS (L, st, Y, U, work, nwork, klist, gp, g, tw, v)
+    return HP_ANY(launch_rows_inv);
+#undef ARGS
 }
 
-// ---- batched launchers (2D generic path): one launch per pass for a size class ----------
-bool launch_batch_rows_fwd(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cur) {
-#define X_(n) if (L == n) { hipLaunchKernelGGL((kb_rows_fwd<n, Cfg<n>::ROW_NT>), dim3(nitems), dim3(Cfg<n>::ROW_NT), 0, st, mem, items, cur); return true; }
-    HP_FFT_SIZES(X_)
-#undef X_
-    return false;
+bool launch_batch_rows_fwd(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cur,
+                           const FftVariant& v) {
+#define ARGS (L, st, mem, items, nitems, cur, v)
+    return HP_ANY(launch_batch_rows_fwd);
+#undef ARGS
 }
 
-bool launch_batch_spectral(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems) {
-#define X_(n) if (L == n) { hipLaunchKernelGGL((kb_spectral<n, Cfg<n>::CB, Cfg<n>::COL_NT>), dim3(nitems), dim3(Cfg<n>::COL_NT), 0, st, mem, items); return true; }
-    HP_FFT_SIZES(X_)
-#undef X_
-    return false;
+bool launch_batch_spectral(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems,
+                           const FftVariant& v) {
+#define ARGS (L, st, mem, items, nitems, v)
+    return HP_ANY(launch_batch_spectral);
+#undef ARGS
 }
 
 bool launch_batch_rows_inv(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, bool conv,
-                           int cslot) {
-#define X_(n) if (L == n) { if (conv) hipLaunchKernelGGL((kb_rows_inv<n, Cfg<n>::ROW_NT, true>), dim3(nitems), dim3(Cfg<n>::ROW_NT), 0, st, mem, items, cslot); \
-                           else hipLaunchKernelGGL((kb_rows_inv<n, Cfg<n>::ROW_NT, false>), dim3(nitems), dim3(Cfg<n>::ROW_NT), 0, st, mem, items, 0); return true; }
-    HP_FFT_SIZES(X_)
-#undef X_
-    return false;
-}
-
-int spectral_block_lines(int L) {
-#define X_(n) if (L == n) return Cfg<n>::CB;
-    HP_FFT_SIZES(X_)
-#undef X_
-    return 0;
+                           int cslot, const FftVariant& v) {
+#define ARGS (L, st, mem, items, nitems, conv, cslot, v)
+    return HP_ANY(launch_batch_rows_inv);
+#undef ARGS
 }
 
 }  // namespace hp

@@ -11,25 +11,29 @@
 namespace hp {
 
 // ---- compile-time radix plan ------------------------------------------------------
-__host__ __device__ constexpr int pick_radix(int m) {
-    return (m % 8 == 0 && m != 16) ? 8 : (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2 : (m % 3 == 0) ? 3 : 1;
+// RP selects the radix order: 0 takes radix 8 first (then 4, 2, 3; 16 splits as 4 x 4),
+// 1 takes radix 4 first (more, lighter stages).  Which is faster depends on the length
+// and the occupancy of the pass; the plan autotuner measures both.
+__host__ __device__ constexpr int pick_radix(int m, int rp = 0) {
+    return rp == 0 ? ((m % 8 == 0 && m != 16) ? 8 : (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2 : (m % 3 == 0) ? 3 : 1)
+                   : ((m % 4 == 0) ? 4 : (m % 2 == 0) ? 2 : (m % 3 == 0) ? 3 : 1);
 }
-__host__ __device__ constexpr int num_stages(int L) {
+__host__ __device__ constexpr int num_stages(int L, int rp = 0) {
     int n = 0;
     while (L > 1) {
-        L /= pick_radix(L);
+        L /= pick_radix(L, rp);
         ++n;
     }
     return n;
 }
-__host__ __device__ constexpr int radix_at(int L, int s) {
-    for (int i = 0; i < s; ++i) L /= pick_radix(L);
-    return pick_radix(L);
+__host__ __device__ constexpr int radix_at(int L, int s, int rp = 0) {
+    for (int i = 0; i < s; ++i) L /= pick_radix(L, rp);
+    return pick_radix(L, rp);
 }
-__host__ __device__ constexpr int ns_at(int L, int s) {  // product of the radices before stage s
+__host__ __device__ constexpr int ns_at(int L, int s, int rp = 0) {  // product of the radices before stage s
     int ns = 1;
     for (int i = 0; i < s; ++i) {
-        int r = pick_radix(L);
+        int r = pick_radix(L, rp);
         ns *= r;
         L /= r;
     }
@@ -92,10 +96,10 @@ __device__ __forceinline__ void dft(float2* v) {
 }
 
 // One Stockham stage over NFFT transforms of length L at buf[f*LP + i], run by NT threads.
-template <int L, int STAGE, int DIR, int NFFT, int NT, int LP>
+template <int L, int STAGE, int DIR, int NFFT, int NT, int LP, int RP>
 __device__ __forceinline__ void fft_stage(float2* buf, const float2* tw) {
-    constexpr int R = radix_at(L, STAGE);
-    constexpr int NS = ns_at(L, STAGE);
+    constexpr int R = radix_at(L, STAGE, RP);
+    constexpr int NS = ns_at(L, STAGE, RP);
     constexpr int NB = L / R;
     constexpr int TOT = NFFT * NB;
     constexpr int Q = (TOT + NT - 1) / NT;
@@ -135,19 +139,19 @@ __device__ __forceinline__ void fft_stage(float2* buf, const float2* tw) {
     __syncthreads();
 }
 
-template <int L, int STAGE, int DIR, int NFFT, int NT, int LP>
+template <int L, int STAGE, int DIR, int NFFT, int NT, int LP, int RP>
 __device__ __forceinline__ void fft_stages(float2* buf, const float2* tw) {
-    if constexpr (STAGE < num_stages(L)) {
-        fft_stage<L, STAGE, DIR, NFFT, NT, LP>(buf, tw);
-        fft_stages<L, STAGE + 1, DIR, NFFT, NT, LP>(buf, tw);
+    if constexpr (STAGE < num_stages(L, RP)) {
+        fft_stage<L, STAGE, DIR, NFFT, NT, LP, RP>(buf, tw);
+        fft_stages<L, STAGE + 1, DIR, NFFT, NT, LP, RP>(buf, tw);
     }
 }
 
-// NFFT in-place transforms of length L (unnormalised), DIR = -1 forward, +1 inverse.
-// Caller synchronises before (inputs written); returns after a barrier.
-template <int L, int DIR, int NFFT, int NT, int LP>
+// NFFT in-place transforms of length L (unnormalised), DIR = -1 forward, +1 inverse, radix
+// order RP.  Caller synchronises before (inputs written); returns after a barrier.
+template <int L, int DIR, int NFFT, int NT, int LP, int RP = 0>
 __device__ __forceinline__ void lds_fft(float2* buf, const float2* tw) {
-    fft_stages<L, 0, DIR, NFFT, NT, LP>(buf, tw);
+    fft_stages<L, 0, DIR, NFFT, NT, LP, RP>(buf, tw);
 }
 
 template <int NT>
@@ -156,11 +160,13 @@ __device__ __forceinline__ void stage_twiddles(float2* tw, const float2* __restr
 }
 
 // Launch shapes per length: threads of a row pass, columns per spectral workgroup and
-// threads of a spectral workgroup.
-template <int L>
+// threads of a spectral workgroup.  CS splits the column block (1: up to 8192 elements per
+// workgroup; 2: half as many columns, less LDS per workgroup, more workgroups).
+template <int L, int CS = 1>
 struct Cfg {
     static constexpr int ROW_NT = L / 8 < 64 ? 64 : (L / 8 > 256 ? 256 : L / 8);
-    static constexpr int CB = 8192 / L < 1 ? 1 : (8192 / L > 64 ? 64 : 8192 / L);
+    static constexpr int CB0 = 8192 / L < 1 ? 1 : (8192 / L > 64 ? 64 : 8192 / L);
+    static constexpr int CB = CB0 / CS < 1 ? 1 : CB0 / CS;
     static constexpr int COL_NT = (CB * L) / 32 < 64 ? 64 : ((CB * L) / 32 > 256 ? 256 : (CB * L) / 32);
 };
 

@@ -82,6 +82,9 @@ struct hp_ctx {
     uint64_t prof_n[HP_K_NUM_CLASSES] = {};
     uint32_t max_radius = 0;
     uint64_t general_steps = 0;
+    hp_plan plan{};        // resolved execution plan
+    hp::FftVariant fv;     // LDS FFT variant of the plan
+    int tile = 0;          // fused stencil tile shape of the plan
     std::string err;
 
     hp_status fail(hp_status st, const std::string& msg) {
@@ -197,26 +200,26 @@ hp_status enqueue_affinity(hp_ctx* c, const float* A) {
     }
     {
         Timed t(c, HP_K_ROWS_FWD);
-        hp::launch_rows_fwd(g.W, c->stream, A, c->X, c->d_src_chan, (int)c->src_chan.size(), g, c->tw[g.W]);
+        hp::launch_rows_fwd(g.W, c->stream, A, c->X, c->d_src_chan, (int)c->src_chan.size(), g, c->tw[g.W], c->fv);
     }
     if (c->is3d) {
         Timed t(c, HP_K_COLS_FWD);
-        hp::launch_cols(g.H, c->stream, -1, c->X, (int)c->src_chan.size(), g, c->tw[g.H]);
+        hp::launch_cols(g.H, c->stream, -1, c->X, (int)c->src_chan.size(), g, c->tw[g.H], c->fv);
     }
     {
         Timed t(c, HP_K_SPECTRAL);
         const int L = c->is3d ? g.D : g.H;
         hp::launch_spectral(L, c->is3d ? 3 : 2, c->stream, c->X, c->Y, c->Kq, c->d_spec_work, c->n_spec_work,
-                            c->d_klist_spec, g, c->tw[L]);
+                            c->d_klist_spec, g, c->tw[L], c->fv);
     }
     if (c->is3d) {
         Timed t(c, HP_K_COLS_INV);
-        hp::launch_cols(g.H, c->stream, +1, c->Y, c->K, g, c->tw[g.H]);
+        hp::launch_cols(g.H, c->stream, +1, c->Y, c->K, g, c->tw[g.H], c->fv);
     }
     {
         Timed t(c, HP_K_ROWS_INV);
         hp::launch_rows_inv(g.W, c->stream, c->Y, c->U, c->d_tgt_work, c->n_tgt_work, c->d_klist_tgt, c->d_gp, g,
-                            c->tw[g.W]);
+                            c->tw[g.W], c->fv);
     }
     HIP_TRY(c, hipGetLastError());
     return HP_OK;
@@ -230,7 +233,7 @@ hp_status enqueue_fast_stencil(hp_ctx* c, const float* A, const float* U, float*
     {
         Timed t(c, HP_K_STENCIL);
         hp::launch_stencil2d(c->stream, A, U, out, c->C, c->g, c->cfg.dt, c->cfg.flow.beta_a, c->cfg.flow.n,
-                             c->cfg.flow.distribution_size, c->d_flags);
+                             c->cfg.flow.distribution_size, c->d_flags, c->tile);
     }
     HIP_TRY(c, hipGetLastError());
     return HP_OK;
@@ -411,7 +414,43 @@ void release(hp_ctx* c) {
     delete c;
 }
 
-hp_status create(const hp_config* cfg, int device, hp_ctx** out) {
+// Plans the autotuner found, per configuration shape (sides, channels, each kernel's source
+// and target, distribution size), for the contexts created later in this process.
+std::mutex g_plan_mu;
+std::map<std::string, hp_plan> g_plan_cache;
+
+std::string plan_key(const hp_config* cfg) {
+    std::string k = std::to_string(cfg->width) + "x" + std::to_string(cfg->height) + "x" +
+                    std::to_string(cfg->depth) + "c" + std::to_string(cfg->channels) + "s" +
+                    std::to_string(cfg->flow.distribution_size) + ":";
+    for (uint32_t i = 0; i < cfg->num_kernels; i++)
+        k += std::to_string(cfg->kernels[i].source_channel) + ">" + std::to_string(cfg->kernels[i].target_channel) + ",";
+    return k;
+}
+
+// The tuned plan of this shape if there is one, else the built-in defaults; then the
+// environment overrides, which always win: HP_SPECTRAL_GROUP (1 per source, 0 per kernel),
+// HP_FFT_PATH=generic (LDS FFTs even where the wave-level path applies).
+hp_plan default_plan(const hp_config* cfg) {
+    hp_plan p{};
+    p.fft_path = 0;
+    p.spectral_group = -1;
+    p.radix_order = 0;
+    p.column_split = 1;
+    p.stencil_tile = 0;
+    {
+        std::lock_guard<std::mutex> lk(g_plan_mu);
+        auto it = g_plan_cache.find(plan_key(cfg));
+        if (it != g_plan_cache.end()) p = it->second;
+    }
+    const char* grp = getenv("HP_SPECTRAL_GROUP");
+    const char* path = getenv("HP_FFT_PATH");
+    if (path && strcmp(path, "generic") == 0) p.fft_path = 1;
+    if (grp) p.spectral_group = atoi(grp);
+    return p;
+}
+
+hp_status create(const hp_config* cfg, int device, const hp_plan* req, hp_ctx** out) {
     *out = nullptr;
     hp_status st = hp::validate_config(cfg, nullptr, nullptr);
     if (st != HP_OK) {
@@ -427,14 +466,16 @@ hp_status create(const hp_config* cfg, int device, hp_ctx** out) {
         g_global_error = "device ordinal out of range";
         return HP_ERR_NO_DEVICE;
     }
-    // plan overrides: HP_SPECTRAL_GROUP (1 per source, 0 per kernel), HP_FFT_PATH=generic
-    // (row-major LDS FFTs even where the wave-level column-major path applies)
-    const char* grp = getenv("HP_SPECTRAL_GROUP");
-    const char* path = getenv("HP_FFT_PATH");
-    const bool allow_wave = !(path && strcmp(path, "generic") == 0);
+    const hp_plan want = req ? *req : default_plan(cfg);
+    if (want.fft_path < 0 || want.fft_path > 1 || want.spectral_group < -1 || want.spectral_group > 1 ||
+        want.radix_order < 0 || want.radix_order > 1 || want.column_split < 1 || want.column_split > 2 ||
+        want.stencil_tile < 0 || want.stencil_tile >= hp::stencil2d_tile_count()) {
+        g_global_error = "plan field out of range";
+        return HP_ERR_SHAPE;
+    }
     hp::MemberPlan plan;
     std::string perr;
-    st = hp::plan_member(cfg, allow_wave, grp ? atoi(grp) : -1, &plan, &perr);
+    st = hp::plan_member(cfg, want.fft_path == 0, want.spectral_group, &plan, &perr);
     if (st != HP_OK) {
         g_global_error = perr;
         return st;
@@ -450,6 +491,12 @@ hp_status create(const hp_config* cfg, int device, hp_ctx** out) {
     c->K = plan.K;
     c->is3d = plan.is3d;
     c->wave = plan.wave;
+    c->plan = want;
+    c->plan.fft_path = plan.wave ? 0 : 1;
+    c->plan.spectral_group = plan.grouped ? 1 : 0;
+    c->fv.rp = want.radix_order;
+    c->fv.cs = want.column_split;
+    c->tile = want.stencil_tile;
     c->g = plan.g;
     const hp::Geo& g = c->g;
     const bool is3d = c->is3d;
@@ -633,7 +680,7 @@ hp_status hp_create(const hp_config* cfg, int device, hp_ctx** out) {
     *out = nullptr;
     if (!cfg) return HP_ERR_NULL;
     try {
-        return create(cfg, device, out);
+        return create(cfg, device, nullptr, out);
     } catch (const std::exception& e) {
         g_global_error = e.what();
         return HP_ERR_HIP;
@@ -743,6 +790,161 @@ hp_status hp_stats(const hp_ctx* ctx, uint32_t* max_gather_radius, uint64_t* gen
     if (max_gather_radius) *max_gather_radius = ctx->max_radius;
     if (general_path_steps) *general_path_steps = ctx->general_steps;
     return HP_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// A new device plan for c, keeping its state, time, step and counters.
+hp_status set_plan(hp_ctx* c, const hp_plan& p) {
+    hp_ctx* n = nullptr;
+    hp_status st = create(&c->cfg, c->device, &p, &n);
+    if (st != HP_OK) return c->fail(st, g_global_error);
+    const size_t bytes = (size_t)c->C * c->g.N * sizeof(float);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(n->A[0], c->A[c->cur], bytes, hipMemcpyDeviceToDevice);
+    if (e != hipSuccess) {
+        release(n);
+        return c->fail(HP_ERR_HIP, std::string("plan change: ") + hipGetErrorString(e));
+    }
+    n->cur = 0;
+    n->time = c->time;
+    n->step = c->step;
+    n->max_radius = c->max_radius;
+    n->general_steps = c->general_steps;
+    n->profiling = c->profiling;
+    std::swap(*c, *n);
+    release(n);  // the old device plan
+    return HP_OK;
+}
+
+bool same_fft(const hp_plan& a, const hp_plan& b) {
+    return a.fft_path == b.fft_path && a.spectral_group == b.spectral_group && a.radix_order == b.radix_order &&
+           a.column_split == b.column_split;
+}
+
+// Coordinate search: every FFT configuration with the current tile, then every tile with the
+// fastest FFT configuration (the two are separate kernels, so their costs add).
+hp_status autotune(hp_ctx* c, uint32_t steps, hp_plan* best_out, uint32_t maxr, hp_plan* tried, double* us_out,
+                   uint32_t* ntried) {
+    if (steps == 0) steps = 20;
+    const size_t CN = (size_t)c->C * c->g.N;
+    std::vector<float> snap(CN), ref, got(CN);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(snap.data(), c->A[c->cur], CN * sizeof(float), hipMemcpyDeviceToHost));
+    const float t0 = c->time;
+    const uint64_t s0 = c->step, g0 = c->general_steps;
+    const uint32_t r0 = c->max_radius;
+    const hp_plan start = c->plan;
+    hipEvent_t ea = nullptr, eb = nullptr;
+    HIP_TRY(c, hipEventCreate(&ea));
+    HIP_TRY(c, hipEventCreate(&eb));
+    uint32_t n = 0;
+    double best_us = INFINITY;
+    hp_plan best = start;
+    auto restore = [&]() -> hp_status {
+        HIP_TRY(c, hipMemcpy(c->A[c->cur], snap.data(), CN * sizeof(float), hipMemcpyHostToDevice));
+        return HP_OK;
+    };
+    auto measure = [&](const hp_plan& p) -> hp_status {
+        hp_status st = set_plan(c, p);
+        if (st != HP_OK) return HP_OK;  // a plan this configuration cannot take is skipped
+        st = restore();
+        if (st == HP_OK) st = advance(c, 2);  // graph capture and first touches
+        if (st == HP_OK) st = restore();
+        if (st != HP_OK) return st;
+        HIP_TRY(c, hipEventRecord(ea, c->stream));
+        st = advance(c, steps);
+        if (st != HP_OK) return st;
+        HIP_TRY(c, hipEventRecord(eb, c->stream));
+        HIP_TRY(c, hipEventSynchronize(eb));
+        float ms = 0.0f;
+        HIP_TRY(c, hipEventElapsedTime(&ms, ea, eb));
+        HIP_TRY(c, hipMemcpy(got.data(), c->A[c->cur], CN * sizeof(float), hipMemcpyDeviceToHost));
+        bool ok = true;
+        if (ref.empty()) {
+            ref = got;
+        } else {
+            for (size_t i = 0; i < CN && ok; i++) ok = fabsf(got[i] - ref[i]) <= 1e-3f;
+        }
+        const double us = 1e3 * ms / steps;
+        if (n < maxr) {
+            if (tried) tried[n] = c->plan;
+            if (us_out) us_out[n] = ok ? us : -us;  // negative: disagreed with the first plan
+        }
+        n++;
+        if (ok && us < best_us) {
+            best_us = us;
+            best = c->plan;
+        }
+        return HP_OK;
+    };
+    hp_status st = HP_OK;
+    std::vector<hp_plan> ffts{start};  // the current plan first: the reference result
+    for (int path = 0; path <= 1; path++)
+        for (int grp = 1; grp >= 0; grp--)
+            for (int rp = 0; rp <= (path == 1 ? 1 : 0); rp++)
+                for (int cs = 1; cs <= (path == 1 ? 2 : 1); cs++) {
+                    hp_plan p = start;
+                    p.fft_path = path;
+                    p.spectral_group = grp;
+                    p.radix_order = path == 1 ? rp : start.radix_order;
+                    p.column_split = path == 1 ? cs : start.column_split;
+                    if (path == 0 && (c->is3d || !hp::wave_fft_supported(c->g.W, c->g.H))) continue;
+                    if (!same_fft(p, start)) ffts.push_back(p);
+                }
+    for (const hp_plan& p : ffts)
+        if (st == HP_OK) st = measure(p);
+    if (st == HP_OK && fast_stencil_ok(c)) {
+        const hp_plan fft_best = best;
+        for (int t = 0; t < hp::stencil2d_tile_count() && st == HP_OK; t++) {
+            if (t == fft_best.stencil_tile) continue;
+            hp_plan p = fft_best;
+            p.stencil_tile = t;
+            st = measure(p);
+        }
+    }
+    (void)hipEventDestroy(ea);
+    (void)hipEventDestroy(eb);
+    if (st != HP_OK) return st;
+    st = set_plan(c, best);
+    if (st != HP_OK) return st;
+    st = restore();
+    if (st != HP_OK) return st;
+    c->time = t0;
+    c->step = s0;
+    c->general_steps = g0;
+    c->max_radius = r0;
+    {
+        std::lock_guard<std::mutex> lk(g_plan_mu);
+        g_plan_cache[plan_key(&c->cfg)] = best;
+    }
+    if (best_out) *best_out = best;
+    if (ntried) *ntried = n;
+    return HP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+hp_status hp_get_plan(const hp_ctx* ctx, hp_plan* out) {
+    if (!ctx || !out) return HP_ERR_NULL;
+    *out = ctx->plan;
+    return HP_OK;
+}
+
+hp_status hp_set_plan(hp_ctx* ctx, const hp_plan* plan) {
+    if (!plan) return HP_ERR_NULL;
+    return guarded(ctx, [&]() -> hp_status { return set_plan(ctx, *plan); });
+}
+
+hp_status hp_autotune(hp_ctx* ctx, uint32_t steps, hp_plan* best, uint32_t max_results, hp_plan* tried,
+                      double* us_per_step, uint32_t* n_tried) {
+    return guarded(ctx, [&]() -> hp_status {
+        return autotune(ctx, steps, best, max_results, tried, us_per_step, n_tried);
+    });
 }
 
 hp_status hp_profile_enable(hp_ctx* ctx, int enable) {

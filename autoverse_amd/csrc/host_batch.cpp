@@ -87,6 +87,12 @@ struct hp_batch {
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     uint32_t max_radius = 0;
     uint64_t general_steps = 0;
+    // plan: LDS FFT variant and stencil tile shared by every member
+    hp::FftVariant fv;
+    int tile = 0;
+    std::map<std::pair<int, int>, std::vector<uint32_t>> by_size;      // (W, H) -> members
+    std::map<std::pair<int, bool>, std::vector<uint32_t>> by_stencil;  // (C, pow2 area) -> members
+    std::vector<void*> item_allocs;  // work-item lists of the current plan
     // tape
     float* d_stage = nullptr;     // contiguous device staging of every member's C*N values
     float* h_stage = nullptr;     // pinned host twin of d_stage
@@ -160,17 +166,19 @@ hp_status twiddles(hp_batch* b, int L, const float2** out) {
 // cslot; rows_only: the forward row pass alone (the adjoint's source spectra).
 void enqueue_fft_passes(hp_batch* b, int cur, bool conv, int cslot = 0, bool rows_only = false) {
     for (const SizeClass& sc : b->classes) {
-        if (sc.rows_fwd.n) hp::launch_batch_rows_fwd(sc.W, b->stream, b->d_desc, sc.rows_fwd.d, sc.rows_fwd.n, cur);
+        if (sc.rows_fwd.n)
+            hp::launch_batch_rows_fwd(sc.W, b->stream, b->d_desc, sc.rows_fwd.d, sc.rows_fwd.n, cur, b->fv);
         if (rows_only) continue;
-        if (sc.spectral.n) hp::launch_batch_spectral(sc.H, b->stream, b->d_desc, sc.spectral.d, sc.spectral.n);
+        if (sc.spectral.n) hp::launch_batch_spectral(sc.H, b->stream, b->d_desc, sc.spectral.d, sc.spectral.n, b->fv);
         if (sc.rows_inv.n)
-            hp::launch_batch_rows_inv(sc.W, b->stream, b->d_desc, sc.rows_inv.d, sc.rows_inv.n, conv, cslot);
+            hp::launch_batch_rows_inv(sc.W, b->stream, b->d_desc, sc.rows_inv.d, sc.rows_inv.n, conv, cslot, b->fv);
     }
 }
 
 void enqueue_stencils(hp_batch* b, int cur) {
     for (const StencilGroup& gr : b->stencil_groups)
-        hp::launch_batch_stencil2d(b->stream, gr.C, gr.pow2, b->d_desc, gr.items.d, gr.items.n, cur, b->d_flags);
+        hp::launch_batch_stencil2d(b->stream, gr.C, gr.pow2, b->d_desc, gr.items.d, gr.items.n, cur, b->d_flags,
+                                   b->tile);
 }
 
 hp_status enqueue_step(hp_batch* b, int cur) {
@@ -204,7 +212,8 @@ hp_status member_stencil(hp_batch* b, int i, int cur) {
     bool general = !hp::stencil2d_supported(m.g, s);
     if (!general) {
         hp::launch_clear_ints(b->stream, b->d_flags, 1);
-        hp::launch_stencil2d(b->stream, m.A[cur], m.U, m.A[1 - cur], m.C, m.g, m.dt, m.beta_a, m.nexp, s, b->d_flags);
+        hp::launch_stencil2d(b->stream, m.A[cur], m.U, m.A[1 - cur], m.C, m.g, m.dt, m.beta_a, m.nexp, s, b->d_flags,
+                             b->tile);
         int f = 0;
         BTRY(b, hipMemcpyAsync(&f, b->d_flags, sizeof(int), hipMemcpyDeviceToHost, b->stream));
         BTRY(b, hipStreamSynchronize(b->stream));
@@ -625,11 +634,76 @@ hp_status tape_backward(hp_batch* b, const float* const* seed, float* const* gA0
     return HP_OK;
 }
 
+// Work-item lists of the current plan: per size class the FFT passes' items (the spectral
+// pass's column blocks depend on the plan's column split) and per stencil group the tiles of
+// the plan's tile shape.
+hp_status build_items(hp_batch* b) {
+    for (void* p : b->item_allocs) (void)hipFree(p);
+    b->item_allocs.clear();
+    b->classes.clear();
+    b->stencil_groups.clear();
+    std::vector<void*>* L = &b->item_allocs;
+    for (auto& kv : b->by_size) {
+        SizeClass scl{kv.first.first, kv.first.second, {}, {}, {}, {}, {}};
+        std::vector<int4> rf, sp, ri, arf, asp;
+        const int CB = hp::spectral_block_lines(scl.H, b->fv);
+        const int ACB = std::max(1, hp::adj_spectral_block_lines(scl.H));
+        for (uint32_t i : kv.second) {
+            const hp::MemberPlan& p = b->members[i].plan;
+            const int pairs = (p.g.H + 1) / 2;
+            if (p.K > 0) {
+                for (int sl = 0; sl < (int)p.src_chan.size(); sl++)
+                    for (int pr = 0; pr < pairs; pr++) rf.push_back(make_int4((int)i, pr, sl, 0));
+                for (int wi = 0; wi < (int)p.spec_work.size(); wi++)
+                    for (int blk = 0; blk < (p.g.Wh + CB - 1) / CB; blk++) sp.push_back(make_int4((int)i, blk, wi, 0));
+                for (int wi = 0; wi < (int)p.tgt_work.size(); wi++)
+                    for (int pr = 0; pr < pairs; pr++) ri.push_back(make_int4((int)i, pr, wi, 0));
+                for (int kk = 0; kk < p.K; kk++)
+                    for (int pr = 0; pr < pairs; pr++) arf.push_back(make_int4((int)i, pr, kk, 0));
+                for (int wi = 0; wi < (int)p.spec_work.size(); wi++)
+                    for (int blk = 0; blk < (p.g.Wh + ACB - 1) / ACB; blk++)
+                        asp.push_back(make_int4((int)i, blk, wi, 0));
+            }
+        }
+        STRY(bupload(b, &scl.rows_fwd.d, rf, L));
+        scl.rows_fwd.n = (int)rf.size();
+        STRY(bupload(b, &scl.spectral.d, sp, L));
+        scl.spectral.n = (int)sp.size();
+        STRY(bupload(b, &scl.rows_inv.d, ri, L));
+        scl.rows_inv.n = (int)ri.size();
+        STRY(bupload(b, &scl.adj_rows_fwd.d, arf, L));
+        scl.adj_rows_fwd.n = (int)arf.size();
+        STRY(bupload(b, &scl.adj_spectral.d, asp, L));
+        scl.adj_spectral.n = (int)asp.size();
+        b->classes.push_back(scl);
+    }
+    const int TX = hp::stencil2d_tile_w(b->tile), TY = hp::stencil2d_tile_h(b->tile);
+    for (auto& kv : b->by_stencil) {
+        StencilGroup gr{kv.first.first, kv.first.second, {}};
+        std::vector<int4> it;
+        for (uint32_t i : kv.second) {
+            const hp::Geo& g = b->members[i].plan.g;
+            const int nt = ((g.W + TX - 1) / TX) * ((g.H + TY - 1) / TY);
+            for (int t = 0; t < nt; t++) it.push_back(make_int4((int)i, t, 0, 0));
+        }
+        STRY(bupload(b, &gr.items.d, it, L));
+        gr.items.n = (int)it.size();
+        b->stencil_groups.push_back(gr);
+    }
+    for (int p = 0; p < 2; p++)
+        if (b->graph[p]) {
+            (void)hipGraphExecDestroy(b->graph[p]);
+            b->graph[p] = nullptr;
+        }
+    return HP_OK;
+}
+
 void release(hp_batch* b) {
     if (!b) return;
     for (int p = 0; p < 2; p++)
         if (b->graph[p]) (void)hipGraphExecDestroy(b->graph[p]);
     for (void* p : b->tape_allocs) (void)hipFree(p);
+    for (void* p : b->item_allocs) (void)hipFree(p);
     for (void* p : b->allocs) (void)hipFree(p);
     if (b->h_stage) (void)hipHostFree(b->h_stage);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -682,8 +756,8 @@ hp_status create(const hp_config* cfgs, uint32_t count, int device, hp_batch** o
         }                                \
     } while (0)
     b->desc.resize(count);
-    std::map<std::pair<int, int>, std::vector<uint32_t>> by_size;
-    std::map<std::pair<int, bool>, std::vector<uint32_t>> by_stencil;
+    std::map<std::pair<int, int>, std::vector<uint32_t>>& by_size = b->by_size;
+    std::map<std::pair<int, bool>, std::vector<uint32_t>>& by_stencil = b->by_stencil;
     std::vector<int4> copy_it, tiles, wdft;
     const int ATX = hp::adj_tile_w(), ATY = hp::adj_tile_h();
     for (uint32_t i = 0; i < count; i++) {
@@ -752,54 +826,7 @@ hp_status create(const hp_config* cfgs, uint32_t count, int device, hp_batch** o
     b->adj_tiles.n = (int)tiles.size();
     TRY(bupload(b, &b->wdft_items.d, wdft));
     b->wdft_items.n = (int)wdft.size();
-    // work items per size class
-    for (auto& kv : by_size) {
-        SizeClass scl{kv.first.first, kv.first.second, {}, {}, {}, {}, {}};
-        std::vector<int4> rf, sp, ri, arf, asp;
-        const int CB = hp::spectral_block_lines(scl.H);
-        const int ACB = std::max(1, hp::adj_spectral_block_lines(scl.H));
-        for (uint32_t i : kv.second) {
-            const hp::MemberPlan& p = b->members[i].plan;
-            const int pairs = (p.g.H + 1) / 2;
-            if (p.K > 0) {
-                for (int sl = 0; sl < (int)p.src_chan.size(); sl++)
-                    for (int pr = 0; pr < pairs; pr++) rf.push_back(make_int4((int)i, pr, sl, 0));
-                for (int wi = 0; wi < (int)p.spec_work.size(); wi++)
-                    for (int blk = 0; blk < (p.g.Wh + CB - 1) / CB; blk++) sp.push_back(make_int4((int)i, blk, wi, 0));
-                for (int wi = 0; wi < (int)p.tgt_work.size(); wi++)
-                    for (int pr = 0; pr < pairs; pr++) ri.push_back(make_int4((int)i, pr, wi, 0));
-                for (int kk = 0; kk < p.K; kk++)
-                    for (int pr = 0; pr < pairs; pr++) arf.push_back(make_int4((int)i, pr, kk, 0));
-                for (int wi = 0; wi < (int)p.spec_work.size(); wi++)
-                    for (int blk = 0; blk < (p.g.Wh + ACB - 1) / ACB; blk++)
-                        asp.push_back(make_int4((int)i, blk, wi, 0));
-            }
-        }
-        TRY(bupload(b, &scl.rows_fwd.d, rf));
-        scl.rows_fwd.n = (int)rf.size();
-        TRY(bupload(b, &scl.spectral.d, sp));
-        scl.spectral.n = (int)sp.size();
-        TRY(bupload(b, &scl.rows_inv.d, ri));
-        scl.rows_inv.n = (int)ri.size();
-        TRY(bupload(b, &scl.adj_rows_fwd.d, arf));
-        scl.adj_rows_fwd.n = (int)arf.size();
-        TRY(bupload(b, &scl.adj_spectral.d, asp));
-        scl.adj_spectral.n = (int)asp.size();
-        b->classes.push_back(scl);
-    }
-    const int TX = hp::stencil2d_tile_w(), TY = hp::stencil2d_tile_h();
-    for (auto& kv : by_stencil) {
-        StencilGroup gr{kv.first.first, kv.first.second, {}};
-        std::vector<int4> it;
-        for (uint32_t i : kv.second) {
-            const hp::Geo& g = b->members[i].plan.g;
-            const int nt = ((g.W + TX - 1) / TX) * ((g.H + TY - 1) / TY);
-            for (int t = 0; t < nt; t++) it.push_back(make_int4((int)i, t, 0, 0));
-        }
-        TRY(bupload(b, &gr.items.d, it));
-        gr.items.n = (int)it.size();
-        b->stencil_groups.push_back(gr);
-    }
+    TRY(build_items(b));
     size_t fused = 0;
     for (auto& kv : by_stencil) fused += kv.second.size();
     if (fused != count) {
@@ -817,6 +844,143 @@ hp_status create(const hp_config* cfgs, uint32_t count, int device, hp_batch** o
     BTRY(b, hipDeviceSynchronize());
 #undef TRY
     *out = b;
+    return HP_OK;
+}
+
+hp_status batch_set_plan(hp_batch* b, const hp_plan& p) {
+    if (p.radix_order < 0 || p.radix_order > 1 || p.column_split < 1 || p.column_split > 2 || p.stencil_tile < 0 ||
+        p.stencil_tile >= hp::stencil2d_tile_count())
+        return b->fail(HP_ERR_SHAPE, "plan field out of range");
+    BTRY(b, hipStreamSynchronize(b->stream));
+    b->fv.rp = p.radix_order;
+    b->fv.cs = p.column_split;
+    b->tile = p.stencil_tile;
+    return build_items(b);
+}
+
+hp_plan batch_plan(const hp_batch* b) {
+    hp_plan p{};
+    p.fft_path = 1;
+    p.spectral_group = 1;
+    p.radix_order = b->fv.rp;
+    p.column_split = b->fv.cs;
+    p.stencil_tile = b->tile;
+    return p;
+}
+
+// Coordinate search as for a single context: the four LDS FFT variants with the current
+// tile, then the tiles with the fastest variant.  Every member's state is restored after.
+hp_status batch_autotune(hp_batch* b, uint32_t steps, hp_plan* best_out, uint32_t maxr, hp_plan* tried,
+                         double* us_out, uint32_t* ntried) {
+    if (steps == 0) steps = 20;
+    const size_t M = b->members.size();
+    std::vector<float> t0(M);
+    std::vector<uint64_t> s0(M);
+    for (size_t i = 0; i < M; i++) {
+        t0[i] = b->members[i].time;
+        s0[i] = b->members[i].step;
+    }
+    const uint64_t g0 = b->general_steps;
+    const uint32_t r0 = b->max_radius;
+    const bool had_tape = b->tape_recorded;
+    // every member's state into one temporary buffer (advance() uses the snapshot slots itself)
+    size_t total = 0;
+    std::vector<size_t> off(M);
+    for (size_t i = 0; i < M; i++) {
+        off[i] = total;
+        total += (size_t)b->desc[i].C * b->desc[i].g.N;
+    }
+    float* keep = nullptr;
+    BTRY(b, hipMalloc((void**)&keep, std::max<size_t>(1, total) * sizeof(float)));
+    auto save = [&](bool back) -> hp_status {
+        for (size_t i = 0; i < M; i++) {
+            const hp::MemberDesc& m = b->desc[i];
+            const size_t bytes = (size_t)m.C * m.g.N * sizeof(float);
+            if (back) BTRY(b, hipMemcpyAsync(m.A[b->cur], keep + off[i], bytes, hipMemcpyDeviceToDevice, b->stream));
+            else BTRY(b, hipMemcpyAsync(keep + off[i], m.A[b->cur], bytes, hipMemcpyDeviceToDevice, b->stream));
+        }
+        return HP_OK;
+    };
+    hp_status sst = save(false);
+    if (sst != HP_OK) {
+        (void)hipFree(keep);
+        return sst;
+    }
+    hipEvent_t ea = nullptr, eb = nullptr;
+    BTRY(b, hipEventCreate(&ea));
+    BTRY(b, hipEventCreate(&eb));
+    std::vector<double> ref, mom(3 * M);
+    uint32_t n = 0;
+    double best_us = INFINITY;
+    const hp_plan start = batch_plan(b);
+    hp_plan best = start;
+    auto measure = [&](const hp_plan& p) -> hp_status {
+        STRY(batch_set_plan(b, p));
+        STRY(save(true));
+        STRY(advance(b, 2));
+        STRY(save(true));
+        BTRY(b, hipEventRecord(ea, b->stream));
+        STRY(advance(b, steps));
+        BTRY(b, hipEventRecord(eb, b->stream));
+        BTRY(b, hipEventSynchronize(eb));
+        float ms = 0.0f;
+        BTRY(b, hipEventElapsedTime(&ms, ea, eb));
+        hp::launch_batch_moments(b->stream, b->d_desc, (int)M, b->cur, b->d_moments);
+        BTRY(b, hipMemcpyAsync(mom.data(), b->d_moments, mom.size() * sizeof(double), hipMemcpyDeviceToHost,
+                               b->stream));
+        BTRY(b, hipStreamSynchronize(b->stream));
+        bool ok = true;
+        if (ref.empty()) {
+            ref = mom;
+        } else {
+            for (size_t j = 0; j < mom.size() && ok; j++)
+                ok = fabs(mom[j] - ref[j]) <= 1e-4 * std::max(1.0, fabs(ref[j]));
+        }
+        const double us = 1e3 * ms / steps;
+        if (n < maxr) {
+            if (tried) tried[n] = p;
+            if (us_out) us_out[n] = ok ? us : -us;
+        }
+        n++;
+        if (ok && us < best_us) {
+            best_us = us;
+            best = p;
+        }
+        return HP_OK;
+    };
+    hp_status st = measure(start);
+    for (int rp = 0; rp <= 1 && st == HP_OK; rp++)
+        for (int cs = 1; cs <= 2 && st == HP_OK; cs++) {
+            hp_plan p = start;
+            p.radix_order = rp;
+            p.column_split = cs;
+            if (rp == start.radix_order && cs == start.column_split) continue;
+            st = measure(p);
+        }
+    const hp_plan fft_best = best;
+    for (int t = 0; t < hp::stencil2d_tile_count() && st == HP_OK; t++) {
+        if (t == fft_best.stencil_tile) continue;
+        hp_plan p = fft_best;
+        p.stencil_tile = t;
+        st = measure(p);
+    }
+    (void)hipEventDestroy(ea);
+    (void)hipEventDestroy(eb);
+    if (st == HP_OK) st = batch_set_plan(b, best);
+    if (st == HP_OK) st = save(true);
+    if (st == HP_OK && hipStreamSynchronize(b->stream) != hipSuccess) st = b->fail(HP_ERR_HIP, "synchronise");
+    (void)hipFree(keep);
+    if (st != HP_OK) return st;
+    for (size_t i = 0; i < M; i++) {
+        b->members[i].time = t0[i];
+        b->members[i].step = s0[i];
+    }
+    b->general_steps = g0;
+    b->max_radius = r0;
+    (void)had_tape;
+    b->tape_recorded = false;  // tuning stepped the population: a tape must be recorded again
+    if (best_out) *best_out = best;
+    if (ntried) *ntried = n;
     return HP_OK;
 }
 
@@ -954,6 +1118,24 @@ hp_status hp_batch_theta_count(const hp_batch* b, uint32_t member, uint32_t* cou
     if (member >= b->members.size()) return HP_ERR_SHAPE;
     *count = (uint32_t)theta_count(b->members[member]);
     return HP_OK;
+}
+
+hp_status hp_batch_get_plan(const hp_batch* b, hp_plan* out) {
+    if (!b || !out) return HP_ERR_NULL;
+    *out = batch_plan(b);
+    return HP_OK;
+}
+
+hp_status hp_batch_set_plan(hp_batch* b, const hp_plan* plan) {
+    if (!plan) return HP_ERR_NULL;
+    return guarded(b, [&]() -> hp_status { return batch_set_plan(b, *plan); });
+}
+
+hp_status hp_batch_autotune(hp_batch* b, uint32_t steps, hp_plan* best, uint32_t max_results, hp_plan* tried,
+                            double* us_per_step, uint32_t* n_tried) {
+    return guarded(b, [&]() -> hp_status {
+        return batch_autotune(b, steps, best, max_results, tried, us_per_step, n_tried);
+    });
 }
 
 hp_status hp_batch_tape_forward(hp_batch* b, uint64_t steps, uint32_t checkpoint_every, uint32_t* used_every) {

@@ -18,6 +18,7 @@ struct MemberPlan {
     int C = 0, K = 0;
     bool is3d = false;
     bool wave = false;  // 2D sides 256..2048: column-major spectra, wave-level FFTs
+    bool grouped = true;  // spectral work: one item per source (else one per kernel)
     std::vector<float> kq;
     std::vector<int> src_chan;  // source slot -> channel
     std::vector<SpecWork> spec_work;

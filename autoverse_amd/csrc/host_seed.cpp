@@ -200,6 +200,7 @@ hp_status plan_member(const hp_config* cfg, bool allow_wave, int group, MemberPl
     // or one item per kernel (more workgroups; the forward transform is redone)
     const int blocks_per_source = p.wave ? (g.Wh + 3) / 4 : 1;
     const bool grouped = group >= 0 ? group != 0 : (!p.wave || blocks_per_source * (long long)p.src_chan.size() >= 256);
+    p.grouped = grouped;
     if (grouped) {
         for (size_t sl = 0; sl < p.src_chan.size(); sl++) {
             int first = (int)p.klist_spec.size(), cnt = 0;

@@ -44,6 +44,13 @@ struct Geo {
     long long spec;   // complex elements of one spectrum: D*H*Wp
 };
 
+// Tunable shape of the LDS FFT passes: radix order (0: radix 8 first, 1: radix 4 first) and
+// the split of the spectral pass's column block (1 or 2).
+struct FftVariant {
+    int rp = 0;
+    int cs = 1;
+};
+
 // One member of a batched population: its buffers and geometry, in device memory.
 struct MemberDesc {
     float* A[2];                 // state ping-pong
@@ -82,21 +89,27 @@ struct MemberDesc {
 // Each returns false if the length is not one of the compiled FFT sizes.
 bool fft_supported(int L);
 bool launch_rows_fwd(int L, hipStream_t st, const float* A, float2* X, const int* src_chan, int nsrc,
-                     const Geo& g, const float2* tw);
-bool launch_cols(int L, hipStream_t st, int dir, float2* X, int nspec, const Geo& g, const float2* tw);
+                     const Geo& g, const float2* tw, const FftVariant& v = FftVariant());
+bool launch_cols(int L, hipStream_t st, int dir, float2* X, int nspec, const Geo& g, const float2* tw,
+                 const FftVariant& v = FftVariant());
 bool launch_spectral(int L, int dim, hipStream_t st, const float2* X, float2* Y, const float* Kq,
-                     const SpecWork* work, int nwork, const int* klist, const Geo& g, const float2* tw);
+                     const SpecWork* work, int nwork, const int* klist, const Geo& g, const float2* tw,
+                     const FftVariant& v = FftVariant());
 bool launch_rows_inv(int L, hipStream_t st, const float2* Y, float* U, const TgtWork* work, int nwork,
-                     const int* klist, const GrowthParams* gp, const Geo& g, const float2* tw);
+                     const int* klist, const GrowthParams* gp, const Geo& g, const float2* tw,
+                     const FftVariant& v = FftVariant());
 
 // Batched 2D passes over a size class: items[i] = (member, block, slot/work, -).
-bool launch_batch_rows_fwd(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cur);
-bool launch_batch_spectral(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems);
+bool launch_batch_rows_fwd(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cur,
+                           const FftVariant& v = FftVariant());
+bool launch_batch_spectral(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems,
+                           const FftVariant& v = FftVariant());
 // conv: also store each kernel's convolution into MemberDesc::conv slot cslot (K*N each),
 // for the adjoint.
 bool launch_batch_rows_inv(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems,
-                           bool conv = false, int cslot = 0);
-int spectral_block_lines(int L);  // columns per spectral workgroup for length-L columns
+                           bool conv = false, int cslot = 0, const FftVariant& v = FftVariant());
+// columns per spectral workgroup for length-L columns
+int spectral_block_lines(int L, const FftVariant& v = FftVariant());
 
 // ---- launchers implemented in wave_fft.hip (2D, sides 256..2048, column-major spectra) ----
 bool wave_fft_supported(int W, int H);
@@ -115,13 +128,15 @@ bool stencil2d_supported(const Geo& g, float s);
 // Batched fused stencil: items[i] = (member, tile); members of one launch share C and the
 // power-of-two-area property.
 bool launch_batch_stencil2d(hipStream_t st, int C, bool pow2, const MemberDesc* mem, const int4* items, int nitems,
-                            int cur, int* flag);
-int stencil2d_tile_w();
+                            int cur, int* flag, int tile = 0);
+// Tile shapes of the fused stencil: variant t in [0, stencil2d_tile_count()).
+int stencil2d_tile_count();
+int stencil2d_tile_w(int tile = 0);
 // Per member (sum m, sum x m, sum y m) of the channel sum over cells with m > 0, FP64.
 void launch_batch_moments(hipStream_t st, const MemberDesc* mem, int nmem, int cur, double* out);
-int stencil2d_tile_h();
+int stencil2d_tile_h(int tile = 0);
 bool launch_stencil2d(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g,
-                      float dt, float beta_a, float nexp, float s, int* flag);
+                      float dt, float beta_a, float nexp, float s, int* flag, int tile = 0);
 bool launch_stencil3d(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g,
                       float dt, float beta_a, float nexp, float s, int* flag);
 // General path: per-source destination boxes (+ needed radius via atomicMax)...

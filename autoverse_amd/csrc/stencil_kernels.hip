@@ -47,35 +47,40 @@ namespace hp {
 // wrapped order, so every destination adds its contributions in the scatter's source raster
 // order -- also at the periodic edges, where the visiting order is rotated.
 // ---------------------------------------------------------------------------------------------
-#ifndef HP_ST_TX
-#define HP_ST_TX 64  // destination tile width
-#endif
-#ifndef HP_ST_TY
-#define HP_ST_TY 16  // destination tile height
-#endif
-#ifndef HP_ST_NT
-#define HP_ST_NT 512  // threads per workgroup
-#endif
-
-namespace st2 {
-constexpr int TX = HP_ST_TX, TY = HP_ST_TY, NT = HP_ST_NT, SR = TX * TY / NT;
-constexpr int SW = TX + 3, SH = TY + 3;  // source region: offsets -1 .. +2 around the tile
-constexpr int LW = SW + 2, LH = SH + 2;  // + Sobel halo
-constexpr int NSRC = SW * SH;
-constexpr int NLOAD = LW * LH;
-constexpr int LPT = (NLOAD + NT - 1) / NT;  // load-region cells per thread
-constexpr int SPT = (NSRC + NT - 1) / NT;   // sources per thread
-constexpr int SROWS = SR + 3;               // source rows of one strip
-static_assert(TX * TY == NT * SR && NT % TX == 0, "one strip per thread");
-}  // namespace st2
+// Tile shapes (the plan autotuner picks one per configuration): TX x TY destinations per
+// workgroup of NT threads, SR = TX*TY/NT destinations per thread strip.
+template <int TX_, int TY_, int NT_>
+struct StTile {
+    static constexpr int TX = TX_, TY = TY_, NT = NT_, SR = TX * TY / NT;
+    static constexpr int SW = TX + 3, SH = TY + 3;  // source region: offsets -1 .. +2 around the tile
+    static constexpr int LW = SW + 2, LH = SH + 2;  // + Sobel halo
+    static constexpr int NSRC = SW * SH;
+    static constexpr int NLOAD = LW * LH;
+    static constexpr int LPT = (NLOAD + NT - 1) / NT;  // load-region cells per thread
+    static constexpr int SPT = (NSRC + NT - 1) / NT;   // sources per thread
+    static constexpr int SROWS = SR + 3;               // source rows of one strip
+    static_assert(TX * TY == NT * SR && NT % TX == 0, "one strip per thread");
+};
+using StTile0 = StTile<64, 16, 512>;
+using StTile1 = StTile<64, 8, 256>;
+using StTile2 = StTile<32, 16, 256>;
+using StTile3 = StTile<128, 8, 512>;
+constexpr int kStencilTiles = 4;
+static_assert(StTile0::SROWS == StTile1::SROWS && StTile0::SROWS == StTile2::SROWS &&
+                  StTile0::SROWS == StTile3::SROWS,
+              "every tile shape covers the same grids");
 
 // CC: channel count known at compile time (1..4), or 0 for any count (masses re-read per channel).
-template <int CC, bool POW2>
+template <int CC, bool POW2, class TL>
 __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, const float* __restrict__ U,
                                                float* __restrict__ Aout, int Cdyn, int W, int H, long long N,
                                                int tiles_x, int tile, float dt, float beta_a, float nexp, float s,
                                                float ta, float inv_ta, int* __restrict__ flag) {
-    using namespace st2;
+    constexpr int TX = TL::TX, TY = TL::TY, NT = TL::NT, SR = TL::SR, SW = TL::SW;
+    constexpr int LW = TL::LW, NSRC = TL::NSRC, NLOAD = TL::NLOAD, LPT = TL::LPT, SPT = TL::SPT;
+    constexpr int SROWS = TL::SROWS;
+    (void)TY;
+    (void)SROWS;
     const int C = CC > 0 ? CC : Cdyn;
     __shared__ float sSum[NLOAD];
     __shared__ float sU[2][NLOAD];
@@ -285,40 +290,64 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
     if (local_flag) atomicOr(flag, local_flag);
 }
 
-template <int CC, bool POW2>
-__global__ __launch_bounds__(st2::NT) void k_stencil2d(const float* __restrict__ A, const float* __restrict__ U,
-                                                       float* __restrict__ Aout, int Cdyn, int W, int H, long long N,
-                                                       int tiles_x, int ntiles, float dt, float beta_a, float nexp,
-                                                       float s, float ta, float inv_ta, int* __restrict__ flag) {
+template <int CC, bool POW2, class TL>
+__global__ __launch_bounds__(TL::NT) void k_stencil2d(const float* __restrict__ A, const float* __restrict__ U,
+                                                      float* __restrict__ Aout, int Cdyn, int W, int H, long long N,
+                                                      int tiles_x, int ntiles, float dt, float beta_a, float nexp,
+                                                      float s, float ta, float inv_ta, int* __restrict__ flag) {
     // XCD-aware tile order: workgroup b runs on XCD b % 8; each XCD gets a contiguous run of
     // tiles so that neighbouring tiles (which share halo rows) meet in the same L2.
     const int b = blockIdx.x;
     const int per = (ntiles + 7) / 8;
     const int tile = (b % 8) * per + b / 8;
     if (tile >= ntiles) return;
-    stencil2d_body<CC, POW2>(A, U, Aout, Cdyn, W, H, N, tiles_x, tile, dt, beta_a, nexp, s, ta, inv_ta, flag);
+    stencil2d_body<CC, POW2, TL>(A, U, Aout, Cdyn, W, H, N, tiles_x, tile, dt, beta_a, nexp, s, ta, inv_ta, flag);
 }
 
 // batched: item = (member, tile); every member of one launch has the same channel count class
-template <int CC, bool POW2>
-__global__ __launch_bounds__(st2::NT) void kb_stencil2d(const MemberDesc* __restrict__ mem,
-                                                        const int4* __restrict__ items, int cur,
-                                                        int* __restrict__ flag) {
+template <int CC, bool POW2, class TL>
+__global__ __launch_bounds__(TL::NT) void kb_stencil2d(const MemberDesc* __restrict__ mem,
+                                                       const int4* __restrict__ items, int cur,
+                                                       int* __restrict__ flag) {
     const int4 it = items[blockIdx.x];
     const MemberDesc& m = mem[it.x];
     const float ta = (2.0f * m.s) * (2.0f * m.s);
-    const int tiles_x = (m.g.W + st2::TX - 1) / st2::TX;
-    stencil2d_body<CC, POW2>(m.A[cur], m.U, m.A[1 - cur], m.C, m.g.W, m.g.H, m.g.N, tiles_x, it.y, m.dt, m.beta_a,
-                             m.nexp, m.s, ta, 1.0f / ta, flag);
+    const int tiles_x = (m.g.W + TL::TX - 1) / TL::TX;
+    stencil2d_body<CC, POW2, TL>(m.A[cur], m.U, m.A[1 - cur], m.C, m.g.W, m.g.H, m.g.N, tiles_x, it.y, m.dt,
+                                 m.beta_a, m.nexp, m.s, ta, 1.0f / ta, flag);
 }
 
-bool launch_batch_stencil2d(hipStream_t st, int C, bool pow2, const MemberDesc* mem, const int4* items, int nitems,
-                            int cur, int* flag) {
-#define B_(cc)                                                                                                  \
-    if (pow2)                                                                                                   \
-        hipLaunchKernelGGL((kb_stencil2d<cc, true>), dim3(nitems), dim3(st2::NT), 0, st, mem, items, cur, flag); \
-    else                                                                                                        \
-        hipLaunchKernelGGL((kb_stencil2d<cc, false>), dim3(nitems), dim3(st2::NT), 0, st, mem, items, cur, flag);
+// HP_TILE binds the run-time tile variant to the compile-time tile type TL of the call.
+#define HP_TILE(t, ...)                          \
+    do {                                         \
+        switch (t) {                             \
+        case 1: {                                \
+            using TL = StTile1;                  \
+            __VA_ARGS__;                         \
+        } break;                                 \
+        case 2: {                                \
+            using TL = StTile2;                  \
+            __VA_ARGS__;                         \
+        } break;                                 \
+        case 3: {                                \
+            using TL = StTile3;                  \
+            __VA_ARGS__;                         \
+        } break;                                 \
+        default: {                               \
+            using TL = StTile0;                  \
+            __VA_ARGS__;                         \
+        }                                        \
+        }                                        \
+    } while (0)
+
+template <class TL>
+static void launch_kb_st2(hipStream_t st, int C, bool pow2, const MemberDesc* mem, const int4* items, int nitems,
+                          int cur, int* flag) {
+#define B_(cc)                                                                                                   \
+    if (pow2)                                                                                                    \
+        hipLaunchKernelGGL((kb_stencil2d<cc, true, TL>), dim3(nitems), dim3(TL::NT), 0, st, mem, items, cur, flag); \
+    else                                                                                                         \
+        hipLaunchKernelGGL((kb_stencil2d<cc, false, TL>), dim3(nitems), dim3(TL::NT), 0, st, mem, items, cur, flag);
     switch (C) {
     case 1: B_(1) break;
     case 2: B_(2) break;
@@ -327,22 +356,36 @@ bool launch_batch_stencil2d(hipStream_t st, int C, bool pow2, const MemberDesc* 
     default: B_(0)
     }
 #undef B_
+}
+
+bool launch_batch_stencil2d(hipStream_t st, int C, bool pow2, const MemberDesc* mem, const int4* items, int nitems,
+                            int cur, int* flag, int tile) {
+    HP_TILE(tile, launch_kb_st2<TL>(st, C, pow2, mem, items, nitems, cur, flag));
     return true;
 }
 
-int stencil2d_tile_w() { return st2::TX; }
-int stencil2d_tile_h() { return st2::TY; }
+int stencil2d_tile_w(int tile) {
+    int w = 0;
+    HP_TILE(tile, w = TL::TX);
+    return w;
+}
+int stencil2d_tile_h(int tile) {
+    int h = 0;
+    HP_TILE(tile, h = TL::TY);
+    return h;
+}
+int stencil2d_tile_count() { return kStencilTiles; }
 
-template <int CC>
+template <int CC, class TL>
 static void launch_st2(hipStream_t st, bool pow2, dim3 grid, const float* A, const float* U, float* Aout, int C,
                        const Geo& g, int tiles_x, int ntiles, float dt, float beta_a, float nexp, float s,
                        float ta, float inv_ta, int* flag) {
     if (pow2)
-        hipLaunchKernelGGL((k_stencil2d<CC, true>), grid, dim3(st2::NT), 0, st, A, U, Aout, C, g.W, g.H, g.N, tiles_x,
-                           ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag);
+        hipLaunchKernelGGL((k_stencil2d<CC, true, TL>), grid, dim3(TL::NT), 0, st, A, U, Aout, C, g.W, g.H, g.N,
+                           tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag);
     else
-        hipLaunchKernelGGL((k_stencil2d<CC, false>), grid, dim3(st2::NT), 0, st, A, U, Aout, C, g.W, g.H, g.N, tiles_x,
-                           ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag);
+        hipLaunchKernelGGL((k_stencil2d<CC, false, TL>), grid, dim3(TL::NT), 0, st, A, U, Aout, C, g.W, g.H, g.N,
+                           tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag);
 }
 
 bool stencil2d_supported(const Geo& g, float s) {
@@ -350,28 +393,33 @@ bool stencil2d_supported(const Geo& g, float s) {
     // the 16-source window assumes s <= 1; the point-distribution fallback (total area < 1e-10)
     // lives in the general path
     const float ta = (2.0f * s) * (2.0f * s);
-    return g.D == 1 && g.W >= 4 && g.H >= st2::SROWS && s > 0.0f && s <= 1.0f && ta >= 1e-10f;
+    return g.D == 1 && g.W >= 4 && g.H >= StTile0::SROWS && s > 0.0f && s <= 1.0f && ta >= 1e-10f;
 }
 
-bool launch_stencil2d(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g, float dt,
-                      float beta_a, float nexp, float s, int* flag) {
-    using namespace st2;
-    if (!stencil2d_supported(g, s)) return false;
+template <class TL>
+static void launch_k_st2(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g, float dt,
+                         float beta_a, float nexp, float s, int* flag) {
     const float ta = (2.0f * s) * (2.0f * s);
     int e;
     const float fr = frexpf(ta, &e);
     const bool pow2 = fr == 0.5f && ta >= 1.17549435e-38f && ta < INFINITY;
     const float inv_ta = 1.0f / ta;
-    const int tiles_x = (g.W + TX - 1) / TX, tiles_y = (g.H + TY - 1) / TY;
+    const int tiles_x = (g.W + TL::TX - 1) / TL::TX, tiles_y = (g.H + TL::TY - 1) / TL::TY;
     const int ntiles = tiles_x * tiles_y;
     const dim3 grid((ntiles + 7) / 8 * 8);
     switch (C) {
-    case 1: launch_st2<1>(st, pow2, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
-    case 2: launch_st2<2>(st, pow2, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
-    case 3: launch_st2<3>(st, pow2, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
-    case 4: launch_st2<4>(st, pow2, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
-    default: launch_st2<0>(st, pow2, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag);
+    case 1: launch_st2<1, TL>(st, pow2, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
+    case 2: launch_st2<2, TL>(st, pow2, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
+    case 3: launch_st2<3, TL>(st, pow2, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
+    case 4: launch_st2<4, TL>(st, pow2, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
+    default: launch_st2<0, TL>(st, pow2, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag);
     }
+}
+
+bool launch_stencil2d(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g, float dt,
+                      float beta_a, float nexp, float s, int* flag, int tile) {
+    if (!stencil2d_supported(g, s)) return false;
+    HP_TILE(tile, launch_k_st2<TL>(st, A, U, Aout, C, g, dt, beta_a, nexp, s, flag));
     return true;
 }
 

@@ -115,6 +115,23 @@ class HipPopulation:
         self._check(self._lib.hp_batch_stats(self._ptr, ctypes.byref(r), ctypes.byref(g)), "hp_batch_stats")
         return r.value, g.value
 
+    # -- execution plan ----------------------------------------------------------------------
+    def plan(self) -> dict:
+        p = _abi.HpPlan()
+        self._check(self._lib.hp_batch_get_plan(self._ptr, ctypes.byref(p)), "hp_batch_get_plan")
+        return p.as_dict()
+
+    def set_plan(self, plan: dict) -> None:
+        p = _abi.HpPlan.from_dict({**self.plan(), **plan})
+        self._check(self._lib.hp_batch_set_plan(self._ptr, ctypes.byref(p)), "hp_batch_set_plan")
+
+    def autotune(self, steps: int = 20):
+        """Time the LDS FFT variants and stencil tiles on the population from its current states
+        (restored afterwards); keeps the fastest.  Returns (best plan, [(plan, us per step)])."""
+        st, best, table = _abi.autotune_call(self._lib.hp_batch_autotune, self._ptr, steps)
+        self._check(st, "hp_batch_autotune")
+        return best, table
+
     # -- gradient tape ---------------------------------------------------------------------
     def theta_count(self, member: int) -> int:
         n = ctypes.c_uint32(0)

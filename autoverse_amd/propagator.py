@@ -134,6 +134,26 @@ class HipPropagator:
         """CpuPropagator::config (propagator.rs:430-432)."""
         return self._config
 
+    # -- execution plan (no reference counterpart) ---------------------------------------------
+    def plan(self) -> dict:
+        """The resolved plan: fft_path, spectral_group, radix_order, column_split, stencil_tile."""
+        p = _abi.HpPlan()
+        self._check(self._lib.hp_get_plan(self._ptr, ctypes.byref(p)), "hp_get_plan")
+        return p.as_dict()
+
+    def set_plan(self, plan: dict) -> None:
+        """Rebuild the device plan (the state, time and step are kept)."""
+        p = _abi.HpPlan.from_dict({**self.plan(), **plan})
+        self._check(self._lib.hp_set_plan(self._ptr, ctypes.byref(p)), "hp_set_plan")
+
+    def autotune(self, steps: int = 20):
+        """Time every candidate plan from the current state (restored afterwards), keep the
+        fastest, and remember it for later propagators of the same shape.  Returns (best plan,
+        [(plan, us per step or None if it disagreed with the first plan)])."""
+        st, best, table = _abi.autotune_call(self._lib.hp_autotune, self._ptr, steps)
+        self._check(st, "hp_autotune")
+        return best, table
+
     def _ptrs(self, state: SimulationState):
         if len(state.channels) != self._config.channels:
             raise ValueError("channel count mismatch")

@@ -198,6 +198,34 @@ hp_status hp_mass_f64(hp_ctx* ctx, double* per_channel, double* total);
  * take the general reintegration path (a flow too large for the fused stencil). */
 hp_status hp_stats(const hp_ctx* ctx, uint32_t* max_gather_radius, uint64_t* general_path_steps);
 
+/* ---- Execution plan and its autotuner ------------------------------------------
+ * No counterpart in the reference (CpuPropagator has one code path).  A plan picks how
+ * the step runs on the device; every plan computes the same step: tile shapes and spectral
+ * grouping change nothing bit for bit, FFT variants only the FFT's rounding.
+ * hp_create uses the plan the autotuner last found for a configuration of the same shape in
+ * this process, else built-in defaults. */
+typedef struct hp_plan {
+    int32_t fft_path;       /* 0: wave-level column-major FFTs where the sides allow (2D, 256..2048),
+                               1: LDS FFTs (always available) */
+    int32_t spectral_group; /* 1: one spectral work item per source channel, 0: one per kernel,
+                               -1: chosen from the work size */
+    int32_t radix_order;    /* LDS FFTs: 0 radix 8 first, 1 radix 4 first */
+    int32_t column_split;   /* LDS spectral pass: 1 or 2 (half the columns per workgroup) */
+    int32_t stencil_tile;   /* fused 2D stencil tile: 0 64x16, 1 64x8, 2 32x16, 3 128x8 */
+} hp_plan;
+
+/* The context's resolved plan (fft_path and spectral_group as actually used). */
+hp_status hp_get_plan(const hp_ctx* ctx, hp_plan* out);
+/* Rebuild the device plan; the state, time and step are kept. */
+hp_status hp_set_plan(hp_ctx* ctx, const hp_plan* plan);
+/* Time every candidate plan over `steps` steps from the current state (which is restored
+ * afterwards, with time and step), keep the fastest whose result agrees with the default
+ * plan's to 1e-3, and remember it for later contexts of the same shape.  Up to max_results
+ * candidates are reported in tried / us_per_step (either may be NULL); *n_tried (optional)
+ * receives how many were timed. */
+hp_status hp_autotune(hp_ctx* ctx, uint32_t steps, hp_plan* best, uint32_t max_results, hp_plan* tried,
+                      double* us_per_step, uint32_t* n_tried);
+
 /* ---- Per-kernel timing (HIP events on the context's stream) ------------------ */
 /* Kernel classes of the step, in launch order. */
 typedef enum hp_kernel_class {
@@ -285,6 +313,13 @@ hp_status hp_batch_stats(const hp_batch* b, uint32_t* max_gather_radius, uint64_
  * *used_every (optional) receives the interval used.
  * Limits: sides 2^a * 3^b up to 1024; every kernel's support 2R+1 within the grid. */
 hp_status hp_batch_theta_count(const hp_batch* b, uint32_t member, uint32_t* count);
+/* Batch plans: the LDS FFT variant and the stencil tile shared by all members (fft_path and
+ * spectral_group are fixed for populations: LDS FFTs, one spectral item per source).
+ * hp_batch_autotune times the candidates from the current states and restores them. */
+hp_status hp_batch_get_plan(const hp_batch* b, hp_plan* out);
+hp_status hp_batch_set_plan(hp_batch* b, const hp_plan* plan);
+hp_status hp_batch_autotune(hp_batch* b, uint32_t steps, hp_plan* best, uint32_t max_results, hp_plan* tried,
+                            double* us_per_step, uint32_t* n_tried);
 hp_status hp_batch_tape_forward(hp_batch* b, uint64_t steps, uint32_t checkpoint_every, uint32_t* used_every);
 hp_status hp_batch_tape_backward(hp_batch* b, const float* const* dl_dfinal, float* const* dl_dinitial,
                                  double* dtheta);
