@@ -37,6 +37,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the plan autotuner (run the built-in default plan)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline sample length")
     ap.add_argument("--population", action="store_true",
                     help="population mode: 256 uneven members stepped as one batch vs one lone "
@@ -73,6 +75,13 @@ def main() -> int:
     prop = av.HipPropagator(cfg)
     prop.set_state(state)
     prop.advance(args.warmup)
+    tune = None
+    if not args.no_tune:  # untimed: pick the fastest plan on this device, state restored after
+        best, table = prop.autotune(steps=20)
+        tune = {"plan": best, "candidates": len(table),
+                "default_us_per_step": table[0][1] if table else None,
+                "best_us_per_step": min(us for _, us in table if us is not None) if table else None}
+    plan = prop.plan()
 
     # timed pass: K steps replayed from hipGraphs, bracketed by device synchronisation
     t0 = time.perf_counter()
@@ -135,7 +144,7 @@ def main() -> int:
                    "width": cfg.width, "height": cfg.height, "channels": cfg.channels,
                    "kernels": len(cfg.kernels), "kernel_radius": cfg.kernel_radius,
                    "max_gather_radius": radius, "general_path_steps": general,
-                   "mass_f64_after": mass},
+                   "mass_f64_after": mass, "plan": plan, "autotune": tune},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -163,6 +172,9 @@ def population_bench(args) -> int:
     pop = HipPopulation(cfgs)
     pop.set_seeds(seeds)
     pop.advance(args.warmup)
+    plan = None
+    if not args.no_tune:  # untimed, states restored after
+        plan, _ = pop.autotune(steps=10)
     m0 = pop.mass_f64()
     c0, _ = pop.centroids()
     t0 = time.perf_counter()
@@ -193,6 +205,8 @@ def population_bench(args) -> int:
     # chosen by the library from free HBM), then the backward pass for L = sum <seed, A_T>
     # with a fixed random seed
     pop = HipPopulation(cfgs)
+    if plan is not None:
+        pop.set_plan(plan)
     pop.set_seeds(seeds)
     pop.advance(args.warmup)
     rng = np.random.default_rng(7)
@@ -225,7 +239,7 @@ def population_bench(args) -> int:
         "data": "synthetic",
         "config": {"workload": "population of 256: side U{64,128,192,256}, 1-3 channels, every "
                                "source->target kernel, R=15, dt=0.2", "members": len(cfgs), "cells": cells,
-                   "max_gather_radius": radius, "general_path_steps": general},
+                   "max_gather_radius": radius, "general_path_steps": general, "plan": plan},
         "per_instance": {"value": per_inst, "unit": "cell-steps/s", "ms_per_step": secs_i * 1e3 / K,
                          "path": "one HipPropagator per member, advanced in turn"},
         "speedup_vs_per_instance": value / per_inst,
