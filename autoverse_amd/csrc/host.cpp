@@ -226,14 +226,19 @@ hp_status enqueue_affinity(hp_ctx* c, const float* A) {
 }
 
 bool fast_stencil_ok(const hp_ctx* c) {
-    return !c->is3d && hp::stencil2d_supported(c->g, c->cfg.flow.distribution_size);
+    return c->is3d ? hp::stencil3d_supported(c->g, c->cfg.flow.distribution_size)
+                   : hp::stencil2d_supported(c->g, c->cfg.flow.distribution_size);
 }
 
 hp_status enqueue_fast_stencil(hp_ctx* c, const float* A, const float* U, float* out) {
     {
         Timed t(c, HP_K_STENCIL);
-        hp::launch_stencil2d(c->stream, A, U, out, c->C, c->g, c->cfg.dt, c->cfg.flow.beta_a, c->cfg.flow.n,
-                             c->cfg.flow.distribution_size, c->d_flags, c->tile);
+        if (c->is3d)
+            hp::launch_stencil3d(c->stream, A, U, out, c->C, c->g, c->cfg.dt, c->cfg.flow.beta_a, c->cfg.flow.n,
+                                 c->cfg.flow.distribution_size, c->d_flags);
+        else
+            hp::launch_stencil2d(c->stream, A, U, out, c->C, c->g, c->cfg.dt, c->cfg.flow.beta_a, c->cfg.flow.n,
+                                 c->cfg.flow.distribution_size, c->d_flags, c->tile);
     }
     HIP_TRY(c, hipGetLastError());
     return HP_OK;

@@ -137,6 +137,7 @@ void launch_batch_moments(hipStream_t st, const MemberDesc* mem, int nmem, int c
 int stencil2d_tile_h(int tile = 0);
 bool launch_stencil2d(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g,
                       float dt, float beta_a, float nexp, float s, int* flag, int tile = 0);
+bool stencil3d_supported(const Geo& g, float s);
 bool launch_stencil3d(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g,
                       float dt, float beta_a, float nexp, float s, int* flag);
 // General path: per-source destination boxes (+ needed radius via atomicMax)...

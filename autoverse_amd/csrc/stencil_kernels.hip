@@ -423,9 +423,216 @@ bool launch_stencil2d(hipStream_t st, const float* A, const float* U, float* Aou
     return true;
 }
 
-bool launch_stencil3d(hipStream_t, const float*, const float*, float*, int, const Geo&, float, float, float, float,
-                      int*) {
-    return false;  // 3D runs through the general path
+// ---------------------------------------------------------------------------------------------
+// Fused 3D stencil: channel sum, 27-point Sobel of the sum and of U (gradient3d.rs:75-138,
+// same grouping, / 32), alpha, flow (flow3d.rs:50-73), per-source box and its four overlaps
+// per axis (reintegration3d.rs:104-176), and the 64-source gather.
+//
+// A source at x whose box has |dt F| < 1 and s <= 1 reaches cells x-2 .. x+1 on every axis,
+// so a destination receives from offsets o = source - dest in {-1, 0, 1, 2}^3.  Every
+// destination visits them in ascending wrapped raster order (z, then y, then x; rotated at
+// the periodic edges), the scatter's order, and adds m * (((ow*oh)*od) / V) exactly as the
+// reference forms cell_mass; zero-volume pairs add +-0 and change nothing.  Any owned source
+// whose box reaches further sets flag bit 0 (bit 1 for non-finite values) and the host reruns
+// the step through the general path.
+// ---------------------------------------------------------------------------------------------
+namespace st3 {
+constexpr int TX = 16, TY = 4, TZ = 4, NT = TX * TY * TZ;
+constexpr int SX = TX + 3, SY = TY + 3, SZ = TZ + 3;  // sources: offsets -1 .. +2 around the tile
+constexpr int LX = SX + 2, LY = SY + 2, LZ = SZ + 2;  // + Sobel halo
+constexpr int NSRC = SX * SY * SZ;
+constexpr int NLOAD = LX * LY * LZ;
+constexpr int SPT = (NSRC + NT - 1) / NT;
+}  // namespace st3
+
+// gradient3d.rs:75-138 on an LDS box: P(dz, dy, dx) = L[i + dz*sz + dy*sy + dx].
+__device__ __forceinline__ void sobel27_lds(const float* L, int i, int sy, int sz, float& vx, float& vy, float& vz) {
+    auto P = [&](int dz, int dy, int dx) -> float { return L[i + dz * sz + dy * sy + dx]; };
+    const float p_mmm = P(-1, -1, -1), p_mm0 = P(-1, -1, 0), p_mmp = P(-1, -1, 1);
+    const float p_m0m = P(-1, 0, -1), p_m00 = P(-1, 0, 0), p_m0p = P(-1, 0, 1);
+    const float p_mpm = P(-1, 1, -1), p_mp0 = P(-1, 1, 0), p_mpp = P(-1, 1, 1);
+    const float p_0mm = P(0, -1, -1), p_0m0 = P(0, -1, 0), p_0mp = P(0, -1, 1);
+    const float p_00m = P(0, 0, -1), p_00p = P(0, 0, 1);
+    const float p_0pm = P(0, 1, -1), p_0p0 = P(0, 1, 0), p_0pp = P(0, 1, 1);
+    const float p_pmm = P(1, -1, -1), p_pm0 = P(1, -1, 0), p_pmp = P(1, -1, 1);
+    const float p_p0m = P(1, 0, -1), p_p00 = P(1, 0, 0), p_p0p = P(1, 0, 1);
+    const float p_ppm = P(1, 1, -1), p_pp0 = P(1, 1, 0), p_ppp = P(1, 1, 1);
+    const float gx_zm = -1.0f * (1.0f * p_mmm + 2.0f * p_m0m + 1.0f * p_mpm) + 1.0f * (1.0f * p_mmp + 2.0f * p_m0p + 1.0f * p_mpp);
+    const float gx_z0 = -1.0f * (1.0f * p_0mm + 2.0f * p_00m + 1.0f * p_0pm) + 1.0f * (1.0f * p_0mp + 2.0f * p_00p + 1.0f * p_0pp);
+    const float gx_zp = -1.0f * (1.0f * p_pmm + 2.0f * p_p0m + 1.0f * p_ppm) + 1.0f * (1.0f * p_pmp + 2.0f * p_p0p + 1.0f * p_ppp);
+    vx = (1.0f * gx_zm + 2.0f * gx_z0 + 1.0f * gx_zp) / 32.0f;
+    const float gy_zm = -1.0f * (1.0f * p_mmm + 2.0f * p_mm0 + 1.0f * p_mmp) + 1.0f * (1.0f * p_mpm + 2.0f * p_mp0 + 1.0f * p_mpp);
+    const float gy_z0 = -1.0f * (1.0f * p_0mm + 2.0f * p_0m0 + 1.0f * p_0mp) + 1.0f * (1.0f * p_0pm + 2.0f * p_0p0 + 1.0f * p_0pp);
+    const float gy_zp = -1.0f * (1.0f * p_pmm + 2.0f * p_pm0 + 1.0f * p_pmp) + 1.0f * (1.0f * p_ppm + 2.0f * p_pp0 + 1.0f * p_ppp);
+    vy = (1.0f * gy_zm + 2.0f * gy_z0 + 1.0f * gy_zp) / 32.0f;
+    const float gz_xm = -1.0f * (1.0f * p_mmm + 2.0f * p_m0m + 1.0f * p_mpm) + 1.0f * (1.0f * p_pmm + 2.0f * p_p0m + 1.0f * p_ppm);
+    const float gz_x0 = -1.0f * (1.0f * p_mm0 + 2.0f * p_m00 + 1.0f * p_mp0) + 1.0f * (1.0f * p_pm0 + 2.0f * p_p00 + 1.0f * p_pp0);
+    const float gz_xp = -1.0f * (1.0f * p_mmp + 2.0f * p_m0p + 1.0f * p_mpp) + 1.0f * (1.0f * p_pmp + 2.0f * p_p0p + 1.0f * p_ppp);
+    vz = (1.0f * gz_xm + 2.0f * gz_x0 + 1.0f * gz_xp) / 32.0f;
+}
+
+// first visited offset index of the ascending wrapped order of g + {-1, 0, 1, 2} on n cells
+__device__ __forceinline__ int wrap_rotation(int g, int n) {
+    return g < 1 ? 1 - g : (g + 2 >= n ? 4 - (g + 2 - (n - 1)) : 0);
+}
+
+template <bool POW2>
+__global__ __launch_bounds__(st3::NT) void k_stencil3d(const float* __restrict__ A, const float* __restrict__ U,
+                                                       float* __restrict__ Aout, int C, int W, int H, int D,
+                                                       long long N, int tiles_x, int tiles_y, int ntiles, float dt,
+                                                       float beta_a, float nexp, float s, float tv, float inv_tv,
+                                                       int* __restrict__ flag) {
+    using namespace st3;
+    __shared__ float sSum[NLOAD];
+    __shared__ float sUc[NLOAD];
+    __shared__ float sAl[NSRC], sGx[NSRC], sGy[NSRC], sGz[NSRC];
+    __shared__ float sM[NSRC];
+    __shared__ float sOX[4][NSRC], sOY[4][NSRC], sOZ[4][NSRC];  // overlap j of each source, j-major
+    const int b = blockIdx.x;
+    const int per = (ntiles + 7) / 8;
+    const int tile = (b % 8) * per + b / 8;  // XCD-aware order, as the 2D kernel
+    if (tile >= ntiles) return;
+    const int x0 = (tile % tiles_x) * TX, y0 = ((tile / tiles_x) % tiles_y) * TY, z0 = (tile / (tiles_x * tiles_y)) * TZ;
+    const int tid = threadIdx.x;
+    const int HW = W * H;
+    auto gidx = [&](int zz, int yy, int xx) -> long long {
+        return (long long)wrap_near(zz, D) * HW + (long long)wrap_near(yy, H) * W + wrap_near(xx, W);
+    };
+    // channel sum over the load region (propagator3d.rs: 0 + A_0 + A_1 + ...)
+    for (int t = tid; t < NLOAD; t += NT) {
+        const int lz = t / (LX * LY), r = t - lz * LX * LY, ly = r / LX, lx = r - ly * LX;
+        const long long gi = gidx(z0 - 2 + lz, y0 - 2 + ly, x0 - 2 + lx);
+        float v = 0.0f;
+        for (int c = 0; c < C; ++c) v += A[(long long)c * N + gi];
+        sSum[t] = v;
+    }
+    __syncthreads();
+    // channel-independent source data: alpha and the mass gradient
+#pragma unroll 1
+    for (int q = 0; q < SPT; ++q) {
+        const int i = tid + q * NT;
+        if (i >= NSRC) break;
+        const int sz = i / (SX * SY), r = i - sz * SX * SY, sy = r / SX, sx = r - sy * SX;
+        const int li = ((sz + 1) * LY + (sy + 1)) * LX + (sx + 1);
+        float gx, gy, gz;
+        sobel27_lds(sSum, li, LX, LX * LY, gx, gy, gz);
+        sGx[i] = gx;
+        sGy[i] = gy;
+        sGz[i] = gz;
+        sAl[i] = alpha_of(sSum[li], beta_a, nexp);
+    }
+    // destination of this thread and its visiting order
+    const int lx = tid % TX, ly = (tid / TX) % TY, lz = tid / (TX * TY);
+    const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+    const bool inside = gx < W && gy < H && gz < D;
+    const int rx = wrap_rotation(gx, W), ry = wrap_rotation(gy, H), rz = wrap_rotation(gz, D);
+    int local_flag = 0;
+    for (int c = 0; c < C; ++c) {
+        const float* Uc = U + (long long)c * N;
+        __syncthreads();  // previous channel's gather has read the source data
+        for (int t = tid; t < NLOAD; t += NT) {
+            const int tz = t / (LX * LY), r = t - tz * LX * LY, ty = r / LX, tx = r - ty * LX;
+            sUc[t] = Uc[gidx(z0 - 2 + tz, y0 - 2 + ty, x0 - 2 + tx)];
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int q = 0; q < SPT; ++q) {
+            const int i = tid + q * NT;
+            if (i >= NSRC) break;
+            const int sz = i / (SX * SY), r = i - sz * SX * SY, sy = r / SX, sx = r - sy * SX;
+            const int ux = x0 - 1 + sx, uy = y0 - 1 + sy, uz = z0 - 1 + sz;
+            const int gxs = wrap_near(ux, W), gys = wrap_near(uy, H), gzs = wrap_near(uz, D);
+            const float m = A[(long long)c * N + (long long)gzs * HW + (long long)gys * W + gxs];
+            if (fabsf(m) < 1e-10f) {  // reintegration3d.rs:77: a skipped source adds nothing
+                sM[i] = 0.0f;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) sOX[j][i] = sOY[j][i] = sOZ[j][i] = 0.0f;
+                continue;
+            }
+            const int li = ((sz + 1) * LY + (sy + 1)) * LX + (sx + 1);
+            float gux, guy, guz;
+            sobel27_lds(sUc, li, LX, LX * LY, gux, guy, guz);
+            const float al = sAl[i];
+            const float oma = 1.0f - al;
+            const float fx = oma * gux - al * sGx[i];
+            const float fy = oma * guy - al * sGy[i];
+            const float fz = oma * guz - al * sGz[i];
+            const float dxp = (float)gxs + dt * fx;  // reintegration3d.rs:83-85
+            const float dyp = (float)gys + dt * fy;
+            const float dzp = (float)gzs + dt * fz;
+            const float xm = dxp - s, xM = dxp + s, ym = dyp - s, yM = dyp + s, zm = dzp - s, zM = dzp + s;
+            float o[3][4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {  // cells gxs + 1 - j of the source's frame
+                o[0][j] = overlap1((float)(gxs + 1 - j), (float)(gxs + 2 - j), xm, xM);
+                o[1][j] = overlap1((float)(gys + 1 - j), (float)(gys + 2 - j), ym, yM);
+                o[2][j] = overlap1((float)(gzs + 1 - j), (float)(gzs + 2 - j), zm, zM);
+            }
+            sM[i] = m;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                sOX[j][i] = o[0][j];
+                sOY[j][i] = o[1][j];
+                sOZ[j][i] = o[2][j];
+            }
+            if (ux >= x0 && ux < x0 + TX && uy >= y0 && uy < y0 + TY && uz >= z0 && uz < z0 + TZ && ux < W && uy < H &&
+                uz < D) {
+                if (!(isfinite(xm) && isfinite(xM) && isfinite(ym) && isfinite(yM) && isfinite(zm) && isfinite(zM) &&
+                      isfinite(m)))
+                    local_flag |= 2;
+                else if (floorf(xm) < (float)(gxs - 2) || ceilf(xM) > (float)(gxs + 2) || floorf(ym) < (float)(gys - 2) ||
+                         ceilf(yM) > (float)(gys + 2) || floorf(zm) < (float)(gzs - 2) || ceilf(zM) > (float)(gzs + 2))
+                    local_flag |= 1;
+            }
+        }
+        __syncthreads();
+        if (!inside) continue;
+        float acc = 0.0f;
+        // the destination is cell gxs - o of a source's frame: that source's overlap index 1 + o
+        for (int az = 0; az < 4; ++az) {
+            const int oz = ((az + rz) & 3) - 1;
+            const float* wzs = sOZ[1 + oz];
+            for (int ay = 0; ay < 4; ++ay) {
+                const int oy = ((ay + ry) & 3) - 1;
+                const float* wys = sOY[1 + oy];
+                const int row = ((lz + 1 + oz) * SY + (ly + 1 + oy)) * SX + lx + 1;
+#pragma unroll
+                for (int ax = 0; ax < 4; ++ax) {
+                    const int ox = ((ax + rx) & 3) - 1;
+                    const int si = row + ox;
+                    const float vol = sOX[1 + ox][si] * wys[si] * wzs[si];
+                    const float frac = POW2 ? vol * inv_tv : vol / tv;
+                    acc += sM[si] * frac;
+                }
+            }
+        }
+        Aout[(long long)c * N + (long long)gz * HW + (long long)gy * W + gx] = acc;
+    }
+    if (local_flag) atomicOr(flag, local_flag);
+}
+
+bool stencil3d_supported(const Geo& g, float s) {
+    const float tv = (2.0f * s) * (2.0f * s) * (2.0f * s);
+    return g.D > 1 && g.W >= 4 && g.H >= 4 && g.D >= 4 && s > 0.0f && s <= 1.0f && tv >= 1e-10f;
+}
+
+bool launch_stencil3d(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g, float dt,
+                      float beta_a, float nexp, float s, int* flag) {
+    using namespace st3;
+    if (!stencil3d_supported(g, s)) return false;
+    const float tv = (2.0f * s) * (2.0f * s) * (2.0f * s);  // reintegration3d.rs:134
+    int e;
+    const bool pow2 = frexpf(tv, &e) == 0.5f && tv >= 1.17549435e-38f && tv < INFINITY;
+    const int tiles_x = (g.W + TX - 1) / TX, tiles_y = (g.H + TY - 1) / TY, tiles_z = (g.D + TZ - 1) / TZ;
+    const int ntiles = tiles_x * tiles_y * tiles_z;
+    const dim3 grid((ntiles + 7) / 8 * 8);
+    if (pow2)
+        hipLaunchKernelGGL((k_stencil3d<true>), grid, dim3(NT), 0, st, A, U, Aout, C, g.W, g.H, g.D, g.N, tiles_x,
+                           tiles_y, ntiles, dt, beta_a, nexp, s, tv, 1.0f / tv, flag);
+    else
+        hipLaunchKernelGGL((k_stencil3d<false>), grid, dim3(NT), 0, st, A, U, Aout, C, g.W, g.H, g.D, g.N, tiles_x,
+                           tiles_y, ntiles, dt, beta_a, nexp, s, tv, 1.0f / tv, flag);
+    return true;
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -49,6 +49,8 @@ def main() -> int:
                          "--grad-steps steps with a checkpoint every --grad-every, then the backward pass")
     ap.add_argument("--grad-steps", type=int, default=64)
     ap.add_argument("--grad-every", type=int, default=8)
+    ap.add_argument("--sphere3d", action="store_true",
+                    help="3D mode: 128^3, one channel, one shell kernel (prints its own JSON line)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of 256 MiB ten times (calibrates rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE against a kernel of known traffic)")
@@ -63,6 +65,8 @@ def main() -> int:
         return population_bench(args)
     if args.grad:
         return grad_bench(args)
+    if args.sphere3d:
+        return sphere3d_bench(args)
 
     import numpy as np
 
@@ -300,6 +304,47 @@ def grad_bench(args) -> int:
         "backward_over_forward": (t2 - t1) / secs_plain,
         "gradients_finite": all(bool(np.all(np.isfinite(t))) for t in dth),
     }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def sphere3d_bench(args) -> int:
+    """3D workload: 128^3, one channel, one shell kernel, R=10, dt=0.1, Gaussian sphere seed
+    (autoverse_amd/workloads.sphere_3d).  Steps default to 200 here."""
+    import numpy as np
+
+    import autoverse_amd as av
+    from autoverse_amd import workloads
+
+    cfg, seed = workloads.sphere_3d(128)
+    N = cfg.width * cfg.height * cfg.depth
+    K = args.steps if args.steps != 1000 else 200
+    prop = av.HipPropagator(cfg)
+    prop.set_state(av.SimulationState.from_seed(seed, cfg))
+    prop.advance(args.warmup)
+    tune = None
+    if not args.no_tune:
+        best, table = prop.autotune(steps=10)
+        tune = {"plan": best, "candidates": len(table)}
+    t0 = time.perf_counter()
+    prop.advance(K)
+    secs = time.perf_counter() - t0
+    radius, general = prop.stats()
+    prop.profile_enable(True)
+    prop.advance(min(K, 50))
+    prof = prop.profile_read()
+    prop.profile_enable(False)
+    kbytes = prop.kernel_bytes()
+    per_kernel = {k: {"launches": n, "avg_us": round(ms * 1e3 / n, 3), "bytes_per_launch": kbytes[k]}
+                  for k, (ms, n) in prof.items() if n}
+    _, mass = prop.mass_f64()
+    prop.close()
+    out = {"metric": "cell-steps/s at 128^3 (3D HipPropagator forward step)", "value": N * K / secs,
+           "unit": "cell-steps/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+           "ms_per_step": secs * 1e3 / K, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "sphere 128^3, 1 channel, shell kernel R=10, dt=0.1", "plan": tune,
+                      "max_gather_radius": radius, "general_path_steps": general, "mass_f64_after": mass},
+           "per_kernel": per_kernel}
     print(json.dumps(out), flush=True)
     return 0
 

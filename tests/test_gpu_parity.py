@@ -243,3 +243,31 @@ def test_3d_small_vs_oracle(oracle):
     np.testing.assert_array_equal(got_nxt[0], nxt[0])
     assert ea <= AFF_TOL
     assert e <= 1e-4
+
+
+@pytest.mark.parametrize("shape,channels,dt", [((24, 16, 8), 2, 0.1), ((8, 8, 8), 1, 0.1), ((32, 24, 16), 3, 0.2),
+                                               ((16, 16, 16), 1, 40.0)])
+def test_3d_stencil_bitwise(oracle, shape, channels, dt):
+    """The fused 3D stencil (64-source gather) on random states: the oracle's next state bit for
+    bit; a large dt leaves the fused window and must come back through the general path."""
+    from autoverse_amd.schema import FlowConfig, SimulationConfig
+    W, H, D = shape
+    _say(f"[3d_stencil] {shape} C={channels} dt={dt}")
+    rng = np.random.default_rng(W * H + D + channels)
+    ks = workloads._random_kernels(rng, channels)
+    cfg = SimulationConfig(width=W, height=H, depth=D, channels=channels, dt=dt, kernel_radius=3, kernels=ks,
+                           flow=FlowConfig(1.0, 2.0, 1.0))
+    ch = [rng.random(W * H * D).astype(np.float32) for _ in range(channels)]
+    p = oracle.Propagator(cfg, np.float32)
+    aff = p.affinity(ch)
+    want = p.stencil(ch, aff)
+    with av.HipPropagator(cfg) as hp:
+        got = hp.stencil(ch, aff)
+        r, g = hp.stats()
+    for c in range(channels):
+        np.testing.assert_array_equal(got[c], want[c])
+    if dt >= 3.0:
+        assert g >= 1  # the fused window was left: the general path produced the result
+    else:
+        assert g == 0
+
