@@ -227,7 +227,7 @@ hp_status enqueue_affinity(hp_ctx* c, const float* A) {
 
 bool fast_stencil_ok(const hp_ctx* c) {
     return c->is3d ? hp::stencil3d_supported(c->g, c->cfg.flow.distribution_size)
-                   : hp::stencil2d_supported(c->g, c->cfg.flow.distribution_size);
+                   : hp::stencil2d_supported(c->g, c->cfg.flow.distribution_size, c->tile);
 }
 
 hp_status enqueue_fast_stencil(hp_ctx* c, const float* A, const float* U, float* out) {
@@ -502,6 +502,11 @@ hp_status create(const hp_config* cfg, int device, const hp_plan* req, hp_ctx** 
     c->fv.rp = want.radix_order;
     c->fv.cs = want.column_split;
     c->tile = want.stencil_tile;
+    // a tile whose strips need more rows than the grid has falls back to the default shape
+    if (!plan.is3d && !hp::stencil2d_supported(plan.g, cfg->flow.distribution_size, c->tile) &&
+        hp::stencil2d_supported(plan.g, cfg->flow.distribution_size, 0))
+        c->tile = 0;
+    c->plan.stencil_tile = c->tile;
     c->g = plan.g;
     const hp::Geo& g = c->g;
     const bool is3d = c->is3d;

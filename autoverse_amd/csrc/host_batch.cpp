@@ -209,7 +209,7 @@ hp_status member_stencil(hp_batch* b, int i, int cur) {
     const hp::MemberDesc& m = b->desc[i];
     const HostMember& hm = b->members[i];
     const float s = hm.cfg.flow.distribution_size;
-    bool general = !hp::stencil2d_supported(m.g, s);
+    bool general = !hp::stencil2d_supported(m.g, s, b->tile);
     if (!general) {
         hp::launch_clear_ints(b->stream, b->d_flags, 1);
         hp::launch_stencil2d(b->stream, m.A[cur], m.U, m.A[1 - cur], m.C, m.g, m.dt, m.beta_a, m.nexp, s, b->d_flags,
@@ -851,6 +851,9 @@ hp_status batch_set_plan(hp_batch* b, const hp_plan& p) {
     if (p.radix_order < 0 || p.radix_order > 1 || p.column_split < 1 || p.column_split > 2 || p.stencil_tile < 0 ||
         p.stencil_tile >= hp::stencil2d_tile_count())
         return b->fail(HP_ERR_SHAPE, "plan field out of range");
+    for (const HostMember& hm : b->members)
+        if (!hp::stencil2d_supported(hm.plan.g, hm.cfg.flow.distribution_size, p.stencil_tile))
+            return b->fail(HP_ERR_UNSUPPORTED_SIZE, "stencil tile does not fit every member's grid");
     BTRY(b, hipStreamSynchronize(b->stream));
     b->fv.rp = p.radix_order;
     b->fv.cs = p.column_split;
@@ -915,7 +918,7 @@ hp_status batch_autotune(hp_batch* b, uint32_t steps, hp_plan* best_out, uint32_
     const hp_plan start = batch_plan(b);
     hp_plan best = start;
     auto measure = [&](const hp_plan& p) -> hp_status {
-        STRY(batch_set_plan(b, p));
+        if (batch_set_plan(b, p) == HP_ERR_UNSUPPORTED_SIZE) return HP_OK;  // skipped candidate
         STRY(save(true));
         STRY(advance(b, 2));
         STRY(save(true));

@@ -124,7 +124,7 @@ bool launch_rows_inv_t(hipStream_t st, const float2* YT, float* U, const TgtWork
 // Fused channel sum + Sobel + flow + gather reintegration with a fixed gather
 // radius; sets *flag when any active source's box leaves that radius (bit 0) or
 // is not finite (bit 1).  Returns false if the configuration is not covered.
-bool stencil2d_supported(const Geo& g, float s);
+bool stencil2d_supported(const Geo& g, float s, int tile = 0);
 // Batched fused stencil: items[i] = (member, tile); members of one launch share C and the
 // power-of-two-area property.
 bool launch_batch_stencil2d(hipStream_t st, int C, bool pow2, const MemberDesc* mem, const int4* items, int nitems,

@@ -65,10 +65,9 @@ using StTile0 = StTile<64, 16, 512>;
 using StTile1 = StTile<64, 8, 256>;
 using StTile2 = StTile<32, 16, 256>;
 using StTile3 = StTile<128, 8, 512>;
-constexpr int kStencilTiles = 4;
-static_assert(StTile0::SROWS == StTile1::SROWS && StTile0::SROWS == StTile2::SROWS &&
-                  StTile0::SROWS == StTile3::SROWS,
-              "every tile shape covers the same grids");
+using StTile4 = StTile<64, 16, 256>;  // 4-destination strips: fewer source loads per destination
+using StTile5 = StTile<128, 8, 256>;
+constexpr int kStencilTiles = 6;
 
 // CC: channel count known at compile time (1..4), or 0 for any count (masses re-read per channel).
 template <int CC, bool POW2, class TL>
@@ -200,7 +199,10 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
                 ox[j] = overlap1((float)ix, (float)(ix + 1), xm, xM);
                 oyv[j] = overlap1((float)iy, (float)(iy + 1), ym, yM);
             }
-            sM[i] = m;
+            // With (2s)^2 a power of two the scaling by its inverse is exact (overlaps are 0 or at
+            // least ~6e-8 and |m| >= 1e-10, so nothing is subnormal): m * ((ow*oh) * inv) equals
+            // (m * inv) * (ow*oh) bit for bit, and the gather saves a multiply per pair.
+            sM[i] = POW2 ? m * inv_ta : m;
             sOX[0][i] = ox[0];
             sOX[1][i] = ox[1];
             sOX[2][i] = ox[2];
@@ -248,8 +250,7 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
                         for (int r = 0; r < SR; ++r) {
                             if (a - r >= 0 && a - r <= 3) {
                                 const float area = ow * ohv[a - r];
-                                const float frac = POW2 ? area * inv_ta : area / ta;
-                                acc[r] += m * frac;
+                                acc[r] += POW2 ? m * area : m * (area / ta);
                             }
                         }
                     }
@@ -273,8 +274,7 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
                             if (jy >= 0 && jy <= 3) {
                                 const float oh = jy == 0 ? oh4.x : jy == 1 ? oh4.y : jy == 2 ? oh4.z : oh4.w;
                                 const float area = ow * oh;
-                                const float frac = POW2 ? area * inv_ta : area / ta;
-                                acc[r] += m * frac;
+                                acc[r] += POW2 ? m * area : m * (area / ta);
                             }
                         }
                     }
@@ -333,6 +333,14 @@ __global__ __launch_bounds__(TL::NT) void kb_stencil2d(const MemberDesc* __restr
             using TL = StTile3;                  \
             __VA_ARGS__;                         \
         } break;                                 \
+        case 4: {                                \
+            using TL = StTile4;                  \
+            __VA_ARGS__;                         \
+        } break;                                 \
+        case 5: {                                \
+            using TL = StTile5;                  \
+            __VA_ARGS__;                         \
+        } break;                                 \
         default: {                               \
             using TL = StTile0;                  \
             __VA_ARGS__;                         \
@@ -388,12 +396,14 @@ static void launch_st2(hipStream_t st, bool pow2, dim3 grid, const float* A, con
                            tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag);
 }
 
-bool stencil2d_supported(const Geo& g, float s) {
+bool stencil2d_supported(const Geo& g, float s, int tile) {
     // the rotated visiting orders need every strip's source rows / columns to be distinct cells;
     // the 16-source window assumes s <= 1; the point-distribution fallback (total area < 1e-10)
     // lives in the general path
     const float ta = (2.0f * s) * (2.0f * s);
-    return g.D == 1 && g.W >= 4 && g.H >= StTile0::SROWS && s > 0.0f && s <= 1.0f && ta >= 1e-10f;
+    int srows = 0;
+    HP_TILE(tile, srows = TL::SROWS);
+    return g.D == 1 && g.W >= 4 && g.H >= srows && s > 0.0f && s <= 1.0f && ta >= 1e-10f;
 }
 
 template <class TL>
@@ -418,7 +428,7 @@ static void launch_k_st2(hipStream_t st, const float* A, const float* U, float* 
 
 bool launch_stencil2d(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g, float dt,
                       float beta_a, float nexp, float s, int* flag, int tile) {
-    if (!stencil2d_supported(g, s)) return false;
+    if (!stencil2d_supported(g, s, tile)) return false;
     HP_TILE(tile, launch_k_st2<TL>(st, A, U, Aout, C, g, dt, beta_a, nexp, s, flag));
     return true;
 }

@@ -4,9 +4,9 @@
 // the growth epilogue; CachedConvolver::convolve_with_kernel_scratch, fft.rs:288-324,
 // and growth_accumulate, growth.rs:44-59), laid out for the GPU differently:
 //
-//   * spectra are stored column-major, XT[slot][kx][y] -- one kx column of H complex
-//     values is one contiguous line, so the y pass reads and writes whole lines with
-//     unit-stride, fully coalesced accesses;
+//   * spectra are stored blocked column-major, XT[slot][y / 8][kx][y % 8]: the 8-row tile of
+//     a row pass is one contiguous run, and a y-pass workgroup (4 adjacent kx columns) reads
+//     and writes 256-byte runs per 8 rows;
 //   * every line transform is done by ONE wavefront: lane l holds the E = L/64 elements
 //     l, l+64, l+128, ... in registers, the radix-16/8/4 Stockham stages run in registers
 //     and exchange data through a per-wave LDS line (no workgroup barriers), and the first
@@ -167,8 +167,8 @@ __device__ __forceinline__ void wave_fft(float2 (&e)[L / WAVE], float2* line, co
 }
 
 // XCD-aware block order for the row passes: workgroup b runs on XCD b % 8, so give every XCD a
-// contiguous run of row blocks -- adjacent blocks read the two halves of the same 128-byte
-// lines of a column-major spectrum, and then meet in the same L2.
+// contiguous run of row blocks (neighbouring tiles of the spectrum and of the state meet in the
+// same L2).
 __device__ __forceinline__ bool xcd_block(int nbx, int total, int& bx, int& by) {
     const int b = blockIdx.x;
     const int per = (total + 7) / 8;
@@ -183,8 +183,8 @@ constexpr int WAVES = 4;  // waves per workgroup
 constexpr int NT = WAVES * WAVE;
 constexpr int ROWS = 2 * WAVES;  // rows per workgroup in the row passes (one packed pair per wave)
 
-// ---- pass 1: R2C of 8 rows, written column-major ------------------------------------------
-// grid (H / 8, nsrc).  XT[slot][kx][y], kx < W/2+1, line pitch H.
+// ---- pass 1: R2C of 8 rows, written as one blocked column-major tile ------------------------
+// grid (H / 8, nsrc).  XT[slot][y / 8][kx][y % 8], kx < W/2+1.
 template <int L>
 __global__ __launch_bounds__(NT) void k_rows_fwd_t(const float* __restrict__ A, float2* __restrict__ XT,
                                                    const int* __restrict__ src_chan, int H, long long N,
@@ -232,11 +232,9 @@ __global__ __launch_bounds__(NT) void k_rows_fwd_t(const float* __restrict__ A, 
         }
     }
     __syncthreads();
-    float2* xs = XT + (long long)s * spec + ybase;
-    for (int t = threadIdx.x; t < LH * ROWS; t += NT) {
-        const int k = t / ROWS, r = t % ROWS;
-        xs[(long long)k * H + r] = tile[t];
-    }
+    // blocked column-major: the tile [kx][8 rows] is one contiguous run of the spectrum
+    float2* xs = XT + (long long)s * spec + (long long)bx * LH * ROWS;
+    for (int t = threadIdx.x; t < LH * ROWS; t += NT) xs[t] = tile[t];
 }
 
 // ---- pass 2: y FFT of one kx column per wave, x folded kernel spectrum, inverse y FFT ------
@@ -256,10 +254,13 @@ __global__ __launch_bounds__(NT) void k_spectral_t(const float2* __restrict__ XT
     __syncthreads();
     if (kx >= Wh) return;  // after the only block-wide barrier
     float2* line = lines + wave * LINE;
-    const float2* xs = XT + (long long)wk.src_slot * spec + (long long)kx * L;
+    // blocked column-major element (kx, y): (y / 8) * Wh * 8 + kx * 8 + y % 8; lane + 64 i = y
+    const long long col = (long long)(lane >> 3) * Wh * ROWS + (long long)kx * ROWS + (lane & 7);
+    const long long ystep = (long long)(WAVE / ROWS) * Wh * ROWS;  // 64 rows further
+    const float2* xs = XT + (long long)wk.src_slot * spec + col;
     float2 f[E];
 #pragma unroll
-    for (int i = 0; i < E; ++i) f[i] = xs[lane + WAVE * i];
+    for (int i = 0; i < E; ++i) f[i] = xs[i * ystep];
     wave_fft<L, -1>(f, line, tw);
     for (int j = 0; j < wk.k_count; ++j) {
         const int k = klist[wk.k_first + j];
@@ -272,9 +273,9 @@ __global__ __launch_bounds__(NT) void k_spectral_t(const float2* __restrict__ XT
             e[i] = make_float2(f[i].x * kv, f[i].y * kv);
         }
         wave_fft<L, +1>(e, line, tw);
-        float2* ys = YT + (long long)k * spec + (long long)kx * L;
+        float2* ys = YT + (long long)k * spec + col;
 #pragma unroll
-        for (int i = 0; i < E; ++i) ys[lane + WAVE * i] = e[i];
+        for (int i = 0; i < E; ++i) ys[i * ystep] = e[i];
     }
 }
 
@@ -306,12 +307,12 @@ __global__ __launch_bounds__(NT) void k_rows_inv_t(const float2* __restrict__ YT
     // the next kernel's tile is loaded into registers while the current one is transformed
     constexpr int TPT = (LH * ROWS + NT - 1) / NT;
     float2 pre[TPT];
-    auto fetch = [&](int k) {
-        const float2* ys = YT + (long long)k * spec + ybase;
+    auto fetch = [&](int k) {  // the block's tile is one contiguous run (blocked column-major)
+        const float2* ys = YT + (long long)k * spec + (long long)bx * LH * ROWS;
 #pragma unroll
         for (int q = 0; q < TPT; ++q) {
             const int t = threadIdx.x + q * NT;
-            if (t < LH * ROWS) pre[q] = ys[(long long)(t / ROWS) * H + (t % ROWS)];
+            if (t < LH * ROWS) pre[q] = ys[t];
         }
     };
     if (wk.k_count > 0) fetch(klist[wk.k_first]);

@@ -211,7 +211,8 @@ typedef struct hp_plan {
                                -1: chosen from the work size */
     int32_t radix_order;    /* LDS FFTs: 0 radix 8 first, 1 radix 4 first */
     int32_t column_split;   /* LDS spectral pass: 1 or 2 (half the columns per workgroup) */
-    int32_t stencil_tile;   /* fused 2D stencil tile: 0 64x16, 1 64x8, 2 32x16, 3 128x8 */
+    int32_t stencil_tile;   /* fused 2D stencil tile (destinations / threads): 0 64x16/512, 1 64x8/256,
+                               2 32x16/256, 3 128x8/512, 4 64x16/256, 5 128x8/256 */
 } hp_plan;
 
 /* The context's resolved plan (fft_path and spectral_group as actually used). */

@@ -29,7 +29,7 @@ def _evolve(O, cfg, seed, steps):
     return p, ch
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
 def test_every_stencil_tile_is_bitwise_exact(oracle, tile):
     _say(f"[tile] {tile}")
     cfg, sd = workloads.multispecies(size=96, channels=3, seed=12)
