@@ -21,6 +21,15 @@ __device__ __forceinline__ int wrap_near(int c, int n) {
     return c;
 }
 
+// Wrap c into [0, n).  `multi` is uniform over the workgroup: false when the caller knows
+// c lies in [-n, 2n) (one conditional step, no loop and no division), true for grids so
+// small that a tile window spans them more than once.
+__device__ __forceinline__ int wrap_tile(int c, int n, bool multi) {
+    if (multi) return wrapi(c, n);
+    c = c < 0 ? c + n : c;
+    return c >= n ? c - n : c;
+}
+
 // Rust `f as i32`: saturating, NaN -> 0.
 __device__ __forceinline__ int sat_i32(float v) {
     if (v != v) return 0;

@@ -59,6 +59,11 @@ struct StTile {
     static constexpr int LPT = (NLOAD + NT - 1) / NT;  // load-region cells per thread
     static constexpr int SPT = (NSRC + NT - 1) / NT;   // sources per thread
     static constexpr int SROWS = SR + 3;               // source rows of one strip
+    // LDS: two affinity tiles, and per source its mass and four x and four y overlaps
+    static constexpr int LDS = 4 * (2 * NLOAD + 9 * NSRC);
+    static constexpr int WG_PER_CU = 163840 / LDS;  // 160 KiB of LDS per CU
+    // waves per SIMD that LDS admits: registers are capped so that they do not admit fewer
+    static constexpr int WAVES_PER_EU = WG_PER_CU * NT / 256 < 1 ? 1 : (WG_PER_CU * NT / 256 > 8 ? 8 : WG_PER_CU * NT / 256);
     static_assert(TX * TY == NT * SR && NT % TX == 0, "one strip per thread");
 };
 using StTile0 = StTile<64, 16, 512>;
@@ -70,6 +75,13 @@ using StTile5 = StTile<128, 8, 256>;
 constexpr int kStencilTiles = 6;
 
 // CC: channel count known at compile time (1..4), or 0 for any count (masses re-read per channel).
+//
+// Per-thread work is laid out so that nothing channel-independent is recomputed per channel:
+// the alpha phase leaves each owned source's 1 - alpha, alpha * dSum/dx and alpha * dSum/dy in
+// registers (the thread that owns source i in the alpha phase owns it in every channel's source
+// phase), and the four y overlaps are stored planar so that the gather reads only the ones a
+// strip row uses.  The channel-sum tile shares memory with the second affinity buffer, which
+// is first written after the alpha phase has read it.
 template <int CC, bool POW2, class TL>
 __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, const float* __restrict__ U,
                                                float* __restrict__ Aout, int Cdyn, int W, int H, long long N,
@@ -78,18 +90,18 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
     constexpr int TX = TL::TX, TY = TL::TY, NT = TL::NT, SR = TL::SR, SW = TL::SW;
     constexpr int LW = TL::LW, NSRC = TL::NSRC, NLOAD = TL::NLOAD, LPT = TL::LPT, SPT = TL::SPT;
     constexpr int SROWS = TL::SROWS;
-    (void)TY;
-    (void)SROWS;
     const int C = CC > 0 ? CC : Cdyn;
-    __shared__ float sSum[NLOAD];
     __shared__ float sU[2][NLOAD];
-    __shared__ float sAl[NSRC], sGx[NSRC], sGy[NSRC];
     __shared__ float sM[NSRC];
     __shared__ float sOX[4][NSRC];
-    __shared__ float4 sOY[NSRC];
+    __shared__ float sOY[4][NSRC];
+    float* const sSum = sU[1];
 
     const int x0 = (tile % tiles_x) * TX, y0 = (tile / tiles_x) * TY;
     const int tid = threadIdx.x;
+    // load-region coordinates lie in [-2, n + TX + 1): one wrap step suffices unless the grid is
+    // narrower than the tile plus its halo
+    const bool multi = W < TX + 2 || H < TY + 2;
 
     // ---- every global read of the tile, issued together ----
     int goff[LPT];
@@ -97,21 +109,22 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
     for (int q = 0; q < LPT; ++q) {
         const int i = tid + q * NT;
         const int ly = i / LW, lx = i - ly * LW;
-        goff[q] = i < NLOAD ? wrap_near(y0 - 2 + ly, H) * W + wrap_near(x0 - 2 + lx, W) : 0;
+        goff[q] = i < NLOAD ? wrap_tile(y0 - 2 + ly, H, multi) * W + wrap_tile(x0 - 2 + lx, W, multi) : 0;
     }
-    int soff[SPT];
+    int sgx[SPT], sgy[SPT];  // wrapped grid coordinates of the owned sources
 #pragma unroll
     for (int q = 0; q < SPT; ++q) {
         const int i = tid + q * NT;
         const int sy = i / SW, sx = i - sy * SW;
-        soff[q] = i < NSRC ? wrap_near(y0 - 1 + sy, H) * W + wrap_near(x0 - 1 + sx, W) : 0;
+        sgx[q] = i < NSRC ? wrap_tile(x0 - 1 + sx, W, multi) : 0;
+        sgy[q] = i < NSRC ? wrap_tile(y0 - 1 + sy, H, multi) : 0;
     }
     float msrc[CC > 0 ? CC : 1][SPT];
     if (CC > 0) {
 #pragma unroll
         for (int c = 0; c < (CC > 0 ? CC : 1); ++c)
 #pragma unroll
-            for (int q = 0; q < SPT; ++q) msrc[c][q] = A[(long long)c * N + soff[q]];
+            for (int q = 0; q < SPT; ++q) msrc[c][q] = A[(long long)c * N + (long long)sgy[q] * W + sgx[q]];
     }
     float upre[LPT];
 #pragma unroll
@@ -132,19 +145,32 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
         }
     }
     __syncthreads();
+    // mass gradient and alpha of the owned sources (flow.rs:68-87: F = (1-a) dU - a dSum).  The
+    // loop stays rolled (alpha_of may expand to a powf); each thread parks its own results in
+    // the overlap planes, which are free until the first source phase, and reads them back
+    // into registers -- the same thread writes and reads every entry, so no barrier is needed.
 #pragma unroll 1
-    for (int i = tid; i < NSRC; i += NT) {  // mass gradient and alpha over the source region
+    for (int i = tid; i < NSRC; i += NT) {
         const int sy = i / SW, sx = i - sy * SW;
         const float* r0 = sSum + sy * LW + sx;
         const float* r1 = r0 + LW;
         const float* r2 = r1 + LW;
         float gx, gy;
         sobel9(r0[0], r0[1], r0[2], r1[0], r1[2], r2[0], r2[1], r2[2], gx, gy);
-        sGx[i] = gx;
-        sGy[i] = gy;
-        sAl[i] = alpha_of(r1[1], beta_a, nexp);
+        const float al = alpha_of(r1[1], beta_a, nexp);
+        sOX[0][i] = 1.0f - al;
+        sOX[1][i] = al * gx;
+        sOX[2][i] = al * gy;
     }
-    __syncthreads();
+    float aOm[SPT], aGx[SPT], aGy[SPT];
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) {
+        const int i = tid + q * NT;
+        const bool ok = i < NSRC;
+        aOm[q] = ok ? sOX[0][i] : 0.0f;
+        aGx[q] = ok ? sOX[1][i] : 0.0f;
+        aGy[q] = ok ? sOX[2][i] : 0.0f;
+    }
 
     // strip geometry: destination column gdx, rows gdy0 .. gdy0 + SR - 1
     const int tx = tid % TX, ty0 = (tid / TX) * SR;
@@ -154,6 +180,7 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
     const int roty = gdy0 < 1 ? 1 - gdy0 : (gdy0 + SR + 1 >= H ? SROWS - (gdy0 + SR + 1 - (H - 1)) : 0);
 
     int local_flag = 0;
+#pragma unroll 1
     for (int c = 0; c < C; ++c) {
         const float* su = sU[c & 1];
         // ---- sources: flow, box, overlaps ----
@@ -162,8 +189,6 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
             const int i = tid + q * NT;
             if (i >= NSRC) break;
             const int sy = i / SW, sx = i - sy * SW;
-            const int ux = x0 - 1 + sx, uy = y0 - 1 + sy;  // unwrapped source coordinates
-            const int gxs = wrap_near(ux, W), gys = wrap_near(uy, H);
             float m;
             if (CC > 0) {
                 m = 0.0f;
@@ -171,12 +196,12 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
                 for (int cc = 0; cc < (CC > 0 ? CC : 1); ++cc)
                     if (cc == c) m = msrc[cc][q];
             } else {
-                m = A[(long long)c * N + soff[q]];
+                m = A[(long long)c * N + (long long)sgy[q] * W + sgx[q]];
             }
             if (fabsf(m) < 1e-10f) {  // reintegration.rs:76: a skipped source adds nothing
                 sM[i] = 0.0f;
-                sOX[0][i] = sOX[1][i] = sOX[2][i] = sOX[3][i] = 0.0f;
-                sOY[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) sOX[j][i] = sOY[j][i] = 0.0f;
                 continue;
             }
             const float* r0 = su + sy * LW + sx;
@@ -184,35 +209,33 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
             const float* r2 = r1 + LW;
             float gux, guy;
             sobel9(r0[0], r0[1], r0[2], r1[0], r1[2], r2[0], r2[1], r2[2], gux, guy);
-            const float al = sAl[i];
-            const float oma = 1.0f - al;
-            const float fx = oma * gux - al * sGx[i];
-            const float fy = oma * guy - al * sGy[i];
-            const float dxp = (float)gxs + dt * fx;  // reintegration.rs:81-82
-            const float dyp = (float)gys + dt * fy;
+            const float fx = aOm[q] * gux - aGx[q];
+            const float fy = aOm[q] * guy - aGy[q];
+            // integer-valued floats below 2^24: (float)(g + k) == (float)g + k exactly
+            const float cx = (float)sgx[q], cy = (float)sgy[q];
+            const float dxp = cx + dt * fx;  // reintegration.rs:81-82
+            const float dyp = cy + dt * fy;
             const float xm = dxp - s, xM = dxp + s, ym = dyp - s, yM = dyp + s;
-            // overlaps with destination cells gxs - o, o = j - 1 (cell index in the source's frame)
-            float ox[4], oyv[4];
+            // overlaps with destination cells g + 1 - j, j = o + 1 (o = source - dest): cell
+            // [c + 1 - j, c + 2 - j)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int ix = gxs + 1 - j, iy = gys + 1 - j;
-                ox[j] = overlap1((float)ix, (float)(ix + 1), xm, xM);
-                oyv[j] = overlap1((float)iy, (float)(iy + 1), ym, yM);
+                sOX[j][i] = overlap1(cx + (float)(1 - j), cx + (float)(2 - j), xm, xM);
+                sOY[j][i] = overlap1(cy + (float)(1 - j), cy + (float)(2 - j), ym, yM);
             }
             // With (2s)^2 a power of two the scaling by its inverse is exact (overlaps are 0 or at
             // least ~6e-8 and |m| >= 1e-10, so nothing is subnormal): m * ((ow*oh) * inv) equals
             // (m * inv) * (ow*oh) bit for bit, and the gather saves a multiply per pair.
             sM[i] = POW2 ? m * inv_ta : m;
-            sOX[0][i] = ox[0];
-            sOX[1][i] = ox[1];
-            sOX[2][i] = ox[2];
-            sOX[3][i] = ox[3];
-            sOY[i] = make_float4(oyv[0], oyv[1], oyv[2], oyv[3]);
+            // every source is checked by the one tile that owns it
+            const int ux = x0 - 1 + sx, uy = y0 - 1 + sy;
             if (ux >= x0 && ux < x0 + TX && uy >= y0 && uy < y0 + TY && ux < W && uy < H) {
-                if (!(isfinite(xm) && isfinite(xM) && isfinite(ym) && isfinite(yM) && isfinite(m))) local_flag |= 2;
-                else if (floorf(xm) < (float)(gxs - 2) || ceilf(xM) > (float)(gxs + 2) ||
-                         floorf(ym) < (float)(gys - 2) || ceilf(yM) > (float)(gys + 2))
-                    local_flag |= 1;
+                // floor(xm) < c - 2  <=>  xm < c - 2 for integer c (likewise ceil); a non-finite
+                // value fails the comparisons and is classified on the slow side
+                const bool inside = xm >= cx - 2.0f && xM <= cx + 2.0f && ym >= cy - 2.0f && yM <= cy + 2.0f &&
+                                    fabsf(m) <= 3.40282347e38f;
+                if (!inside)
+                    local_flag |= (isfinite(xm) && isfinite(xM) && isfinite(ym) && isfinite(yM) && isfinite(m)) ? 1 : 2;
             }
         }
         // next channel's affinity: into registers now, into the other LDS buffer after the barrier
@@ -230,56 +253,53 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
         // ---- gather: strip of SR destinations ----
         // A zero-area pair contributes m * 0 = +-0; added to an accumulator that started at +0 it
         // changes no bit, so every pair is added without the reference's `area > 0` test.
-        if (gdx < W) {
-            float acc[SR];
+        // Waves with no periodic edge in reach take the unrotated path, whose LDS addresses are
+        // one base plus constants; waves touching an edge visit in rotated order.
+        float acc[SR];
 #pragma unroll
-            for (int r = 0; r < SR; ++r) acc[r] = 0.0f;
-            if (roty == 0) {
+        for (int r = 0; r < SR; ++r) acc[r] = 0.0f;
+        if (__all(rotx == 0 && roty == 0 && gdx < W)) {
+            const int base = ty0 * SW + tx;  // source (row 0, column o = -1) of the strip
 #pragma unroll
-                for (int a = 0; a < SROWS; ++a) {  // strip source row a <-> grid row gdy0 - 1 + a
-                    const int srow = (ty0 + a) * SW + tx + 1;
+            for (int a = 0; a < SROWS; ++a) {  // strip source row a <-> grid row gdy0 - 1 + a
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const int jx = (i + rotx) & 3;  // plane j = o + 1, o = source - dest
-                        const int si = srow + jx - 1;
-                        const float m = sM[si];
-                        const float ow = sOX[jx][si];
-                        const float4 oh4 = sOY[si];
-                        const float ohv[4] = {oh4.x, oh4.y, oh4.z, oh4.w};
+                for (int j = 0; j < 4; ++j) {  // plane j = o + 1, o = source - dest
+                    const int si = base + a * SW + j;
+                    const float m = sM[si];
+                    const float ow = sOX[j][si];
 #pragma unroll
-                        for (int r = 0; r < SR; ++r) {
-                            if (a - r >= 0 && a - r <= 3) {
-                                const float area = ow * ohv[a - r];
-                                acc[r] += POW2 ? m * area : m * (area / ta);
-                            }
-                        }
-                    }
-                }
-            } else {
-#pragma unroll 1
-                for (int it = 0; it < SROWS; ++it) {
-                    int a = it + roty;
-                    if (a >= SROWS) a -= SROWS;
-                    const int srow = (ty0 + a) * SW + tx + 1;
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const int jx = (i + rotx) & 3;
-                        const int si = srow + jx - 1;
-                        const float m = sM[si];
-                        const float ow = sOX[jx][si];
-                        const float4 oh4 = sOY[si];
-#pragma unroll
-                        for (int r = 0; r < SR; ++r) {
-                            const int jy = a - r;  // o_y + 1
-                            if (jy >= 0 && jy <= 3) {
-                                const float oh = jy == 0 ? oh4.x : jy == 1 ? oh4.y : jy == 2 ? oh4.z : oh4.w;
-                                const float area = ow * oh;
-                                acc[r] += POW2 ? m * area : m * (area / ta);
-                            }
+                    for (int r = 0; r < SR; ++r) {
+                        if (a - r >= 0 && a - r <= 3) {
+                            const float area = ow * sOY[a - r][si];
+                            acc[r] += POW2 ? m * area : m * (area / ta);
                         }
                     }
                 }
             }
+        } else if (gdx < W) {
+#pragma unroll 1
+            for (int it = 0; it < SROWS; ++it) {
+                int a = it + roty;
+                if (a >= SROWS) a -= SROWS;
+                const int srow = (ty0 + a) * SW + tx + 1;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int jx = (i + rotx) & 3;
+                    const int si = srow + jx - 1;
+                    const float m = sM[si];
+                    const float ow = sOX[jx][si];
+#pragma unroll
+                    for (int r = 0; r < SR; ++r) {
+                        const int jy = a - r;  // o_y + 1
+                        if (jy >= 0 && jy <= 3) {
+                            const float area = ow * sOY[jy][si];
+                            acc[r] += POW2 ? m * area : m * (area / ta);
+                        }
+                    }
+                }
+            }
+        }
+        if (gdx < W) {
             float* Oc = Aout + (long long)c * N;
 #pragma unroll
             for (int r = 0; r < SR; ++r)
@@ -291,7 +311,7 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
 }
 
 template <int CC, bool POW2, class TL>
-__global__ __launch_bounds__(TL::NT) void k_stencil2d(const float* __restrict__ A, const float* __restrict__ U,
+__global__ __launch_bounds__(TL::NT, TL::WAVES_PER_EU) void k_stencil2d(const float* __restrict__ A, const float* __restrict__ U,
                                                       float* __restrict__ Aout, int Cdyn, int W, int H, long long N,
                                                       int tiles_x, int ntiles, float dt, float beta_a, float nexp,
                                                       float s, float ta, float inv_ta, int* __restrict__ flag) {
@@ -306,7 +326,7 @@ __global__ __launch_bounds__(TL::NT) void k_stencil2d(const float* __restrict__ 
 
 // batched: item = (member, tile); every member of one launch has the same channel count class
 template <int CC, bool POW2, class TL>
-__global__ __launch_bounds__(TL::NT) void kb_stencil2d(const MemberDesc* __restrict__ mem,
+__global__ __launch_bounds__(TL::NT, TL::WAVES_PER_EU) void kb_stencil2d(const MemberDesc* __restrict__ mem,
                                                        const int4* __restrict__ items, int cur,
                                                        int* __restrict__ flag) {
     const int4 it = items[blockIdx.x];

@@ -4,17 +4,18 @@
 // the growth epilogue; CachedConvolver::convolve_with_kernel_scratch, fft.rs:288-324,
 // and growth_accumulate, growth.rs:44-59), laid out for the GPU differently:
 //
-//   * spectra are stored blocked column-major, XT[slot][y / 8][kx][y % 8]: the 8-row tile of
-//     a row pass is one contiguous run, and a y-pass workgroup (4 adjacent kx columns) reads
-//     and writes 256-byte runs per 8 rows;
-//   * every line transform is done by ONE wavefront: lane l holds the E = L/64 elements
-//     l, l+64, l+128, ... in registers, the radix-16/8/4 Stockham stages run in registers
+//   * every line transform is done by ONE wavefront: lane l holds the E = n/64 elements
+//     l, l+64, l+128, ... in registers, the radix-16/8/4/2 Stockham stages run in registers
 //     and exchange data through a per-wave LDS line (no workgroup barriers), and the first
 //     stage reads / last stage writes that same register layout, so a forward transform's
 //     output feeds the kernel-spectrum product and the inverse transform directly;
-//   * the row passes work on 8 rows per workgroup (4 packed row pairs, one per wave) and
-//     move the transposition through an LDS tile, so their global accesses are 64-byte
-//     runs per kx column and whole rows in the spatial domain.
+//   * a row of L reals is transformed by one wavefront as an L/2-point complex FFT of its
+//     (even, odd) pairs, followed by the split into the half spectrum (and the reverse for
+//     the inverse), so the row passes run one wavefront per row: twice the wavefronts of a
+//     two-rows-per-FFT packing, each with half the work and half the LDS line;
+//   * spectra are stored blocked column-major, XT[slot][y / 4][kx][y % 4]: the 4-row tile of
+//     a row-pass workgroup is one contiguous run, and a y-pass workgroup (4 adjacent kx
+//     columns) reads and writes 128-byte runs per 4 rows.
 #include "hp_common.hpp"
 
 namespace hp {
@@ -22,9 +23,11 @@ namespace wv {
 
 constexpr int WAVE = 64;
 
-__host__ __device__ constexpr int stages(int L) { return L == 256 ? 4 : 3; }
+// Radix plans of the wave FFT, by length: every stage's radix divides the 64-lane element
+// count E = n / 64.
+__host__ __device__ constexpr int stages(int L) { return L == 128 ? 7 : L == 256 ? 4 : 3; }
 __host__ __device__ constexpr int radix(int L, int s) {
-    return L == 256 ? 4 : L == 512 ? 8 : L == 1024 ? (s < 2 ? 16 : 4) : (s < 2 ? 16 : 8);
+    return L == 128 ? 2 : L == 256 ? 4 : L == 512 ? 8 : L == 1024 ? (s < 2 ? 16 : 4) : (s < 2 ? 16 : 8);
 }
 __host__ __device__ constexpr int ns_before(int L, int s) {
     int ns = 1;
@@ -56,7 +59,11 @@ __device__ __forceinline__ void dft4(float2& a, float2& b, float2& c, float2& d)
 
 template <int R, int DIR>
 __device__ __forceinline__ void dft(float2* v) {
-    if constexpr (R == 4) {
+    if constexpr (R == 2) {
+        const float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 4) {
         dft4<DIR>(v[0], v[1], v[2], v[3]);
     } else if constexpr (R == 8) {
         float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
@@ -114,9 +121,35 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// Per-stage twiddle tables.  Stage S >= 1 multiplies input r of butterfly k by
+// W_L^(r k TSTEP); its table holds those values at [(r - 1) NS + k] (copies of entries of the
+// full table exp(-2 pi i m / L), so the arithmetic is unchanged), which a wave reads at
+// consecutive addresses instead of strided ones.
+__host__ __device__ constexpr int tw_stage_size(int L, int s) { return s == 0 ? 0 : (radix(L, s) - 1) * ns_before(L, s); }
+__host__ __device__ constexpr int tw_stage_off(int L, int s) {
+    int o = 0;
+    for (int i = 0; i < s; ++i) o += tw_stage_size(L, i);
+    return o;
+}
+__host__ __device__ constexpr int tw_total(int L) { return tw_stage_off(L, stages(L)); }
+
+// Fill the per-stage tables of length L (workgroup-cooperative) from `twg`, the table of a
+// length STRIDE * L: its entry STRIDE * m is exp(-2 pi i m / L), bit for bit.
+template <int L, int NTH, int STRIDE>
+__device__ __forceinline__ void stage_twiddles(float2* tw, const float2* __restrict__ twg) {
+    for (int idx = threadIdx.x; idx < tw_total(L); idx += NTH) {
+        int s = 1;
+        while (s + 1 < stages(L) && idx >= tw_stage_off(L, s + 1)) ++s;
+        const int ns = ns_before(L, s), tstep = L / (ns * radix(L, s));
+        const int rel = idx - tw_stage_off(L, s);
+        const int r = rel / ns + 1, k = rel - (r - 1) * ns;
+        tw[idx] = twg[STRIDE * (r * k * tstep)];
+    }
+}
+
 // One wavefront transforms one line of L elements held as e[i] = x[lane + 64 i].
-// `line` is this wave's private LDS line (padded(L) float2), `tw` the shared table
-// exp(-2 pi i m / L).  Output in the same layout.  Unnormalised; DIR -1 forward.
+// `line` is this wave's private LDS line (padded(L) float2), `tw` the per-stage twiddle
+// tables.  Output in the same layout.  Unnormalised; DIR -1 forward.
 template <int L, int S, int DIR>
 __device__ __forceinline__ void wave_stage(float2 (&e)[L / WAVE], float2* line, const float2* tw) {
     constexpr int E = L / WAVE;
@@ -124,8 +157,9 @@ __device__ __forceinline__ void wave_stage(float2 (&e)[L / WAVE], float2* line, 
     constexpr int Q = E / R;
     constexpr int NS = ns_before(L, S);
     constexpr int NB = L / R;
-    constexpr int TSTEP = L / (NS * R);
+    constexpr int TOFF = tw_stage_off(L, S);
     constexpr bool FIRST = S == 0, LAST = S == stages(L) - 1;
+    static_assert(Q * R == E, "radix must divide the per-lane element count");
     const int lane = threadIdx.x & (WAVE - 1);
     float2 v[Q][R];
 #pragma unroll
@@ -143,7 +177,7 @@ __device__ __forceinline__ void wave_stage(float2 (&e)[L / WAVE], float2* line, 
         if (NS > 1) {
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                float2 w = tw[r * k * TSTEP];
+                float2 w = tw[TOFF + (r - 1) * NS + k];
                 if (DIR > 0) w.y = -w.y;
                 v[q][r] = cmul(v[q][r], w);
             }
@@ -158,12 +192,17 @@ __device__ __forceinline__ void wave_stage(float2 (&e)[L / WAVE], float2* line, 
     if (!LAST) wave_sync();
 }
 
+template <int L, int S, int DIR>
+__device__ __forceinline__ void wave_stages(float2 (&e)[L / WAVE], float2* line, const float2* tw) {
+    if constexpr (S < stages(L)) {
+        wave_stage<L, S, DIR>(e, line, tw);
+        wave_stages<L, S + 1, DIR>(e, line, tw);
+    }
+}
+
 template <int L, int DIR>
 __device__ __forceinline__ void wave_fft(float2 (&e)[L / WAVE], float2* line, const float2* tw) {
-    wave_stage<L, 0, DIR>(e, line, tw);
-    wave_stage<L, 1, DIR>(e, line, tw);
-    wave_stage<L, 2, DIR>(e, line, tw);
-    if constexpr (stages(L) > 3) wave_stage<L, 3, DIR>(e, line, tw);
+    wave_stages<L, 0, DIR>(e, line, tw);
 }
 
 // XCD-aware block order for the row passes: workgroup b runs on XCD b % 8, so give every XCD a
@@ -179,62 +218,61 @@ __device__ __forceinline__ bool xcd_block(int nbx, int total, int& bx, int& by) 
     return true;
 }
 
-constexpr int WAVES = 4;  // waves per workgroup
+constexpr int WAVES = 4;      // waves per workgroup
 constexpr int NT = WAVES * WAVE;
-constexpr int ROWS = 2 * WAVES;  // rows per workgroup in the row passes (one packed pair per wave)
+constexpr int RB = WAVES;     // rows per spectrum block = rows per row-pass workgroup (one per wave)
+constexpr int TP = RB + 1;    // LDS pitch of a row tile's kx entries: lanes at consecutive kx hit distinct banks
 
-// ---- pass 1: R2C of 8 rows, written as one blocked column-major tile ------------------------
-// grid (H / 8, nsrc).  XT[slot][y / 8][kx][y % 8], kx < W/2+1.
+// ---- pass 1: R2C of 4 rows (one per wave), written as one blocked column-major tile --------
+// grid (H / 4 * nsrc).  XT[slot][y / 4][kx][y % 4], kx < W/2+1.  twg: the length-L table.
+//
+// Row x (L reals) as z[n] = x[2n] + i x[2n+1], n < M = L/2; Z = FFT_M(z).  With
+// E[k] = (Z[k] + conj Z[M-k]) / 2 and O[k] = -i (Z[k] - conj Z[M-k]) / 2 (the transforms of the
+// even and odd samples), X[k] = E[k] + W_L^k O[k] for k < M and X[M] = Re Z[0] - Im Z[0].
 template <int L>
 __global__ __launch_bounds__(NT) void k_rows_fwd_t(const float* __restrict__ A, float2* __restrict__ XT,
                                                    const int* __restrict__ src_chan, int H, long long N,
                                                    long long spec, int nblocks, const float2* __restrict__ twg) {
-    constexpr int E = L / WAVE, LH = L / 2 + 1;
-    constexpr int LINE = padded(L);
-    __shared__ float2 tw[L];
-    __shared__ float2 lines[WAVES * LINE > LH * ROWS ? WAVES * LINE : LH * ROWS];  // FFT lines, then the tile
+    constexpr int M = L / 2, E = M / WAVE, LH = L / 2 + 1;
+    constexpr int LINE = padded(M);
+    __shared__ float2 tw[tw_total(M)];
+    __shared__ float2 lines[WAVES * LINE > LH * TP ? WAVES * LINE : LH * TP];  // FFT lines, then the tile
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     int bx, s;
-    if (!xcd_block(H / ROWS, nblocks, bx, s)) return;
-    const int ybase = bx * ROWS;
-    for (int i = threadIdx.x; i < L; i += NT) tw[i] = twg[i];
-    const float* a0 = A + (long long)src_chan[s] * N + (long long)(ybase + 2 * wave) * L;
-    const float* a1 = a0 + L;
-    float2 e[E];
+    if (!xcd_block(H / RB, nblocks, bx, s)) return;
+    stage_twiddles<M, NT, 2>(tw, twg);
+    const float2* row =
+        reinterpret_cast<const float2*>(A + (long long)src_chan[s] * N + (long long)(bx * RB + wave) * L);
+    float2 e[E], w[E];
 #pragma unroll
-    for (int i = 0; i < E; ++i) e[i] = make_float2(a0[lane + WAVE * i], a1[lane + WAVE * i]);
+    for (int i = 0; i < E; ++i) {
+        e[i] = row[lane + WAVE * i];
+        w[i] = twg[lane + WAVE * i];  // W_L^k, k = lane + 64 i
+    }
     __syncthreads();  // twiddles staged
     float2* line = lines + wave * LINE;
-    wave_fft<L, -1>(e, line, tw);
-    // untangle Z = X0 + i X1 (Hermitian halves): needs Z[L - k]
+    wave_fft<M, -1>(e, line, tw);
 #pragma unroll
     for (int i = 0; i < E; ++i) line[pad(lane + WAVE * i)] = e[i];
     wave_sync();
-    float2 x0[E / 2 + 1], x1[E / 2 + 1];
+    float2 xk[E];
 #pragma unroll
-    for (int i = 0; i <= E / 2; ++i) {
+    for (int i = 0; i < E; ++i) {
         const int k = lane + WAVE * i;
-        if (k <= L / 2) {
-            const float2 zk = line[pad(k)];
-            const float2 zm = line[pad(k == 0 ? 0 : L - k)];
-            x0[i] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-            x1[i] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
-        }
+        const float2 zk = e[i], zm = line[pad(k == 0 ? 0 : M - k)];
+        const float2 ev = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+        const float2 od = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+        xk[i] = cadd(ev, cmul(w[i], od));
     }
     __syncthreads();  // all waves are done with their lines before the tile overwrites them
-    float2* tile = lines;  // [kx][ROWS]
+    float2* tile = lines;  // [kx][TP], rows 0..RB-1 used
 #pragma unroll
-    for (int i = 0; i <= E / 2; ++i) {
-        const int k = lane + WAVE * i;
-        if (k <= L / 2) {
-            tile[k * ROWS + 2 * wave] = x0[i];
-            tile[k * ROWS + 2 * wave + 1] = x1[i];
-        }
-    }
+    for (int i = 0; i < E; ++i) tile[(lane + WAVE * i) * TP + wave] = xk[i];
+    if (lane == 0) tile[M * TP + wave] = make_float2(e[0].x - e[0].y, 0.0f);
     __syncthreads();
-    // blocked column-major: the tile [kx][8 rows] is one contiguous run of the spectrum
-    float2* xs = XT + (long long)s * spec + (long long)bx * LH * ROWS;
-    for (int t = threadIdx.x; t < LH * ROWS; t += NT) xs[t] = tile[t];
+    // blocked column-major: the tile [kx][4 rows] is one contiguous run of the spectrum
+    float2* xs = XT + (long long)s * spec + (long long)bx * LH * RB;
+    for (int t = threadIdx.x; t < LH * RB; t += NT) xs[t] = tile[(t / RB) * TP + t % RB];
 }
 
 // ---- pass 2: y FFT of one kx column per wave, x folded kernel spectrum, inverse y FFT ------
@@ -245,18 +283,18 @@ __global__ __launch_bounds__(NT) void k_spectral_t(const float2* __restrict__ XT
                                                    const int* __restrict__ klist, int Wh, long long spec,
                                                    long long kq_stride, const float2* __restrict__ twg) {
     constexpr int E = L / WAVE, LINE = padded(L), HQ = L / 2 + 1;
-    __shared__ float2 tw[L];
+    __shared__ float2 tw[tw_total(L)];
     __shared__ float2 lines[WAVES * LINE];
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const SpecWork wk = work[blockIdx.y];
     const int kx = blockIdx.x * WAVES + wave;
-    for (int i = threadIdx.x; i < L; i += NT) tw[i] = twg[i];
+    stage_twiddles<L, NT, 1>(tw, twg);
     __syncthreads();
     if (kx >= Wh) return;  // after the only block-wide barrier
     float2* line = lines + wave * LINE;
-    // blocked column-major element (kx, y): (y / 8) * Wh * 8 + kx * 8 + y % 8; lane + 64 i = y
-    const long long col = (long long)(lane >> 3) * Wh * ROWS + (long long)kx * ROWS + (lane & 7);
-    const long long ystep = (long long)(WAVE / ROWS) * Wh * ROWS;  // 64 rows further
+    // blocked column-major element (kx, y): (y / 4) * Wh * 4 + kx * 4 + y % 4; lane + 64 i = y
+    const long long col = (long long)(lane / RB) * Wh * RB + (long long)kx * RB + (lane % RB);
+    const long long ystep = (long long)(WAVE / RB) * Wh * RB;  // 64 rows further
     const float2* xs = XT + (long long)wk.src_slot * spec + col;
     float2 f[E];
 #pragma unroll
@@ -279,40 +317,44 @@ __global__ __launch_bounds__(NT) void k_spectral_t(const float2* __restrict__ XT
     }
 }
 
-// ---- pass 3: C2R of 8 rows per kernel + growth + weighted accumulate ------------------------
-// grid (H / 8, nwork).
+// ---- pass 3: C2R of 4 rows (one per wave) per kernel + growth + weighted accumulate ---------
+// grid (H / 4 * nwork).  Inverse of pass 1's split: with X[k], k <= M, of a real row,
+// Z[k] = (X[k] + conj X[M-k]) + i (X[k] - conj X[M-k]) W_L^-k, and the unnormalised
+// IFFT_M(Z)[n] = L (x[2n] + i x[2n+1]) -- the unnormalised length-L inverse, as the LDS path
+// produces it.
 template <int L>
 __global__ __launch_bounds__(NT) void k_rows_inv_t(const float2* __restrict__ YT, float* __restrict__ U,
                                                    const TgtWork* __restrict__ work, const int* __restrict__ klist,
                                                    const GrowthParams* __restrict__ gp, int H, long long N,
                                                    long long spec, float scale, int nblocks,
                                                    const float2* __restrict__ twg) {
-    constexpr int E = L / WAVE, LINE = padded(L), LH = L / 2 + 1;
-    __shared__ float2 tw[L];
+    constexpr int M = L / 2, E = M / WAVE, LINE = padded(M), LH = L / 2 + 1;
+    __shared__ float2 tw[tw_total(M)];
     // the staging tile and the per-wave FFT lines share memory: the tile is read completely
     // into registers (and a barrier passed) before any wave starts transforming
-    __shared__ float2 buf[WAVES * LINE > LH * ROWS ? WAVES * LINE : LH * ROWS];
+    __shared__ float2 buf[WAVES * LINE > LH * TP ? WAVES * LINE : LH * TP];
     float2* tile = buf;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     int bx, wi;
-    if (!xcd_block(H / ROWS, nblocks, bx, wi)) return;
+    if (!xcd_block(H / RB, nblocks, bx, wi)) return;
     const TgtWork wk = work[wi];
-    const int ybase = bx * ROWS;
-    for (int i = threadIdx.x; i < L; i += NT) tw[i] = twg[i];
+    stage_twiddles<M, NT, 2>(tw, twg);
+    float2 w[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) w[i] = twg[lane + WAVE * i];  // W_L^k, k = lane + 64 i
     float acc0[E], acc1[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) acc0[i] = acc1[i] = 0.0f;
     float2* line = buf + wave * LINE;
-    const int r0 = 2 * wave;
     // the next kernel's tile is loaded into registers while the current one is transformed
-    constexpr int TPT = (LH * ROWS + NT - 1) / NT;
+    constexpr int TPT = (LH * RB + NT - 1) / NT;
     float2 pre[TPT];
     auto fetch = [&](int k) {  // the block's tile is one contiguous run (blocked column-major)
-        const float2* ys = YT + (long long)k * spec + (long long)bx * LH * ROWS;
+        const float2* ys = YT + (long long)k * spec + (long long)bx * LH * RB;
 #pragma unroll
         for (int q = 0; q < TPT; ++q) {
             const int t = threadIdx.x + q * NT;
-            if (t < LH * ROWS) pre[q] = ys[t];
+            if (t < LH * RB) pre[q] = ys[t];
         }
     };
     if (wk.k_count > 0) fetch(klist[wk.k_first]);
@@ -322,29 +364,24 @@ __global__ __launch_bounds__(NT) void k_rows_inv_t(const float2* __restrict__ YT
 #pragma unroll
         for (int q = 0; q < TPT; ++q) {
             const int t = threadIdx.x + q * NT;
-            if (t < LH * ROWS) tile[t] = pre[q];
+            if (t < LH * RB) tile[(t / RB) * TP + t % RB] = pre[q];
         }
         __syncthreads();
         if (j + 1 < wk.k_count) fetch(klist[wk.k_first + j + 1]);
         float2 e[E];
 #pragma unroll
         for (int i = 0; i < E; ++i) {
-            // Hermitian completion of the two half rows; packed line Z = X0 + i X1
             const int n = lane + WAVE * i;
-            if (n <= L / 2) {
-                const float2 a = tile[n * ROWS + r0], b = tile[n * ROWS + r0 + 1];
-                e[i] = make_float2(a.x - b.y, a.y + b.x);
-            } else {
-                const float2 a = tile[(L - n) * ROWS + r0], b = tile[(L - n) * ROWS + r0 + 1];
-                e[i] = make_float2(a.x + b.y, b.x - a.y);
-            }
+            const float2 a = tile[n * TP + wave], b = tile[(M - n) * TP + wave];  // X[k], X[M - k]
+            const float2 sm = make_float2(a.x + b.x, a.y - b.y);
+            const float2 t = cmul(make_float2(a.x - b.x, a.y + b.y), make_float2(w[i].x, -w[i].y));
+            e[i] = make_float2(sm.x - t.y, sm.y + t.x);
         }
-        __syncthreads();  // every wave holds its rows: the tile memory becomes FFT lines
-        wave_fft<L, +1>(e, line, tw);
+        __syncthreads();  // every wave holds its row: the tile memory becomes FFT lines
+        wave_fft<M, +1>(e, line, tw);
         const GrowthParams p = gp[k];
 #pragma unroll
         for (int i = 0; i < E; ++i) {
-#pragma clang fp contract(off)
             // growth_accumulate (growth.rs:51-57): g = 2 exp(-d*d*inv) - 1; t += w*g
             const float c0 = e[i].x * scale, c1 = e[i].y * scale;
             const float d0 = c0 - p.mu, d1 = c1 - p.mu;
@@ -354,18 +391,15 @@ __global__ __launch_bounds__(NT) void k_rows_inv_t(const float2* __restrict__ YT
             acc1[i] += p.weight * g1;
         }
     }
-    float* u0 = U + (long long)wk.target * N + (long long)(ybase + r0) * L;
+    float2* u = reinterpret_cast<float2*>(U + (long long)wk.target * N + (long long)(bx * RB + wave) * L);
 #pragma unroll
-    for (int i = 0; i < E; ++i) {
-        u0[lane + WAVE * i] = acc0[i];
-        u0[L + lane + WAVE * i] = acc1[i];
-    }
+    for (int i = 0; i < E; ++i) u[lane + WAVE * i] = make_float2(acc0[i], acc1[i]);  // cells 2n, 2n + 1
 }
 
 template <int L>
 void rows_fwd_t(hipStream_t st, const float* A, float2* XT, const int* src_chan, int nsrc, const Geo& g,
                 const float2* tw) {
-    const int nblocks = g.H / ROWS * nsrc;
+    const int nblocks = g.H / RB * nsrc;
     hipLaunchKernelGGL(k_rows_fwd_t<L>, dim3((nblocks + 7) / 8 * 8), dim3(NT), 0, st, A, XT, src_chan, g.H, g.N, g.spec,
                        nblocks, tw);
 }
@@ -382,7 +416,7 @@ template <int L>
 void rows_inv_t(hipStream_t st, const float2* YT, float* U, const TgtWork* work, int nwork, const int* klist,
                 const GrowthParams* gp, const Geo& g, const float2* tw) {
     const float scale = 1.0f / (float)(g.N);  // fft.rs:389: 1 / (W*H) as f32
-    const int nblocks = g.H / ROWS * nwork;
+    const int nblocks = g.H / RB * nwork;
     hipLaunchKernelGGL(k_rows_inv_t<L>, dim3((nblocks + 7) / 8 * 8), dim3(NT), 0, st, YT, U, work, klist, gp, g.H, g.N,
                        g.spec, scale, nblocks, tw);
 }

@@ -33,10 +33,13 @@ def _members(sides, chans, seed):
     return out
 
 
-def _lone(cfg, seed, steps, monkeypatch, stats=False):
+def _lone(cfg, seed, steps, monkeypatch, plan, stats=False):
+    """The lone trajectory under the population's plan (the process-wide plan cache may hold
+    a tuned plan for this shape from an earlier test, with another FFT rounding)."""
     monkeypatch.setenv("HP_FFT_PATH", "generic")
     monkeypatch.setenv("HP_SPECTRAL_GROUP", "1")
     with av.HipPropagator(cfg) as hp:
+        hp.set_plan(plan)
         st = av.SimulationState.from_seed(seed, cfg)
         hp.run(st, steps)
         _, mass = hp.mass_f64()
@@ -66,9 +69,10 @@ def test_population_members_match_lone_propagators_bitwise(monkeypatch):
         got = [pop.get_state(i) for i in range(len(mem))]
         masses = pop.mass_f64()
         r, g = pop.stats()
+        plan = pop.plan()
     assert g == 0
     for i, (cfg, sd) in enumerate(mem):
-        want, wmass = _lone(cfg, sd, 15, monkeypatch)
+        want, wmass = _lone(cfg, sd, 15, monkeypatch, plan)
         assert got[i].step == 15 and got[i].time == want.time
         for c in range(cfg.channels):
             np.testing.assert_array_equal(got[i].channels[c], want.channels[c], err_msg=f"member {i} ch {c}")
@@ -132,12 +136,13 @@ def test_population_general_path_replay(monkeypatch):
         pop.advance(4)
         got = [pop.get_state(i) for i in range(len(mem))]
         r, g = pop.stats()
-    _, _, (lr, lg) = _lone(mem[1][0], mem[1][1], 4, monkeypatch, stats=True)
+        plan = pop.plan()
+    _, _, (lr, lg) = _lone(mem[1][0], mem[1][1], 4, monkeypatch, plan, stats=True)
     _say(f"  batch: radius {r}, general steps {g}; lone member 1: radius {lr}, general steps {lg}")
     assert lg >= 1, "the case must leave the fused radius"
     assert g >= 1 and r == lr
     for i, (cfg, sd) in enumerate(mem):
-        want, _ = _lone(cfg, sd, 4, monkeypatch)
+        want, _ = _lone(cfg, sd, 4, monkeypatch, plan)
         for c in range(cfg.channels):
             np.testing.assert_array_equal(got[i].channels[c], want.channels[c], err_msg=f"member {i} ch {c}")
 
