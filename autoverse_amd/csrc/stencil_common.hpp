@@ -67,4 +67,13 @@ __device__ __forceinline__ float overlap1(float c0, float c1, float lo, float hi
     return fmaxf(fminf(c1, hi) - fmaxf(c0, lo), 0.0f);
 }
 
+// The same overlap as clamp(hi, c0, c1) - clamp(lo, c0, c1), two v_med3_f32 and a subtract.
+// For finite lo <= hi and c0 < c1 it equals overlap1 bit for bit: inside the cell both forms
+// subtract the same operands, and a box that misses the cell clamps both ends to the same
+// bound (+0) where overlap1 clamps a negative difference to +0.  A non-finite box may differ;
+// the fused stencils flag such a source and the host discards the step.
+__device__ __forceinline__ float overlap_clamped(float c0, float c1, float lo, float hi) {
+    return __builtin_amdgcn_fmed3f(hi, c0, c1) - __builtin_amdgcn_fmed3f(lo, c0, c1);
+}
+
 }  // namespace hp

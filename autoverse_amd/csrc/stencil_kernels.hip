@@ -220,8 +220,8 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
             // [c + 1 - j, c + 2 - j)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                sOX[j][i] = overlap1(cx + (float)(1 - j), cx + (float)(2 - j), xm, xM);
-                sOY[j][i] = overlap1(cy + (float)(1 - j), cy + (float)(2 - j), ym, yM);
+                sOX[j][i] = overlap_clamped(cx + (float)(1 - j), cx + (float)(2 - j), xm, xM);
+                sOY[j][i] = overlap_clamped(cy + (float)(1 - j), cy + (float)(2 - j), ym, yM);
             }
             // With (2s)^2 a power of two the scaling by its inverse is exact (overlaps are 0 or at
             // least ~6e-8 and |m| >= 1e-10, so nothing is subnormal): m * ((ow*oh) * inv) equals
@@ -594,9 +594,9 @@ __global__ __launch_bounds__(st3::NT) void k_stencil3d(const float* __restrict__
             float o[3][4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {  // cells gxs + 1 - j of the source's frame
-                o[0][j] = overlap1((float)(gxs + 1 - j), (float)(gxs + 2 - j), xm, xM);
-                o[1][j] = overlap1((float)(gys + 1 - j), (float)(gys + 2 - j), ym, yM);
-                o[2][j] = overlap1((float)(gzs + 1 - j), (float)(gzs + 2 - j), zm, zM);
+                o[0][j] = overlap_clamped((float)(gxs + 1 - j), (float)(gxs + 2 - j), xm, xM);
+                o[1][j] = overlap_clamped((float)(gys + 1 - j), (float)(gys + 2 - j), ym, yM);
+                o[2][j] = overlap_clamped((float)(gzs + 1 - j), (float)(gzs + 2 - j), zm, zM);
             }
             sM[i] = m;
 #pragma unroll

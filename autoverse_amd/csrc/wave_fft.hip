@@ -275,45 +275,91 @@ __global__ __launch_bounds__(NT) void k_rows_fwd_t(const float* __restrict__ A, 
     for (int t = threadIdx.x; t < LH * RB; t += NT) xs[t] = tile[(t / RB) * TP + t % RB];
 }
 
-// ---- pass 2: y FFT of one kx column per wave, x folded kernel spectrum, inverse y FFT ------
-// grid (ceil(Wh / 4), nwork).  KqT[k][kx][kyf], kyf < H/2+1.
+// ---- pass 2: y FFT of one kx column per wave pair, x folded kernel spectrum, inverse y FFT ----
+// grid ceil(Wh / 2) * nwork (rounded up to a multiple of 8); a workgroup is two columns x two halves.  KqT[k][kx][kyf],
+// kyf < H/2+1.
+//
+// A column of L = 2M is split over two wavefronts.  Forward (radix-2 decimation in frequency):
+// a[n] = x[n] + x[n+M], b[n] = (x[n] - x[n+M]) W_L^n, and FFT_M(a)[k] = X[2k],
+// FFT_M(b)[k] = X[2k+1].  Inverse (decimation in time) of the product P: with
+// e = IFFT_M(P[2k]) and o = IFFT_M(P[2k+1]) W_L^-n (unnormalised), y[n] = e[n] + o[n] and
+// y[n+M] = e[n] - o[n].  Each butterfly exchanges one half through LDS.
 template <int L>
 __global__ __launch_bounds__(NT) void k_spectral_t(const float2* __restrict__ XT, float2* __restrict__ YT,
                                                    const float* __restrict__ KqT, const SpecWork* __restrict__ work,
                                                    const int* __restrict__ klist, int Wh, long long spec,
-                                                   long long kq_stride, const float2* __restrict__ twg) {
-    constexpr int E = L / WAVE, LINE = padded(L), HQ = L / 2 + 1;
-    __shared__ float2 tw[tw_total(L)];
+                                                   long long kq_stride, int ncp, int nblocks,
+                                                   const float2* __restrict__ twg) {
+    constexpr int M = L / 2, E = M / WAVE, LINE = padded(M), HQ = L / 2 + 1;
+    __shared__ float2 tw[tw_total(M)];
     __shared__ float2 lines[WAVES * LINE];
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const SpecWork wk = work[blockIdx.y];
-    const int kx = blockIdx.x * WAVES + wave;
-    stage_twiddles<L, NT, 1>(tw, twg);
-    __syncthreads();
-    if (kx >= Wh) return;  // after the only block-wide barrier
+    const int h = wave & 1;                    // half: 0 -> rows/outputs n, even k; 1 -> n + M, odd k
+    // XCD-aware order: the workgroups that share the 128-byte lines of a row block (adjacent
+    // column pairs) run on the same XCD, so the lines are completed in one L2
+    int cp, wi;
+    if (!xcd_block(ncp, nblocks, cp, wi)) return;  // before any barrier: the whole workgroup leaves
+    const SpecWork wk = work[wi];
+    const int kx = cp * (WAVES / 2) + (wave >> 1);
+    const bool active = kx < Wh;               // both halves of a column agree
+    stage_twiddles<M, NT, 2>(tw, twg);
     float2* line = lines + wave * LINE;
-    // blocked column-major element (kx, y): (y / 4) * Wh * 4 + kx * 4 + y % 4; lane + 64 i = y
-    const long long col = (long long)(lane / RB) * Wh * RB + (long long)kx * RB + (lane % RB);
+    const float2* other = lines + (wave ^ 1) * LINE;
+    // blocked column-major element (kx, y): (y / 4) * Wh * 4 + kx * 4 + y % 4; y = h M + lane + 64 i
+    const int kxc = active ? kx : 0;
+    const long long col = (long long)((h * M) / RB + lane / RB) * Wh * RB + (long long)kxc * RB + (lane % RB);
     const long long ystep = (long long)(WAVE / RB) * Wh * RB;  // 64 rows further
+    float2 f[E], w[E];
     const float2* xs = XT + (long long)wk.src_slot * spec + col;
-    float2 f[E];
 #pragma unroll
-    for (int i = 0; i < E; ++i) f[i] = xs[i * ystep];
-    wave_fft<L, -1>(f, line, tw);
-    for (int j = 0; j < wk.k_count; ++j) {
-        const int k = klist[wk.k_first + j];
-        const float* kq = KqT + (long long)k * kq_stride + (long long)kx * HQ;
-        float2 e[E];
+    for (int i = 0; i < E; ++i) {
+        f[i] = active ? xs[i * ystep] : make_float2(0.0f, 0.0f);
+        w[i] = twg[lane + WAVE * i];  // W_L^n, n = lane + 64 i
+    }
+    // the next kernel's spectrum values (ky = 2k + h) are loaded while the current one works
+    float kvn[E];
+    auto fetch = [&](int k) {
+        const float* kq = KqT + (long long)k * kq_stride + (long long)kxc * HQ;
 #pragma unroll
         for (int i = 0; i < E; ++i) {
-            const int ky = lane + WAVE * i;
-            const float kv = kq[ky <= L / 2 ? ky : L - ky];
-            e[i] = make_float2(f[i].x * kv, f[i].y * kv);
+            const int ky = 2 * (lane + WAVE * i) + h;
+            kvn[i] = kq[ky <= L / 2 ? ky : L - ky];
         }
-        wave_fft<L, +1>(e, line, tw);
+    };
+    if (wk.k_count > 0) fetch(klist[wk.k_first]);
+    // forward butterfly across the halves
+#pragma unroll
+    for (int i = 0; i < E; ++i) line[lane + WAVE * i] = f[i];
+    __syncthreads();  // both halves (and the twiddle tables) are in LDS
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+        const float2 o = other[lane + WAVE * i];
+        f[i] = h == 0 ? cadd(f[i], o) : cmul(csub(o, f[i]), w[i]);
+    }
+    __syncthreads();  // the partner has read this line before the FFT reuses it
+    wave_fft<M, -1>(f, line, tw);
+    for (int j = 0; j < wk.k_count; ++j) {
+        const int k = klist[wk.k_first + j];
+        float2 e[E];
+#pragma unroll
+        for (int i = 0; i < E; ++i) e[i] = make_float2(f[i].x * kvn[i], f[i].y * kvn[i]);
+        if (j + 1 < wk.k_count) fetch(klist[wk.k_first + j + 1]);
+        wave_fft<M, +1>(e, line, tw);  // own line only: the partner finished reading it
+        if (h == 1) {
+#pragma unroll
+            for (int i = 0; i < E; ++i) e[i] = cmul(e[i], make_float2(w[i].x, -w[i].y));
+        }
+#pragma unroll
+        for (int i = 0; i < E; ++i) line[lane + WAVE * i] = e[i];
+        __syncthreads();  // both halves written
         float2* ys = YT + (long long)k * spec + col;
 #pragma unroll
-        for (int i = 0; i < E; ++i) ys[i * ystep] = e[i];
+        for (int i = 0; i < E; ++i) {
+            const float2 o = other[lane + WAVE * i];
+            const float2 y = h == 0 ? cadd(e[i], o) : csub(o, e[i]);
+            if (active) ys[i * ystep] = y;
+        }
+        __syncthreads();  // the partner has read this line before the next FFT reuses it
     }
 }
 
@@ -380,13 +426,16 @@ __global__ __launch_bounds__(NT) void k_rows_inv_t(const float2* __restrict__ YT
         __syncthreads();  // every wave holds its row: the tile memory becomes FFT lines
         wave_fft<M, +1>(e, line, tw);
         const GrowthParams p = gp[k];
+        // exp(-d^2 inv) as 2^(-d^2 inv log2 e) on the hardware exp2 (v_exp_f32): the growth
+        // value differs from a libm expf by < 1e-7 absolute, far inside the FFT's own rounding
+        const float k2 = p.inv * 1.44269504088896341f;
 #pragma unroll
         for (int i = 0; i < E; ++i) {
             // growth_accumulate (growth.rs:51-57): g = 2 exp(-d*d*inv) - 1; t += w*g
             const float c0 = e[i].x * scale, c1 = e[i].y * scale;
             const float d0 = c0 - p.mu, d1 = c1 - p.mu;
-            const float g0 = 2.0f * expf(-d0 * d0 * p.inv) - 1.0f;
-            const float g1 = 2.0f * expf(-d1 * d1 * p.inv) - 1.0f;
+            const float g0 = 2.0f * __builtin_amdgcn_exp2f(-(d0 * d0) * k2) - 1.0f;
+            const float g1 = 2.0f * __builtin_amdgcn_exp2f(-(d1 * d1) * k2) - 1.0f;
             acc0[i] += p.weight * g0;
             acc1[i] += p.weight * g1;
         }
@@ -408,8 +457,10 @@ template <int L>
 void spectral_t(hipStream_t st, const float2* XT, float2* YT, const float* KqT, const SpecWork* work, int nwork,
                 const int* klist, const Geo& g, const float2* tw) {
     const long long kq_stride = (long long)g.Wh * g.Hq;
-    hipLaunchKernelGGL(k_spectral_t<L>, dim3((g.Wh + WAVES - 1) / WAVES, nwork), dim3(NT), 0, st, XT, YT, KqT, work,
-                       klist, g.Wh, g.spec, kq_stride, tw);
+    const int ncp = (g.Wh + WAVES / 2 - 1) / (WAVES / 2);  // column pairs
+    const int nblocks = ncp * nwork;
+    hipLaunchKernelGGL(k_spectral_t<L>, dim3((nblocks + 7) / 8 * 8), dim3(NT), 0, st, XT, YT, KqT, work, klist, g.Wh,
+                       g.spec, kq_stride, ncp, nblocks, tw);
 }
 
 template <int L>

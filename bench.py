@@ -40,6 +40,8 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the plan autotuner (run the built-in default plan)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline sample length")
+    ap.add_argument("--plan", type=str, default=None,
+                    help='fix plan fields instead of tuning, as JSON, e.g. \'{"stencil_tile": 2}\'')
     ap.add_argument("--population", action="store_true",
                     help="population mode: 256 uneven members stepped as one batch vs one lone "
                          "propagator per member (prints its own JSON line)")
@@ -80,7 +82,9 @@ def main() -> int:
     prop.set_state(state)
     prop.advance(args.warmup)
     tune = None
-    if not args.no_tune:  # untimed: pick the fastest plan on this device, state restored after
+    if args.plan:
+        prop.set_plan(json.loads(args.plan))
+    elif not args.no_tune:  # untimed: pick the fastest plan on this device, state restored after
         best, table = prop.autotune(steps=20)
         tune = {"plan": best, "candidates": len(table),
                 "default_us_per_step": table[0][1] if table else None,
@@ -350,7 +354,9 @@ def sphere3d_bench(args) -> int:
 
 
 def copy_probe() -> int:
-    """A plain copy kernel of known size: 64 Mi float32 (256 MiB) read and written, ten times."""
+    """A plain copy kernel of known size: 64 Mi float32 (256 MiB) read and written, ten times
+    (the calibration run for the traffic counters), then copies of the sizes one step's passes
+    move, timed with events: the time a trivial kernel needs for the same bytes."""
     import torch
 
     a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -358,7 +364,21 @@ def copy_probe() -> int:
     for _ in range(10):
         b.copy_(a)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_bytes_each_way": a.numel() * 4, "launches": 10}), flush=True)
+    out = {"copy_probe_bytes_each_way": a.numel() * 4, "launches": 10, "copy_us": {}}
+    for mb in (4, 12.6, 19, 25, 30, 64, 256):
+        n = int(mb * 1e6 / 4)
+        x, y = a[:n], b[:n]
+        for _ in range(3):
+            y.copy_(x)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(20):
+            y.copy_(x)
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / 20
+        out["copy_us"][f"{mb}MB_each_way"] = {"us": round(us, 2), "TBps_read_plus_write": round(2 * n * 4 / us / 1e6, 2)}
+    print(json.dumps(out), flush=True)
     return 0
 
 
