@@ -46,6 +46,8 @@ def main() -> int:
                     help="population mode: 256 uneven members stepped as one batch vs one lone "
                          "propagator per member (prints its own JSON line)")
     ap.add_argument("--pop-steps", type=int, default=200)
+    ap.add_argument("--no-lone", action="store_true",
+                    help="population mode: skip the per-instance comparison pass (for profiling the batch)")
     ap.add_argument("--grad", action="store_true",
                     help="differentiation mode: 32 members of 512^2 x 2 channels, forward tape of "
                          "--grad-steps steps with a checkpoint every --grad-every, then the backward pass")
@@ -195,19 +197,21 @@ def population_bench(args) -> int:
     disp = np.hypot(c1[:, 0] - c0[:, 0], c1[:, 1] - c0[:, 1])
     drift = float(np.max(np.abs(m1 - m0) / np.maximum(m0, 1e-30)))
 
-    props = []
-    for c, s in pop_def:
-        p = av.HipPropagator(c)
-        p.set_state(av.SimulationState.from_seed(s, c))
-        p.advance(args.warmup)
-        props.append(p)
-    print("[population] per-instance pass", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    for p in props:
-        p.advance(K)
-    secs_i = time.perf_counter() - t0
-    for p in props:
-        p.close()
+    secs_i = None
+    if not args.no_lone:
+        props = []
+        for c, s in pop_def:
+            p = av.HipPropagator(c)
+            p.set_state(av.SimulationState.from_seed(s, c))
+            p.advance(args.warmup)
+            props.append(p)
+        print("[population] per-instance pass", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        for p in props:
+            p.advance(K)
+        secs_i = time.perf_counter() - t0
+        for p in props:
+            p.close()
 
     # batched forward + adjoint over the same population: tape of K steps (checkpoint interval
     # chosen by the library from free HBM), then the backward pass for L = sum <seed, A_T>
@@ -230,7 +234,7 @@ def population_bench(args) -> int:
     finite = all(np.all(np.isfinite(t)) for t in dth)
 
     value = cells * K / secs_b
-    per_inst = cells * K / secs_i
+    per_inst = cells * K / secs_i if secs_i else None
     fwd_adj = cells * K / secs_fa
     out = {
         "metric": "cell-steps/s, 256-member population (batched forward step)",
@@ -248,12 +252,14 @@ def population_bench(args) -> int:
         "config": {"workload": "population of 256: side U{64,128,192,256}, 1-3 channels, every "
                                "source->target kernel, R=15, dt=0.2", "members": len(cfgs), "cells": cells,
                    "max_gather_radius": radius, "general_path_steps": general, "plan": plan},
-        "per_instance": {"value": per_inst, "unit": "cell-steps/s", "ms_per_step": secs_i * 1e3 / K,
-                         "path": "one HipPropagator per member, advanced in turn"},
-        "speedup_vs_per_instance": value / per_inst,
+        "per_instance": None if per_inst is None else {
+            "value": per_inst, "unit": "cell-steps/s", "ms_per_step": secs_i * 1e3 / K,
+            "path": "one HipPropagator per member, advanced in turn"},
+        "speedup_vs_per_instance": None if per_inst is None else value / per_inst,
         "forward_adjoint": {"value": fwd_adj, "unit": "cell-steps/s", "steps": K, "checkpoint_every": every,
                             "ms_per_step": secs_fa * 1e3 / K, "tape_forward_ms_per_step": secs_f * 1e3 / K,
-                            "speedup_vs_per_instance_forward": fwd_adj / per_inst, "gradients_finite": finite},
+                            "speedup_vs_per_instance_forward": None if per_inst is None else fwd_adj / per_inst,
+                            "gradients_finite": finite},
         "fitness": {"displacement_mean": float(disp.mean()), "displacement_max": float(disp.max())},
         "mass_drift_max_rel": drift,
     }
