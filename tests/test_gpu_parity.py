@@ -144,19 +144,23 @@ def test_plain256_100_steps(oracle):
     assert abs(mass - float(want64[0].sum())) / m0 < 1e-6
 
 
-def test_step_equals_run_and_is_deterministic(oracle):
-    _say("[step_vs_run]")
-    cfg, sd = _small_multispecies(128, 2, seed=5)
-    st0 = oracle.seed_generate(sd, 128, 128, 1, 2)
-    a = av.SimulationState([c.copy() for c in st0], 128, 128)
-    b = av.SimulationState([c.copy() for c in st0], 128, 128)
-    c = av.SimulationState([x.copy() for x in st0], 128, 128)
+@pytest.mark.parametrize("size,channels", [(128, 2), (512, 3)])
+def test_step_equals_run_and_is_deterministic(oracle, size, channels):
+    """hp_step (one checked step at a time, launched kernel by kernel) against hp_run (hipGraph
+    replay with one flag check per run; 512^2 takes the wave-level FFT path): the same
+    trajectory bit for bit, and repeatable."""
+    _say(f"[step_vs_run] {size}^2 x {channels}")
+    cfg, sd = _small_multispecies(size, channels, seed=5)
+    st0 = oracle.seed_generate(sd, size, size, 1, channels)
+    a = av.SimulationState([c.copy() for c in st0], size, size)
+    b = av.SimulationState([c.copy() for c in st0], size, size)
+    c = av.SimulationState([x.copy() for x in st0], size, size)
     with av.HipPropagator(cfg) as hp:
         for _ in range(12):
             hp.step(a)
         hp.run(b, 12)
         hp.run(c, 12)
-    for i in range(2):
+    for i in range(channels):
         np.testing.assert_array_equal(a.channels[i], b.channels[i])
         np.testing.assert_array_equal(b.channels[i], c.channels[i])
     assert a.step == b.step == 12 and a.time == b.time
