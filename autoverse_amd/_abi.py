@@ -120,6 +120,7 @@ EXPORTS = [
     ("hp_stats", c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint64)]),
     ("hp_profile_enable", c_int, [c_void_p, c_int]),
     ("hp_profile_read", c_int, [c_void_p, POINTER(c_double), POINTER(c_uint64)]),
+    ("hp_profile_repeats", c_int, [c_void_p, c_uint32]),
     ("hp_kernel_bytes", c_int, [c_void_p, POINTER(c_double)]),
     ("hp_advect_mass_2d", c_int, [c_int, FP, FP, FP, FP, c_uint32, c_uint32, c_float, c_float]),
     ("hp_advect_mass_3d", c_int, [c_int, FP, FP, FP, FP, FP, c_uint32, c_uint32, c_uint32, c_float, c_float]),

@@ -204,7 +204,10 @@ class HipPropagator:
         self._check(self._lib.hp_stats(self._ptr, ctypes.byref(r), ctypes.byref(g)), "hp_stats")
         return r.value, g.value
 
-    def profile_enable(self, on: bool) -> None:
+    def profile_enable(self, on: bool, repeats: int = 1) -> None:
+        """Per-pass event timing (graph replay off while on).  `repeats` > 1 launches each
+        out-of-place pass that many times back to back inside its event pair."""
+        self._check(self._lib.hp_profile_repeats(self._ptr, int(repeats)), "hp_profile_repeats")
         self._check(self._lib.hp_profile_enable(self._ptr, int(on)), "hp_profile_enable")
 
     def profile_read(self):

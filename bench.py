@@ -29,6 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+PROF_REPEATS = 10
 
 
 def main() -> int:
@@ -101,8 +102,10 @@ def main() -> int:
     value = N * args.steps / secs
     radius, general = prop.stats()
 
-    # per-kernel pass: the same K steps with an event pair around every launch
-    prop.profile_enable(True)
+    # per-kernel pass: the same K steps with an event pair around every pass; each pass (all of
+    # them write buffers their inputs do not alias) is launched PROF_REPEATS times back to back
+    # inside its pair, so the event overhead is spread over the repeats
+    prop.profile_enable(True, repeats=PROF_REPEATS)
     prop.advance(args.steps)
     prof = prop.profile_read()
     prop.profile_enable(False)

@@ -243,6 +243,11 @@ typedef enum hp_kernel_class {
 /* Turn per-kernel event timing on or off (off by default; on adds one event pair
  * per launch and disables graph replay).  Enabling resets the counters. */
 hp_status hp_profile_enable(hp_ctx* ctx, int enable);
+/* Profiling aid, no reference counterpart: in profiling mode, launch each pass that writes
+ * only buffers its inputs do not alias (the wave-path FFT passes, the fused stencils) `reps`
+ * times back to back inside its event pair, so that the per-launch average carries 1/reps of
+ * the event overhead.  1..1000; default 1. */
+hp_status hp_profile_repeats(hp_ctx* ctx, uint32_t reps);
 /* Total milliseconds and launch count per class since enabling. */
 hp_status hp_profile_read(hp_ctx* ctx, double* ms /*HP_K_NUM_CLASSES*/, uint64_t* launches /*HP_K_NUM_CLASSES*/);
 /* Algorithmic HBM bytes one launch of each class moves for this config. */
