@@ -84,6 +84,7 @@ def main() -> int:
     prop = av.HipPropagator(cfg)
     prop.set_state(state)
     prop.advance(args.warmup)
+    _, mass0 = prop.mass_f64()
     tune = None
     if args.plan:
         prop.set_plan(json.loads(args.plan))
@@ -135,6 +136,17 @@ def main() -> int:
     # FP64 mass of the device state (conservation check, reported)
     _, mass = prop.mass_f64()
     prop.close()
+    drift = (mass - mass0) / mass0 if mass0 else None
+
+    # the whole step against the survey's byte model and against a plain device copy
+    step_bytes = step_model_bytes(cfg)
+    step_gbs = step_bytes / (secs / args.steps) / 1e9
+    copy_gbs = copy_bandwidth_gbs()
+    step_model = {"bytes_per_step": step_bytes, "GBs": round(step_gbs, 1),
+                  "frac_of_peak": round(step_gbs / HBM_PEAK_GBS, 4),
+                  "copy_GBs": round(copy_gbs, 1) if copy_gbs else None,
+                  "frac_of_copy": round(step_gbs / copy_gbs, 4) if copy_gbs else None,
+                  "copy": "hipMemcpy device to device, 32 MiB each way, read + write bytes"}
 
     cpu = None
     if not args.no_cpu:
@@ -157,8 +169,10 @@ def main() -> int:
                    "width": cfg.width, "height": cfg.height, "channels": cfg.channels,
                    "kernels": len(cfg.kernels), "kernel_radius": cfg.kernel_radius,
                    "max_gather_radius": radius, "general_path_steps": general,
-                   "mass_f64_after": mass, "plan": plan, "autotune": tune},
+                   "mass_f64_after": mass, "mass_rel_drift": drift,
+                   "mass_drift_steps": 2 * args.steps, "plan": plan, "autotune": tune},
         "roofline": roofline,
+        "step_model": step_model,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -360,6 +374,59 @@ def sphere3d_bench(args) -> int:
            "per_kernel": per_kernel}
     print(json.dumps(out), flush=True)
     return 0
+
+
+def step_model_bytes(cfg) -> int:
+    """Algorithmic bytes of one step in the survey's model for a grid whose half spectrum takes
+    two passes: C_s (4N + 3S) + 4 K S + 4 C N + 12 C N, S = 8 (W/2+1) H D."""
+    n = cfg.width * cfg.height * max(cfg.depth, 1)
+    s = 8 * (cfg.width // 2 + 1) * cfg.height * max(cfg.depth, 1)
+    cs = len({k.source_channel for k in cfg.kernels})
+    k = len(cfg.kernels)
+    c = cfg.channels
+    return cs * (4 * n + 3 * s) + 4 * k * s + 4 * c * n + 12 * c * n
+
+
+def copy_bandwidth_gbs(nbytes: int = 32 << 20, reps: int = 20):
+    """Read + write GB/s of hipMemcpy device to device (the runtime's copy kernel) at a size that,
+    like one step's buffers, fits the Infinity Cache; None if the runtime is not loadable."""
+    import ctypes
+    from ctypes import byref, c_float, c_int, c_size_t, c_void_p
+
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        return None
+    hip.hipMalloc.argtypes = [ctypes.POINTER(c_void_p), c_size_t]
+    hip.hipFree.argtypes = [c_void_p]
+    hip.hipMemcpyAsync.argtypes = [c_void_p, c_void_p, c_size_t, c_int, c_void_p]
+    hip.hipEventCreate.argtypes = [ctypes.POINTER(c_void_p)]
+    hip.hipEventRecord.argtypes = [c_void_p, c_void_p]
+    hip.hipEventSynchronize.argtypes = [c_void_p]
+    hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(c_float), c_void_p, c_void_p]
+    hip.hipEventDestroy.argtypes = [c_void_p]
+    a, b, e0, e1 = c_void_p(), c_void_p(), c_void_p(), c_void_p()
+    if hip.hipMalloc(byref(a), nbytes) != 0 or hip.hipMalloc(byref(b), nbytes) != 0:
+        return None
+    try:
+        hip.hipEventCreate(byref(e0))
+        hip.hipEventCreate(byref(e1))
+        for _ in range(3):
+            hip.hipMemcpyAsync(b, a, nbytes, 3, None)  # 3 = hipMemcpyDeviceToDevice
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            hip.hipMemcpyAsync(b, a, nbytes, 3, None)
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = c_float()
+        if hip.hipEventElapsedTime(byref(ms), e0, e1) != 0 or ms.value <= 0:
+            return None
+        return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
+    finally:
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+        hip.hipFree(a)
+        hip.hipFree(b)
 
 
 def copy_probe() -> int:

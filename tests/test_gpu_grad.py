@@ -5,7 +5,8 @@ Bars:
     component within 1e-3 relative -- or, for the few ill-conditioned components (d/dsigma
     carries 1/sigma^3 ~ 1e6 over sums that cancel), within 3x the deviation of the oracle's
     own f32 adjoint from its FP64 adjoint, i.e. as accurate as f32 arithmetic allows;
-  * dL/dtheta within 1e-3 relative of FP64 central differences of the oracle forward;
+  * dL/dtheta within 1e-3 relative of FP64 central differences of the oracle forward, and for
+    the L2-to-a-perturbed-target loss, 64 sampled coordinates at eps = 1e-4;
   * dL/dA_0 within 1e-3 of the FP64 oracle's, relative to its largest entry;
   * the checkpoint interval and batching change nothing, bit for bit;
   * the derivative of total mass (conserved by reintegration) is 1 for every cell and ~0
@@ -95,6 +96,62 @@ def test_grad_matches_fp64_central_differences(oracle):
         fd = (oracle.loss(cfg, st64, T, sd64, tp) - oracle.loss(cfg, st64, T, sd64, tm)) / (2 * e)
         _say(f"  theta[{j}]: gpu {th[j]:.6e} fd {fd:.6e}")
         assert abs(th[j] - fd) <= REL * abs(fd) + 1e-6 * np.abs(th).max()
+
+
+def test_l2_target_loss_matches_fp64_fd_on_64_coordinates(oracle):
+    """The differentiation workload's loss at a small size: L = ||A_T - target||^2 with the
+    target run under parameters perturbed by N(0, 0.05 |p|); 4 kernels (2 channels), T = 8 with a
+    checkpoint every 4.  64 sampled coordinates -- every parameter and the rest initial-state
+    cells -- against FP64 central differences with eps = 1e-4.  A coordinate counts when its
+    difference quotient is stable (eps and eps/2 agree to 1e-5 relative: no reintegration kink in
+    reach); at least 48 of the 64 must be, and each of those matches to 1e-3 relative."""
+    cfg, st = _case(oracle, 32, 2, seed=11, R=6)
+    T, eps = 8, 1e-4
+    theta = G.theta_of(cfg).astype(np.float32).astype(np.float64)
+    rng = np.random.default_rng(42)
+    tgt_theta = theta + rng.normal(size=theta.size) * 0.05 * np.abs(theta)
+    st64 = [a.astype(np.float64) for a in st]
+    target, _, _ = oracle.Propagator(cfg, np.float64, theta=tgt_theta).run([a.copy() for a in st64], T)
+
+    def L(th, s0):
+        out, _, _ = oracle.Propagator(cfg, np.float64, theta=th).run([a.copy() for a in s0], T)
+        return sum(float(((o - t) ** 2).sum()) for o, t in zip(out, target))
+
+    a_t, _, _ = oracle.Propagator(cfg, np.float64, theta=theta).run([a.copy() for a in st64], T)
+    seed = [(2.0 * (a - t)).astype(np.float32) for a, t in zip(a_t, target)]
+    _, g0, th = _gpu_grad(cfg, st, T, seed, k=4)
+    n = 32 * 32
+    coords = [("theta", j) for j in range(theta.size)]
+    cells = rng.choice(2 * n, size=64 - len(coords), replace=False)
+    coords += [("state", int(i)) for i in cells]
+    assert len(coords) == 64
+
+    def fd(kind, j, e):
+        if kind == "theta":
+            tp, tm = theta.copy(), theta.copy()
+            tp[j] += e
+            tm[j] -= e
+            return (L(tp, st64) - L(tm, st64)) / (2 * e)
+        c, i = divmod(j, n)
+        sp = [a.copy() for a in st64]
+        sm = [a.copy() for a in st64]
+        sp[c][i] += e
+        sm[c][i] -= e
+        return (L(theta, sp) - L(theta, sm)) / (2 * e)
+
+    gscale = max(np.abs(th).max(), max(np.abs(g).max() for g in g0))
+    kept, worst = 0, 0.0
+    for kind, j in coords:
+        f1, f2 = fd(kind, j, eps), fd(kind, j, eps / 2)
+        if abs(f1 - f2) > 1e-5 * max(abs(f1), 1e-6 * gscale):
+            continue  # a kink within eps: the difference quotient is not a derivative
+        kept += 1
+        got = th[j] if kind == "theta" else g0[j // n][j % n]
+        err = abs(got - f1) / max(abs(f1), 1e-3 * gscale)
+        worst = max(worst, err)
+        assert err <= REL, (kind, j, got, f1)
+    _say(f"  {kept} of 64 coordinates smooth at eps = 1e-4; worst relative error {worst:.2e}")
+    assert kept >= 48
 
 
 def test_checkpoint_interval_and_batching_are_bitwise_neutral(oracle):
