@@ -191,16 +191,20 @@ def test_step_equals_run_and_is_deterministic(oracle, size, channels):
 
 
 def test_multispecies_1024_steps(oracle):
-    """The 1024^2 x 3 workload for 10 steps against the f32 oracle."""
-    _say("[multispecies_1024x3 10 steps]")
+    """The 1024^2 x 3 workload for 100 steps: max-abs <= 1e-4 against the f32 oracle; the error
+    against the FP64 oracle is stated (the f32 oracle's own FP64 error is ~1.2e-4 here)."""
+    _say("[multispecies_1024x3 100 steps]")
     cfg, sd = workloads.multispecies()
     st0 = oracle.seed_generate(sd, 1024, 1024, 1, 3)
-    want, _, _ = oracle.Propagator(cfg, np.float32, threads=8).run(st0, 10)
+    want, _, _ = oracle.Propagator(cfg, np.float32, threads=16).run(st0, 100)
+    want64, _, _ = oracle.Propagator(cfg, np.float64, threads=16).run([np.asarray(c, np.float64) for c in st0], 100)
     state = av.SimulationState([c.copy() for c in st0], 1024, 1024)
     with av.HipPropagator(cfg) as hp:
-        hp.run(state, 10)
+        hp.run(state, 100)
     e = max(float(np.abs(state.channels[c] - want[c]).max()) for c in range(3))
-    _say(f"  max|gpu-f32|={e:.3e}")
+    e64 = max(float(np.abs(state.channels[c] - want64[c]).max()) for c in range(3))
+    o64 = max(float(np.abs(np.asarray(want[c], np.float64) - want64[c]).max()) for c in range(3))
+    _say(f"  max|gpu-f32|={e:.3e} max|gpu-f64|={e64:.3e} (f32 oracle vs FP64: {o64:.3e})")
     assert e <= 1e-4
 
 
