@@ -277,6 +277,27 @@ def test_3d_small_vs_oracle(oracle):
     assert e <= 1e-4
 
 
+def test_3d_sphere_128_steps(oracle):
+    """The 3D configuration at its full size, 128^3 with R = 10: 20 steps against the f32 oracle
+    (max-abs <= 1e-4), the FP64 error stated, and the FP64 mass drift."""
+    _say("[3d sphere 128^3, 20 steps]")
+    cfg, sd = workloads.sphere_3d(128)
+    st0 = oracle.seed_generate(sd, 128, 128, 128, 1)
+    want, _, _ = oracle.Propagator(cfg, np.float32, threads=16).run(st0, 20)
+    want64, _, _ = oracle.Propagator(cfg, np.float64, threads=16).run([np.asarray(st0[0], np.float64)], 20)
+    state = av.SimulationState([st0[0].copy()], 128, 128, 128)
+    with av.HipPropagator(cfg) as hp:
+        hp.set_state(state)
+        hp.advance(20)
+        got = hp.get_state()
+        _, mass = hp.mass_f64()
+    e = float(np.abs(got.channels[0] - want[0]).max())
+    e64 = float(np.abs(got.channels[0] - want64[0]).max())
+    m0 = float(np.asarray(st0[0], np.float64).sum())
+    _say(f"  max|gpu-f32|={e:.3e} max|gpu-f64|={e64:.3e} mass drift={(mass - m0) / m0:.3e}")
+    assert e <= 1e-4
+
+
 @pytest.mark.parametrize("shape,channels,dt", [((24, 16, 8), 2, 0.1), ((8, 8, 8), 1, 0.1), ((32, 24, 16), 3, 0.2),
                                                ((16, 16, 16), 1, 40.0)])
 def test_3d_stencil_bitwise(oracle, shape, channels, dt):
