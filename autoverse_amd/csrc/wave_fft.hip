@@ -367,7 +367,10 @@ __global__ __launch_bounds__(NT) void k_spectral_t(const float2* __restrict__ XT
 // grid (H / 4 * nwork).  Inverse of pass 1's split: with X[k], k <= M, of a real row,
 // Z[k] = (X[k] + conj X[M-k]) + i (X[k] - conj X[M-k]) W_L^-k, and the unnormalised
 // IFFT_M(Z)[n] = L (x[2n] + i x[2n+1]) -- the unnormalised length-L inverse, as the LDS path
-// produces it.
+// produces it.  Each wave reads its own row's X[k] and X[M-k] (k = lane + 64 i) straight from
+// the blocked spectrum -- the four waves of a workgroup together cover whole lines -- and holds
+// the next kernel's values in registers while the current one is transformed; the only
+// workgroup barrier is the twiddle staging.
 template <int L>
 __global__ __launch_bounds__(NT) void k_rows_inv_t(const float2* __restrict__ YT, float* __restrict__ U,
                                                    const TgtWork* __restrict__ work, const int* __restrict__ klist,
@@ -376,10 +379,7 @@ __global__ __launch_bounds__(NT) void k_rows_inv_t(const float2* __restrict__ YT
                                                    const float2* __restrict__ twg) {
     constexpr int M = L / 2, E = M / WAVE, LINE = padded(M), LH = L / 2 + 1;
     __shared__ float2 tw[tw_total(M)];
-    // the staging tile and the per-wave FFT lines share memory: the tile is read completely
-    // into registers (and a barrier passed) before any wave starts transforming
-    __shared__ float2 buf[WAVES * LINE > LH * TP ? WAVES * LINE : LH * TP];
-    float2* tile = buf;
+    __shared__ float2 lines[WAVES * LINE];
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     int bx, wi;
     if (!xcd_block(H / RB, nblocks, bx, wi)) return;
@@ -391,39 +391,30 @@ __global__ __launch_bounds__(NT) void k_rows_inv_t(const float2* __restrict__ YT
     float acc0[E], acc1[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) acc0[i] = acc1[i] = 0.0f;
-    float2* line = buf + wave * LINE;
-    // the next kernel's tile is loaded into registers while the current one is transformed
-    constexpr int TPT = (LH * RB + NT - 1) / NT;
-    float2 pre[TPT];
-    auto fetch = [&](int k) {  // the block's tile is one contiguous run (blocked column-major)
-        const float2* ys = YT + (long long)k * spec + (long long)bx * LH * RB;
-#pragma unroll
-        for (int q = 0; q < TPT; ++q) {
-            const int t = threadIdx.x + q * NT;
-            if (t < LH * RB) pre[q] = ys[t];
-        }
-    };
-    if (wk.k_count > 0) fetch(klist[wk.k_first]);
-    for (int j = 0; j < wk.k_count; ++j) {
-        const int k = klist[wk.k_first + j];
-        __syncthreads();  // previous tile fully consumed (and twiddles staged)
-#pragma unroll
-        for (int q = 0; q < TPT; ++q) {
-            const int t = threadIdx.x + q * NT;
-            if (t < LH * RB) tile[(t / RB) * TP + t % RB] = pre[q];
-        }
-        __syncthreads();
-        if (j + 1 < wk.k_count) fetch(klist[wk.k_first + j + 1]);
-        float2 e[E];
+    float2* line = lines + wave * LINE;
+    float2 xa[E], xb[E];  // X[k] and X[M - k] of this wave's row, next kernel
+    auto fetch = [&](int k) {
+        const float2* row = YT + (long long)k * spec + (long long)bx * LH * RB + wave;  // kx at row[kx * RB]
 #pragma unroll
         for (int i = 0; i < E; ++i) {
             const int n = lane + WAVE * i;
-            const float2 a = tile[n * TP + wave], b = tile[(M - n) * TP + wave];  // X[k], X[M - k]
+            xa[i] = row[(long long)n * RB];
+            xb[i] = row[(long long)(M - n) * RB];
+        }
+    };
+    if (wk.k_count > 0) fetch(klist[wk.k_first]);
+    __syncthreads();  // twiddles staged
+    for (int j = 0; j < wk.k_count; ++j) {
+        const int k = klist[wk.k_first + j];
+        float2 e[E];
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const float2 a = xa[i], b = xb[i];
             const float2 sm = make_float2(a.x + b.x, a.y - b.y);
             const float2 t = cmul(make_float2(a.x - b.x, a.y + b.y), make_float2(w[i].x, -w[i].y));
             e[i] = make_float2(sm.x - t.y, sm.y + t.x);
         }
-        __syncthreads();  // every wave holds its row: the tile memory becomes FFT lines
+        if (j + 1 < wk.k_count) fetch(klist[wk.k_first + j + 1]);
         wave_fft<M, +1>(e, line, tw);
         const GrowthParams p = gp[k];
         // exp(-d^2 inv) as 2^(-d^2 inv log2 e) on the hardware exp2 (v_exp_f32): the growth
