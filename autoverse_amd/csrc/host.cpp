@@ -56,7 +56,8 @@ struct hp_ctx {
     float2* X = nullptr;
     float2* Y = nullptr;
     float* Kq = nullptr;
-    std::map<int, float2*> tw;  // FFT length -> twiddle table
+    std::map<int, float2*> tw;   // FFT length -> twiddle table
+    std::map<int, float2*> tws;  // wave path: line length L -> per-stage tables of the L/2-point FFTs
     int* d_src_chan = nullptr;
     hp::SpecWork* d_spec_work = nullptr;
     int n_spec_work = 0;
@@ -196,21 +197,39 @@ hp_status make_twiddles(hp_ctx* c, int L) {
     return upload(c, d, t);
 }
 
+// The wave path's per-stage twiddle tables for the L/2-point transforms of a length-L line,
+// each entry the length-L table's value, computed by the same expression.
+hp_status make_stage_twiddles(hp_ctx* c, int L) {
+    if (c->tws.count(L)) return HP_OK;
+    const std::vector<int> idx = hp::wave_stage_twiddle_index(L / 2);
+    std::vector<float2> t(idx.size());
+    for (size_t i = 0; i < idx.size(); i++) {
+        const int m = 2 * idx[i];
+        const double a = -2.0 * M_PI * (double)m / (double)L;
+        t[i] = make_float2((float)cos(a), (float)sin(a));
+    }
+    float2* d = nullptr;
+    hp_status st = dalloc(c, &d, t.size());
+    if (st != HP_OK) return st;
+    c->tws[L] = d;
+    return upload(c, d, t);
+}
+
 // ---- one step, enqueued on c->stream: A[src] -> A[1-src]; the FFT path leaves U filled ----
 hp_status enqueue_affinity(hp_ctx* c, const float* A) {
     if (c->K == 0) return HP_OK;  // U stays zero (propagator.rs:245-247)
     const hp::Geo& g = c->g;
     if (c->wave) {  // A -> X -> Y -> U, each pass out of place
         timed_pass(c, HP_K_ROWS_FWD, true, [&] {
-            hp::launch_rows_fwd_t(c->stream, A, c->X, c->d_src_chan, (int)c->src_chan.size(), g, c->tw[g.W]);
+            hp::launch_rows_fwd_t(c->stream, A, c->X, c->d_src_chan, (int)c->src_chan.size(), g, c->tw[g.W], c->tws[g.W]);
         });
         timed_pass(c, HP_K_SPECTRAL, true, [&] {
             hp::launch_spectral_t(c->stream, c->X, c->Y, c->Kq, c->d_spec_work, c->n_spec_work, c->d_klist_spec, g,
-                                  c->tw[g.H]);
+                                  c->tw[g.H], c->tws[g.H]);
         });
         timed_pass(c, HP_K_ROWS_INV, true, [&] {
             hp::launch_rows_inv_t(c->stream, c->Y, c->U, c->d_tgt_work, c->n_tgt_work, c->d_klist_tgt, c->d_gp, g,
-                                  c->tw[g.W]);
+                                  c->tw[g.W], c->tws[g.W]);
         });
         HIP_TRY(c, hipGetLastError());
         return HP_OK;
@@ -431,6 +450,7 @@ void release(hp_ctx* c) {
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& kv : c->tw) (void)hipFree(kv.second);
+    for (auto& kv : c->tws) (void)hipFree(kv.second);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -581,6 +601,10 @@ hp_status create(const hp_config* cfg, int device, const hp_plan* req, hp_ctx** 
     TRY(upload(c, c->d_gp, gp));
     TRY(make_twiddles(c, g.W));
     TRY(make_twiddles(c, g.H));
+    if (c->wave) {
+        TRY(make_stage_twiddles(c, g.W));
+        TRY(make_stage_twiddles(c, g.H));
+    }
     if (is3d) TRY(make_twiddles(c, g.D));
     // zero state, affinity (channels no kernel targets keep U = 0) and flags
     if (hipMemset(c->A[0], 0, CN * 4) != hipSuccess || hipMemset(c->A[1], 0, CN * 4) != hipSuccess ||

@@ -5,6 +5,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace hp {
 
 // Work item of the spectral pass: one source spectrum and a run of kernels that
@@ -113,12 +115,16 @@ int spectral_block_lines(int L, const FftVariant& v = FftVariant());
 
 // ---- launchers implemented in wave_fft.hip (2D, sides 256..2048, column-major spectra) ----
 bool wave_fft_supported(int W, int H);
+// Wave path: tw is the length-L table exp(-2 pi i m / L) of the pass's line length L (W for the
+// row passes, H for the spectral pass); tws holds the per-stage tables of the L/2-point
+// transforms, entry i = tw[2 * wave_stage_twiddle_index(L / 2)[i]].
+std::vector<int> wave_stage_twiddle_index(int M);
 bool launch_rows_fwd_t(hipStream_t st, const float* A, float2* XT, const int* src_chan, int nsrc, const Geo& g,
-                       const float2* tw);
+                       const float2* tw, const float2* tws);
 bool launch_spectral_t(hipStream_t st, const float2* XT, float2* YT, const float* KqT, const SpecWork* work,
-                       int nwork, const int* klist, const Geo& g, const float2* tw);
+                       int nwork, const int* klist, const Geo& g, const float2* tw, const float2* tws);
 bool launch_rows_inv_t(hipStream_t st, const float2* YT, float* U, const TgtWork* work, int nwork, const int* klist,
-                       const GrowthParams* gp, const Geo& g, const float2* tw);
+                       const GrowthParams* gp, const Geo& g, const float2* tw, const float2* tws);
 
 // ---- launchers implemented in stencil_kernels.hip ----------------------------------
 // Fused channel sum + Sobel + flow + gather reintegration with a fixed gather

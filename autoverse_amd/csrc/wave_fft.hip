@@ -16,6 +16,8 @@
 //   * spectra are stored blocked column-major, XT[slot][y / 4][kx][y % 4]: the 4-row tile of
 //     a row-pass workgroup is one contiguous run, and a y-pass workgroup (4 adjacent kx
 //     columns) reads and writes 128-byte runs per 4 rows.
+#include <vector>
+
 #include "hp_common.hpp"
 
 namespace hp {
@@ -133,18 +135,20 @@ __host__ __device__ constexpr int tw_stage_off(int L, int s) {
 }
 __host__ __device__ constexpr int tw_total(int L) { return tw_stage_off(L, stages(L)); }
 
-// Fill the per-stage tables of length L (workgroup-cooperative) from `twg`, the table of a
-// length STRIDE * L: its entry STRIDE * m is exp(-2 pi i m / L), bit for bit.
-template <int L, int NTH, int STRIDE>
-__device__ __forceinline__ void stage_twiddles(float2* tw, const float2* __restrict__ twg) {
-    for (int idx = threadIdx.x; idx < tw_total(L); idx += NTH) {
-        int s = 1;
-        while (s + 1 < stages(L) && idx >= tw_stage_off(L, s + 1)) ++s;
-        const int ns = ns_before(L, s), tstep = L / (ns * radix(L, s));
-        const int rel = idx - tw_stage_off(L, s);
-        const int r = rel / ns + 1, k = rel - (r - 1) * ns;
-        tw[idx] = twg[STRIDE * (r * k * tstep)];
-    }
+// Entry idx of the per-stage tables of length L holds exp(-2 pi i m / L) for the m returned here.
+__host__ __device__ constexpr int stage_twiddle_src(int L, int idx) {
+    int s = 1;
+    while (s + 1 < stages(L) && idx >= tw_stage_off(L, s + 1)) ++s;
+    const int ns = ns_before(L, s), tstep = L / (ns * radix(L, s));
+    const int rel = idx - tw_stage_off(L, s);
+    const int r = rel / ns + 1, k = rel - (r - 1) * ns;
+    return r * k * tstep;
+}
+
+// Copy the per-stage tables of length L (built once on the host) into LDS, workgroup-cooperative.
+template <int L, int NTH>
+__device__ __forceinline__ void stage_twiddles(float2* tw, const float2* __restrict__ tws) {
+    for (int idx = threadIdx.x; idx < tw_total(L); idx += NTH) tw[idx] = tws[idx];
 }
 
 // One wavefront transforms one line of L elements held as e[i] = x[lane + 64 i].
@@ -232,7 +236,8 @@ constexpr int TP = RB + 1;    // LDS pitch of a row tile's kx entries: lanes at 
 template <int L>
 __global__ __launch_bounds__(NT) void k_rows_fwd_t(const float* __restrict__ A, float2* __restrict__ XT,
                                                    const int* __restrict__ src_chan, int H, long long N,
-                                                   long long spec, int nblocks, const float2* __restrict__ twg) {
+                                                   long long spec, int nblocks, const float2* __restrict__ twg,
+                                                   const float2* __restrict__ tws) {
     constexpr int M = L / 2, E = M / WAVE, LH = L / 2 + 1;
     constexpr int LINE = padded(M);
     __shared__ float2 tw[tw_total(M)];
@@ -240,7 +245,7 @@ __global__ __launch_bounds__(NT) void k_rows_fwd_t(const float* __restrict__ A, 
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     int bx, s;
     if (!xcd_block(H / RB, nblocks, bx, s)) return;
-    stage_twiddles<M, NT, 2>(tw, twg);
+    stage_twiddles<M, NT>(tw, tws);
     const float2* row =
         reinterpret_cast<const float2*>(A + (long long)src_chan[s] * N + (long long)(bx * RB + wave) * L);
     float2 e[E], w[E];
@@ -289,7 +294,7 @@ __global__ __launch_bounds__(NT) void k_spectral_t(const float2* __restrict__ XT
                                                    const float* __restrict__ KqT, const SpecWork* __restrict__ work,
                                                    const int* __restrict__ klist, int Wh, long long spec,
                                                    long long kq_stride, int ncp, int nblocks,
-                                                   const float2* __restrict__ twg) {
+                                                   const float2* __restrict__ twg, const float2* __restrict__ tws) {
     constexpr int M = L / 2, E = M / WAVE, LINE = padded(M), HQ = L / 2 + 1;
     __shared__ float2 tw[tw_total(M)];
     __shared__ float2 lines[WAVES * LINE];
@@ -302,7 +307,7 @@ __global__ __launch_bounds__(NT) void k_spectral_t(const float2* __restrict__ XT
     const SpecWork wk = work[wi];
     const int kx = cp * (WAVES / 2) + (wave >> 1);
     const bool active = kx < Wh;               // both halves of a column agree
-    stage_twiddles<M, NT, 2>(tw, twg);
+    stage_twiddles<M, NT>(tw, tws);
     float2* line = lines + wave * LINE;
     const float2* other = lines + (wave ^ 1) * LINE;
     // blocked column-major element (kx, y): (y / 4) * Wh * 4 + kx * 4 + y % 4; y = h M + lane + 64 i
@@ -376,7 +381,7 @@ __global__ __launch_bounds__(NT) void k_rows_inv_t(const float2* __restrict__ YT
                                                    const TgtWork* __restrict__ work, const int* __restrict__ klist,
                                                    const GrowthParams* __restrict__ gp, int H, long long N,
                                                    long long spec, float scale, int nblocks,
-                                                   const float2* __restrict__ twg) {
+                                                   const float2* __restrict__ twg, const float2* __restrict__ tws) {
     constexpr int M = L / 2, E = M / WAVE, LINE = padded(M), LH = L / 2 + 1;
     __shared__ float2 tw[tw_total(M)];
     __shared__ float2 lines[WAVES * LINE];
@@ -384,7 +389,7 @@ __global__ __launch_bounds__(NT) void k_rows_inv_t(const float2* __restrict__ YT
     int bx, wi;
     if (!xcd_block(H / RB, nblocks, bx, wi)) return;
     const TgtWork wk = work[wi];
-    stage_twiddles<M, NT, 2>(tw, twg);
+    stage_twiddles<M, NT>(tw, tws);
     float2 w[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) w[i] = twg[lane + WAVE * i];  // W_L^k, k = lane + 64 i
@@ -438,29 +443,29 @@ __global__ __launch_bounds__(NT) void k_rows_inv_t(const float2* __restrict__ YT
 
 template <int L>
 void rows_fwd_t(hipStream_t st, const float* A, float2* XT, const int* src_chan, int nsrc, const Geo& g,
-                const float2* tw) {
+                const float2* tw, const float2* tws) {
     const int nblocks = g.H / RB * nsrc;
     hipLaunchKernelGGL(k_rows_fwd_t<L>, dim3((nblocks + 7) / 8 * 8), dim3(NT), 0, st, A, XT, src_chan, g.H, g.N, g.spec,
-                       nblocks, tw);
+                       nblocks, tw, tws);
 }
 
 template <int L>
 void spectral_t(hipStream_t st, const float2* XT, float2* YT, const float* KqT, const SpecWork* work, int nwork,
-                const int* klist, const Geo& g, const float2* tw) {
+                const int* klist, const Geo& g, const float2* tw, const float2* tws) {
     const long long kq_stride = (long long)g.Wh * g.Hq;
     const int ncp = (g.Wh + WAVES / 2 - 1) / (WAVES / 2);  // column pairs
     const int nblocks = ncp * nwork;
     hipLaunchKernelGGL(k_spectral_t<L>, dim3((nblocks + 7) / 8 * 8), dim3(NT), 0, st, XT, YT, KqT, work, klist, g.Wh,
-                       g.spec, kq_stride, ncp, nblocks, tw);
+                       g.spec, kq_stride, ncp, nblocks, tw, tws);
 }
 
 template <int L>
 void rows_inv_t(hipStream_t st, const float2* YT, float* U, const TgtWork* work, int nwork, const int* klist,
-                const GrowthParams* gp, const Geo& g, const float2* tw) {
+                const GrowthParams* gp, const Geo& g, const float2* tw, const float2* tws) {
     const float scale = 1.0f / (float)(g.N);  // fft.rs:389: 1 / (W*H) as f32
     const int nblocks = g.H / RB * nwork;
     hipLaunchKernelGGL(k_rows_inv_t<L>, dim3((nblocks + 7) / 8 * 8), dim3(NT), 0, st, YT, U, work, klist, gp, g.H, g.N,
-                       g.spec, scale, nblocks, tw);
+                       g.spec, scale, nblocks, tw, tws);
 }
 
 }  // namespace wv
@@ -472,25 +477,33 @@ bool wave_fft_supported(int W, int H) {
 
 #define HP_WAVE_SIZES(X) X(256) X(512) X(1024) X(2048)
 
+std::vector<int> wave_stage_twiddle_index(int L) {
+    std::vector<int> idx;
+#define X_(n) if (L == n) for (int i = 0; i < wv::tw_total(n); ++i) idx.push_back(wv::stage_twiddle_src(n, i));
+    X_(128) X_(256) X_(512) X_(1024) X_(2048)
+#undef X_
+    return idx;
+}
+
 bool launch_rows_fwd_t(hipStream_t st, const float* A, float2* XT, const int* src_chan, int nsrc, const Geo& g,
-                       const float2* tw) {
-#define X_(n) if (g.W == n) { wv::rows_fwd_t<n>(st, A, XT, src_chan, nsrc, g, tw); return true; }
+                       const float2* tw, const float2* tws) {
+#define X_(n) if (g.W == n) { wv::rows_fwd_t<n>(st, A, XT, src_chan, nsrc, g, tw, tws); return true; }
     HP_WAVE_SIZES(X_)
 #undef X_
     return false;
 }
 
 bool launch_spectral_t(hipStream_t st, const float2* XT, float2* YT, const float* KqT, const SpecWork* work,
-                       int nwork, const int* klist, const Geo& g, const float2* tw) {
-#define X_(n) if (g.H == n) { wv::spectral_t<n>(st, XT, YT, KqT, work, nwork, klist, g, tw); return true; }
+                       int nwork, const int* klist, const Geo& g, const float2* tw, const float2* tws) {
+#define X_(n) if (g.H == n) { wv::spectral_t<n>(st, XT, YT, KqT, work, nwork, klist, g, tw, tws); return true; }
     HP_WAVE_SIZES(X_)
 #undef X_
     return false;
 }
 
 bool launch_rows_inv_t(hipStream_t st, const float2* YT, float* U, const TgtWork* work, int nwork, const int* klist,
-                       const GrowthParams* gp, const Geo& g, const float2* tw) {
-#define X_(n) if (g.W == n) { wv::rows_inv_t<n>(st, YT, U, work, nwork, klist, gp, g, tw); return true; }
+                       const GrowthParams* gp, const Geo& g, const float2* tw, const float2* tws) {
+#define X_(n) if (g.W == n) { wv::rows_inv_t<n>(st, YT, U, work, nwork, klist, gp, g, tw, tws); return true; }
     HP_WAVE_SIZES(X_)
 #undef X_
     return false;
