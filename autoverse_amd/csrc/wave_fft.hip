@@ -4,18 +4,21 @@
 // the growth epilogue; CachedConvolver::convolve_with_kernel_scratch, fft.rs:288-324,
 // and growth_accumulate, growth.rs:44-59), laid out for the GPU differently:
 //
-//   * every line transform is done by ONE wavefront: lane l holds the E = n/64 elements
-//     l, l+64, l+128, ... in registers, the radix-16/8/4/2 Stockham stages run in registers
-//     and exchange data through a per-wave LDS line (no workgroup barriers), and the first
-//     stage reads / last stage writes that same register layout, so a forward transform's
-//     output feeds the kernel-spectrum product and the inverse transform directly;
+//   * the transforms are wavefront FFTs: lane l holds the E = n/64 elements l, l+64, l+128,
+//     ... in registers, the radix-16/8/4/2 Stockham stages run in registers and exchange data
+//     through a per-wave LDS line (no workgroup barriers), and the first stage reads / last
+//     stage writes that same register layout, so a forward transform's output feeds the
+//     kernel-spectrum product and the inverse transform directly;
 //   * a row of L reals is transformed by one wavefront as an L/2-point complex FFT of its
 //     (even, odd) pairs, followed by the split into the half spectrum (and the reverse for
-//     the inverse), so the row passes run one wavefront per row: twice the wavefronts of a
-//     two-rows-per-FFT packing, each with half the work and half the LDS line;
+//     the inverse): one wavefront per row, each with half the work and half the LDS line of
+//     a two-rows-per-FFT packing;
+//   * a kx column of H is transformed by two wavefronts, one H/2-point FFT each, joined by a
+//     radix-2 butterfly exchanged through LDS;
 //   * spectra are stored blocked column-major, XT[slot][y / 4][kx][y % 4]: the 4-row tile of
-//     a row-pass workgroup is one contiguous run, and a y-pass workgroup (4 adjacent kx
-//     columns) reads and writes 128-byte runs per 4 rows.
+//     a row-pass workgroup is one contiguous run, and a y-pass workgroup (2 adjacent kx
+//     columns, 2 wavefronts each) reads and writes 64-byte runs per 4 rows, its XCD
+//     neighbour the other half of each 128-byte line.
 #include <vector>
 
 #include "hp_common.hpp"
