@@ -76,7 +76,8 @@ struct hp_ctx {
     int cur = 0;
     float time = 0.0f;
     uint64_t step = 0;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graph[2] = {nullptr, nullptr};       // one step from parity b
+    hipGraphExec_t graph_run[2] = {nullptr, nullptr};   // kGraphSteps steps from parity b
     bool profiling = false;
     int prof_reps = 1;  // back-to-back repeats of an out-of-place pass inside its event bracket
     std::vector<Launch> launches;
@@ -338,17 +339,37 @@ hp_status safe_step(hp_ctx* c) {
     return HP_OK;
 }
 
+// Steps per replay of the long graphs: an even count, so a replay returns to its parity and
+// long runs pay one graph launch per kGraphSteps steps.
+constexpr int kGraphSteps = 16;
+
+hp_status capture_steps(hp_ctx* c, int b, int n, hipGraphExec_t* out) {
+    hipGraph_t graph = nullptr;
+    HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hp_status st = HP_OK;
+    for (int i = 0; i < n && st == HP_OK; i++) {
+        const int p = b ^ (i & 1);
+        st = enqueue_affinity(c, c->A[p]);
+        if (st == HP_OK) st = enqueue_fast_stencil(c, c->A[p], c->U, c->A[1 - p]);
+    }
+    hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+    if (st != HP_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    if (ec != hipSuccess) return c->fail(HP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+    hipError_t e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return c->fail(HP_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    return HP_OK;
+}
+
 hp_status build_graphs(hp_ctx* c) {
     for (int b = 0; b < 2; b++) {
-        if (c->graph[b]) continue;
-        hipGraph_t graph = nullptr;
-        HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_affinity(c, c->A[b]);
-        enqueue_fast_stencil(c, c->A[b], c->U, c->A[1 - b]);
-        HIP_TRY(c, hipStreamEndCapture(c->stream, &graph));
-        hipError_t e = hipGraphInstantiate(&c->graph[b], graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) return c->fail(HP_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        hp_status st = HP_OK;
+        if (!c->graph[b]) st = capture_steps(c, b, 1, &c->graph[b]);
+        if (st == HP_OK && !c->graph_run[b]) st = capture_steps(c, b, kGraphSteps, &c->graph_run[b]);
+        if (st != HP_OK) return st;
     }
     return HP_OK;
 }
@@ -375,18 +396,25 @@ hp_status advance(hp_ctx* c, uint64_t steps) {
         hp_status st = build_graphs(c);
         if (st != HP_OK) return st;
     }
-    for (uint64_t i = 0; i < steps; i++) {
+    for (uint64_t i = 0; i < steps;) {
         if (c->profiling) {
             float* A = c->A[c->cur];
             hp_status st = enqueue_affinity(c, A);
             if (st == HP_OK) st = enqueue_fast_stencil(c, A, c->U, c->A[1 - c->cur]);
             if (st != HP_OK) return st;
+        } else if (steps - i >= (uint64_t)kGraphSteps) {
+            HIP_TRY(c, hipGraphLaunch(c->graph_run[c->cur], c->stream));
+            for (int k = 0; k < kGraphSteps; k++) c->time += c->cfg.dt;  // f32, one dt per step
+            c->step += kGraphSteps;
+            i += kGraphSteps;
+            continue;  // an even count of steps: the parity is unchanged
         } else {
             HIP_TRY(c, hipGraphLaunch(c->graph[c->cur], c->stream));
         }
         c->cur = 1 - c->cur;
         c->time += c->cfg.dt;
         c->step += 1;
+        i += 1;
     }
     int f = 0;
     HIP_TRY(c, hipMemcpyAsync(&f, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -437,8 +465,10 @@ hp_status download(hp_ctx* c, const float* src, float* const* ch) {
 
 void release(hp_ctx* c) {
     if (!c) return;
-    for (int b = 0; b < 2; b++)
+    for (int b = 0; b < 2; b++) {
         if (c->graph[b]) (void)hipGraphExecDestroy(c->graph[b]);
+        if (c->graph_run[b]) (void)hipGraphExecDestroy(c->graph_run[b]);
+    }
     for (auto& l : c->launches) {
         (void)hipEventDestroy(l.a);
         (void)hipEventDestroy(l.b);
