@@ -300,7 +300,7 @@ hp_status tape_check_member(hp_batch* b, size_t i) {
     const std::string who = "member " + std::to_string(i) + ": ";
     if (!hp::adj_fft_supported(p.g.W) || !hp::adj_fft_supported(p.g.H) || p.g.W > hp::window_dft_max_side() ||
         p.g.H > hp::window_dft_max_side())
-        return b->fail(HP_ERR_UNSUPPORTED_SIZE, who + "gradient tapes need sides 2^a * 3^b up to " +
+        return b->fail(HP_ERR_UNSUPPORTED_SIZE, who + "gradient tapes need sides 2^k or 3 * 2^k up to " +
                                                    std::to_string(hp::window_dft_max_side()));
     if (p.spec_work.size() != p.src_chan.size())
         return b->fail(HP_ERR_UNSUPPORTED_SIZE, who + "gradient tapes need one spectral item per source");

@@ -152,7 +152,7 @@ hp_status plan_member(const hp_config* cfg, bool allow_wave, int group, MemberPl
     const bool is3d = cfg->depth > 1;
     const uint32_t W = cfg->width, H = cfg->height, D = cfg->depth;
     if (!fft_supported((int)W) || !fft_supported((int)H) || (is3d && !fft_supported((int)D))) {
-        *err = "grid sides must be 2^a * 3^b between 8 and 2048";
+        *err = "grid sides must be 2^k or 3 * 2^k between 8 and 2048";
         return HP_ERR_UNSUPPORTED_SIZE;
     }
     p.C = (int)cfg->channels;

@@ -223,17 +223,20 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
                 sOX[j][i] = overlap_clamped(cx + (float)(1 - j), cx + (float)(2 - j), xm, xM);
                 sOY[j][i] = overlap_clamped(cy + (float)(1 - j), cy + (float)(2 - j), ym, yM);
             }
-            // With (2s)^2 a power of two the scaling by its inverse is exact (overlaps are 0 or at
-            // least ~6e-8 and |m| >= 1e-10, so nothing is subnormal): m * ((ow*oh) * inv) equals
-            // (m * inv) * (ow*oh) bit for bit, and the gather saves a multiply per pair.
-            sM[i] = POW2 ? m * inv_ta : m;
+            // With (2s)^2 a power of two the scaling by its inverse is exact (the masses skipped
+            // above are the only ones below 1e-10, every nonzero overlap is a multiple of an ulp
+            // of s or of a coordinate, so no product is subnormal before the last one): m * ((ow*oh)
+            // * inv) equals (m * inv) * (ow*oh) bit for bit, and the gather saves a multiply per
+            // pair.  A scaled mass that overflows is sent to the general path below.
+            const float sm = POW2 ? m * inv_ta : m;
+            sM[i] = sm;
             // every source is checked by the one tile that owns it
             const int ux = x0 - 1 + sx, uy = y0 - 1 + sy;
             if (ux >= x0 && ux < x0 + TX && uy >= y0 && uy < y0 + TY && ux < W && uy < H) {
                 // floor(xm) < c - 2  <=>  xm < c - 2 for integer c (likewise ceil); a non-finite
                 // value fails the comparisons and is classified on the slow side
                 const bool inside = xm >= cx - 2.0f && xM <= cx + 2.0f && ym >= cy - 2.0f && yM <= cy + 2.0f &&
-                                    fabsf(m) <= 3.40282347e38f;
+                                    fabsf(sm) <= 3.40282347e38f;
                 if (!inside)
                     local_flag |= (isfinite(xm) && isfinite(xM) && isfinite(ym) && isfinite(yM) && isfinite(m)) ? 1 : 2;
             }
@@ -465,14 +468,27 @@ bool launch_stencil2d(hipStream_t st, const float* A, const float* U, float* Aou
 // reference forms cell_mass; zero-volume pairs add +-0 and change nothing.  Any owned source
 // whose box reaches further sets flag bit 0 (bit 1 for non-finite values) and the host reruns
 // the step through the general path.
+//
+// Work split.  A workgroup owns a 16 x 8 x 8 destination tile (its 19 x 11 x 11 source region
+// is 2.25 sources per destination).  LDS holds one affinity tile and, per source, its mass and
+// four overlaps per axis; the channel-sum tile shares memory with the z overlaps, which are
+// first written after the alpha phase.  That is 131 KiB: one workgroup of 1024 threads per CU,
+// four waves per SIMD.  Each thread gathers a z strip of SR destinations.
 // ---------------------------------------------------------------------------------------------
 namespace st3 {
-constexpr int TX = 16, TY = 4, TZ = 4, NT = TX * TY * TZ;
-constexpr int SX = TX + 3, SY = TY + 3, SZ = TZ + 3;  // sources: offsets -1 .. +2 around the tile
-constexpr int LX = SX + 2, LY = SY + 2, LZ = SZ + 2;  // + Sobel halo
-constexpr int NSRC = SX * SY * SZ;
-constexpr int NLOAD = LX * LY * LZ;
-constexpr int SPT = (NSRC + NT - 1) / NT;
+template <int SR_>
+struct Tile {
+    static constexpr int TX = 16, TY = 8, TZ = 8, SR = SR_, NT = TX * TY * TZ / SR;
+    static constexpr int SX = TX + 3, SY = TY + 3, SZ = TZ + 3;  // sources: offsets -1 .. +2 around the tile
+    static constexpr int LX = SX + 2, LY = SY + 2, LZ = SZ + 2;  // + Sobel halo
+    static constexpr int NSRC = SX * SY * SZ;
+    static constexpr int NLOAD = LX * LY * LZ;
+    static constexpr int LPT = (NLOAD + NT - 1) / NT;  // load-region cells per thread
+    static constexpr int SPT = (NSRC + NT - 1) / NT;   // sources per thread
+    static constexpr int SROWS = SR + 3;               // source planes of one strip
+    static_assert(TZ % SR == 0 && NLOAD <= 4 * NSRC, "strip and channel-sum aliasing");
+};
+using Tile3 = Tile<1>;
 }  // namespace st3
 
 // gradient3d.rs:75-138 on an LDS box: P(dz, dy, dx) = L[i + dz*sz + dy*sy + dx].
@@ -506,18 +522,20 @@ __device__ __forceinline__ int wrap_rotation(int g, int n) {
     return g < 1 ? 1 - g : (g + 2 >= n ? 4 - (g + 2 - (n - 1)) : 0);
 }
 
-template <bool POW2>
-__global__ __launch_bounds__(st3::NT) void k_stencil3d(const float* __restrict__ A, const float* __restrict__ U,
-                                                       float* __restrict__ Aout, int C, int W, int H, int D,
-                                                       long long N, int tiles_x, int tiles_y, int ntiles, float dt,
-                                                       float beta_a, float nexp, float s, float tv, float inv_tv,
-                                                       int* __restrict__ flag) {
-    using namespace st3;
-    __shared__ float sSum[NLOAD];
-    __shared__ float sUc[NLOAD];
-    __shared__ float sAl[NSRC], sGx[NSRC], sGy[NSRC], sGz[NSRC];
+template <bool POW2, class TL>
+__global__ __launch_bounds__(TL::NT) void k_stencil3d(const float* __restrict__ A, const float* __restrict__ U,
+                                                    float* __restrict__ Aout, int C, int W, int H, int D, int N,
+                                                    int tiles_x, int tiles_y, int ntiles, float dt, float beta_a,
+                                                    float nexp, float s, float tv, float inv_tv,
+                                                    int* __restrict__ flag) {
+    constexpr int TX = TL::TX, TY = TL::TY, TZ = TL::TZ, SR = TL::SR, NT = TL::NT;
+    constexpr int SX = TL::SX, SY = TL::SY, LX = TL::LX, LY = TL::LY;
+    constexpr int NSRC = TL::NSRC, NLOAD = TL::NLOAD, LPT = TL::LPT, SPT = TL::SPT, SROWS = TL::SROWS;
+    __shared__ float sU[NLOAD];
     __shared__ float sM[NSRC];
-    __shared__ float sOX[4][NSRC], sOY[4][NSRC], sOZ[4][NSRC];  // overlap j of each source, j-major
+    __shared__ float sOX[4 * NSRC], sOY[4 * NSRC], sOZ[4 * NSRC];  // overlap j of source i at [j * NSRC + i]
+    float* const sSum = sOZ;
+
     const int b = blockIdx.x;
     const int per = (ntiles + 7) / 8;
     const int tile = (b % 8) * per + b / 8;  // XCD-aware order, as the 2D kernel
@@ -525,143 +543,256 @@ __global__ __launch_bounds__(st3::NT) void k_stencil3d(const float* __restrict__
     const int x0 = (tile % tiles_x) * TX, y0 = ((tile / tiles_x) % tiles_y) * TY, z0 = (tile / (tiles_x * tiles_y)) * TZ;
     const int tid = threadIdx.x;
     const int HW = W * H;
-    auto gidx = [&](int zz, int yy, int xx) -> long long {
-        return (long long)wrap_near(zz, D) * HW + (long long)wrap_near(yy, H) * W + wrap_near(xx, W);
-    };
-    // channel sum over the load region (propagator3d.rs: 0 + A_0 + A_1 + ...)
-    for (int t = tid; t < NLOAD; t += NT) {
-        const int lz = t / (LX * LY), r = t - lz * LX * LY, ly = r / LX, lx = r - ly * LX;
-        const long long gi = gidx(z0 - 2 + lz, y0 - 2 + ly, x0 - 2 + lx);
-        float v = 0.0f;
-        for (int c = 0; c < C; ++c) v += A[(long long)c * N + gi];
-        sSum[t] = v;
+    // load-region coordinates lie in [-2, n + T + 1): one wrap step suffices unless the grid is
+    // smaller than the tile plus its halo
+    const bool multi = W < TX + 2 || H < TY + 2 || D < TZ + 2;
+
+    // ---- every global read of the tile, issued together ----
+    int goff[LPT];
+#pragma unroll
+    for (int q = 0; q < LPT; ++q) {
+        const int i = tid + q * NT;
+        const int lz = i / (LX * LY), r = i - lz * (LX * LY), ly = r / LX, lx = r - ly * LX;
+        goff[q] = i < NLOAD ? wrap_tile(z0 - 2 + lz, D, multi) * HW + wrap_tile(y0 - 2 + ly, H, multi) * W +
+                                  wrap_tile(x0 - 2 + lx, W, multi)
+                            : 0;
     }
-    __syncthreads();
-    // channel-independent source data: alpha and the mass gradient
-#pragma unroll 1
+    int soff[SPT];  // grid offsets of the owned sources
+#pragma unroll
     for (int q = 0; q < SPT; ++q) {
         const int i = tid + q * NT;
-        if (i >= NSRC) break;
-        const int sz = i / (SX * SY), r = i - sz * SX * SY, sy = r / SX, sx = r - sy * SX;
+        const int sz = i / (SX * SY), r = i - sz * (SX * SY), sy = r / SX, sx = r - sy * SX;
+        soff[q] = i < NSRC ? wrap_tile(z0 - 1 + sz, D, multi) * HW + wrap_tile(y0 - 1 + sy, H, multi) * W +
+                                 wrap_tile(x0 - 1 + sx, W, multi)
+                           : 0;
+    }
+    float msrc[SPT];
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) msrc[q] = A[soff[q]];
+    float upre[LPT];
+#pragma unroll
+    for (int q = 0; q < LPT; ++q) upre[q] = U[goff[q]];
+#pragma unroll
+    for (int q = 0; q < LPT; ++q) {  // channel sum (propagator3d.rs: 0 + A_0 + A_1 + ...)
+        float sum = 0.0f;
+        for (int c = 0; c < C; ++c) sum += A[(long long)c * N + goff[q]];
+        const int i = tid + q * NT;
+        if (i < NLOAD) {
+            sSum[i] = sum;
+            sU[i] = upre[q];
+        }
+    }
+    __syncthreads();
+    // alpha and the mass gradient of the owned sources.  The loop stays rolled (alpha_of may
+    // expand to a powf); each thread parks its own results in the x overlap planes, which are
+    // free until the first source phase, and reads them back into registers -- the same thread
+    // writes and reads every entry, so no barrier is needed.
+#pragma unroll 1
+    for (int i = tid; i < NSRC; i += NT) {
+        const int sz = i / (SX * SY), r = i - sz * (SX * SY), sy = r / SX, sx = r - sy * SX;
         const int li = ((sz + 1) * LY + (sy + 1)) * LX + (sx + 1);
         float gx, gy, gz;
         sobel27_lds(sSum, li, LX, LX * LY, gx, gy, gz);
-        sGx[i] = gx;
-        sGy[i] = gy;
-        sGz[i] = gz;
-        sAl[i] = alpha_of(sSum[li], beta_a, nexp);
+        const float al = alpha_of(sSum[li], beta_a, nexp);
+        sOX[i] = 1.0f - al;
+        sOX[NSRC + i] = al * gx;
+        sOX[2 * NSRC + i] = al * gy;
+        sOX[3 * NSRC + i] = al * gz;
     }
-    // destination of this thread and its visiting order
-    const int lx = tid % TX, ly = (tid / TX) % TY, lz = tid / (TX * TY);
-    const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
-    const bool inside = gx < W && gy < H && gz < D;
-    const int rx = wrap_rotation(gx, W), ry = wrap_rotation(gy, H), rz = wrap_rotation(gz, D);
+    float aOm[SPT], aGx[SPT], aGy[SPT], aGz[SPT];
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) {
+        const int i = tid + q * NT;
+        const bool ok = i < NSRC;
+        aOm[q] = ok ? sOX[i] : 0.0f;
+        aGx[q] = ok ? sOX[NSRC + i] : 0.0f;
+        aGy[q] = ok ? sOX[2 * NSRC + i] : 0.0f;
+        aGz[q] = ok ? sOX[3 * NSRC + i] : 0.0f;
+    }
+    __syncthreads();  // the channel sum (in the z overlap planes) has been read
+
+    // strip geometry: destination (gdx, gdy), planes gdz0 .. gdz0 + SR - 1
+    const int lx = tid % TX, ly = (tid / TX) % TY, lzs = tid / (TX * TY);
+    const int gdx = x0 + lx, gdy = y0 + ly, gdz0 = z0 + lzs * SR;
+    const bool inside = gdx < W && gdy < H && gdz0 < D;
+    const int rx = wrap_rotation(gdx, W), ry = wrap_rotation(gdy, H);
+    // ascending wrapped order of the strip's planes: start with the first that does not wrap low
+    const int rz = gdz0 < 1 ? 1 - gdz0 : (gdz0 + SR + 1 >= D ? SROWS - (gdz0 + SR + 1 - (D - 1)) : 0);
+    const int base = (lzs * SR * SY + ly) * SX + lx;  // source (plane 0, row 0, column 0) of the strip
+
     int local_flag = 0;
-    for (int c = 0; c < C; ++c) {
-        const float* Uc = U + (long long)c * N;
-        __syncthreads();  // previous channel's gather has read the source data
-        for (int t = tid; t < NLOAD; t += NT) {
-            const int tz = t / (LX * LY), r = t - tz * LX * LY, ty = r / LX, tx = r - ty * LX;
-            sUc[t] = Uc[gidx(z0 - 2 + tz, y0 - 2 + ty, x0 - 2 + tx)];
-        }
-        __syncthreads();
 #pragma unroll 1
+    for (int c = 0; c < C; ++c) {
+        // ---- sources: flow, box, overlaps ----
+#pragma unroll
         for (int q = 0; q < SPT; ++q) {
             const int i = tid + q * NT;
             if (i >= NSRC) break;
-            const int sz = i / (SX * SY), r = i - sz * SX * SY, sy = r / SX, sx = r - sy * SX;
-            const int ux = x0 - 1 + sx, uy = y0 - 1 + sy, uz = z0 - 1 + sz;
-            const int gxs = wrap_near(ux, W), gys = wrap_near(uy, H), gzs = wrap_near(uz, D);
-            const float m = A[(long long)c * N + (long long)gzs * HW + (long long)gys * W + gxs];
+            const float m = msrc[q];
             if (fabsf(m) < 1e-10f) {  // reintegration3d.rs:77: a skipped source adds nothing
                 sM[i] = 0.0f;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) sOX[j][i] = sOY[j][i] = sOZ[j][i] = 0.0f;
+                for (int j = 0; j < 4; ++j) sOX[j * NSRC + i] = sOY[j * NSRC + i] = sOZ[j * NSRC + i] = 0.0f;
                 continue;
             }
+            const int sz = i / (SX * SY), r = i - sz * (SX * SY), sy = r / SX, sx = r - sy * SX;
             const int li = ((sz + 1) * LY + (sy + 1)) * LX + (sx + 1);
             float gux, guy, guz;
-            sobel27_lds(sUc, li, LX, LX * LY, gux, guy, guz);
-            const float al = sAl[i];
-            const float oma = 1.0f - al;
-            const float fx = oma * gux - al * sGx[i];
-            const float fy = oma * guy - al * sGy[i];
-            const float fz = oma * guz - al * sGz[i];
-            const float dxp = (float)gxs + dt * fx;  // reintegration3d.rs:83-85
-            const float dyp = (float)gys + dt * fy;
-            const float dzp = (float)gzs + dt * fz;
+            sobel27_lds(sU, li, LX, LX * LY, gux, guy, guz);
+            const float fx = aOm[q] * gux - aGx[q];
+            const float fy = aOm[q] * guy - aGy[q];
+            const float fz = aOm[q] * guz - aGz[q];
+            // wrapped grid coordinates: integer-valued floats below 2^24
+            const int gxs = wrap_tile(x0 - 1 + sx, W, multi), gys = wrap_tile(y0 - 1 + sy, H, multi),
+                      gzs = wrap_tile(z0 - 1 + sz, D, multi);
+            const float cx = (float)gxs, cy = (float)gys, cz = (float)gzs;
+            const float dxp = cx + dt * fx;  // reintegration3d.rs:83-85
+            const float dyp = cy + dt * fy;
+            const float dzp = cz + dt * fz;
             const float xm = dxp - s, xM = dxp + s, ym = dyp - s, yM = dyp + s, zm = dzp - s, zM = dzp + s;
-            float o[3][4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {  // cells gxs + 1 - j of the source's frame
-                o[0][j] = overlap_clamped((float)(gxs + 1 - j), (float)(gxs + 2 - j), xm, xM);
-                o[1][j] = overlap_clamped((float)(gys + 1 - j), (float)(gys + 2 - j), ym, yM);
-                o[2][j] = overlap_clamped((float)(gzs + 1 - j), (float)(gzs + 2 - j), zm, zM);
-            }
-            sM[i] = m;
+            // overlaps with destination cells g + 1 - j, j = o + 1 (o = source - dest)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                sOX[j][i] = o[0][j];
-                sOY[j][i] = o[1][j];
-                sOZ[j][i] = o[2][j];
+                sOX[j * NSRC + i] = overlap_clamped(cx + (float)(1 - j), cx + (float)(2 - j), xm, xM);
+                sOY[j * NSRC + i] = overlap_clamped(cy + (float)(1 - j), cy + (float)(2 - j), ym, yM);
+                sOZ[j * NSRC + i] = overlap_clamped(cz + (float)(1 - j), cz + (float)(2 - j), zm, zM);
             }
+            // With (2s)^3 a power of two the scaling by its inverse is exact (the masses skipped
+            // above are the only ones below 1e-10, every nonzero overlap is a multiple of an ulp
+            // of s or of a coordinate, so no product is subnormal before the last one): m * (vol *
+            // inv) equals (m * inv) * vol bit for bit, and the gather saves a multiply per pair.
+            // A scaled mass that overflows is sent to the general path below.
+            const float sm = POW2 ? m * inv_tv : m;
+            sM[i] = sm;
+            // every source is checked by the one tile that owns it
+            const int ux = x0 - 1 + sx, uy = y0 - 1 + sy, uz = z0 - 1 + sz;
             if (ux >= x0 && ux < x0 + TX && uy >= y0 && uy < y0 + TY && uz >= z0 && uz < z0 + TZ && ux < W && uy < H &&
                 uz < D) {
-                if (!(isfinite(xm) && isfinite(xM) && isfinite(ym) && isfinite(yM) && isfinite(zm) && isfinite(zM) &&
-                      isfinite(m)))
-                    local_flag |= 2;
-                else if (floorf(xm) < (float)(gxs - 2) || ceilf(xM) > (float)(gxs + 2) || floorf(ym) < (float)(gys - 2) ||
-                         ceilf(yM) > (float)(gys + 2) || floorf(zm) < (float)(gzs - 2) || ceilf(zM) > (float)(gzs + 2))
-                    local_flag |= 1;
+                // floor(xm) < c - 2  <=>  xm < c - 2 for integer c (likewise ceil); a non-finite
+                // value fails the comparisons and is classified on the slow side
+                const bool ok = xm >= cx - 2.0f && xM <= cx + 2.0f && ym >= cy - 2.0f && yM <= cy + 2.0f &&
+                                zm >= cz - 2.0f && zM <= cz + 2.0f && fabsf(sm) <= 3.40282347e38f;
+                if (!ok)
+                    local_flag |= (isfinite(xm) && isfinite(xM) && isfinite(ym) && isfinite(yM) && isfinite(zm) &&
+                                   isfinite(zM) && isfinite(m))
+                                      ? 1
+                                      : 2;
             }
         }
-        __syncthreads();
-        if (!inside) continue;
-        float acc = 0.0f;
-        // the destination is cell gxs - o of a source's frame: that source's overlap index 1 + o
-        for (int az = 0; az < 4; ++az) {
-            const int oz = ((az + rz) & 3) - 1;
-            const float* wzs = sOZ[1 + oz];
-            for (int ay = 0; ay < 4; ++ay) {
-                const int oy = ((ay + ry) & 3) - 1;
-                const float* wys = sOY[1 + oy];
-                const int row = ((lz + 1 + oz) * SY + (ly + 1 + oy)) * SX + lx + 1;
+        // next channel's affinity and masses: into registers now, the affinity into LDS after the
+        // barrier (the gather does not read it)
+        if (c + 1 < C) {
+            const float* Un = U + (long long)(c + 1) * N;
 #pragma unroll
-                for (int ax = 0; ax < 4; ++ax) {
-                    const int ox = ((ax + rx) & 3) - 1;
-                    const int si = row + ox;
-                    const float vol = sOX[1 + ox][si] * wys[si] * wzs[si];
-                    const float frac = POW2 ? vol * inv_tv : vol / tv;
-                    acc += sM[si] * frac;
+            for (int q = 0; q < LPT; ++q) upre[q] = Un[goff[q]];
+            const float* An = A + (long long)(c + 1) * N;
+#pragma unroll
+            for (int q = 0; q < SPT; ++q) msrc[q] = An[soff[q]];
+        }
+        __syncthreads();
+        if (c + 1 < C) {
+#pragma unroll
+            for (int q = 0; q < LPT; ++q)
+                if (tid + q * NT < NLOAD) sU[tid + q * NT] = upre[q];
+        }
+        // ---- gather: strip of SR destinations ----
+        // A zero-volume pair contributes m * 0 = +-0; added to an accumulator that started at +0
+        // it changes no bit, so every pair is added without the reference's `volume > 0` test.
+        // Waves with no periodic edge in reach take the unrotated path, whose LDS addresses are
+        // one base plus constants; waves touching an edge visit in rotated order.
+        float acc[SR];
+#pragma unroll
+        for (int r = 0; r < SR; ++r) acc[r] = 0.0f;
+        if (__all(rx == 0 && ry == 0 && rz == 0 && inside)) {
+#pragma unroll
+            for (int a = 0; a < SROWS; ++a) {  // strip source plane a <-> grid plane gdz0 - 1 + a
+#pragma unroll
+                for (int jy = 0; jy < 4; ++jy) {
+#pragma unroll
+                    for (int jx = 0; jx < 4; ++jx) {
+                        const int si = base + (a * SY + jy) * SX + jx;
+                        const float m = sM[si];
+                        const float area = sOX[jx * NSRC + si] * sOY[jy * NSRC + si];
+#pragma unroll
+                        for (int r = 0; r < SR; ++r) {
+                            if (a - r >= 0 && a - r <= 3) {
+                                const float vol = area * sOZ[(a - r) * NSRC + si];
+                                acc[r] += POW2 ? m * vol : m * (vol / tv);
+                            }
+                        }
+                    }
+                }
+            }
+        } else if (inside) {
+            int xo[4], yo[4];  // overlap index visited in slot k
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                xo[k] = (k + rx) & 3;
+                yo[k] = (k + ry) & 3;
+            }
+#pragma unroll 1
+            for (int it = 0; it < SROWS; ++it) {
+                int a = it + rz;
+                if (a >= SROWS) a -= SROWS;
+                const int pb = base + a * SY * SX;
+#pragma unroll
+                for (int ky = 0; ky < 4; ++ky) {
+                    const int jy = yo[ky];
+                    const int rb = pb + jy * SX;
+#pragma unroll
+                    for (int kx = 0; kx < 4; ++kx) {
+                        const int jx = xo[kx];
+                        const int si = rb + jx;
+                        const float m = sM[si];
+                        const float area = sOX[jx * NSRC + si] * sOY[jy * NSRC + si];
+#pragma unroll
+                        for (int r = 0; r < SR; ++r) {
+                            const int jz = a - r;
+                            if (jz >= 0 && jz <= 3) {
+                                const float vol = area * sOZ[jz * NSRC + si];
+                                acc[r] += POW2 ? m * vol : m * (vol / tv);
+                            }
+                        }
+                    }
                 }
             }
         }
-        Aout[(long long)c * N + (long long)gz * HW + (long long)gy * W + gx] = acc;
+        if (inside) {
+            float* Oc = Aout + (long long)c * N + gdy * W + gdx;
+#pragma unroll
+            for (int r = 0; r < SR; ++r)
+                if (gdz0 + r < D) Oc[(gdz0 + r) * HW] = acc[r];
+        }
+        if (c + 1 < C) __syncthreads();  // the next channel's source phase overwrites the overlaps
     }
     if (local_flag) atomicOr(flag, local_flag);
 }
 
 bool stencil3d_supported(const Geo& g, float s) {
+    // distinct cells in every strip's window; int offsets; the 64-source window assumes s <= 1
     const float tv = (2.0f * s) * (2.0f * s) * (2.0f * s);
-    return g.D > 1 && g.W >= 4 && g.H >= 4 && g.D >= 4 && s > 0.0f && s <= 1.0f && tv >= 1e-10f;
+    return g.D > 1 && g.W >= 4 && g.H >= 4 && g.D >= st3::Tile3::SROWS && g.N <= 0x7fffffffLL && s > 0.0f &&
+           s <= 1.0f && tv >= 1e-10f;
 }
 
 bool launch_stencil3d(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g, float dt,
                       float beta_a, float nexp, float s, int* flag) {
-    using namespace st3;
+    using TL = st3::Tile3;
     if (!stencil3d_supported(g, s)) return false;
     const float tv = (2.0f * s) * (2.0f * s) * (2.0f * s);  // reintegration3d.rs:134
     int e;
     const bool pow2 = frexpf(tv, &e) == 0.5f && tv >= 1.17549435e-38f && tv < INFINITY;
-    const int tiles_x = (g.W + TX - 1) / TX, tiles_y = (g.H + TY - 1) / TY, tiles_z = (g.D + TZ - 1) / TZ;
+    const int tiles_x = (g.W + TL::TX - 1) / TL::TX, tiles_y = (g.H + TL::TY - 1) / TL::TY,
+              tiles_z = (g.D + TL::TZ - 1) / TL::TZ;
     const int ntiles = tiles_x * tiles_y * tiles_z;
     const dim3 grid((ntiles + 7) / 8 * 8);
+    const int N = (int)g.N;
     if (pow2)
-        hipLaunchKernelGGL((k_stencil3d<true>), grid, dim3(NT), 0, st, A, U, Aout, C, g.W, g.H, g.D, g.N, tiles_x,
+        hipLaunchKernelGGL((k_stencil3d<true, TL>), grid, dim3(TL::NT), 0, st, A, U, Aout, C, g.W, g.H, g.D, N, tiles_x,
                            tiles_y, ntiles, dt, beta_a, nexp, s, tv, 1.0f / tv, flag);
     else
-        hipLaunchKernelGGL((k_stencil3d<false>), grid, dim3(NT), 0, st, A, U, Aout, C, g.W, g.H, g.D, g.N, tiles_x,
-                           tiles_y, ntiles, dt, beta_a, nexp, s, tv, 1.0f / tv, flag);
+        hipLaunchKernelGGL((k_stencil3d<false, TL>), grid, dim3(TL::NT), 0, st, A, U, Aout, C, g.W, g.H, g.D, N,
+                           tiles_x, tiles_y, ntiles, dt, beta_a, nexp, s, tv, 1.0f / tv, flag);
     return true;
 }
 

@@ -317,7 +317,7 @@ hp_status hp_batch_stats(const hp_batch* b, uint32_t* max_gather_radius, uint64_
  * checkpoint_every = 0 picks the interval from free device memory: every step (no
  * recomputation) when the whole trajectory fits in half of it, else about sqrt(steps);
  * *used_every (optional) receives the interval used.
- * Limits: sides 2^a * 3^b up to 1024; every kernel's support 2R+1 within the grid. */
+ * Limits: sides 2^k or 3 * 2^k (8 .. 1024); every kernel's support 2R+1 within the grid. */
 hp_status hp_batch_theta_count(const hp_batch* b, uint32_t member, uint32_t* count);
 /* Batch plans: the LDS FFT variant and the stencil tile shared by all members (fft_path and
  * spectral_group are fixed for populations: LDS FFTs, one spectral item per source).

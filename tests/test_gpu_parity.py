@@ -298,18 +298,22 @@ def test_3d_sphere_128_steps(oracle):
     assert e <= 1e-4
 
 
-@pytest.mark.parametrize("shape,channels,dt", [((24, 16, 8), 2, 0.1), ((8, 8, 8), 1, 0.1), ((32, 24, 16), 3, 0.2),
-                                               ((16, 16, 16), 1, 40.0)])
-def test_3d_stencil_bitwise(oracle, shape, channels, dt):
+@pytest.mark.parametrize("shape,channels,dt,s", [((24, 16, 8), 2, 0.1, 1.0), ((8, 8, 8), 1, 0.1, 1.0),
+                                                 ((32, 24, 16), 3, 0.2, 1.0), ((16, 16, 16), 1, 40.0, 1.0),
+                                                 ((48, 24, 16), 2, 0.1, 0.75), ((64, 48, 32), 1, 0.1, 0.5),
+                                                 ((48, 32, 24), 3, 0.15, 0.6), ((24, 96, 8), 1, 0.2, 1.0)])
+def test_3d_stencil_bitwise(oracle, shape, channels, dt, s):
     """The fused 3D stencil (64-source gather) on random states: the oracle's next state bit for
-    bit; a large dt leaves the fused window and must come back through the general path."""
+    bit, with (2s)^3 a power of two and not, on grids that are and are not tile multiples (so
+    that both the unrotated and the edge-rotated gathers run); a large dt leaves the fused window
+    and must come back through the general path."""
     from autoverse_amd.schema import FlowConfig, SimulationConfig
     W, H, D = shape
-    _say(f"[3d_stencil] {shape} C={channels} dt={dt}")
+    _say(f"[3d_stencil] {shape} C={channels} dt={dt} s={s}")
     rng = np.random.default_rng(W * H + D + channels)
     ks = workloads._random_kernels(rng, channels)
     cfg = SimulationConfig(width=W, height=H, depth=D, channels=channels, dt=dt, kernel_radius=3, kernels=ks,
-                           flow=FlowConfig(1.0, 2.0, 1.0))
+                           flow=FlowConfig(1.0, 2.0, s))
     ch = [rng.random(W * H * D).astype(np.float32) for _ in range(channels)]
     p = oracle.Propagator(cfg, np.float32)
     aff = p.affinity(ch)
@@ -324,3 +328,29 @@ def test_3d_stencil_bitwise(oracle, shape, channels, dt):
     else:
         assert g == 0
 
+
+
+@pytest.mark.parametrize("shape", [(64, 64, 1), (32, 32, 24)])
+def test_stencil_huge_mass_leaves_fused_path(oracle, shape):
+    """With (2s)^d a power of two the fused stencils pre-scale each mass by 1/(2s)^d; for
+    s = 0.25 and a mass of 1e38 that product overflows while the reference's m * (V_o / V)
+    does not.  The source must be sent to the general path and the result stay bit-exact."""
+    from autoverse_amd.schema import FlowConfig, SimulationConfig
+    W, H, D = shape
+    _say(f"[huge_mass] {shape}")
+    rng = np.random.default_rng(17)
+    ks = workloads._random_kernels(rng, 1)
+    cfg = SimulationConfig(width=W, height=H, depth=D, channels=1, dt=0.1, kernel_radius=3, kernels=ks,
+                           flow=FlowConfig(1.0, 2.0, 0.25))
+    a = np.zeros((D, H, W), np.float32)
+    a[: max(1, D // 4), : H // 4, :] = rng.random((max(1, D // 4), H // 4, W)).astype(np.float32)
+    a[D // 2, H // 2, W // 2] = 1e38
+    ch = [a.reshape(-1)]
+    aff = [(0.1 * rng.random(W * H * D)).astype(np.float32)]
+    want = oracle.Propagator(cfg, np.float32).stencil(ch, aff)
+    with av.HipPropagator(cfg) as hp:
+        got = hp.stencil(ch, aff)
+        r, g = hp.stats()
+    assert np.isfinite(want[0]).all()
+    np.testing.assert_array_equal(got[0], want[0])
+    assert g >= 1
