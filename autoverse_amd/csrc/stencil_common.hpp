@@ -30,6 +30,17 @@ __device__ __forceinline__ int wrap_tile(int c, int n, bool multi) {
     return c >= n ? c - n : c;
 }
 
+// The reference's source test (reintegration.rs:76, reintegration3d.rs:77): a mass with
+// |m| < 1e-10 is skipped; NaN is not.
+__device__ __forceinline__ bool active_mass(float m) { return !(fabsf(m) < 1e-10f); }
+
+// OR of v over the 64 lanes of the wave.
+__device__ __forceinline__ unsigned wave_or(unsigned v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v |= (unsigned)__shfl_xor((int)v, o);
+    return v;
+}
+
 // Rust `f as i32`: saturating, NaN -> 0.
 __device__ __forceinline__ int sat_i32(float v) {
     if (v != v) return 0;

@@ -27,6 +27,18 @@
 
 namespace hp {
 
+// Tile order over the 8 XCDs (workgroup b runs on XCD b % 8, each with its own L2).  Rows of
+// tiles (tiles_x consecutive tiles along x) are dealt to the XCDs in turn: an XCD's L2 sees
+// runs of x-neighbours, which share halo columns, and every XCD gets rows from the whole grid,
+// so work that depends on the data (regions with no mass skip most of theirs) spreads evenly.
+// Padding workgroups get a tile index past the end.
+__device__ __forceinline__ int xcd_row_tile(int b, int tiles_x, int nrows) {
+    const int j = b >> 3, q = j / tiles_x;
+    const int row = (b & 7) + 8 * q, col = j - q * tiles_x;
+    return row < nrows ? row * tiles_x + col : 0x7fffffff;
+}
+static int xcd_row_grid(int tiles_x, int nrows) { return 8 * ((nrows + 7) / 8) * tiles_x; }
+
 // ---------------------------------------------------------------------------------------------
 // Fused 2D stencil.
 //
@@ -62,8 +74,10 @@ struct StTile {
     // LDS: two affinity tiles, and per source its mass and four x and four y overlaps
     static constexpr int LDS = 4 * (2 * NLOAD + 9 * NSRC);
     static constexpr int WG_PER_CU = 163840 / LDS;  // 160 KiB of LDS per CU
-    // waves per SIMD that LDS admits: registers are capped so that they do not admit fewer
-    static constexpr int WAVES_PER_EU = WG_PER_CU * NT / 256 < 1 ? 1 : (WG_PER_CU * NT / 256 > 8 ? 8 : WG_PER_CU * NT / 256);
+    // waves per SIMD that LDS admits, at most 4: registers are capped so that they do not admit
+    // fewer, and never below 128 per lane (a tighter cap makes the multi-channel bodies spill;
+    // a body that needs fewer registers still runs as many waves as LDS admits)
+    static constexpr int WAVES_PER_EU = WG_PER_CU * NT / 256 < 1 ? 1 : (WG_PER_CU * NT / 256 > 4 ? 4 : WG_PER_CU * NT / 256);
     static_assert(TX * TY == NT * SR && NT % TX == 0, "one strip per thread");
 };
 using StTile0 = StTile<64, 16, 512>;
@@ -82,6 +96,11 @@ constexpr int kStencilTiles = 6;
 // phase), and the four y overlaps are stored planar so that the gather reads only the ones a
 // strip row uses.  The channel-sum tile shares memory with the second affinity buffer, which
 // is first written after the alpha phase has read it.
+//
+// Empty regions: the reference skips every source with |m| < 1e-10, so a channel none of whose
+// sources in the tile's region passes that test sends no mass into the tile and its outputs are
+// +0, as the reference's.  Such a channel skips its source and gather phases (the test is
+// uniform over the workgroup); a tile with no active channel stores zeros after its loads.
 template <int CC, bool POW2, class TL>
 __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, const float* __restrict__ U,
                                                float* __restrict__ Aout, int Cdyn, int W, int H, long long N,
@@ -95,6 +114,7 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
     __shared__ float sM[NSRC];
     __shared__ float sOX[4][NSRC];
     __shared__ float sOY[4][NSRC];
+    __shared__ unsigned sAct[NT / 64];
     float* const sSum = sU[1];
 
     const int x0 = (tile % tiles_x) * TX, y0 = (tile / tiles_x) * TY;
@@ -126,6 +146,19 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
 #pragma unroll
             for (int q = 0; q < SPT; ++q) msrc[c][q] = A[(long long)c * N + (long long)sgy[q] * W + sgx[q]];
     }
+    // bit c: channel c has a source in the region that the reference does not skip.  A channel
+    // without one receives no mass anywhere in the tile.  (With the channel count known only at
+    // run time the masses are not at hand and every channel counts as active.)
+    unsigned act_t = CC > 0 ? 0u : ~0u;
+    if (CC > 0) {
+#pragma unroll
+        for (int c = 0; c < (CC > 0 ? CC : 1); ++c)
+#pragma unroll
+            for (int q = 0; q < SPT; ++q)
+                if (tid + q * NT < NSRC && active_mass(msrc[c][q])) act_t |= 1u << c;
+        act_t = wave_or(act_t);
+    }
+    if ((tid & 63) == 0) sAct[tid >> 6] = act_t;
     float upre[LPT];
 #pragma unroll
     for (int q = 0; q < LPT; ++q) upre[q] = U[goff[q]];
@@ -145,6 +178,22 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
         }
     }
     __syncthreads();
+    unsigned act = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) act |= sAct[w];
+    if (act == 0) {  // no mass reaches the tile: every output is +0, as the reference's
+        const int gdx = x0 + tid % TX, gdy0 = y0 + (tid / TX) * SR;
+        if (gdx < W) {
+#pragma unroll
+            for (int c = 0; c < (CC > 0 ? CC : 1); ++c) {
+                float* Oc = Aout + (long long)c * N;
+#pragma unroll
+                for (int r = 0; r < SR; ++r)
+                    if (gdy0 + r < H) Oc[(long long)(gdy0 + r) * W + gdx] = 0.0f;
+            }
+        }
+        return;
+    }
     // mass gradient and alpha of the owned sources (flow.rs:68-87: F = (1-a) dU - a dSum).  The
     // loop stays rolled (alpha_of may expand to a powf); each thread parks its own results in
     // the overlap planes, which are free until the first source phase, and reads them back
@@ -183,11 +232,12 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
 #pragma unroll 1
     for (int c = 0; c < C; ++c) {
         const float* su = sU[c & 1];
+        const bool live = (act >> (c & 31)) & 1u;  // uniform over the workgroup; all ones for CC = 0
         // ---- sources: flow, box, overlaps ----
 #pragma unroll
         for (int q = 0; q < SPT; ++q) {
             const int i = tid + q * NT;
-            if (i >= NSRC) break;
+            if (i >= NSRC || !live) break;
             const int sy = i / SW, sx = i - sy * SW;
             float m;
             if (CC > 0) {
@@ -261,7 +311,8 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
         float acc[SR];
 #pragma unroll
         for (int r = 0; r < SR; ++r) acc[r] = 0.0f;
-        if (__all(rotx == 0 && roty == 0 && gdx < W)) {
+        if (!live) {
+        } else if (__all(rotx == 0 && roty == 0 && gdx < W)) {
             const int base = ty0 * SW + tx;  // source (row 0, column o = -1) of the strip
 #pragma unroll
             for (int a = 0; a < SROWS; ++a) {  // strip source row a <-> grid row gdy0 - 1 + a
@@ -318,11 +369,7 @@ __global__ __launch_bounds__(TL::NT, TL::WAVES_PER_EU) void k_stencil2d(const fl
                                                       float* __restrict__ Aout, int Cdyn, int W, int H, long long N,
                                                       int tiles_x, int ntiles, float dt, float beta_a, float nexp,
                                                       float s, float ta, float inv_ta, int* __restrict__ flag) {
-    // XCD-aware tile order: workgroup b runs on XCD b % 8; each XCD gets a contiguous run of
-    // tiles so that neighbouring tiles (which share halo rows) meet in the same L2.
-    const int b = blockIdx.x;
-    const int per = (ntiles + 7) / 8;
-    const int tile = (b % 8) * per + b / 8;
+    const int tile = xcd_row_tile(blockIdx.x, tiles_x, ntiles / tiles_x);
     if (tile >= ntiles) return;
     stencil2d_body<CC, POW2, TL>(A, U, Aout, Cdyn, W, H, N, tiles_x, tile, dt, beta_a, nexp, s, ta, inv_ta, flag);
 }
@@ -439,7 +486,7 @@ static void launch_k_st2(hipStream_t st, const float* A, const float* U, float* 
     const float inv_ta = 1.0f / ta;
     const int tiles_x = (g.W + TL::TX - 1) / TL::TX, tiles_y = (g.H + TL::TY - 1) / TL::TY;
     const int ntiles = tiles_x * tiles_y;
-    const dim3 grid((ntiles + 7) / 8 * 8);
+    const dim3 grid(xcd_row_grid(tiles_x, ntiles / tiles_x));
     switch (C) {
     case 1: launch_st2<1, TL>(st, pow2, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
     case 2: launch_st2<2, TL>(st, pow2, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
@@ -474,11 +521,17 @@ bool launch_stencil2d(hipStream_t st, const float* A, const float* U, float* Aou
 // four overlaps per axis; the channel-sum tile shares memory with the z overlaps, which are
 // first written after the alpha phase.  That is 131 KiB: one workgroup of 1024 threads per CU,
 // four waves per SIMD.  Each thread gathers a z strip of SR destinations.
+//
+// Empty regions.  The reference skips every source with |m| < 1e-10; a channel none of whose
+// sources in the tile's region passes that test sends no mass into the tile, whose outputs are
+// then +0, as the reference's.  Such a channel skips the source and gather phases (the test is
+// uniform over the workgroup), and a tile with no active channel skips everything after its
+// loads.
 // ---------------------------------------------------------------------------------------------
 namespace st3 {
-template <int SR_>
+template <int TX_, int TY_, int TZ_, int SR_>
 struct Tile {
-    static constexpr int TX = 16, TY = 8, TZ = 8, SR = SR_, NT = TX * TY * TZ / SR;
+    static constexpr int TX = TX_, TY = TY_, TZ = TZ_, SR = SR_, NT = TX * TY * TZ / SR;
     static constexpr int SX = TX + 3, SY = TY + 3, SZ = TZ + 3;  // sources: offsets -1 .. +2 around the tile
     static constexpr int LX = SX + 2, LY = SY + 2, LZ = SZ + 2;  // + Sobel halo
     static constexpr int NSRC = SX * SY * SZ;
@@ -486,9 +539,10 @@ struct Tile {
     static constexpr int LPT = (NLOAD + NT - 1) / NT;  // load-region cells per thread
     static constexpr int SPT = (NSRC + NT - 1) / NT;   // sources per thread
     static constexpr int SROWS = SR + 3;               // source planes of one strip
-    static_assert(TZ % SR == 0 && NLOAD <= 4 * NSRC, "strip and channel-sum aliasing");
+    static constexpr int LDS = 4 * (NLOAD + 13 * NSRC);
+    static_assert(TZ % SR == 0 && NLOAD <= 4 * NSRC && LDS <= 163840, "strip, channel-sum aliasing, LDS");
 };
-using Tile3 = Tile<1>;
+using Tile3 = Tile<16, 8, 8, 1>;
 }  // namespace st3
 
 // gradient3d.rs:75-138 on an LDS box: P(dz, dy, dx) = L[i + dz*sz + dy*sy + dx].
@@ -534,15 +588,18 @@ __global__ __launch_bounds__(TL::NT) void k_stencil3d(const float* __restrict__ 
     __shared__ float sU[NLOAD];
     __shared__ float sM[NSRC];
     __shared__ float sOX[4 * NSRC], sOY[4 * NSRC], sOZ[4 * NSRC];  // overlap j of source i at [j * NSRC + i]
+    __shared__ unsigned sAct[NT / 64];
     float* const sSum = sOZ;
 
-    const int b = blockIdx.x;
-    const int per = (ntiles + 7) / 8;
-    const int tile = (b % 8) * per + b / 8;  // XCD-aware order, as the 2D kernel
+    const int tile = xcd_row_tile(blockIdx.x, tiles_x, ntiles / tiles_x);
     if (tile >= ntiles) return;
     const int x0 = (tile % tiles_x) * TX, y0 = ((tile / tiles_x) % tiles_y) * TY, z0 = (tile / (tiles_x * tiles_y)) * TZ;
     const int tid = threadIdx.x;
     const int HW = W * H;
+    // strip geometry: destination (gdx, gdy), planes gdz0 .. gdz0 + SR - 1
+    const int lx = tid % TX, ly = (tid / TX) % TY, lzs = tid / (TX * TY);
+    const int gdx = x0 + lx, gdy = y0 + ly, gdz0 = z0 + lzs * SR;
+    const bool inside = gdx < W && gdy < H && gdz0 < D;
     // load-region coordinates lie in [-2, n + T + 1): one wrap step suffices unless the grid is
     // smaller than the tile plus its halo
     const bool multi = W < TX + 2 || H < TY + 2 || D < TZ + 2;
@@ -569,6 +626,20 @@ __global__ __launch_bounds__(TL::NT) void k_stencil3d(const float* __restrict__ 
     float msrc[SPT];
 #pragma unroll
     for (int q = 0; q < SPT; ++q) msrc[q] = A[soff[q]];
+    // bit c: channel c has a source in the region that the reference does not skip (channels
+    // past 31 count as active).  A channel without one receives no mass anywhere in the tile.
+    unsigned act_t = 0;
+#pragma unroll
+    for (int q = 0; q < SPT; ++q)
+        if (tid + q * NT < NSRC && active_mass(msrc[q])) act_t = 1u;
+    for (int c = 1; c < C && c < 32; ++c) {
+        const float* Ac = A + (long long)c * N;
+#pragma unroll
+        for (int q = 0; q < SPT; ++q)
+            if (tid + q * NT < NSRC && active_mass(Ac[soff[q]])) act_t |= 1u << c;
+    }
+    act_t = wave_or(act_t);
+    if ((tid & 63) == 0) sAct[tid >> 6] = act_t;
     float upre[LPT];
 #pragma unroll
     for (int q = 0; q < LPT; ++q) upre[q] = U[goff[q]];
@@ -583,6 +654,20 @@ __global__ __launch_bounds__(TL::NT) void k_stencil3d(const float* __restrict__ 
         }
     }
     __syncthreads();
+    unsigned act = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) act |= sAct[w];
+    if (act == 0 && C <= 32) {  // no mass reaches the tile: every output is +0, as the reference's
+        if (inside) {
+            for (int c = 0; c < C; ++c) {
+                float* Oc = Aout + (long long)c * N + gdy * W + gdx;
+#pragma unroll
+                for (int r = 0; r < SR; ++r)
+                    if (gdz0 + r < D) Oc[(gdz0 + r) * HW] = 0.0f;
+            }
+        }
+        return;
+    }
     // alpha and the mass gradient of the owned sources.  The loop stays rolled (alpha_of may
     // expand to a powf); each thread parks its own results in the x overlap planes, which are
     // free until the first source phase, and reads them back into registers -- the same thread
@@ -611,10 +696,6 @@ __global__ __launch_bounds__(TL::NT) void k_stencil3d(const float* __restrict__ 
     }
     __syncthreads();  // the channel sum (in the z overlap planes) has been read
 
-    // strip geometry: destination (gdx, gdy), planes gdz0 .. gdz0 + SR - 1
-    const int lx = tid % TX, ly = (tid / TX) % TY, lzs = tid / (TX * TY);
-    const int gdx = x0 + lx, gdy = y0 + ly, gdz0 = z0 + lzs * SR;
-    const bool inside = gdx < W && gdy < H && gdz0 < D;
     const int rx = wrap_rotation(gdx, W), ry = wrap_rotation(gdy, H);
     // ascending wrapped order of the strip's planes: start with the first that does not wrap low
     const int rz = gdz0 < 1 ? 1 - gdz0 : (gdz0 + SR + 1 >= D ? SROWS - (gdz0 + SR + 1 - (D - 1)) : 0);
@@ -623,11 +704,12 @@ __global__ __launch_bounds__(TL::NT) void k_stencil3d(const float* __restrict__ 
     int local_flag = 0;
 #pragma unroll 1
     for (int c = 0; c < C; ++c) {
+        const bool live = c >= 32 || ((act >> c) & 1u);  // uniform over the workgroup
         // ---- sources: flow, box, overlaps ----
 #pragma unroll
         for (int q = 0; q < SPT; ++q) {
             const int i = tid + q * NT;
-            if (i >= NSRC) break;
+            if (i >= NSRC || !live) break;
             const float m = msrc[q];
             if (fabsf(m) < 1e-10f) {  // reintegration3d.rs:77: a skipped source adds nothing
                 sM[i] = 0.0f;
@@ -703,7 +785,8 @@ __global__ __launch_bounds__(TL::NT) void k_stencil3d(const float* __restrict__ 
         float acc[SR];
 #pragma unroll
         for (int r = 0; r < SR; ++r) acc[r] = 0.0f;
-        if (__all(rx == 0 && ry == 0 && rz == 0 && inside)) {
+        if (!live) {
+        } else if (__all(rx == 0 && ry == 0 && rz == 0 && inside)) {
 #pragma unroll
             for (int a = 0; a < SROWS; ++a) {  // strip source plane a <-> grid plane gdz0 - 1 + a
 #pragma unroll
@@ -785,7 +868,7 @@ bool launch_stencil3d(hipStream_t st, const float* A, const float* U, float* Aou
     const int tiles_x = (g.W + TL::TX - 1) / TL::TX, tiles_y = (g.H + TL::TY - 1) / TL::TY,
               tiles_z = (g.D + TL::TZ - 1) / TL::TZ;
     const int ntiles = tiles_x * tiles_y * tiles_z;
-    const dim3 grid((ntiles + 7) / 8 * 8);
+    const dim3 grid(xcd_row_grid(tiles_x, ntiles / tiles_x));
     const int N = (int)g.N;
     if (pow2)
         hipLaunchKernelGGL((k_stencil3d<true, TL>), grid, dim3(TL::NT), 0, st, A, U, Aout, C, g.W, g.H, g.D, N, tiles_x,

@@ -354,3 +354,37 @@ def test_stencil_huge_mass_leaves_fused_path(oracle, shape):
     assert np.isfinite(want[0]).all()
     np.testing.assert_array_equal(got[0], want[0])
     assert g >= 1
+
+
+@pytest.mark.parametrize("shape", [(128, 128, 1), (48, 32, 24)])
+def test_stencil_sparse_sources_bitwise(oracle, shape):
+    """Isolated masses on and around tile boundaries (so that some tiles hold their only active
+    sources in the halo), a second channel that is zero everywhere, and a third whose one mass
+    sits at the far corner: tiles with no active source store +0 and the rest match the oracle
+    bit for bit."""
+    from autoverse_amd.schema import FlowConfig, SimulationConfig
+    W, H, D = shape
+    _say(f"[sparse_sources] {shape}")
+    rng = np.random.default_rng(23)
+    ks = workloads._random_kernels(rng, 3)
+    cfg = SimulationConfig(width=W, height=H, depth=D, channels=3, dt=0.2, kernel_radius=3, kernels=ks,
+                           flow=FlowConfig(1.0, 2.0, 1.0))
+    a = np.zeros((3, D, H, W), np.float32)
+    xs = [x for x in (0, 1, 15, 16, 17, 31, 32, 33, 63, 64, 65, W - 2, W - 1) if x < W]
+    ys = [y for y in (0, 7, 8, 9, 15, 16, 17, H - 2, H - 1) if y < H]
+    zs = [z for z in (0, 7, 8, 9, D - 1) if z < D] if D > 1 else [0]
+    for z in zs[::2]:
+        for y in ys[::2]:
+            for x in xs[::3]:
+                a[0, z, y, x] = 0.25 + 0.5 * rng.random()
+    a[2, D - 1, H - 1, W - 1] = 0.7
+    ch = [a[c].reshape(-1) for c in range(3)]
+    aff = [(0.2 * rng.random(W * H * D) - 0.1).astype(np.float32) for _ in range(3)]
+    want = oracle.Propagator(cfg, np.float32).stencil(ch, aff)
+    with av.HipPropagator(cfg) as hp:
+        got = hp.stencil(ch, aff)
+        r, g = hp.stats()
+    for c in range(3):
+        np.testing.assert_array_equal(got[c], want[c])
+    assert not got[1].any() and got[0].any() and got[2].any()
+    assert g == 0
