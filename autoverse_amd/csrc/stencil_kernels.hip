@@ -516,11 +516,12 @@ bool launch_stencil2d(hipStream_t st, const float* A, const float* U, float* Aou
 // whose box reaches further sets flag bit 0 (bit 1 for non-finite values) and the host reruns
 // the step through the general path.
 //
-// Work split.  A workgroup owns a 16 x 8 x 8 destination tile (its 19 x 11 x 11 source region
-// is 2.25 sources per destination).  LDS holds one affinity tile and, per source, its mass and
-// four overlaps per axis; the channel-sum tile shares memory with the z overlaps, which are
-// first written after the alpha phase.  That is 131 KiB: one workgroup of 1024 threads per CU,
-// four waves per SIMD.  Each thread gathers a z strip of SR destinations.
+// Work split.  A workgroup owns a 32 x 8 x 4 destination tile (its 35 x 11 x 7 source region is
+// 2.6 sources per destination; a half-wave's gather reads one contiguous source row, which keeps
+// LDS bank conflicts lower than 16-wide tiles do).  LDS holds one affinity tile and, per source,
+// its mass and four overlaps per axis; the channel-sum tile shares memory with the z overlaps,
+// which are first written after the alpha phase.  That is 154 KiB: one workgroup of 1024
+// threads per CU, four waves per SIMD.  Each thread gathers a z strip of SR destinations.
 //
 // Empty regions.  The reference skips every source with |m| < 1e-10; a channel none of whose
 // sources in the tile's region passes that test sends no mass into the tile, whose outputs are
@@ -542,7 +543,7 @@ struct Tile {
     static constexpr int LDS = 4 * (NLOAD + 13 * NSRC);
     static_assert(TZ % SR == 0 && NLOAD <= 4 * NSRC && LDS <= 163840, "strip, channel-sum aliasing, LDS");
 };
-using Tile3 = Tile<16, 8, 8, 1>;
+using Tile3 = Tile<32, 8, 4, 1>;
 }  // namespace st3
 
 // gradient3d.rs:75-138 on an LDS box: P(dz, dy, dx) = L[i + dz*sz + dy*sy + dx].
