@@ -388,3 +388,29 @@ def test_stencil_sparse_sources_bitwise(oracle, shape):
         np.testing.assert_array_equal(got[c], want[c])
     assert not got[1].any() and got[0].any() and got[2].any()
     assert g == 0
+
+
+@pytest.mark.parametrize("shape", [(96, 64, 1), (32, 24, 16)])
+def test_stencil_five_channels_bitwise(oracle, shape):
+    """Five channels: the 2D stencil's body for a channel count known only at run time (masses
+    re-read per channel, every channel treated as active) and the 3D kernel's per-channel mass
+    prefetch; one channel is zero everywhere."""
+    from autoverse_amd.schema import FlowConfig, SimulationConfig
+    W, H, D = shape
+    C = 5
+    _say(f"[five_channels] {shape}")
+    rng = np.random.default_rng(31)
+    ks = workloads._random_kernels(rng, C)
+    cfg = SimulationConfig(width=W, height=H, depth=D, channels=C, dt=0.2, kernel_radius=3, kernels=ks,
+                           flow=FlowConfig(1.0, 2.0, 1.0))
+    ch = [rng.random(W * H * D).astype(np.float32) for _ in range(C)]
+    ch[3][:] = 0.0
+    aff = [(0.2 * rng.random(W * H * D) - 0.1).astype(np.float32) for _ in range(C)]
+    want = oracle.Propagator(cfg, np.float32).stencil(ch, aff)
+    with av.HipPropagator(cfg) as hp:
+        got = hp.stencil(ch, aff)
+        r, g = hp.stats()
+    for c in range(C):
+        np.testing.assert_array_equal(got[c], want[c])
+    assert not got[3].any()
+    assert g == 0
