@@ -332,7 +332,7 @@ inline void rows_fwd(hipStream_t st, const float* A, float2* X, const int* src_c
 
 template <int L, int RP, int CS>
 inline void cols(hipStream_t st, int dir, float2* X, int nspec, const Geo& g, const float2* tw) {
-    constexpr int CB = Cfg<L, CS>::CB, NT = Cfg<L, CS>::COL_NT;
+    constexpr int CB = Cfg<L, CS>::CBC3, NT = Cfg<L, CS>::COLC3_NT;
     dim3 grid((g.Wh + CB - 1) / CB, g.D, nspec);
     if (dir < 0)
         hipLaunchKernelGGL((k_cols<L, CB, NT, -1, RP>), grid, dim3(NT), 0, st, X, g.Wh, g.Wp, g.spec, tw);
@@ -343,13 +343,14 @@ inline void cols(hipStream_t st, int dir, float2* X, int nspec, const Geo& g, co
 template <int L, int RP, int CS>
 inline void spectral(int dim, hipStream_t st, const float2* X, float2* Y, const float* Kq, const SpecWork* work,
               int nwork, const int* klist, const Geo& g, const float2* tw) {
-    constexpr int CB = Cfg<L, CS>::CB, NT = Cfg<L, CS>::COL_NT;
     const long long kq_stride = (long long)g.Dq * g.Hq * g.Wh;
     if (dim == 2) {
+        constexpr int CB = Cfg<L, CS>::CB, NT = Cfg<L, CS>::COL_NT;
         dim3 grid((g.Wh + CB - 1) / CB, nwork);
         hipLaunchKernelGGL((k_spectral<L, CB, NT, 2, RP>), grid, dim3(NT), 0, st, X, Y, Kq, work, klist, g.Wh, g.Wp,
                            g.H, g.Hq, g.spec, kq_stride, tw);
     } else {
+        constexpr int CB = Cfg<L, CS>::CBS3, NT = Cfg<L, CS>::SPEC3_NT;
         const long long lines = (long long)g.H * g.Wp;
         dim3 grid((unsigned)((lines + CB - 1) / CB), nwork);
         hipLaunchKernelGGL((k_spectral<L, CB, NT, 3, RP>), grid, dim3(NT), 0, st, X, Y, Kq, work, klist, g.Wh, g.Wp,
