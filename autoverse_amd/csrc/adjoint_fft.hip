@@ -80,7 +80,7 @@ __global__ __launch_bounds__(NT) void kb_adj_rows_fwd(const MemberDesc* __restri
 // thread) stay in registers.
 template <int L>
 struct AdjCfg {
-    static constexpr int CB = Cfg<L>::CB / 4 < 1 ? 1 : Cfg<L>::CB / 4;
+    static constexpr int CB = Cfg<L>::CBP / 4 < 1 ? 1 : Cfg<L>::CBP / 4;
     static constexpr int NT = (CB * L) / 8 < 64 ? 64 : ((CB * L) / 8 > 256 ? 256 : (CB * L) / 8);
     static_assert((CB * L) % NT == 0, "adjoint column tile must split evenly");
 };

@@ -166,16 +166,23 @@ template <int L, int CS = 1>
 struct Cfg {
     static constexpr int ROW_NT = L / 8 < 64 ? 64 : (L / 8 > 256 ? 256 : L / 8);
     static constexpr int CB0 = 8192 / L < 1 ? 1 : (8192 / L > 64 ? 64 : 8192 / L);
-    static constexpr int CB = CB0 / CS < 1 ? 1 : CB0 / CS;
-    static constexpr int COL_NT = (CB * L) / 32 < 64 ? 64 : ((CB * L) / 32 > 256 ? 256 : (CB * L) / 32);
+    static constexpr int CBP = CB0 / CS < 1 ? 1 : CB0 / CS;  // columns per block before balancing
+    // The 2D spectral pass: a square grid's L/2 + 1 half-spectrum columns in near-equal blocks
+    // of at most CBP (65 columns as 33 + 32 rather than 64 + 1), and threads rounded up to whole
+    // wavefronts so that no thread holds more than 32 elements.
+    static constexpr int WH = L / 2 + 1;
+    static constexpr int NBLK = (WH + CBP - 1) / CBP;
+    static constexpr int CB = (WH + NBLK - 1) / NBLK;
+    static constexpr int NT_RAW = ((CB * L) / 32 + 63) / 64 * 64;
+    static constexpr int COL_NT = NT_RAW < 64 ? 64 : (NT_RAW > 256 ? 256 : NT_RAW);
     // The 3D line passes run narrower workgroups, so that a 128^3 grid spreads over several
     // hundred to a thousand workgroups on the 256 CUs instead of 160-256 single-wave ones: the y
     // column pass at most 16 lines and a thread per radix-8 butterfly, the z spectral pass at
     // most 8 lines and a thread per radix-4 butterfly (the fastest shapes measured at 128^3).
     // A line's arithmetic does not depend on the launch shape.
-    static constexpr int CBC3 = CB < 16 ? CB : 16;
+    static constexpr int CBC3 = CBP < 16 ? CBP : 16;
     static constexpr int COLC3_NT = (CBC3 * L) / 8 < 64 ? 64 : ((CBC3 * L) / 8 > 256 ? 256 : (CBC3 * L) / 8);
-    static constexpr int CBS3 = CB < 8 ? CB : 8;
+    static constexpr int CBS3 = CBP < 8 ? CBP : 8;
     static constexpr int SPEC3_NT = (CBS3 * L) / 4 < 64 ? 64 : ((CBS3 * L) / 4 > 256 ? 256 : (CBS3 * L) / 4);
 };
 

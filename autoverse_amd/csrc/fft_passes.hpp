@@ -110,8 +110,9 @@ __device__ __forceinline__ void spectral_body(const float2* __restrict__ X, floa
                                               long long spec, long long kq_stride, const float2* __restrict__ twg,
                                               int blk, int wi) {
     constexpr int LP = L + 1;
-    constexpr int PER = CB * L / NT;
-    static_assert((CB * L) % NT == 0, "tile must split evenly over the threads");
+    constexpr int TOT = CB * L;
+    constexpr int PER = (TOT + NT - 1) / NT;  // elements per thread; EVEN: no thread runs past TOT
+    constexpr bool EVEN = TOT % NT == 0;
     __shared__ float2 buf[CB * LP];
     __shared__ float2 tw[L];
     const SpecWork wk = work[wi];
@@ -125,6 +126,8 @@ __device__ __forceinline__ void spectral_body(const float2* __restrict__ X, floa
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int i = threadIdx.x + q * NT;
+        koff[q] = -1;
+        if (!EVEN && i >= TOT) continue;
         const int c = i % CB, e = i / CB;
         const long long line = line0 + c;
         int kx, ky;
@@ -152,7 +155,7 @@ __device__ __forceinline__ void spectral_body(const float2* __restrict__ X, floa
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int i = threadIdx.x + q * NT;
-        reg[q] = buf[(i % CB) * LP + i / CB];
+        reg[q] = EVEN || i < TOT ? buf[(i % CB) * LP + i / CB] : make_float2(0.f, 0.f);
     }
     for (int j = 0; j < wk.k_count; ++j) {
         const int k = klist[wk.k_first + j];
@@ -161,6 +164,7 @@ __device__ __forceinline__ void spectral_body(const float2* __restrict__ X, floa
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int i = threadIdx.x + q * NT;
+            if (!EVEN && i >= TOT) continue;
             const float kv = koff[q] >= 0 ? kq[koff[q]] : 0.0f;
             buf[(i % CB) * LP + i / CB] = make_float2(reg[q].x * kv, reg[q].y * kv);
         }
