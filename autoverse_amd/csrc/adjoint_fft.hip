@@ -76,12 +76,12 @@ __global__ __launch_bounds__(NT) void kb_adj_rows_fwd(const MemberDesc* __restri
 }
 
 // Launch shape of the adjoint column pass: a quarter of the forward's columns per
-// workgroup, so that the source column, the accumulator and the offsets (8 values each per
+// workgroup, so that the source column, the accumulator and the offsets (4 values each per
 // thread) stay in registers.
 template <int L>
 struct AdjCfg {
     static constexpr int CB = Cfg<L>::CBP / 4 < 1 ? 1 : Cfg<L>::CBP / 4;
-    static constexpr int NT = (CB * L) / 8 < 64 ? 64 : ((CB * L) / 8 > 256 ? 256 : (CB * L) / 8);
+    static constexpr int NT = (CB * L) / 4 < 64 ? 64 : ((CB * L) / 4 > 512 ? 512 : (CB * L) / 4);
     static_assert((CB * L) % NT == 0, "adjoint column tile must split evenly");
 };
 

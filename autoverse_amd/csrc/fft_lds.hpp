@@ -168,13 +168,16 @@ struct Cfg {
     static constexpr int CB0 = 8192 / L < 1 ? 1 : (8192 / L > 64 ? 64 : 8192 / L);
     static constexpr int CBP = CB0 / CS < 1 ? 1 : CB0 / CS;  // columns per block before balancing
     // The 2D spectral pass: a square grid's L/2 + 1 half-spectrum columns in near-equal blocks
-    // of at most CBP (65 columns as 33 + 32 rather than 64 + 1), and threads rounded up to whole
-    // wavefronts so that no thread holds more than 32 elements.
+    // of at most CBP (65 columns as 33 + 32 rather than 64 + 1), and about 4 elements per thread
+    // (whole wavefronts, up to 1024 threads): fewer elements held per thread keep registers
+    // low and occupancy up (at 32 per thread the pass ran one wave per SIMD; the 256-member
+    // population's step went from 703 to 544 us).
     static constexpr int WH = L / 2 + 1;
     static constexpr int NBLK = (WH + CBP - 1) / CBP;
     static constexpr int CB = (WH + NBLK - 1) / NBLK;
-    static constexpr int NT_RAW = ((CB * L) / 32 + 63) / 64 * 64;
-    static constexpr int COL_NT = NT_RAW < 64 ? 64 : (NT_RAW > 256 ? 256 : NT_RAW);
+    static constexpr int NT_RAW = ((CB * L) / 4 + 63) / 64 * 64;
+    static constexpr int NT_MAX = L % 3 == 0 ? 512 : 1024;  // radix-3 stages spill at 1024 threads
+    static constexpr int COL_NT = NT_RAW < 64 ? 64 : (NT_RAW > NT_MAX ? NT_MAX : NT_RAW);
     // The 3D line passes run narrower workgroups, so that a 128^3 grid spreads over several
     // hundred to a thousand workgroups on the 256 CUs instead of 160-256 single-wave ones: the y
     // column pass at most 16 lines and a thread per radix-8 butterfly, the z spectral pass at
