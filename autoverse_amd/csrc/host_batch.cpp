@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/flowlenia_hip.h"
@@ -534,6 +535,8 @@ void member_theta(const HostMember& hm, const std::vector<double>& pacc, const s
         off += (size_t)S * S;
         std::vector<double> v((size_t)S * S, 0.0);
         std::vector<char> in((size_t)S * S, 0);
+        const size_t nr = kc.ring_count;
+        std::vector<double> ex((size_t)S * S * nr, 0.0);  // each tap's ring exponentials, reused below
         double sum = 0.0, dot = 0.0;
         for (int y = 0; y < S; y++)
             for (int x = 0; x < S; x++) {
@@ -544,7 +547,9 @@ void member_theta(const HostMember& hm, const std::vector<double>& pacc, const s
                 for (uint32_t r = 0; r < kc.ring_count; r++) {
                     const hp_ring& rg = hm.rings[kc.ring_offset + r];
                     const double rd = rho - rg.distance, w = rg.width;
-                    val += rg.amplitude * exp(-rd * rd / (2.0 * w * w));
+                    const double e = exp(-rd * rd / (2.0 * w * w));
+                    ex[((size_t)y * S + x) * nr + r] = e;
+                    val += rg.amplitude * e;
                 }
                 v[(size_t)y * S + x] = val;
                 in[(size_t)y * S + x] = 1;
@@ -561,7 +566,7 @@ void member_theta(const HostMember& hm, const std::vector<double>& pacc, const s
                 for (uint32_t r = 0; r < kc.ring_count; r++) {
                     const hp_ring& rg = hm.rings[kc.ring_offset + r];
                     const double a = rg.amplitude, rd = rho - rg.distance, w = rg.width;
-                    const double e = exp(-rd * rd / (2.0 * w * w));
+                    const double e = ex[((size_t)y * S + x) * nr + r];
                     double* tr = th + 3 * (size_t)K + 3 * (size_t)(kc.ring_offset + r);
                     tr[0] += dv * e;
                     tr[1] += dv * a * e * rd / (w * w);
@@ -611,25 +616,47 @@ hp_status tape_backward(hp_batch* b, const float* const* seed, float* const* gA0
     copy_states(b, b->tape_cur, (int)nck, false);  // the population holds A_T again
     b->cur = b->tape_cur;
     BTRY(b, hipStreamSynchronize(b->stream));
-    ci = 0;
-    size_t off = 0, poff = 0, woff = 0;
-    for (size_t i = 0; i < b->members.size(); i++) {
+    // Every member's outputs are its own ranges of gA0 and dtheta, so the members are spread
+    // over up to 16 host threads (the FP64 ring chain rule is most of the host's work here).
+    const size_t M = b->members.size();
+    std::vector<size_t> cbase(M), off(M), poff(M), woff(M);
+    for (size_t i = 0, c0 = 0, o = 0, po = 0, wo = 0; i < M; i++) {
+        const HostMember& hm = b->members[i];
+        cbase[i] = c0;
+        off[i] = o;
+        poff[i] = po;
+        woff[i] = wo;
+        c0 += b->desc[i].C;
+        o += theta_count(hm);
+        po += (size_t)hm.cfg.num_kernels * ((b->desc[i].g.H + 1) / 2) * 3;
+        for (int r : hm.radius) wo += (size_t)(2 * r + 1) * (2 * r + 1);
+    }
+    auto member_out = [&](size_t i) {
         const hp::MemberDesc& m = b->desc[i];
         const HostMember& hm = b->members[i];
-        for (int c = 0; c < m.C; c++, ci++)
-            if (gA0 && gA0[ci])
-                memcpy(gA0[ci], b->h_stage + m.stage_off + (size_t)c * m.g.N, (size_t)m.g.N * sizeof(float));
-        const size_t pn = (size_t)hm.cfg.num_kernels * ((m.g.H + 1) / 2) * 3;
-        size_t wn = 0;
-        for (int r : hm.radius) wn += (size_t)(2 * r + 1) * (2 * r + 1);
+        for (int c = 0; c < m.C; c++)
+            if (gA0 && gA0[cbase[i] + c])
+                memcpy(gA0[cbase[i] + c], b->h_stage + m.stage_off + (size_t)c * m.g.N, (size_t)m.g.N * sizeof(float));
         if (dtheta) {
-            std::vector<double> pm(pacc.begin() + poff, pacc.begin() + poff + pn);
-            std::vector<double> wm(dkwin.begin() + woff, dkwin.begin() + woff + wn);
-            member_theta(hm, pm, wm, dtheta + off);
+            const size_t pn = (size_t)hm.cfg.num_kernels * ((m.g.H + 1) / 2) * 3;
+            size_t wn = 0;
+            for (int r : hm.radius) wn += (size_t)(2 * r + 1) * (2 * r + 1);
+            std::vector<double> pm(pacc.begin() + poff[i], pacc.begin() + poff[i] + pn);
+            std::vector<double> wm(dkwin.begin() + woff[i], dkwin.begin() + woff[i] + wn);
+            member_theta(hm, pm, wm, dtheta + off[i]);
         }
-        off += theta_count(hm);
-        poff += pn;
-        woff += wn;
+    };
+    const unsigned hw = std::thread::hardware_concurrency();
+    const size_t nth = std::min<size_t>(M, std::max(1u, std::min(16u, hw)));
+    if (nth <= 1) {
+        for (size_t i = 0; i < M; i++) member_out(i);
+    } else {
+        std::vector<std::thread> pool;
+        for (size_t t = 0; t < nth; t++)
+            pool.emplace_back([&, t] {
+                for (size_t i = t; i < M; i += nth) member_out(i);
+            });
+        for (std::thread& th : pool) th.join();
     }
     return HP_OK;
 }
