@@ -223,6 +223,31 @@ __global__ __launch_bounds__(256) void kb_tape_copy(const MemberDesc* __restrict
     for (long long j = (long long)it.y * 256 + threadIdx.x; j < n; j += (long long)it.z * 256) dst[j] = src[j];
 }
 
+// U of a taped step to / from its tape slot (items as kb_tape_copy).
+__global__ __launch_bounds__(256) void kb_u_copy(const MemberDesc* __restrict__ mem, const int4* __restrict__ items,
+                                                int slot, int to_tape) {
+    const int4 it = items[blockIdx.x];
+    const MemberDesc& m = mem[it.x];
+    const long long n = (long long)m.C * m.g.N;
+    float* t = m.tape_u + (long long)slot * n;
+    const float* src = to_tape ? m.U : t;
+    float* dst = to_tape ? t : m.U;
+    for (long long j = (long long)it.y * 256 + threadIdx.x; j < n; j += (long long)it.z * 256) dst[j] = src[j];
+}
+
+// Point each member's A[0] at tape slot `slot` and its U at tape_u slot `u_slot` (a negative
+// slot leaves that pointer): a backward step then reads its state and affinity where the tape
+// keeps them, and a taped forward step writes its U straight into its slot.  One thread per
+// member; the host restores the descriptors afterwards.
+__global__ __launch_bounds__(256) void kb_point_slots(MemberDesc* __restrict__ mem, int n, int slot, int u_slot) {
+    const int i = blockIdx.x * 256 + (int)threadIdx.x;
+    if (i >= n) return;
+    MemberDesc& m = mem[i];
+    const long long cn = (long long)m.C * m.g.N;
+    if (slot >= 0) m.A[0] = m.tape + (long long)slot * cn;
+    if (u_slot >= 0) m.U = m.tape_u + (long long)u_slot * cn;
+}
+
 // Clear the accumulators of a backward pass: P, pacc and the kernel-tap windows.
 __global__ __launch_bounds__(256) void kb_tape_clear(const MemberDesc* __restrict__ mem,
                                                     const int4* __restrict__ items) {
@@ -241,6 +266,14 @@ __global__ __launch_bounds__(256) void kb_tape_clear(const MemberDesc* __restric
 void launch_tape_copy(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cur, int slot,
                       bool to_tape) {
     hipLaunchKernelGGL(kb_tape_copy, dim3(nitems), dim3(256), 0, st, mem, items, cur, slot, to_tape ? 1 : 0);
+}
+
+void launch_point_slots(hipStream_t st, MemberDesc* mem, int n, int slot, int u_slot) {
+    hipLaunchKernelGGL(kb_point_slots, dim3((n + 255) / 256), dim3(256), 0, st, mem, n, slot, u_slot);
+}
+
+void launch_u_copy(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int slot, bool to_tape) {
+    hipLaunchKernelGGL(kb_u_copy, dim3(nitems), dim3(256), 0, st, mem, items, slot, to_tape ? 1 : 0);
 }
 
 void launch_tape_clear(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems) {

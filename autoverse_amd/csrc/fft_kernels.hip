@@ -1,59 +1,8 @@
 // fft_kernels.hip -- public launchers of the LDS FFT passes: each length goes to the
-// translation unit (fft_g*.hip) that instantiates its kernels; and the adjoint's rebuild of
-// U from stored convolutions, which shares growth_term with the inverse row pass.
+// translation unit (fft_g*.hip) that instantiates its kernels.
 #include "fft_passes.hpp"
 
 namespace hp {
-
-// U of a taped step from its stored convolutions (conv slot cslot): per target channel the
-// kernels in index order, acc from 0 -- the inverse row pass's epilogue on the same values.
-// items = (member, part, nparts) striding over the member's N cells.  A thread takes UC cells
-// at a time, so that each kernel's parameters are read once per UC cells and UC independent
-// loads are in flight; the tables come in through restrict parameters, which lets the
-// compiler keep them across the stores to U.
-constexpr int UC = 4;
-__device__ __forceinline__ void u_from_conv_cells(const float* __restrict__ conv, float* __restrict__ U,
-                                                  const TgtWork* __restrict__ tgt, const int* __restrict__ klist,
-                                                  const GrowthParams* __restrict__ gp, int ntgt, long long N,
-                                                  long long i0, long long stride) {
-    for (long long ib = i0; ib < N; ib += stride * UC) {
-        for (int w = 0; w < ntgt; ++w) {
-            const TgtWork wk = tgt[w];
-            float acc[UC];
-#pragma unroll
-            for (int u = 0; u < UC; ++u) acc[u] = 0.0f;
-            for (int j = 0; j < wk.k_count; ++j) {
-                const int k = klist[wk.k_first + j];
-                const GrowthParams p = gp[k];
-                const float* ck = conv + (long long)k * N;
-#pragma unroll
-                for (int u = 0; u < UC; ++u) {
-                    const long long i = ib + u * stride;
-                    if (i < N) acc[u] += growth_term(ck[i], p);
-                }
-            }
-            float* ut = U + (long long)wk.target * N;
-#pragma unroll
-            for (int u = 0; u < UC; ++u) {
-                const long long i = ib + u * stride;
-                if (i < N) ut[i] = acc[u];
-            }
-        }
-    }
-}
-
-__global__ __launch_bounds__(256) void kb_u_from_conv(const MemberDesc* __restrict__ mem,
-                                                     const int4* __restrict__ items, int cslot) {
-    const int4 it = items[blockIdx.x];
-    const MemberDesc& m = mem[it.x];
-    const long long N = m.g.N;
-    u_from_conv_cells(m.conv + (long long)cslot * m.K * N, m.U, m.tgt_work, m.klist_tgt, m.gp, m.ntgt, N,
-                      (long long)it.y * 256 + threadIdx.x, (long long)it.z * 256);
-}
-
-void launch_u_from_conv(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cslot) {
-    hipLaunchKernelGGL(kb_u_from_conv, dim3(nitems), dim3(256), 0, st, mem, items, cslot);
-}
 
 #define HP_GROUPS(M) M(g0) M(g1) M(g2) M(g3)
 #define HP_DECL(G)                                                                                               \

@@ -202,8 +202,7 @@ __global__ __launch_bounds__(NT) void kb_spectral(const MemberDesc* __restrict__
 // ---- pass 3: C2R along x on pairs of rows + growth + weighted accumulate -----------------
 // grid (ceil(rows/2), nwork); work[w] is a target channel and its kernels in index order.
 // One kernel's growth term w * G(c) (growth_accumulate, growth.rs:51-57: g = 2 exp(-d*d*inv) - 1,
-// t += w*g).  The inverse row pass and the adjoint's rebuild of U both call this one function
-// in this one translation unit, so the two produce bit-identical U.
+// t += w*g).  A taped step's U is the one this pass wrote, kept in the tape for the adjoint.
 __device__ __forceinline__ float growth_term(float c, const GrowthParams& p) {
 #pragma clang fp contract(off)
     const float d = c - p.mu;

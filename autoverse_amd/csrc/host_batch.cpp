@@ -165,7 +165,9 @@ hp_status twiddles(hp_batch* b, int L, const float2** out) {
 
 // The three FFT passes of a step from A[cur]; conv: also store the convolutions in conv slot
 // cslot; rows_only: the forward row pass alone (the adjoint's source spectra).
-void enqueue_fft_passes(hp_batch* b, int cur, bool conv, int cslot = 0, bool rows_only = false) {
+// save_u: also keep the step's U in tape_u slot cslot (a tape with stored convolutions).
+void enqueue_fft_passes(hp_batch* b, int cur, bool conv, int cslot = 0, bool rows_only = false,
+                        bool save_u = false) {
     for (const SizeClass& sc : b->classes) {
         if (sc.rows_fwd.n)
             hp::launch_batch_rows_fwd(sc.W, b->stream, b->d_desc, sc.rows_fwd.d, sc.rows_fwd.n, cur, b->fv);
@@ -174,6 +176,7 @@ void enqueue_fft_passes(hp_batch* b, int cur, bool conv, int cslot = 0, bool row
         if (sc.rows_inv.n)
             hp::launch_batch_rows_inv(sc.W, b->stream, b->d_desc, sc.rows_inv.d, sc.rows_inv.n, conv, cslot, b->fv);
     }
+    if (save_u && !rows_only) hp::launch_u_copy(b->stream, b->d_desc, b->copy_items.d, b->copy_items.n, cslot, true);
 }
 
 void enqueue_stencils(hp_batch* b, int cur) {
@@ -245,7 +248,7 @@ hp_status member_stencil(hp_batch* b, int i, int cur) {
 
 // One step with every member's stencil checked (A[cur] -> A[1 - cur]).
 hp_status step_checked(hp_batch* b, int cur, bool conv = false, int cslot = 0) {
-    enqueue_fft_passes(b, cur, conv, cslot);
+    enqueue_fft_passes(b, cur, conv, cslot, false, conv);
     for (size_t i = 0; i < b->members.size(); i++) STRY(member_stencil(b, (int)i, cur));
     return HP_OK;
 }
@@ -332,6 +335,7 @@ hp_status ensure_tape(hp_batch* b, uint64_t T, uint32_t k, bool store_conv) {
         STRY(balloc(b, &d.lam[1], CN, L));
         STRY(balloc(b, &d.lamU, CN, L));
         STRY(balloc(b, &d.conv, cslots * K * N, L));
+        STRY(balloc(b, &d.tape_u, cslots * CN, L));  // as many slots as conv: conv_cap_slots covers both
         STRY(balloc(b, &d.gscr, (2 * (size_t)p.C + 3) * N, L));
         STRY(balloc(b, &d.P, K * (size_t)p.g.spec, L));
         STRY(balloc(b, &d.tape, slots * CN, L));
@@ -393,10 +397,10 @@ hp_status ensure_tape(hp_batch* b, uint64_t T, uint32_t k, bool store_conv) {
 void auto_layout(hp_batch* b, uint64_t T, uint32_t* k, bool* store_conv) {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
-    double set = 0.0, cset = 0.0;  // bytes of one state / one step's convolutions, all members
+    double set = 0.0, cset = 0.0;  // bytes of one state / one step's convolutions and U, all members
     for (const HostMember& hm : b->members) {
         set += 4.0 * hm.plan.C * (double)hm.plan.g.N;
-        cset += 4.0 * std::max(hm.plan.K, 0) * (double)hm.plan.g.N;
+        cset += 4.0 * (std::max(hm.plan.K, 0) + hm.plan.C) * (double)hm.plan.g.N;
     }
     const double budget = 0.5 * (double)free_b;
     *store_conv = false;
@@ -430,6 +434,9 @@ hp_status tape_forward(hp_batch* b, uint64_t T, uint32_t k) {
             if (pass == 1) {
                 STRY(step_checked(b, cur, store_conv, (int)t));
             } else if (store_conv) {
+                // the step's U goes straight into its tape slot (no graph runs in this loop;
+                // the descriptors are restored after it)
+                hp::launch_point_slots(b->stream, b->d_desc, (int)b->members.size(), -1, (int)t);
                 enqueue_fft_passes(b, cur, true, (int)t);
                 enqueue_stencils(b, cur);
             } else {
@@ -437,6 +444,9 @@ hp_status tape_forward(hp_batch* b, uint64_t T, uint32_t k) {
             }
             cur = 1 - cur;
         }
+        if (pass == 0 && store_conv)
+            BTRY(b, hipMemcpyAsync(b->d_desc, b->desc.data(), b->desc.size() * sizeof(hp::MemberDesc),
+                                   hipMemcpyHostToDevice, b->stream));
         BTRY(b, hipGetLastError());
         copy_states(b, cur, (int)nck, true);
         b->cur = cur;
@@ -487,12 +497,13 @@ hp_status recompute_segment(hp_batch* b, uint64_t j, uint64_t t0, uint64_t t1) {
 // Adjoint of step t whose input state is in A[0]; lam[li] holds dL/dA_{t+1}, lam[1-li]
 // receives dL/dA_t.  With stored convolutions only the forward row pass (the source spectra)
 // and U are rebuilt; otherwise the step's FFT passes run again, keeping the convolutions.
-hp_status backward_step(hp_batch* b, int li, uint64_t t) {
+// u_in_place: the descriptors already point U at the step's tape slot.
+hp_status backward_step(hp_batch* b, int li, uint64_t t, bool u_in_place = false) {
     int cslot = 0;
     if (b->tape_conv) {
         cslot = (int)t;
         enqueue_fft_passes(b, 0, false, 0, true);
-        hp::launch_u_from_conv(b->stream, b->d_desc, b->copy_items.d, b->copy_items.n, cslot);
+        if (!u_in_place) hp::launch_u_copy(b->stream, b->d_desc, b->copy_items.d, b->copy_items.n, cslot, false);
     } else {
         enqueue_fft_passes(b, 0, true, 0);
     }
@@ -576,31 +587,64 @@ void member_theta(const HostMember& hm, const std::vector<double>& pacc, const s
     }
 }
 
+// f(i) for every member i over up to 16 host threads (callers give each member its own outputs).
+template <class F>
+void parallel_members(size_t M, const F& f) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const size_t nth = std::min<size_t>(M, std::max(1u, std::min(16u, hw)));
+    if (nth <= 1) {
+        for (size_t i = 0; i < M; i++) f(i);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (size_t t = 0; t < nth; t++)
+        pool.emplace_back([&, t] {
+            for (size_t i = t; i < M; i += nth) f(i);
+        });
+    for (std::thread& th : pool) th.join();
+}
+
 hp_status tape_backward(hp_batch* b, const float* const* seed, float* const* gA0, double* dtheta) {
     if (!b->tape_recorded) return b->fail(HP_ERR_SHAPE, "no recorded tape: call hp_batch_tape_forward first");
     const uint64_t T = b->tape_T, k = b->tape_k, nck = num_checkpoints(T, b->tape_k);
     size_t ci = 0;
+    std::vector<size_t> cfirst(b->members.size());
     for (size_t i = 0; i < b->members.size(); i++) {
-        const hp::MemberDesc& m = b->desc[i];
-        for (int c = 0; c < m.C; c++, ci++) {
+        cfirst[i] = ci;
+        for (int c = 0; c < b->desc[i].C; c++, ci++)
             if (!seed[ci]) return b->fail(HP_ERR_NULL, "dL/dA_T pointer is NULL");
-            memcpy(b->h_stage + m.stage_off + (size_t)c * m.g.N, seed[ci], (size_t)m.g.N * sizeof(float));
-        }
     }
+    parallel_members(b->members.size(), [&](size_t i) {
+        const hp::MemberDesc& m = b->desc[i];
+        for (int c = 0; c < m.C; c++)
+            memcpy(b->h_stage + m.stage_off + (size_t)c * m.g.N, seed[cfirst[i] + c], (size_t)m.g.N * sizeof(float));
+    });
     BTRY(b, hipMemcpyAsync(b->d_stage, b->h_stage, b->stage_n * sizeof(float), hipMemcpyHostToDevice, b->stream));
     hp::launch_stage_lam(b->stream, b->d_desc, b->copy_items.d, b->copy_items.n, b->d_stage, 0, false);
     hp::launch_tape_clear(b->stream, b->d_desc, b->copy_items.d, b->copy_items.n);
+    // With a checkpoint every step and stored convolutions nothing is recomputed and no graph
+    // runs in the loop: each backward step reads its state and U straight from their tape slots
+    // (the descriptors point there) instead of copying them into A[0] and U.  The descriptors
+    // are restored right after the loop.
+    const bool direct = k == 1 && b->tape_conv;
     int li = 0;
-    for (uint64_t j = nck; j-- > 0;) {
+    hp_status lst = HP_OK;
+    for (uint64_t j = nck; j-- > 0 && lst == HP_OK;) {
         const uint64_t t0 = j * k, t1 = std::min<uint64_t>(T, t0 + k);
-        STRY(recompute_segment(b, j, t0, t1));
+        lst = recompute_segment(b, j, t0, t1);
         const int seg = (int)(nck + 1);
-        for (uint64_t t = t1; t-- > t0;) {
-            copy_states(b, 0, t == t0 ? (int)j : seg + (int)(t - t0), false);
-            STRY(backward_step(b, li, t));
+        for (uint64_t t = t1; t-- > t0 && lst == HP_OK;) {
+            const int slot = t == t0 ? (int)j : seg + (int)(t - t0);
+            if (direct) hp::launch_point_slots(b->stream, b->d_desc, (int)b->members.size(), slot, (int)t);
+            else copy_states(b, 0, slot, false);
+            lst = backward_step(b, li, t, direct);
             li = 1 - li;
         }
     }
+    if (direct)  // also after a failed step: later calls must see the members' own buffers
+        BTRY(b, hipMemcpyAsync(b->d_desc, b->desc.data(), b->desc.size() * sizeof(hp::MemberDesc),
+                               hipMemcpyHostToDevice, b->stream));
+    if (lst != HP_OK) return lst;
     hp::launch_window_dft(b->stream, b->d_desc, b->wdft_items.d, b->wdft_items.n);
     BTRY(b, hipGetLastError());
     // results: one copy each for dL/dA_0, the growth sums and the kernel-tap windows
@@ -646,18 +690,7 @@ hp_status tape_backward(hp_batch* b, const float* const* seed, float* const* gA0
             member_theta(hm, pm, wm, dtheta + off[i]);
         }
     };
-    const unsigned hw = std::thread::hardware_concurrency();
-    const size_t nth = std::min<size_t>(M, std::max(1u, std::min(16u, hw)));
-    if (nth <= 1) {
-        for (size_t i = 0; i < M; i++) member_out(i);
-    } else {
-        std::vector<std::thread> pool;
-        for (size_t t = 0; t < nth; t++)
-            pool.emplace_back([&, t] {
-                for (size_t i = t; i < M; i += nth) member_out(i);
-            });
-        for (std::thread& th : pool) th.join();
-    }
+    parallel_members(M, member_out);
     return HP_OK;
 }
 

@@ -75,6 +75,7 @@ struct MemberDesc {
     const int2* kmap;            // per kernel (source channel, target channel)
     float* lam[2];               // dL/dA ping-pong, C*N each
     float* conv;                 // convolution outputs, K*N per slot (one slot, or one per taped step)
+    float* tape_u;               // U of each taped step, C*N per slot (with stored convolutions)
     float* gscr;                 // (2C+3)*N: dL/d(grad U_c) x, y per channel; dL/d(grad A_sum) x, y;
                                  //   dL/dA_sum through alpha
     float* lamU;                 // C*N: dL/dU
@@ -181,9 +182,11 @@ void launch_tape_clear(hipStream_t st, const MemberDesc* mem, const int4* items,
 // items as launch_tape_copy.  to_stage: stage[stage_off + j] <- lam[li][j], else the reverse.
 void launch_stage_lam(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, float* stage, int li,
                       bool to_stage);
-// items as launch_tape_copy.  U of a step from its stored convolutions (conv slot cslot):
-// the forward's growth accumulate, same f32 operations in the same kernel order.
-void launch_u_from_conv(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cslot);
+// items as launch_tape_copy.  to_tape: tape_u[slot] <- U, else U <- tape_u[slot].
+void launch_u_copy(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int slot, bool to_tape);
+// Every member's A[0] -> tape[slot] and U -> tape_u[u_slot] (descriptor pointers only; a
+// negative slot leaves that pointer).
+void launch_point_slots(hipStream_t st, MemberDesc* mem, int n, int slot, int u_slot);
 // ---- gradient tape: adjoint_fft.hip ---------------------------------------------------------
 bool adj_fft_supported(int L);
 // items = (member, row pair, kernel); reads conv slot cslot

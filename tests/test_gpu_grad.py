@@ -243,3 +243,33 @@ def test_total_mass_gradient_at_full_differentiation_size():
             assert ratio < 1e-2
         else:
             assert d.mean() < 1e-4
+
+
+def test_stored_convolution_tape_matches_recompute():
+    """The automatic layout keeps every step's state, convolutions and U (each backward step
+    then reads them straight from their tape slots); a checkpoint every step without stored
+    convolutions recomputes the FFT passes in the backward instead.  The two forwards run
+    different instantiations of the inverse row pass (with and without the convolution store),
+    whose growth epilogues may round differently in the last bit, so the bar is closeness: a
+    wrong slot or a stale U would miss it by orders of magnitude."""
+    pop_def = workloads.population(6, seed=0x5EED7)
+    cfgs = [c for c, _ in pop_def]
+    rng = np.random.default_rng(5)
+    seed = [[rng.standard_normal(c.width * c.height).astype(np.float32) for _ in range(c.channels)] for c in cfgs]
+    out = {}
+    for k in (0, 1):
+        _say(f"[tape_layout] 6 members, T=5, checkpoint_every={k}")
+        with HipPopulation(cfgs) as pop:
+            pop.set_seeds([s for _, s in pop_def])
+            pop.advance(3)
+            used = pop.tape_forward(5, k)
+            g0, th = pop.tape_backward(seed)
+            after = [pop.get_state(i).channels for i in range(len(cfgs))]
+        out[k] = (used, g0, th, after)
+    assert out[0][0] == 1 and out[1][0] == 1
+    ds = max(float(np.abs(a - b).max()) for m in range(len(cfgs)) for a, b in zip(out[0][3][m], out[1][3][m]))
+    dg = max(float(np.abs(a - b).max()) / (1.0 + float(np.abs(b).max()))
+             for m in range(len(cfgs)) for a, b in zip(out[0][1][m], out[1][1][m]))
+    dt = max(float(np.abs(a - b).max()) / (1e-12 + float(np.abs(b).max())) for a, b in zip(out[0][2], out[1][2]))
+    _say(f"  states max|d| {ds:.2e}; dL/dA0 max|d|/(1+max|g|) {dg:.2e}; dL/dtheta max|d|/max|th| {dt:.2e}")
+    assert ds <= 1e-5 and dg <= 1e-4 and dt <= 1e-3
