@@ -235,17 +235,21 @@ __global__ __launch_bounds__(256) void kb_u_copy(const MemberDesc* __restrict__ 
     for (long long j = (long long)it.y * 256 + threadIdx.x; j < n; j += (long long)it.z * 256) dst[j] = src[j];
 }
 
-// Point each member's A[0] at tape slot `slot` and its U at tape_u slot `u_slot` (a negative
-// slot leaves that pointer): a backward step then reads its state and affinity where the tape
-// keeps them, and a taped forward step writes its U straight into its slot.  One thread per
-// member; the host restores the descriptors afterwards.
+// Point each member's A[0] at tape slot `slot`, and its U and source spectra X at tape_u and
+// tape_x slot `u_slot` (a negative slot leaves those pointers): a taped forward step writes its
+// U and X straight into their slots, and a backward step reads its state, U and X where the
+// tape keeps them (X is then overwritten in place, each slot serving one step).  One thread
+// per member; the host restores the descriptors afterwards.
 __global__ __launch_bounds__(256) void kb_point_slots(MemberDesc* __restrict__ mem, int n, int slot, int u_slot) {
     const int i = blockIdx.x * 256 + (int)threadIdx.x;
     if (i >= n) return;
     MemberDesc& m = mem[i];
     const long long cn = (long long)m.C * m.g.N;
     if (slot >= 0) m.A[0] = m.tape + (long long)slot * cn;
-    if (u_slot >= 0) m.U = m.tape_u + (long long)u_slot * cn;
+    if (u_slot >= 0) {
+        m.U = m.tape_u + (long long)u_slot * cn;
+        m.X = m.tape_x + (long long)u_slot * m.nsrc * m.g.spec;
+    }
 }
 
 // Clear the accumulators of a backward pass: P, pacc and the kernel-tap windows.

@@ -165,9 +165,7 @@ hp_status twiddles(hp_batch* b, int L, const float2** out) {
 
 // The three FFT passes of a step from A[cur]; conv: also store the convolutions in conv slot
 // cslot; rows_only: the forward row pass alone (the adjoint's source spectra).
-// save_u: also keep the step's U in tape_u slot cslot (a tape with stored convolutions).
-void enqueue_fft_passes(hp_batch* b, int cur, bool conv, int cslot = 0, bool rows_only = false,
-                        bool save_u = false) {
+void enqueue_fft_passes(hp_batch* b, int cur, bool conv, int cslot = 0, bool rows_only = false) {
     for (const SizeClass& sc : b->classes) {
         if (sc.rows_fwd.n)
             hp::launch_batch_rows_fwd(sc.W, b->stream, b->d_desc, sc.rows_fwd.d, sc.rows_fwd.n, cur, b->fv);
@@ -176,7 +174,6 @@ void enqueue_fft_passes(hp_batch* b, int cur, bool conv, int cslot = 0, bool row
         if (sc.rows_inv.n)
             hp::launch_batch_rows_inv(sc.W, b->stream, b->d_desc, sc.rows_inv.d, sc.rows_inv.n, conv, cslot, b->fv);
     }
-    if (save_u && !rows_only) hp::launch_u_copy(b->stream, b->d_desc, b->copy_items.d, b->copy_items.n, cslot, true);
 }
 
 void enqueue_stencils(hp_batch* b, int cur) {
@@ -248,7 +245,7 @@ hp_status member_stencil(hp_batch* b, int i, int cur) {
 
 // One step with every member's stencil checked (A[cur] -> A[1 - cur]).
 hp_status step_checked(hp_batch* b, int cur, bool conv = false, int cslot = 0) {
-    enqueue_fft_passes(b, cur, conv, cslot, false, conv);
+    enqueue_fft_passes(b, cur, conv, cslot);
     for (size_t i = 0; i < b->members.size(); i++) STRY(member_stencil(b, (int)i, cur));
     return HP_OK;
 }
@@ -335,7 +332,8 @@ hp_status ensure_tape(hp_batch* b, uint64_t T, uint32_t k, bool store_conv) {
         STRY(balloc(b, &d.lam[1], CN, L));
         STRY(balloc(b, &d.lamU, CN, L));
         STRY(balloc(b, &d.conv, cslots * K * N, L));
-        STRY(balloc(b, &d.tape_u, cslots * CN, L));  // as many slots as conv: conv_cap_slots covers both
+        STRY(balloc(b, &d.tape_u, cslots * CN, L));  // as many slots as conv: conv_cap_slots covers these
+        STRY(balloc(b, &d.tape_x, cslots * std::max<size_t>(1, p.src_chan.size()) * (size_t)p.g.spec, L));
         STRY(balloc(b, &d.gscr, (2 * (size_t)p.C + 3) * N, L));
         STRY(balloc(b, &d.P, K * (size_t)p.g.spec, L));
         STRY(balloc(b, &d.tape, slots * CN, L));
@@ -400,7 +398,8 @@ void auto_layout(hp_batch* b, uint64_t T, uint32_t* k, bool* store_conv) {
     double set = 0.0, cset = 0.0;  // bytes of one state / one step's convolutions and U, all members
     for (const HostMember& hm : b->members) {
         set += 4.0 * hm.plan.C * (double)hm.plan.g.N;
-        cset += 4.0 * (std::max(hm.plan.K, 0) + hm.plan.C) * (double)hm.plan.g.N;
+        cset += 4.0 * (std::max(hm.plan.K, 0) + hm.plan.C) * (double)hm.plan.g.N +
+                8.0 * (double)hm.plan.src_chan.size() * (double)hm.plan.g.spec;
     }
     const double budget = 0.5 * (double)free_b;
     *store_conv = false;
@@ -429,24 +428,26 @@ hp_status tape_forward(hp_batch* b, uint64_t T, uint32_t k) {
         int cur = start;
         if (pass == 1) copy_states(b, start, 0, false);  // back to the first checkpoint
         hp::launch_clear_ints(b->stream, b->d_flags, 1);
-        for (uint64_t t = 0; t < T; t++) {
+        hp_status lst = HP_OK;
+        for (uint64_t t = 0; t < T && lst == HP_OK; t++) {
             if (t % k == 0) copy_states(b, cur, (int)(t / k), true);
+            // with stored convolutions the step's U and source spectra go straight into their
+            // tape slots (no graph runs in this loop; the descriptors are restored after it)
+            if (store_conv) hp::launch_point_slots(b->stream, b->d_desc, (int)b->members.size(), -1, (int)t);
             if (pass == 1) {
-                STRY(step_checked(b, cur, store_conv, (int)t));
+                lst = step_checked(b, cur, store_conv, (int)t);
             } else if (store_conv) {
-                // the step's U goes straight into its tape slot (no graph runs in this loop;
-                // the descriptors are restored after it)
-                hp::launch_point_slots(b->stream, b->d_desc, (int)b->members.size(), -1, (int)t);
                 enqueue_fft_passes(b, cur, true, (int)t);
                 enqueue_stencils(b, cur);
-            } else {
-                BTRY(b, hipGraphLaunch(b->graph[cur], b->stream));
+            } else if (hipGraphLaunch(b->graph[cur], b->stream) != hipSuccess) {
+                lst = b->fail(HP_ERR_HIP, "hipGraphLaunch failed");
             }
             cur = 1 - cur;
         }
-        if (pass == 0 && store_conv)
+        if (store_conv)
             BTRY(b, hipMemcpyAsync(b->d_desc, b->desc.data(), b->desc.size() * sizeof(hp::MemberDesc),
                                    hipMemcpyHostToDevice, b->stream));
+        if (lst != HP_OK) return lst;
         BTRY(b, hipGetLastError());
         copy_states(b, cur, (int)nck, true);
         b->cur = cur;
@@ -502,8 +503,10 @@ hp_status backward_step(hp_batch* b, int li, uint64_t t, bool u_in_place = false
     int cslot = 0;
     if (b->tape_conv) {
         cslot = (int)t;
-        enqueue_fft_passes(b, 0, false, 0, true);
-        if (!u_in_place) hp::launch_u_copy(b->stream, b->d_desc, b->copy_items.d, b->copy_items.n, cslot, false);
+        if (!u_in_place) {  // otherwise the descriptors point X and U at the step's slots
+            enqueue_fft_passes(b, 0, false, 0, true);
+            hp::launch_u_copy(b->stream, b->d_desc, b->copy_items.d, b->copy_items.n, cslot, false);
+        }
     } else {
         enqueue_fft_passes(b, 0, true, 0);
     }

@@ -76,6 +76,7 @@ struct MemberDesc {
     float* lam[2];               // dL/dA ping-pong, C*N each
     float* conv;                 // convolution outputs, K*N per slot (one slot, or one per taped step)
     float* tape_u;               // U of each taped step, C*N per slot (with stored convolutions)
+    float2* tape_x;              // source half spectra of each taped step, nsrc*spec per slot (likewise)
     float* gscr;                 // (2C+3)*N: dL/d(grad U_c) x, y per channel; dL/d(grad A_sum) x, y;
                                  //   dL/dA_sum through alpha
     float* lamU;                 // C*N: dL/dU
@@ -184,8 +185,8 @@ void launch_stage_lam(hipStream_t st, const MemberDesc* mem, const int4* items, 
                       bool to_stage);
 // items as launch_tape_copy.  to_tape: tape_u[slot] <- U, else U <- tape_u[slot].
 void launch_u_copy(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int slot, bool to_tape);
-// Every member's A[0] -> tape[slot] and U -> tape_u[u_slot] (descriptor pointers only; a
-// negative slot leaves that pointer).
+// Every member's A[0] -> tape[slot], U -> tape_u[u_slot] and X -> tape_x[u_slot] (descriptor
+// pointers only; a negative slot leaves those pointers).
 void launch_point_slots(hipStream_t st, MemberDesc* mem, int n, int slot, int u_slot);
 // ---- gradient tape: adjoint_fft.hip ---------------------------------------------------------
 bool adj_fft_supported(int L);
