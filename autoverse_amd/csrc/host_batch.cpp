@@ -206,14 +206,16 @@ hp_status build_graphs(hp_batch* b) {
 }
 
 // One member's stencil through the checked single-member paths (replay after a flagged run).
-hp_status member_stencil(hp_batch* b, int i, int cur) {
+// U: the affinity to read (null: the member's own U buffer).
+hp_status member_stencil(hp_batch* b, int i, int cur, const float* U = nullptr) {
     const hp::MemberDesc& m = b->desc[i];
     const HostMember& hm = b->members[i];
+    const float* u = U ? U : m.U;
     const float s = hm.cfg.flow.distribution_size;
     bool general = !hp::stencil2d_supported(m.g, s, b->tile);
     if (!general) {
         hp::launch_clear_ints(b->stream, b->d_flags, 1);
-        hp::launch_stencil2d(b->stream, m.A[cur], m.U, m.A[1 - cur], m.C, m.g, m.dt, m.beta_a, m.nexp, s, b->d_flags,
+        hp::launch_stencil2d(b->stream, m.A[cur], u, m.A[1 - cur], m.C, m.g, m.dt, m.beta_a, m.nexp, s, b->d_flags,
                              b->tile);
         int f = 0;
         BTRY(b, hipMemcpyAsync(&f, b->d_flags, sizeof(int), hipMemcpyDeviceToHost, b->stream));
@@ -226,7 +228,7 @@ hp_status member_stencil(hp_batch* b, int i, int cur) {
         return HP_OK;
     }
     hp::launch_clear_ints(b->stream, b->d_flags + 1, 2);
-    hp::launch_boxes(b->stream, m.A[cur], m.U, m.C, m.g, m.dt, m.beta_a, m.nexp, s, b->box_lo, b->box_hi,
+    hp::launch_boxes(b->stream, m.A[cur], u, m.C, m.g, m.dt, m.beta_a, m.nexp, s, b->box_lo, b->box_hi,
                      b->d_flags + 1);
     int h[2] = {0, 0};
     BTRY(b, hipMemcpyAsync(h, b->d_flags + 1, sizeof(h), hipMemcpyDeviceToHost, b->stream));
@@ -243,10 +245,16 @@ hp_status member_stencil(hp_batch* b, int i, int cur) {
     return HP_OK;
 }
 
-// One step with every member's stencil checked (A[cur] -> A[1 - cur]).
-hp_status step_checked(hp_batch* b, int cur, bool conv = false, int cslot = 0) {
+// One step with every member's stencil checked (A[cur] -> A[1 - cur]).  u_slot >= 0: the device
+// descriptors point U at tape_u slot u_slot (a taped forward), where the FFT passes leave it, so
+// the host-launched stencils read it there too.
+hp_status step_checked(hp_batch* b, int cur, bool conv = false, int cslot = 0, int u_slot = -1) {
     enqueue_fft_passes(b, cur, conv, cslot);
-    for (size_t i = 0; i < b->members.size(); i++) STRY(member_stencil(b, (int)i, cur));
+    for (size_t i = 0; i < b->members.size(); i++) {
+        const hp::MemberDesc& m = b->desc[i];
+        const float* u = u_slot >= 0 ? m.tape_u + (size_t)u_slot * m.C * m.g.N : nullptr;
+        STRY(member_stencil(b, (int)i, cur, u));
+    }
     return HP_OK;
 }
 
@@ -435,7 +443,7 @@ hp_status tape_forward(hp_batch* b, uint64_t T, uint32_t k) {
             // tape slots (no graph runs in this loop; the descriptors are restored after it)
             if (store_conv) hp::launch_point_slots(b->stream, b->d_desc, (int)b->members.size(), -1, (int)t);
             if (pass == 1) {
-                lst = step_checked(b, cur, store_conv, (int)t);
+                lst = step_checked(b, cur, store_conv, (int)t, store_conv ? (int)t : -1);
             } else if (store_conv) {
                 enqueue_fft_passes(b, cur, true, (int)t);
                 enqueue_stencils(b, cur);

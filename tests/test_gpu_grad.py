@@ -245,20 +245,26 @@ def test_total_mass_gradient_at_full_differentiation_size():
             assert d.mean() < 1e-4
 
 
-def test_stored_convolution_tape_matches_recompute():
-    """The automatic layout keeps every step's state, convolutions and U (each backward step
-    then reads them straight from their tape slots); a checkpoint every step without stored
-    convolutions recomputes the FFT passes in the backward instead.  The two forwards run
-    different instantiations of the inverse row pass (with and without the convolution store),
-    whose growth epilogues may round differently in the last bit, so the bar is closeness: a
-    wrong slot or a stale U would miss it by orders of magnitude."""
+@pytest.mark.parametrize("dt", [None, 3.0])
+def test_stored_convolution_tape_matches_recompute(dt):
+    """The automatic layout keeps every step's state, convolutions, U and source spectra (each
+    backward step then reads them straight from their tape slots); a checkpoint every step
+    without stored convolutions recomputes the FFT passes in the backward instead.  The two
+    forwards run different instantiations of the inverse row pass (with and without the
+    convolution store), whose growth epilogues may round differently in the last bit, so the
+    bar is closeness: a wrong slot or a stale U would miss it by orders of magnitude.  dt = 3
+    pushes flows past the fused stencil's radius, so the taped forward is replayed with
+    checked steps and the general path."""
     pop_def = workloads.population(6, seed=0x5EED7)
     cfgs = [c for c, _ in pop_def]
+    if dt is not None:
+        for c in cfgs:
+            c.dt = dt
     rng = np.random.default_rng(5)
     seed = [[rng.standard_normal(c.width * c.height).astype(np.float32) for _ in range(c.channels)] for c in cfgs]
     out = {}
     for k in (0, 1):
-        _say(f"[tape_layout] 6 members, T=5, checkpoint_every={k}")
+        _say(f"[tape_layout] 6 members, T=5, dt={dt}, checkpoint_every={k}")
         with HipPopulation(cfgs) as pop:
             pop.set_seeds([s for _, s in pop_def])
             pop.advance(3)
