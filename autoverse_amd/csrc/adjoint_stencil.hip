@@ -123,21 +123,52 @@ __global__ __launch_bounds__(adj::NT) void kb_adj_flow(const MemberDesc* __restr
             const float xm = dxp - s, xM = dxp + s, ym = dyp - s, yM = dyp + s;
             const int ixl = max(sat_i32(floorf(xm)), -W), ixh = min(sat_i32(ceilf(xM)), 2 * W);
             const int iyl = max(sat_i32(floorf(ym)), -H), iyh = min(sat_i32(ceilf(yM)), 2 * H);
-            for (int iy = iyl; iy <= iyh; ++iy) {
-                const float cy0 = (float)iy, cy1 = (float)(iy + 1);
-                const float oh = overlap1(cy0, cy1, ym, yM);
-                if (!(oh > 0.0f)) continue;
-                const float doh = (float)(yM < cy1) - (float)(ym > cy0);
-                const float* lrow = ln + (long long)wrap_near(iy, H) * W;
-                for (int ix = ixl; ix <= ixh; ++ix) {
+            if (ixh - ixl <= 4) {
+                // At most five columns (every box with s <= 1): their overlaps, derivatives and
+                // wrapped indices once, reused on every row.  Columns past ixh have zero overlap
+                // and are skipped as the loop below skips them; the order is the loop's.
+                float ow[5], dow[5];
+                int cw[5];
+#pragma unroll
+                for (int j = 0; j < 5; ++j) {
+                    const int ix = ixl + j;
                     const float cx0 = (float)ix, cx1 = (float)(ix + 1);
-                    const float ow = overlap1(cx0, cx1, xm, xM);
-                    if (!(ow * oh > 0.0f)) continue;
-                    const float dow = (float)(xM < cx1) - (float)(xm > cx0);
-                    const float l = lrow[wrap_near(ix, W)];
-                    dm += l * (ow * oh);
-                    gx += l * (dow * oh);
-                    gy += l * (ow * doh);
+                    ow[j] = j <= ixh - ixl ? overlap1(cx0, cx1, xm, xM) : 0.0f;
+                    dow[j] = (float)(xM < cx1) - (float)(xm > cx0);
+                    cw[j] = wrap_near(ix, W);
+                }
+                for (int iy = iyl; iy <= iyh; ++iy) {
+                    const float cy0 = (float)iy, cy1 = (float)(iy + 1);
+                    const float oh = overlap1(cy0, cy1, ym, yM);
+                    if (!(oh > 0.0f)) continue;
+                    const float doh = (float)(yM < cy1) - (float)(ym > cy0);
+                    const float* lrow = ln + (long long)wrap_near(iy, H) * W;
+#pragma unroll
+                    for (int j = 0; j < 5; ++j) {
+                        if (!(ow[j] * oh > 0.0f)) continue;
+                        const float l = lrow[cw[j]];
+                        dm += l * (ow[j] * oh);
+                        gx += l * (dow[j] * oh);
+                        gy += l * (ow[j] * doh);
+                    }
+                }
+            } else {
+                for (int iy = iyl; iy <= iyh; ++iy) {
+                    const float cy0 = (float)iy, cy1 = (float)(iy + 1);
+                    const float oh = overlap1(cy0, cy1, ym, yM);
+                    if (!(oh > 0.0f)) continue;
+                    const float doh = (float)(yM < cy1) - (float)(ym > cy0);
+                    const float* lrow = ln + (long long)wrap_near(iy, H) * W;
+                    for (int ix = ixl; ix <= ixh; ++ix) {
+                        const float cx0 = (float)ix, cx1 = (float)(ix + 1);
+                        const float ow = overlap1(cx0, cx1, xm, xM);
+                        if (!(ow * oh > 0.0f)) continue;
+                        const float dow = (float)(xM < cx1) - (float)(xm > cx0);
+                        const float l = lrow[wrap_near(ix, W)];
+                        dm += l * (ow * oh);
+                        gx += l * (dow * oh);
+                        gy += l * (ow * doh);
+                    }
                 }
             }
             dm *= inv_ta;
