@@ -25,12 +25,20 @@
  *   hp_mass_f64                             <- total_mass_all_channels
  *                                              (src/compute/reintegration.rs:221-228),
  *                                              computed in FP64 instead of f32
- *   hp_advect_mass_2d / hp_sobel_gradient_2d / hp_compute_flow_2d /
- *   hp_growth_accumulate / hp_convolve_2d   <- the stage functions the crate
+ *   hp_advect_mass_2d / hp_advect_mass_3d / hp_sobel_gradient_2d /
+ *   hp_compute_flow_2d / hp_growth_accumulate
+ *                                           <- the stage functions the crate
  *       re-exports from src/compute/mod.rs:5-41 (advect_mass_into
- *       reintegration.rs:60-88, sobel_gradient_into gradient.rs:68-108,
- *       compute_flow_field_into flow.rs:68-87, growth_accumulate growth.rs:44-59,
- *       CachedConvolver::convolve_with_kernel_scratch fft.rs:288-324)
+ *       reintegration.rs:60-88, advect_mass_3d_into reintegration3d.rs:56-101,
+ *       sobel_gradient_into gradient.rs:68-108, compute_flow_field_into
+ *       flow.rs:68-87, growth_accumulate growth.rs:44-59)
+ *   hp_affinity                             <- stage 1 of CpuPropagator::step
+ *       (propagator.rs:244-280): every kernel's
+ *       CachedConvolver::convolve_with_kernel_scratch (fft.rs:288-324) plus
+ *       growth_accumulate, on a context's resident kernel spectra; there is
+ *       no single-convolution entry point
+ *   hp_stencil_with_affinity                <- stages 2-5 of CpuPropagator::step
+ *       (propagator.rs:303-414) with U supplied by the caller
  */
 #ifndef FLOWLENIA_HIP_H
 #define FLOWLENIA_HIP_H
