@@ -144,7 +144,7 @@ def main() -> int:
     prop.close()
     drift = (mass - mass0) / mass0 if mass0 else None
 
-    # the whole step against the survey's byte model and against a plain device copy
+    # the whole step against its algorithmic byte model and against a plain device copy
     step_bytes = step_model_bytes(cfg)
     step_gbs = step_bytes / (secs / args.steps) / 1e9
     copy_gbs = copy_bandwidth_gbs()
@@ -389,8 +389,10 @@ def sphere3d_bench(args) -> int:
 
 
 def step_model_bytes(cfg) -> int:
-    """Algorithmic bytes of one step in the survey's model for a grid whose half spectrum takes
-    two passes: C_s (4N + 3S) + 4 K S + 4 C N + 12 C N, S = 8 (W/2+1) H D."""
+    """Algorithmic bytes of one step, with the half spectrum (S = 8 (W/2+1) H D bytes) taking
+    two passes: C_s (4N + 3S) + 4 K S + 4 C N + 12 C N.  Per source channel: its N f32 cells read
+    and its spectrum written, read and written again; per kernel: a spectrum read; per channel:
+    U written once, then A, U read and A' written by the stencil (12 C N counts those three)."""
     n = cfg.width * cfg.height * max(cfg.depth, 1)
     s = 8 * (cfg.width // 2 + 1) * cfg.height * max(cfg.depth, 1)
     cs = len({k.source_channel for k in cfg.kernels})
