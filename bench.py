@@ -141,6 +141,28 @@ def main() -> int:
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                 "per_kernel": per_kernel}
 
+    # the boundary with host buffers: hp_step and hp_run copy the state in from the caller's
+    # (pageable) per-channel arrays and back out on every call.  PCIe-inclusive rates, reported
+    # beside `value`, never as it.
+    hs = prop.get_state()
+    prop.step(hs)  # untimed first call
+    n_call = 20
+    t0 = time.perf_counter()
+    for _ in range(n_call):
+        prop.step(hs)
+    t1 = time.perf_counter()
+    prop.run(hs, 100)  # untimed first call
+    t2 = time.perf_counter()
+    prop.run(hs, 100)
+    t3 = time.perf_counter()
+    host_boundary = {
+        "hp_step_ms_per_call": round((t1 - t0) * 1e3 / n_call, 3),
+        "hp_step_cell_steps_per_s": N * n_call / (t1 - t0),
+        "hp_run_100_ms_per_call": round((t3 - t2) * 1e3, 3),
+        "hp_run_100_cell_steps_per_s": N * 100 / (t3 - t2),
+        "state_bytes_each_way": 4 * N * cfg.channels,
+        "note": "host arrays copied in and out on every call (pageable numpy buffers, as the "
+                "reference's Vec<f32> channels)"}
     prop.close()
     drift = (mass - mass0) / mass0 if mass0 else None
 
@@ -178,6 +200,7 @@ def main() -> int:
                    "mass_f64_after": mass, "mass_rel_drift": drift,
                    "mass_drift_steps": args.steps, "plan": plan, "autotune": tune},
         "roofline": roofline,
+        "host_boundary": host_boundary,
         "step_model": step_model,
         "cpu_baseline": cpu,
     }
