@@ -22,7 +22,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <exception>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -607,12 +609,28 @@ void parallel_members(size_t M, const F& f) {
         for (size_t i = 0; i < M; i++) f(i);
         return;
     }
-    std::vector<std::thread> pool;
-    for (size_t t = 0; t < nth; t++)
-        pool.emplace_back([&, t] {
+    // An exception in a worker (an allocation in f) is kept and rethrown on the calling thread
+    // after the join, where the entry point's guard turns it into an error status; strides whose
+    // thread could not be started run on the calling thread.
+    std::mutex mu;
+    std::exception_ptr first;
+    auto stride = [&](size_t t) {
+        try {
             for (size_t i = t; i < M; i += nth) f(i);
-        });
+        } catch (...) {
+            std::lock_guard<std::mutex> lk(mu);
+            if (!first) first = std::current_exception();
+        }
+    };
+    std::vector<std::thread> pool;
+    size_t started = 0;
+    try {
+        for (; started < nth; started++) pool.emplace_back(stride, started);
+    } catch (...) {
+    }
+    for (size_t t = started; t < nth; t++) stride(t);
     for (std::thread& th : pool) th.join();
+    if (first) std::rethrow_exception(first);
 }
 
 hp_status tape_backward(hp_batch* b, const float* const* seed, float* const* gA0, double* dtheta) {
