@@ -144,6 +144,35 @@ def test_plain256_100_steps(oracle):
     assert abs(mass - float(want64[0].sum())) / m0 < 1e-6
 
 
+@pytest.mark.parametrize("kind", ["empty", "one_cell"])
+def test_edge_states_vs_oracle(oracle, kind):
+    """Edge states on the 128^2 x 3 multispecies configuration, 10 steps: an all-zero grid (every
+    tile and channel skipped) stays exactly zero, with the oracle's step counter and time; a
+    single cell of mass 1 in channel 1 keeps its mass and matches the f32 oracle to 1e-5."""
+    _say(f"[edge_states {kind}]")
+    cfg, _ = _small_multispecies(128, 3)
+    n = 128 * 128
+    chans = [np.zeros(n, np.float32) for _ in range(3)]
+    if kind == "one_cell":
+        chans[1][37 * 128 + 90] = 1.0
+    want, t32, s32 = oracle.Propagator(cfg, np.float32).run([c.copy() for c in chans], 10)
+    state = av.SimulationState([c.copy() for c in chans], 128, 128)
+    with av.HipPropagator(cfg) as hp:
+        hp.run(state, 10)
+        per, mass = hp.mass_f64()
+    assert state.step == s32 == 10
+    assert state.time == pytest.approx(t32, abs=0)
+    err = max(float(np.abs(state.channels[c] - want[c]).max()) for c in range(3))
+    _say(f"  max|gpu-f32|={err:.3e} mass={mass:.9f}")
+    if kind == "empty":
+        assert all(not np.any(c) for c in state.channels)
+        assert mass == 0.0
+    else:
+        assert err <= 1e-5
+        assert abs(mass - 1.0) < 1e-6
+        assert not np.any(state.channels[0]) and not np.any(state.channels[2])
+
+
 def test_plain256_mass_drift_1000_steps(oracle):
     """Mass conservation over 1000 steps of the plain configuration.  Reintegration in f32 (the
     reference's arithmetic, which the stencil reproduces bit for bit) is not exactly
