@@ -444,13 +444,15 @@ hp_status check_host_state(hp_ctx* c, const float* const* ch) {
     return HP_OK;
 }
 
+// Every channel is checked before any is copied, so a refused state leaves the device state as
+// it was.
 hp_status upload_state(hp_ctx* c, const float* const* ch, float* dst) {
-    for (int i = 0; i < c->C; i++) {
+    for (int i = 0; i < c->C; i++)
         for (long long j = 0; j < c->g.N; j++)
             if (!isfinite(ch[i][j])) return c->fail(HP_ERR_NONFINITE, "input state holds NaN/Inf");
+    for (int i = 0; i < c->C; i++)
         HIP_TRY(c, hipMemcpyAsync(dst + (size_t)i * c->g.N, ch[i], c->g.N * sizeof(float), hipMemcpyHostToDevice,
                                   c->stream));
-    }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return HP_OK;
 }

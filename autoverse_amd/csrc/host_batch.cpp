@@ -1119,13 +1119,15 @@ hp_status hp_batch_set_state(hp_batch* b, uint32_t member, const float* const* c
         if (member >= b->members.size()) return b->fail(HP_ERR_SHAPE, "member index out of range");
         if (!channels) return b->fail(HP_ERR_NULL, "channels is NULL");
         const hp::MemberDesc& m = b->desc[member];
+        // every channel is checked before any is copied: a refused state leaves the member as it was
         for (int c = 0; c < m.C; c++) {
             if (!channels[c]) return b->fail(HP_ERR_NULL, "channel pointer is NULL");
             for (long long j = 0; j < m.g.N; j++)
                 if (!isfinite(channels[c][j])) return b->fail(HP_ERR_NONFINITE, "input state holds NaN/Inf");
+        }
+        for (int c = 0; c < m.C; c++)
             BTRY(b, hipMemcpyAsync(m.A[b->cur] + (size_t)c * m.g.N, channels[c], m.g.N * sizeof(float),
                                    hipMemcpyHostToDevice, b->stream));
-        }
         BTRY(b, hipStreamSynchronize(b->stream));
         b->members[member].time = 0.0f;
         b->members[member].step = 0;

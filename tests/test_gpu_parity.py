@@ -173,6 +173,61 @@ def test_edge_states_vs_oracle(oracle, kind):
         assert not np.any(state.channels[0]) and not np.any(state.channels[2])
 
 
+@pytest.mark.parametrize("bad", [np.nan, np.inf, -np.inf])
+def test_nonfinite_input_is_rejected_and_context_stays_usable(oracle, bad):
+    """A state holding NaN or Inf is refused on the host with HP_ERR_NONFINITE (the reference
+    would carry it through the step); the context then steps a valid state as before."""
+    from autoverse_amd import _abi
+    _say(f"[nonfinite_input {bad}]")
+    cfg, sd = workloads.plain_256()
+    st0 = oracle.seed_generate(sd, 256, 256, 1, 1)
+    want, _, _ = oracle.Propagator(cfg, np.float32).run([st0[0].copy()], 3)
+    poisoned = st0[0].copy()
+    poisoned[1000] = bad
+    with av.HipPropagator(cfg) as hp:
+        with pytest.raises(_abi.HipError) as ei:
+            hp.set_state(av.SimulationState([poisoned], 256, 256))
+        assert ei.value.status == _abi.HP_ERR_NONFINITE
+        with pytest.raises(_abi.HipError) as ei:
+            hp.step(av.SimulationState([poisoned.copy()], 256, 256))
+        assert ei.value.status == _abi.HP_ERR_NONFINITE
+        state = av.SimulationState([st0[0].copy()], 256, 256)
+        hp.run(state, 3)
+    assert state.step == 3
+    assert float(np.abs(state.channels[0] - want[0]).max()) <= 1e-5
+
+
+def test_refused_state_leaves_the_device_state_unchanged(oracle):
+    """Three channels with NaN only in the last one: every channel is checked before any is
+    copied, so the context and a population member both keep their previous state bit for bit."""
+    from autoverse_amd import _abi
+    from autoverse_amd.population import HipPopulation
+    _say("[refused_state_unchanged]")
+    cfg, sd = _small_multispecies(128, 3)
+    good = oracle.seed_generate(sd, 128, 128, 1, 3)
+    bad = [np.full(128 * 128, 0.25, np.float32) for _ in range(3)]
+    bad[2][77] = np.nan
+    with av.HipPropagator(cfg) as hp:
+        hp.set_state(av.SimulationState([c.copy() for c in good], 128, 128))
+        with pytest.raises(_abi.HipError) as ei:
+            hp.set_state(av.SimulationState([c.copy() for c in bad], 128, 128))
+        assert ei.value.status == _abi.HP_ERR_NONFINITE
+        got = hp.get_state()
+    for c in range(3):
+        assert np.array_equal(got.channels[c], good[c])
+    pop = HipPopulation([cfg])
+    try:
+        pop.set_state(0, av.SimulationState([c.copy() for c in good], 128, 128))
+        with pytest.raises(_abi.HipError) as ei:
+            pop.set_state(0, av.SimulationState([c.copy() for c in bad], 128, 128))
+        assert ei.value.status == _abi.HP_ERR_NONFINITE
+        got = pop.get_state(0)
+    finally:
+        pop.close()
+    for c in range(3):
+        assert np.array_equal(got.channels[c], good[c])
+
+
 def test_plain256_mass_drift_1000_steps(oracle):
     """Mass conservation over 1000 steps of the plain configuration.  Reintegration in f32 (the
     reference's arithmetic, which the stencil reproduces bit for bit) is not exactly
