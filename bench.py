@@ -13,7 +13,8 @@ Sobel, flow, reintegration); `value` is cell-steps/s = W*H*steps / seconds.
 
 Prints ONE JSON line.  `roofline` is computed from per-kernel HIP events recorded
 on the library's own stream during a second pass over the same K steps, from the
-same start state (the timed pass runs from hipGraphs without events); `cpu_baseline` times the CPU oracle (oracle/, a
+same start state (the timed pass runs from hipGraphs without events); `host_boundary` times hp_step and hp_run on host
+buffers (PCIe included, never `value`); `cpu_baseline` times the CPU oracle (oracle/, a
 C restatement of the reference CPU propagator with its thread decomposition) on
 a bounded number of steps of the same workload.
 """
