@@ -74,6 +74,7 @@ int32_t orc_sat_i32_f64(double v);
     orc_prop_##S* orc_prop_create_##S(const hp_config* cfg, int threads);                      \
     void orc_prop_destroy_##S(orc_prop_##S* p);                                                \
     void orc_prop_set_threads_##S(orc_prop_##S* p, int threads);                               \
+    void orc_prop_set_line_threads_##S(orc_prop_##S* p, int threads);                          \
     const T* orc_prop_kernel_freq_##S(const orc_prop_##S* p, uint32_t k);                      \
     void orc_prop_affinity_##S(orc_prop_##S* p, T* const* channels, T* const* affinity);        \
     void orc_prop_stencil_##S(orc_prop_##S* p, T* const* channels, T* const* affinity,          \

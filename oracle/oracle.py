@@ -63,6 +63,7 @@ def lib() -> ctypes.CDLL:
                 f"orc_prop_create_{S}": (c_void_p, [POINTER(_abi.HpConfig), c_int]),
                 f"orc_prop_destroy_{S}": (None, [c_void_p]),
                 f"orc_prop_set_threads_{S}": (None, [c_void_p, c_int]),
+                f"orc_prop_set_line_threads_{S}": (None, [c_void_p, c_int]),
                 f"orc_prop_kernel_freq_{S}": (FPx, [c_void_p, c_uint32]),
                 f"orc_prop_affinity_{S}": (None, [c_void_p, PP, PP]),
                 f"orc_prop_stencil_{S}": (None, [c_void_p, PP, PP, PP]),
@@ -270,7 +271,7 @@ class Propagator:
     """CpuPropagator / CpuPropagator3D restated (propagator.rs:124-427), f32 or f64.
     `theta` (float64, layout of theta_of) overrides the config's kernel and ring values."""
 
-    def __init__(self, cfg, dtype=np.float32, threads: int = 1, theta=None):
+    def __init__(self, cfg, dtype=np.float32, threads: int = 1, theta=None, line_threads: int = 1):
         self.cfg = cfg
         self.dtype = np.dtype(dtype)
         self.S = _sfx(dtype)
@@ -284,6 +285,8 @@ class Propagator:
         if not self.ptr:
             raise ValueError("invalid configuration")
         self.n = cfg.width * cfg.height * cfg.depth
+        if line_threads > 1:
+            self.set_line_threads(line_threads)
 
     def close(self):
         if self.ptr:
@@ -298,6 +301,10 @@ class Propagator:
 
     def set_threads(self, t: int):
         getattr(lib(), f"orc_prop_set_threads_{self.S}")(self.ptr, t)
+
+    def set_line_threads(self, t: int):
+        """Workers inside each FFT axis pass and 3D Sobel (results do not depend on it)."""
+        getattr(lib(), f"orc_prop_set_line_threads_{self.S}")(self.ptr, t)
 
     def kernel_freq(self, k: int) -> np.ndarray:
         p = getattr(lib(), f"orc_prop_kernel_freq_{self.S}")(self.ptr, k)

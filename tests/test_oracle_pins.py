@@ -1,0 +1,441 @@
+"""Independent pins of the oracle's step composition, 3D stages and seed patterns -- CPU only.
+
+Each check recomputes the oracle's result from pieces that do not share its code: numpy.fft
+in float64 for the convolutions, numpy closed forms for kernels, growth, Sobel, alpha, flow
+and the seed shapes, and a Python scatter written from the reference's reintegration loops.
+The inputs are chosen so that a wiring mistake changes the result well beyond the tolerance:
+kernels whose source and target channels differ, with different weights, radii and growth
+parameters; channels with different masses; flows of both signs.
+
+Reference files (paths relative to the reference repository):
+  step         src/compute/propagator.rs:239-420, src/compute/propagator3d.rs:119-308
+  kernels      src/compute/kernel.rs:31-117, src/compute/kernel3d.rs:31-134
+  flow (3D)    src/compute/flow3d.rs:50-73
+  reintegrate  src/compute/reintegration.rs:60-174, src/compute/reintegration3d.rs:56-198
+  seeds        src/schema/seed.rs:160-488
+"""
+import math
+
+import numpy as np
+import pytest
+
+from autoverse_amd.schema import (BlobSpec, FlowConfig, KernelConfig, Pattern, RingConfig, Seed,
+                                  SimulationConfig, SphereSpec)
+
+
+# ------------------------------------------------------------------ independent pieces (float64)
+def round_half_away(v: float) -> int:
+    return int(math.floor(abs(v) + 0.5)) * (1 if v >= 0 else -1)
+
+
+def np_kernel(kc: KernelConfig, kernel_radius: int, dim: int) -> np.ndarray:
+    """Ring-sum kernel (kernel.rs:31-86, kernel3d.rs:31-97) in float64, normalised to sum 1."""
+    R = round_half_away(float(np.float32(kc.radius) * np.float32(kernel_radius)))
+    ax = np.arange(-R, R + 1, dtype=np.float64)
+    grids = np.meshgrid(*([ax] * dim), indexing="ij")
+    nd = np.sqrt(sum(g * g for g in grids)) / R
+    v = np.zeros_like(nd)
+    for r in kc.rings:
+        a, d, w = (float(np.float32(x)) for x in (r.amplitude, r.distance, r.width))
+        v += a * np.exp(-(nd - d) ** 2 / (2 * w * w))
+    v[nd > 1] = 0
+    return v / v.sum()
+
+
+def np_pad(k: np.ndarray, shape) -> np.ndarray:
+    """Every tap at offset d from the centre lands at d mod shape (pad_to_size)."""
+    out = np.zeros(shape)
+    R = k.shape[0] // 2
+    for idx in np.ndindex(*k.shape):
+        out[tuple((i - R) % n for i, n in zip(idx, shape))] = k[idx]
+    return out
+
+
+def np_sobel(g: np.ndarray):
+    """Per axis [-1,0,1] derivative times [1,2,1] smoothing on the other axes, periodic:
+    /8 in 2D (gradient.rs:99-102), /32 in 3D (gradient3d.rs:109-138).  Axis order of the
+    result: x first."""
+    nd = g.ndim
+    out = []
+    for ax in reversed(range(nd)):  # x is the last array axis
+        r = (np.roll(g, -1, ax) - np.roll(g, 1, ax))
+        for other in range(nd):
+            if other != ax:
+                r = np.roll(r, 1, other) + 2 * r + np.roll(r, -1, other)
+        out.append(r / (8.0 if nd == 2 else 32.0))
+    return out
+
+
+def np_scatter(cur: np.ndarray, flows, dt: float, s: float) -> np.ndarray:
+    """The reference's scatter (distribute_mass / distribute_mass_3d) in float64: every source
+    with |m| >= 1e-10 spreads m over the cells its box [dest - s, dest + s]^d overlaps, in source
+    raster order, destinations wrapped; a box of measure below 1e-10 puts m on round(dest)."""
+    shape = cur.shape
+    nd = cur.ndim
+    nxt = np.zeros_like(cur)
+    vol = (2 * s) ** nd
+    for idx in np.ndindex(*shape):
+        m = cur[idx]
+        if abs(m) < 1e-10:
+            continue
+        # flows are (fx, fy[, fz]); array index order is (z,) y, x
+        dest = [idx[a] + dt * flows[nd - 1 - a][idx] for a in range(nd)]
+        if vol < 1e-10:
+            cell = tuple(round_half_away(d) % n for d, n in zip(dest, shape))
+            nxt[cell] += m
+            continue
+        ranges = []
+        for a in range(nd):
+            lo, hi = dest[a] - s, dest[a] + s
+            n = shape[a]
+            ranges.append(range(max(math.floor(lo), -n), min(math.ceil(hi), 2 * n) + 1))
+        for cell in np.ndindex(*[len(r) for r in ranges]):
+            area = 1.0
+            for a in range(nd):
+                c = ranges[a][cell[a]]
+                area *= max(min(c + 1, dest[a] + s) - max(c, dest[a] - s), 0.0)
+            if area > 0:
+                tgt = tuple(ranges[a][cell[a]] % shape[a] for a in range(nd))
+                nxt[tgt] += m * (area / vol)
+    return nxt
+
+
+def np_step(cfg: SimulationConfig, chans, kernel_spectra):
+    """One Flow Lenia step in float64 from the pieces above (propagator.rs:239-420)."""
+    f = lambda v: float(np.float32(v))  # noqa: E731 -- config values as the f32 config holds them
+    shape = chans[0].shape
+    U = [np.zeros(shape) for _ in chans]
+    for kc, kh in zip(cfg.kernels, kernel_spectra):
+        conv = np.real(np.fft.ifftn(np.fft.fftn(chans[kc.source_channel]) * kh))
+        mu, sg = f(kc.mu), f(kc.sigma)
+        U[kc.target_channel] += f(kc.weight) * (2 * np.exp(-(conv - mu) ** 2 / (2 * sg * sg)) - 1)
+    asum = sum(chans)
+    ga = np_sobel(asum)
+    beta, n = f(cfg.flow.beta_a), f(cfg.flow.n)
+    al = np.clip((asum / beta) ** n, 0.0, 1.0)
+    out = []
+    for c, A in enumerate(chans):
+        gu = np_sobel(U[c])
+        F = [(1 - al) * gu[i] - al * ga[i] for i in range(len(gu))]
+        out.append(np_scatter(A, F, f(cfg.dt), f(cfg.flow.distribution_size)))
+    return out
+
+
+def mixed_config_2d():
+    """3 channels on 16 x 12; kernels with source != target, different weights, radii and
+    growth parameters, and a kernel per channel whose target differs from its source."""
+    ks = [
+        KernelConfig(1.0, [RingConfig(1.0, 0.5, 0.15)], 0.9, 0.15, 0.02, 0, 1),
+        KernelConfig(0.75, [RingConfig(0.6, 0.3, 0.1), RingConfig(0.8, 0.7, 0.12)], 0.5, 0.2, 0.04, 1, 2),
+        KernelConfig(0.5, [RingConfig(1.0, 0.4, 0.2)], 0.7, 0.1, 0.03, 2, 0),
+        KernelConfig(1.0, [RingConfig(0.5, 0.6, 0.1)], 0.3, 0.25, 0.05, 0, 0),
+        KernelConfig(1.0, [RingConfig(0.9, 0.2, 0.15)], 1.1, 0.12, 0.025, 2, 2),
+    ]
+    return SimulationConfig(width=16, height=12, depth=1, channels=3, dt=0.3, kernel_radius=4,
+                            kernels=ks, flow=FlowConfig(0.8, 2.0, 0.75))
+
+
+def seeded_state(shape, channels, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for c in range(channels):
+        a = rng.uniform(0.0, 1.0 + 0.5 * c, shape)
+        a[rng.uniform(size=shape) < 0.25] = 0.0
+        out.append(a)
+    return out
+
+
+def run_both(oracle, cfg, chans, steps, spectra):
+    want = [a.copy() for a in chans]
+    for _ in range(steps):
+        want = np_step(cfg, want, spectra)
+    got, _, _ = oracle.Propagator(cfg, np.float64).run([a.ravel() for a in chans], steps)
+    return [g.reshape(chans[0].shape) for g in got], want
+
+
+def test_step2d_composition_matches_independent_f64(oracle):
+    """3 steps of the oracle's f64 2D step equal the independent composition to rounding."""
+    cfg = mixed_config_2d()
+    shape = (cfg.height, cfg.width)
+    spectra = [np.fft.fftn(np_pad(np_kernel(k, cfg.kernel_radius, 2), shape)) for k in cfg.kernels]
+    chans = seeded_state(shape, cfg.channels, 21)
+    got, want = run_both(oracle, cfg, chans, 3, spectra)
+    scale = max(np.abs(w).max() for w in want)
+    for g, w in zip(got, want):
+        assert np.abs(g - w).max() <= 1e-12 * scale
+    # the check has teeth: the same composition with growth sent to the source channel instead
+    # of the target differs by far more than the tolerance
+    bad = SimulationConfig(**{**cfg.__dict__})
+    bad.kernels = [KernelConfig(k.radius, k.rings, k.weight, k.mu, k.sigma, k.source_channel, k.source_channel)
+                   for k in cfg.kernels]
+    w_bad = np_step(bad, [a.copy() for a in chans], spectra)
+    g1, _, _ = oracle.Propagator(cfg, np.float64).run([a.ravel() for a in chans], 1)
+    assert max(np.abs(g.reshape(shape) - w).max() for g, w in zip(g1, w_bad)) > 1e-3
+
+
+def test_step2d_composition_large_flow_and_point_fallback(oracle):
+    """Flows well beyond a cell per step (dt 1.5, beta small so alpha saturates and the mass
+    gradient drives) and the point fallback of a tiny distribution size."""
+    for s, dt, beta in ((1.0, 1.5, 0.05), (1e-6, 0.7, 0.5)):
+        cfg = mixed_config_2d()
+        cfg.dt = dt
+        cfg.flow = FlowConfig(beta, 3.0, s)
+        shape = (cfg.height, cfg.width)
+        spectra = [np.fft.fftn(np_pad(np_kernel(k, cfg.kernel_radius, 2), shape)) for k in cfg.kernels]
+        chans = [10.0 * a for a in seeded_state(shape, cfg.channels, 5)]
+        got, want = run_both(oracle, cfg, chans, 2, spectra)
+        scale = max(np.abs(w).max() for w in want)
+        for g, w in zip(got, want):
+            assert np.abs(g - w).max() <= 1e-11 * scale
+
+
+def mixed_config_3d():
+    ks = [
+        KernelConfig(1.0, [RingConfig(1.0, 0.5, 0.15)], 0.9, 0.15, 0.02, 0, 1),
+        KernelConfig(0.5, [RingConfig(0.7, 0.4, 0.2)], 0.6, 0.2, 0.04, 1, 0),
+        KernelConfig(1.0, [RingConfig(0.5, 0.3, 0.1), RingConfig(0.5, 0.8, 0.1)], 0.4, 0.1, 0.03, 1, 1),
+    ]
+    return SimulationConfig(width=8, height=6, depth=5, channels=2, dt=0.4, kernel_radius=2,
+                            kernels=ks, flow=FlowConfig(0.9, 2.0, 0.6))
+
+
+def test_step3d_composition_matches_independent_f64(oracle):
+    cfg = mixed_config_3d()
+    shape = (cfg.depth, cfg.height, cfg.width)
+    spectra = [np.fft.fftn(np_pad(np_kernel(k, cfg.kernel_radius, 3), shape)) for k in cfg.kernels]
+    chans = seeded_state(shape, cfg.channels, 8)
+    got, want = run_both(oracle, cfg, chans, 3, spectra)
+    scale = max(np.abs(w).max() for w in want)
+    for g, w in zip(got, want):
+        assert np.abs(g - w).max() <= 1e-12 * scale
+
+
+def test_flow3d_closed_form(oracle):
+    rng = np.random.default_rng(4)
+    n = 257
+    gu = [rng.normal(size=n) for _ in range(3)]
+    ga = [rng.normal(size=n) for _ in range(3)]
+    m = rng.uniform(0, 2.5, n)
+    m[:5] = [0.0, 1.0, 2.0, 3.0, -1.0]
+    for beta, nexp in ((1.0, 2.0), (2.0, 4.0), (0.7, 2.5), (1.3, 1.0), (1.0, 3.0)):
+        fx, fy, fz = (np.zeros(n) for _ in range(3))
+        oracle.lib().orc_flow3d_f64(*(oracle._p(a) for a in gu + ga + [m]), n, beta, nexp,
+                                    oracle._p(fx), oracle._p(fy), oracle._p(fz))
+        with np.errstate(invalid="ignore"):  # a negative mass to a fractional power is NaN, kept by clamp
+            al = np.clip(np.power(m / beta, nexp), 0, 1)
+        for got, u, a in zip((fx, fy, fz), gu, ga):
+            np.testing.assert_allclose(got, (1 - al) * u - al * a, rtol=1e-13, atol=1e-14)
+    # alpha = 0 passes the affinity gradient, alpha = 1 the negated mass gradient, per axis
+    fx, fy, fz = (np.zeros(3) for _ in range(3))
+    u3 = [np.array([1.0, 2.0, 3.0]) * (i + 1) for i in range(3)]
+    a3 = [np.array([-4.0, 5.0, -6.0]) * (i + 2) for i in range(3)]
+    oracle.lib().orc_flow3d_f64(*(oracle._p(a) for a in u3 + a3 + [np.array([0.0, 9.0, 0.0])]), 3, 1.0, 2.0,
+                                oracle._p(fx), oracle._p(fy), oracle._p(fz))
+    for got, u, a in zip((fx, fy, fz), u3, a3):
+        assert got[0] == u[0] and got[2] == u[2] and got[1] == -a[1]
+
+
+def test_pad3d_places_centre_at_origin(oracle):
+    kc = KernelConfig(1.0, [RingConfig(1.0, 0.5, 0.2)], 1.0, 0.15, 0.015, 0, 0)
+    k = oracle.kernel3d(kc, 2)
+    assert k.shape == (5, 5, 5)
+    for (w, h, d) in ((8, 6, 5), (5, 7, 9)):
+        p = oracle.pad3d(k, w, h, d)
+        want = np.zeros((d, h, w), np.float32)
+        for dz in range(-2, 3):
+            for dy in range(-2, 3):
+                for dx in range(-2, 3):
+                    want[dz % d, dy % h, dx % w] = k[dz + 2, dy + 2, dx + 2]
+        np.testing.assert_array_equal(p, want)
+    np.testing.assert_allclose(oracle.kernel3d(kc, 2, np.float64), np_kernel(kc, 2, 3), rtol=1e-12, atol=1e-15)
+
+
+# ------------------------------------------------------------------ seed patterns (seed.rs:160-488)
+def _f32(v):
+    return np.float32(v)
+
+
+def _grid_f32(shape):
+    """Integer cell coordinates as float32 (x as f32, no +0.5 offset)."""
+    return [a.astype(np.float32) for a in np.meshgrid(*[np.arange(n) for n in shape], indexing="ij")]
+
+
+def test_seed_gaussian_sphere_and_multisphere_closed_form(oracle):
+    W, H, D = 20, 14, 10
+    seed = Seed(Pattern("GaussianSphere", center=(0.4, 0.6, 0.3), radius=0.25, amplitude=0.9, channel=1))
+    out = oracle.seed_generate(seed, W, H, D, 2)
+    z, y, x = _grid_f32((D, H, W))
+    cx, cy, cz = (float(_f32(0.4) * _f32(W)), float(_f32(0.6) * _f32(H)), float(_f32(0.3) * _f32(D)))
+    r = float(_f32(0.25) * _f32(min(W, H, D)))
+    d2 = (x - cx) ** 2.0 + (y - cy) ** 2.0 + (z - cz) ** 2.0
+    want = 0.9 * np.exp(-d2 / (2 * (r / 2) ** 2))
+    assert np.abs(out[0]).max() == 0
+    np.testing.assert_allclose(out[1].reshape(D, H, W), want, rtol=2e-6, atol=1e-37)
+    # MultiSphere adds its spheres, each on its own channel
+    spheres = [SphereSpec((0.3, 0.3, 0.5), 0.2, 1.0, 0), SphereSpec((0.7, 0.6, 0.4), 0.15, 0.5, 0),
+               SphereSpec((0.5, 0.5, 0.5), 0.3, 0.7, 1)]
+    out = oracle.seed_generate(Seed(Pattern("MultiSphere", spheres=spheres)), W, H, D, 2)
+    want = [np.zeros((D, H, W)) for _ in range(2)]
+    for s in spheres:
+        c = [float(_f32(s.center[i]) * _f32(n)) for i, n in enumerate((W, H, D))]
+        r = float(_f32(s.radius) * _f32(min(W, H, D)))
+        d2 = (x - c[0]) ** 2.0 + (y - c[1]) ** 2.0 + (z - c[2]) ** 2.0
+        want[s.channel] += s.amplitude * np.exp(-d2 / (2 * (r / 2) ** 2))
+    for c in range(2):
+        np.testing.assert_allclose(out[c].reshape(D, H, W), want[c], rtol=4e-6, atol=1e-37)
+
+
+def test_seed_blob_in_3d_is_a_sphere_at_mid_depth(oracle):
+    """GaussianBlob on a 3D grid is a Gaussian ball at z = depth/2 with r = radius * min(W, H)."""
+    W, H, D = 16, 12, 6
+    out = oracle.seed_generate(Seed(Pattern("GaussianBlob", center=(0.25, 0.5), radius=0.3, amplitude=1.0,
+                                            channel=0)), W, H, D, 1)
+    z, y, x = _grid_f32((D, H, W))
+    r = float(_f32(0.3) * _f32(min(W, H)))
+    d2 = (x - 4.0) ** 2.0 + (y - 6.0) ** 2.0 + (z - 3.0) ** 2.0
+    np.testing.assert_allclose(out[0].reshape(D, H, W), np.exp(-d2 / (2 * (r / 2) ** 2)), rtol=2e-6, atol=1e-37)
+
+
+def _shell_f32(dist, rin, rout, amp):
+    """Band [rin, rout] with linear falloff over 20% of the width at both edges, in float32
+    operations in the reference's order (no transcendental: bit-exact in numpy float32)."""
+    rin, rout, amp = _f32(rin), _f32(rout), _f32(amp)
+    inside = (dist >= rin) & (dist <= rout)
+    ew = (rout - rin) * _f32(0.2)
+    fall = np.minimum(np.minimum((dist - rin) / ew, _f32(1.0)), np.minimum((rout - dist) / ew, _f32(1.0)))
+    return np.where(inside, amp * fall, _f32(0.0)).astype(np.float32)
+
+
+def test_seed_ring_and_shell_bitwise(oracle):
+    W, H = 40, 30
+    out = oracle.seed_generate(Seed(Pattern("Ring", center=(0.45, 0.55), inner_radius=0.15, outer_radius=0.35,
+                                            amplitude=0.8, channel=0)), W, H, 1, 1)
+    y, x = _grid_f32((H, W))
+    cx, cy = _f32(0.45) * _f32(W), _f32(0.55) * _f32(H)
+    m = _f32(min(W, H))
+    dx, dy = x - cx, y - cy
+    dist = np.sqrt(dx * dx + dy * dy)
+    want = _shell_f32(dist, _f32(0.15) * m, _f32(0.35) * m, 0.8)
+    np.testing.assert_array_equal(out[0].reshape(H, W), want)
+    assert (want > 0).sum() > 100 and want.max() == np.float32(0.8)
+    W, H, D = 18, 16, 14
+    out = oracle.seed_generate(Seed(Pattern("Shell", center=(0.5, 0.4, 0.6), inner_radius=0.2, outer_radius=0.4,
+                                            amplitude=1.0, channel=0)), W, H, D, 1)
+    z, y, x = _grid_f32((D, H, W))
+    c = [_f32(0.5) * _f32(W), _f32(0.4) * _f32(H), _f32(0.6) * _f32(D)]
+    m = _f32(min(W, H, D))
+    dx, dy, dz = x - c[0], y - c[1], z - c[2]
+    dist = np.sqrt(dx * dx + dy * dy + dz * dz)
+    np.testing.assert_array_equal(out[0].reshape(D, H, W), _shell_f32(dist, _f32(0.2) * m, _f32(0.4) * m, 1.0))
+    # Ring on a 3D grid becomes a shell at mid depth with radii scaled by min(W, H)
+    out = oracle.seed_generate(Seed(Pattern("Ring", center=(0.5, 0.5), inner_radius=0.1, outer_radius=0.3,
+                                            amplitude=1.0, channel=0)), W, H, D, 1)
+    c = [_f32(0.5) * _f32(W), _f32(0.5) * _f32(H), _f32(0.5) * _f32(D)]
+    m = _f32(min(W, H))
+    dx, dy, dz = x - c[0], y - c[1], z - c[2]
+    dist = np.sqrt(dx * dx + dy * dy + dz * dz)
+    np.testing.assert_array_equal(out[0].reshape(D, H, W), _shell_f32(dist, _f32(0.1) * m, _f32(0.3) * m, 1.0))
+
+
+def test_seed_torus_closed_form(oracle):
+    W, H, D = 24, 24, 12
+    out = oracle.seed_generate(Seed(Pattern("Torus3D", center=(0.5, 0.5, 0.5), major_radius=0.25,
+                                            minor_radius=0.1, amplitude=1.0, channel=0)), W, H, D, 1)
+    z, y, x = _grid_f32((D, H, W))
+    m = float(_f32(min(W, H, D)))
+    R, r = float(_f32(0.25) * _f32(m)), float(_f32(0.1) * _f32(m))
+    dxy = np.sqrt((x - 12.0) ** 2.0 + (y - 12.0) ** 2.0)
+    t2 = (dxy - R) ** 2 + (z - 6.0) ** 2.0
+    want = np.exp(-t2 / (2 * (r / 2) ** 2))
+    got = out[0].reshape(D, H, W)
+    # values at or below 1e-6 are dropped (seed.rs:478-480); away from that threshold the
+    # closed form holds to f32 rounding
+    clear = np.abs(want - 1e-6) > 1e-8
+    np.testing.assert_allclose(got[clear], np.where(want > 1e-6, want, 0.0)[clear], rtol=2e-6, atol=0)
+    assert (got > 0).sum() > 50 and got.max() > 0.9
+
+
+def test_seed_custom_values(oracle):
+    vals = [(1, 2, 0, 0.5), (3, 0, 1, 0.25), (9, 9, 0, 7.0), (0, 1, 5, 1.0)]  # out of range ones are dropped
+    out = oracle.seed_generate(Seed(Pattern("Custom", values=vals)), 4, 3, 1, 2)
+    want = np.zeros((2, 3, 4), np.float32)
+    want[0, 2, 1] = 0.5
+    want[1, 0, 3] = 0.25
+    np.testing.assert_array_equal(np.stack(out).reshape(2, 3, 4), want)
+    # Custom on a 3D grid writes plane depth/2
+    out = oracle.seed_generate(Seed(Pattern("Custom", values=vals)), 4, 3, 5, 2)
+    want3 = np.zeros((2, 5, 3, 4), np.float32)
+    want3[:, 2] = want
+    np.testing.assert_array_equal(np.stack(out).reshape(2, 5, 3, 4), want3)
+    vals3 = [(1, 2, 3, 0, 0.5), (0, 0, 0, 1, 2.0), (0, 0, 5, 0, 1.0)]
+    out = oracle.seed_generate(Seed(Pattern("Custom3D", values=vals3)), 4, 3, 5, 2)
+    want3 = np.zeros((2, 5, 3, 4), np.float32)
+    want3[0, 3, 2, 1] = 0.5
+    want3[1, 0, 0, 0] = 2.0
+    np.testing.assert_array_equal(np.stack(out).reshape(2, 5, 3, 4), want3)
+
+
+def test_seed_multiblob_adds(oracle):
+    blobs = [BlobSpec((0.3, 0.4), 0.1, 1.0, 0), BlobSpec((0.35, 0.45), 0.2, 0.5, 0), BlobSpec((0.7, 0.2), 0.1, 2.0, 1)]
+    W, H = 32, 24
+    out = oracle.seed_generate(Seed(Pattern("MultiBlob", blobs=blobs)), W, H, 1, 2)
+    y, x = _grid_f32((H, W))
+    want = [np.zeros((H, W)) for _ in range(2)]
+    for b in blobs:
+        cx, cy = float(_f32(b.center[0]) * _f32(W)), float(_f32(b.center[1]) * _f32(H))
+        r = float(_f32(b.radius) * _f32(min(W, H)))
+        want[b.channel] += b.amplitude * np.exp(-((x - cx) ** 2.0 + (y - cy) ** 2.0) / (2 * (r / 2) ** 2))
+    for c in range(2):
+        np.testing.assert_allclose(out[c].reshape(H, W), want[c], rtol=4e-6, atol=1e-37)
+
+
+def typed_scatter(cur: np.ndarray, flows, dt, s, T):
+    """np_scatter in scalar type T with the reference's operation order (reintegration.rs:95-167,
+    reintegration3d.rs:109-198): overlap = max(min(c + 1, hi) - max(c, lo), 0) per axis, the
+    product in axis order x, y, z, fraction = product / (2s)^d, contribution m * fraction."""
+    shape = cur.shape
+    nd = cur.ndim
+    nxt = np.zeros(shape, T)
+    dt, s = T(dt), T(s)
+    two_s = T(2) * s
+    vol = two_s * two_s if nd == 2 else two_s * two_s * two_s
+    for idx in np.ndindex(*shape):
+        m = T(cur[idx])
+        if abs(m) < T(1e-10):
+            continue
+        dest = [T(idx[a]) + dt * T(flows[nd - 1 - a][idx]) for a in range(nd)]
+        lo = [d - s for d in dest]
+        hi = [d + s for d in dest]
+        rng_ = [range(max(math.floor(lo[a]), -shape[a]), min(math.ceil(hi[a]), 2 * shape[a]) + 1)
+                for a in range(nd)]
+        axes = list(reversed(range(nd)))  # x, y(, z) = last array axis first
+        for cell in np.ndindex(*[len(r) for r in rng_]):
+            ov = {}
+            for a in range(nd):
+                c = rng_[a][cell[a]]
+                ov[a] = max(min(T(c + 1), hi[a]) - max(T(c), lo[a]), T(0))
+            area = ov[axes[0]] * ov[axes[1]]
+            if nd == 3:
+                area = area * ov[axes[2]]
+            if area > T(0):
+                tgt = tuple(rng_[a][cell[a]] % shape[a] for a in range(nd))
+                nxt[tgt] = nxt[tgt] + m * (area / vol)
+    return nxt
+
+
+@pytest.mark.parametrize("dt,s,scale", [(0.1, 1.0, 1.0), (0.6, 0.5, 3.0), (1.0, 1.3, 4.0), (0.4, 0.75, 2.0)])
+def test_advect3d_matches_independent_scatter_bitwise(oracle, dt, s, scale):
+    """The f32 3D reintegration equals the typed scatter bit for bit on a 7 x 6 x 5 grid; the
+    f64 one equals it to the last bit as well (same operation order)."""
+    rng = np.random.default_rng(int(dt * 10 + s * 100))
+    shape = (5, 6, 7)
+    cur = rng.uniform(0, 1, shape).astype(np.float32)
+    cur[rng.uniform(size=shape) < 0.3] = 0
+    fx, fy, fz = (rng.normal(0, scale, shape).astype(np.float32) for _ in range(3))
+    got = oracle.advect3d(cur, fx, fy, fz, dt, s)
+    np.testing.assert_array_equal(got, typed_scatter(cur, (fx, fy, fz), dt, s, np.float32))
+    c64 = cur.astype(np.float64)
+    f64 = [a.astype(np.float64) for a in (fx, fy, fz)]
+    got64 = oracle.advect3d(c64, *f64, dt, s)
+    np.testing.assert_allclose(got64, np_scatter(c64, f64, float(np.float64(dt)), float(np.float64(s))),
+                               rtol=0, atol=1e-13)
