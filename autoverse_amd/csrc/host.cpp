@@ -646,6 +646,9 @@ hp_status create(const hp_config* cfg, int device, const hp_plan* req, hp_ctx** 
         g_global_error = "hipMemset failed";
         return HP_ERR_HIP;
     }
+    // the replay graphs are captured here, so that the first advance (and a timed one right
+    // after a plan change) launches them instead of capturing them
+    if (fast_stencil_ok(c)) TRY(build_graphs(c));
 #undef TRY
     *out = c;
     return HP_OK;

@@ -457,7 +457,12 @@ hp_status tape_forward(hp_batch* b, uint64_t T, uint32_t k) {
         if (store_conv)
             BTRY(b, hipMemcpyAsync(b->d_desc, b->desc.data(), b->desc.size() * sizeof(hp::MemberDesc),
                                    hipMemcpyHostToDevice, b->stream));
-        if (lst != HP_OK) return lst;
+        if (lst != HP_OK) {  // the population goes back to the state the call started from
+            copy_states(b, start, 0, false);
+            b->cur = start;
+            (void)hipStreamSynchronize(b->stream);
+            return lst;
+        }
         BTRY(b, hipGetLastError());
         copy_states(b, cur, (int)nck, true);
         b->cur = cur;
@@ -673,7 +678,12 @@ hp_status tape_backward(hp_batch* b, const float* const* seed, float* const* gA0
     if (direct)  // also after a failed step: later calls must see the members' own buffers
         BTRY(b, hipMemcpyAsync(b->d_desc, b->desc.data(), b->desc.size() * sizeof(hp::MemberDesc),
                                hipMemcpyHostToDevice, b->stream));
-    if (lst != HP_OK) return lst;
+    if (lst != HP_OK) {  // A[0]/A[1] served as scratch: the population holds A_T again
+        copy_states(b, b->tape_cur, (int)nck, false);
+        b->cur = b->tape_cur;
+        (void)hipStreamSynchronize(b->stream);
+        return lst;
+    }
     hp::launch_window_dft(b->stream, b->d_desc, b->wdft_items.d, b->wdft_items.n);
     BTRY(b, hipGetLastError());
     // results: one copy each for dL/dA_0, the growth sums and the kernel-tap windows
@@ -727,6 +737,14 @@ hp_status tape_backward(hp_batch* b, const float* const* seed, float* const* gA0
 // pass's column blocks depend on the plan's column split) and per stencil group the tiles of
 // the plan's tile shape.
 hp_status build_items(hp_batch* b) {
+    // the captured graphs hold the old lists as kernel arguments: they go first, so that no
+    // early return below can leave a graph pointing at freed lists
+    BTRY(b, hipStreamSynchronize(b->stream));
+    for (int p = 0; p < 2; p++)
+        if (b->graph[p]) {
+            (void)hipGraphExecDestroy(b->graph[p]);
+            b->graph[p] = nullptr;
+        }
     for (void* p : b->item_allocs) (void)hipFree(p);
     b->item_allocs.clear();
     b->classes.clear();
@@ -779,11 +797,6 @@ hp_status build_items(hp_batch* b) {
         gr.items.n = (int)it.size();
         b->stencil_groups.push_back(gr);
     }
-    for (int p = 0; p < 2; p++)
-        if (b->graph[p]) {
-            (void)hipGraphExecDestroy(b->graph[p]);
-            b->graph[p] = nullptr;
-        }
     return HP_OK;
 }
 
@@ -931,6 +944,7 @@ hp_status create(const hp_config* cfgs, uint32_t count, int device, hp_batch** o
     TRY(balloc(b, &b->d_mass, (size_t)max_c + 1));
     BTRY(b, hipMemset(b->d_flags, 0, 16));
     BTRY(b, hipDeviceSynchronize());
+    TRY(build_graphs(b));
 #undef TRY
     *out = b;
     return HP_OK;
@@ -947,7 +961,8 @@ hp_status batch_set_plan(hp_batch* b, const hp_plan& p) {
     b->fv.rp = p.radix_order;
     b->fv.cs = p.column_split;
     b->tile = p.stencil_tile;
-    return build_items(b);
+    STRY(build_items(b));
+    return build_graphs(b);  // captured now, not in the first advance after the change
 }
 
 hp_plan batch_plan(const hp_batch* b) {
@@ -1007,7 +1022,9 @@ hp_status batch_autotune(hp_batch* b, uint32_t steps, hp_plan* best_out, uint32_
     const hp_plan start = batch_plan(b);
     hp_plan best = start;
     auto measure = [&](const hp_plan& p) -> hp_status {
-        if (batch_set_plan(b, p) == HP_ERR_UNSUPPORTED_SIZE) return HP_OK;  // skipped candidate
+        const hp_status ps = batch_set_plan(b, p);
+        if (ps == HP_ERR_UNSUPPORTED_SIZE) return HP_OK;  // a candidate these members cannot take
+        if (ps != HP_OK) return ps;  // anything else leaves the plan unusable: stop tuning
         STRY(save(true));
         STRY(advance(b, 2));
         STRY(save(true));

@@ -38,7 +38,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline and the FP64 error")
+    ap.add_argument("--no-population", action="store_true",
+                    help="skip the 256-member population object (batched forward+adjoint vs per-instance)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the plan autotuner (run the built-in default plan)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline sample length")
@@ -98,6 +100,10 @@ def main() -> int:
 
     # the state the timed pass starts from, so that the per-kernel pass replays the same K steps
     start = prop.get_state()
+    # untimed: launch every replay graph of both parities once (16-step and single-step graphs
+    # of each parity: 16 + 1 + 16 + 1 steps end on the start parity), then back to the start
+    prop.advance(34)
+    prop.set_state(start)
 
     # timed pass: K steps replayed from hipGraphs, bracketed by device synchronisation
     t0 = time.perf_counter()
@@ -108,6 +114,7 @@ def main() -> int:
     radius, general = prop.stats()
     # FP64 mass of the device state after the timed pass (conservation check, reported)
     _, mass = prop.mass_f64()
+    final = prop.get_state()  # warmup + K steps from the seed
 
     # per-kernel pass: the same K steps from the same start state (the stencil skips empty tiles,
     # so its time depends on how far the pattern has spread), with an event pair around every
@@ -177,9 +184,14 @@ def main() -> int:
                   "frac_of_copy": round(step_gbs / copy_gbs, 4) if copy_gbs else None,
                   "copy": "hipMemcpy device to device, 32 MiB each way, read + write bytes"}
 
-    cpu = None
+    cpu = fp64 = None
     if not args.no_cpu:
+        fp64 = fp64_error(cfg, seed, final, args.warmup + args.steps)
         cpu = cpu_baseline(cfg, seed, args.cpu_seconds)
+
+    population = None
+    if not args.no_population:
+        population = population_measure(args.pop_steps, args.warmup, tune=not args.no_tune, lone=True)
 
     out = {
         "metric": "cell-steps/s at 1024^2x3ch (plain HipPropagator forward step)",
@@ -203,16 +215,77 @@ def main() -> int:
         "roofline": roofline,
         "host_boundary": host_boundary,
         "step_model": step_model,
+        "fp64_error": fp64,
         "cpu_baseline": cpu,
+        "population": population,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def fp64_error(cfg, seed, final, steps_done: int, cap: int = 60):
+    """Max |device - FP64 oracle| after the run's steps from the seed.  The device state after
+    warmup + K steps is compared when that is at most `cap` steps; otherwise a fresh device
+    run of `cap` steps from the seed is.  The f32 oracle's own distance from FP64 is given
+    beside it (the reference's f32 arithmetic drifts from FP64 by that much)."""
+    import numpy as np
+
+    import autoverse_amd as av
+    from oracle import oracle as O
+
+    n = steps_done
+    if n > cap:
+        n = cap
+        with av.HipPropagator(cfg) as p:
+            st = av.SimulationState.from_seed(seed, cfg)
+            p.run(st, n)
+            final = st
+    gpu = [np.asarray(c, np.float64) for c in final.channels]
+    threads = max(1, min(os.cpu_count() or 1, 16, len(cfg.kernels)))
+    st = O.seed_generate(seed, cfg.width, cfg.height, 1, cfg.channels)
+    t0 = time.perf_counter()
+    ref, _, _ = O.Propagator(cfg, np.float64, threads=threads).run([c.astype(np.float64) for c in st], n)
+    t1 = time.perf_counter()
+    o32, _, _ = O.Propagator(cfg, np.float32, threads=threads).run(st, n)
+    err = max(float(np.abs(g - r).max()) for g, r in zip(gpu, ref))
+    e32 = max(float(np.abs(o.astype(np.float64) - r).max()) for o, r in zip(o32, ref))
+    g32 = max(float(np.abs(g - o).max()) for g, o in zip(gpu, o32))
+    scale = max(float(np.abs(r).max()) for r in ref)
+    return {"steps_from_seed": n, "gpu_vs_fp64_max_abs": err, "f32_oracle_vs_fp64_max_abs": e32,
+            "gpu_vs_f32_oracle_max_abs": g32, "fp64_field_max": scale,
+            "fp64_oracle_s": round(t1 - t0, 2)}
 
 
 def population_bench(args) -> int:
     """256 members, sides U{64,128,192,256}, 1-3 channels (autoverse_amd/workloads.population):
     the batched step (one launch per pass per size class, hipGraph replay) against the plain
     per-instance path (one HipPropagator per member, advanced one after the other)."""
+    r = population_measure(args.pop_steps, args.warmup, tune=not args.no_tune, lone=not args.no_lone)
+    out = {
+        "metric": "cell-steps/s, 256-member population (batched forward step)",
+        "value": r["batched_forward"]["value"],
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": r["steps"],
+        "warmup": args.warmup,
+        "ms_per_step": r["batched_forward"]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": r["workload"], "members": r["members"], "cells": r["cells"]},
+        **{k: v for k, v in r.items() if k not in ("workload", "members", "cells", "steps")},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def population_measure(K: int, warmup: int, tune: bool = True, lone: bool = True) -> dict:
+    """The 256-member population: batched forward step, per-instance forward step (one
+    HipPropagator per member, advanced in turn), and batched forward + adjoint (gradient tape
+    of K steps, then the backward pass).  `ratio_fwd_adjoint_vs_per_instance` is the second
+    headline: batched forward+adjoint cell-steps/s over per-instance forward cell-steps/s."""
     import numpy as np
 
     import autoverse_amd as av
@@ -223,15 +296,18 @@ def population_bench(args) -> int:
     cfgs = [c for c, _ in pop_def]
     seeds = [s for _, s in pop_def]
     cells = sum(c.width * c.height for c in cfgs)
-    K = args.pop_steps
     print(f"[population] {len(cfgs)} members, {cells} cells, {K} steps", file=sys.stderr, flush=True)
 
     pop = HipPopulation(cfgs)
     pop.set_seeds(seeds)
-    pop.advance(args.warmup)
+    pop.advance(warmup)
     plan = None
-    if not args.no_tune:  # untimed, states restored after
+    if tune:  # untimed, states restored after
         plan, _ = pop.autotune(steps=10)
+    start = [pop.get_state(i) for i in range(len(cfgs))]
+    pop.advance(2)  # untimed: both step graphs launched once, then back to the start states
+    for i, st in enumerate(start):
+        pop.set_state(i, st)
     m0 = pop.mass_f64()
     c0, _ = pop.centroids()
     t0 = time.perf_counter()
@@ -245,12 +321,12 @@ def population_bench(args) -> int:
     drift = float(np.max(np.abs(m1 - m0) / np.maximum(m0, 1e-30)))
 
     secs_i = None
-    if not args.no_lone:
+    if lone:
         props = []
         for c, s in pop_def:
             p = av.HipPropagator(c)
             p.set_state(av.SimulationState.from_seed(s, c))
-            p.advance(args.warmup)
+            p.advance(warmup)
             props.append(p)
         print("[population] per-instance pass", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
@@ -271,11 +347,11 @@ def population_bench(args) -> int:
     # warm, untimed, as the other two paths are warmed: the first tape allocates its states and
     # convolutions (tens of GB) and builds its launch lists; then back to the same start state
     pop.set_seeds(seeds)
-    pop.advance(args.warmup)
+    pop.advance(warmup)
     pop.tape_forward(K, 0)
     pop.tape_backward(dl)
     pop.set_seeds(seeds)
-    pop.advance(args.warmup)
+    pop.advance(warmup)
     print("[population] forward + adjoint pass", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     every = pop.tape_forward(K, 0)
@@ -289,35 +365,24 @@ def population_bench(args) -> int:
     value = cells * K / secs_b
     per_inst = cells * K / secs_i if secs_i else None
     fwd_adj = cells * K / secs_fa
-    out = {
-        "metric": "cell-steps/s, 256-member population (batched forward step)",
-        "value": value,
-        "unit": "cell-steps/s",
-        "n_gpus": 1,
-        "steps": K,
-        "warmup": args.warmup,
-        "ms_per_step": secs_b * 1e3 / K,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": "population of 256: side U{64,128,192,256}, 1-3 channels, every "
-                               "source->target kernel, R=15, dt=0.2", "members": len(cfgs), "cells": cells,
-                   "max_gather_radius": radius, "general_path_steps": general, "plan": plan},
-        "per_instance": None if per_inst is None else {
+    return {
+        "workload": "population of 256: side U{64,128,192,256}, 1-3 channels, every source->target "
+                    "kernel, R=15, dt=0.2", "members": len(cfgs), "cells": cells, "steps": K,
+        "max_gather_radius": radius, "general_path_steps": general, "plan": plan,
+        "batched_forward": {"value": value, "unit": "cell-steps/s", "ms_per_step": secs_b * 1e3 / K,
+                            "path": "one launch per pass per size class, hipGraph replay"},
+        "per_instance_forward": None if per_inst is None else {
             "value": per_inst, "unit": "cell-steps/s", "ms_per_step": secs_i * 1e3 / K,
             "path": "one HipPropagator per member, advanced in turn"},
-        "speedup_vs_per_instance": None if per_inst is None else value / per_inst,
-        "forward_adjoint": {"value": fwd_adj, "unit": "cell-steps/s", "steps": K, "checkpoint_every": every,
-                            "ms_per_step": secs_fa * 1e3 / K, "tape_forward_ms_per_step": secs_f * 1e3 / K,
-                            "speedup_vs_per_instance_forward": None if per_inst is None else fwd_adj / per_inst,
-                            "gradients_finite": finite},
+        "batched_forward_adjoint": {
+            "value": fwd_adj, "unit": "cell-steps/s", "checkpoint_every": every,
+            "ms_per_step": secs_fa * 1e3 / K, "tape_forward_ms_per_step": secs_f * 1e3 / K,
+            "gradients_finite": finite},
+        "speedup_batched_vs_per_instance": None if per_inst is None else value / per_inst,
+        "ratio_fwd_adjoint_vs_per_instance": None if per_inst is None else fwd_adj / per_inst,
         "fitness": {"displacement_mean": float(disp.mean()), "displacement_max": float(disp.max())},
-        "mass_drift_max_rel": drift,
+        "mass_drift_max_rel_fp64": drift,
     }
-    print(json.dumps(out), flush=True)
-    return 0
 
 
 def grad_bench(args) -> int:
@@ -512,13 +577,51 @@ def cpu_baseline(cfg, seed, target_s: float):
     one = time.perf_counter() - t0
     n = max(1, min(200, int(target_s / max(one, 1e-6))))
     t0 = time.perf_counter()
-    p.run(st, n)
+    st, _, _ = p.run(st, n)
     secs = time.perf_counter() - t0
+    # the same oracle on one thread, 2 steps
+    p.set_threads(1)
+    t0 = time.perf_counter()
+    p.run(st, 2)
+    secs1 = (time.perf_counter() - t0) / 2
     p.close()
     return {"value": cfg.width * cfg.height * n / secs, "unit": "cell-steps/s", "cores": threads,
             "kind": "port",
             "sample": f"{n} steps of the same 1024x1024x3ch workload, f32, {threads} threads "
-                      f"(host has {ncpu} logical CPUs), {secs * 1e3 / n:.1f} ms/step"}
+                      f"(host has {ncpu} logical CPUs), {secs * 1e3 / n:.1f} ms/step",
+            "one_thread": {"value": cfg.width * cfg.height / secs1, "unit": "cell-steps/s",
+                           "ms_per_step": round(secs1 * 1e3, 1), "sample": "2 steps, 1 thread"},
+            "fft_vs_scipy": fft_microbench(cfg.width, cfg.height)}
+
+
+def fft_microbench(w: int, h: int, reps: int = 5):
+    """The oracle's own complex 2D FFT (f32) against scipy.fft.fft2 (pocketfft, complex64, one
+    worker) on the same W x H complex array: ms per transform, so the CPU baseline's FFT can be
+    seen not to be a straw man."""
+    import numpy as np
+
+    from oracle import oracle as O
+    try:
+        import scipy.fft as sf
+    except Exception:
+        return None
+    rng = np.random.default_rng(0)
+    x = (rng.normal(size=(h, w)) + 1j * rng.normal(size=(h, w))).astype(np.complex64)
+    buf = np.empty(2 * x.size, dtype=np.float32)
+    buf[0::2], buf[1::2] = x.real.ravel(), x.imag.ravel()
+    f = O.lib().orc_fft2d_f32
+    f(O._p(buf), w, h, 0)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        f(O._p(buf), w, h, 0)
+    own = (time.perf_counter() - t0) / reps
+    sf.fft2(x, workers=1)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sf.fft2(x, workers=1)
+    ref = (time.perf_counter() - t0) / reps
+    return {"size": f"{w}x{h} complex64", "oracle_ms": round(own * 1e3, 2), "scipy_pocketfft_ms": round(ref * 1e3, 2),
+            "oracle_over_scipy": round(own / ref, 2), "threads": 1}
 
 
 if __name__ == "__main__":
