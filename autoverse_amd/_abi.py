@@ -123,6 +123,8 @@ EXPORTS = [
     ("hp_profile_repeats", c_int, [c_void_p, c_uint32]),
     ("hp_kernel_bytes", c_int, [c_void_p, POINTER(c_double)]),
     ("hp_advect_mass_2d", c_int, [c_int, FP, FP, FP, FP, c_uint32, c_uint32, c_float, c_float]),
+    ("hp_advect_mass_2d_noalloc", c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_uint32,
+                                          c_float, c_float]),
     ("hp_advect_mass_3d", c_int, [c_int, FP, FP, FP, FP, FP, c_uint32, c_uint32, c_uint32, c_float, c_float]),
     ("hp_sobel_gradient_2d", c_int, [c_int, FP, FP, FP, c_uint32, c_uint32]),
     ("hp_compute_flow_2d", c_int, [c_int, FP, FP, FP, FP, FP, c_uint32, HpFlow, FP, FP]),

@@ -38,6 +38,10 @@ struct Launch {
 
 }  // namespace
 
+namespace hp {
+void set_global_error(const std::string& msg) { g_global_error = msg; }
+}  // namespace hp
+
 struct hp_ctx {
     hp_config cfg{};
     std::vector<hp_kernel> kernels;

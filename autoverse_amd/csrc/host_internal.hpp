@@ -34,6 +34,8 @@ struct MemberPlan {
 hp_status plan_member(const hp_config* cfg, bool allow_wave, int group, MemberPlan* out, std::string* err);
 
 hp_status validate_config(const hp_config* cfg, uint32_t* bad_kernel, uint32_t* bad_channel);
+// The message hp_last_global_error returns on this thread (calls that have no context).
+void set_global_error(const std::string& msg);
 uint32_t actual_radius(float radius, uint32_t kernel_radius);
 std::vector<float> build_ring_kernel(const hp_kernel& k, const hp_ring* rings, uint32_t kernel_radius, int dim,
                                      uint32_t* size_out);

@@ -270,6 +270,16 @@ hp_status hp_advect_mass_2d(int device, const float* current, const float* flow_
 hp_status hp_advect_mass_3d(int device, const float* current, const float* flow_x, const float* flow_y,
                             const float* flow_z, float* next, uint32_t width, uint32_t height,
                             uint32_t depth, float dt, float distribution_size);
+/* No counterpart in the reference.  hp_advect_mass_2d for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * `current`, `flow_x`, `flow_y` and `next` are the caller's DEVICE buffers of width*height
+ * floats on `device`; the result is hp_advect_mass_2d's bit for bit.  The per-source boxes and
+ * flags live in a static device scratch of 32 MiB per GPU, so grids above 1,048,576 cells are
+ * refused with HP_ERR_UNSUPPORTED_SIZE.  Calls that overlap on one GPU share that scratch and
+ * race.  Runs on the null stream and synchronises before returning. */
+hp_status hp_advect_mass_2d_noalloc(int device, const float* current, const float* flow_x, const float* flow_y,
+                                    float* next, uint32_t width, uint32_t height, float dt,
+                                    float distribution_size);
 /* sobel_gradient_into (gradient.rs:68-108). */
 hp_status hp_sobel_gradient_2d(int device, const float* grid, float* grad_x, float* grad_y,
                                uint32_t width, uint32_t height);
