@@ -99,6 +99,31 @@ class HpSeed(ctypes.Structure):
                 ("num_values", c_uint32), ("values", POINTER(HpCustomValue))]
 
 
+class HpEvalConfig(ctypes.Structure):
+    _fields_ = [("steps", c_uint64), ("warmup_steps", c_uint64), ("sample_interval", c_uint64)]
+
+
+class HpMetric(ctypes.Structure):
+    _fields_ = [("kind", c_int32), ("weight", c_float), ("period", c_uint64), ("tolerance", c_float),
+                ("min_displacement", c_float), ("max_period", c_uint64), ("threshold", c_float)]
+
+
+class HpBehavior(ctypes.Structure):
+    _fields_ = [("final_mass", c_float), ("initial_mass", c_float), ("total_displacement", c_float),
+                ("final_radius", c_float), ("active_cells", c_uint64), ("max_activation", c_float)]
+
+
+class HpSample(ctypes.Structure):
+    _fields_ = [("center_x", c_float), ("center_y", c_float), ("radius", c_float), ("mass", c_float),
+                ("max_value", c_float), ("active_cells", c_uint32)]
+
+
+# hp_metric_kind, by the serde tag of FitnessMetric (src/schema/evolution.rs:319-375)
+METRIC_KINDS = {
+    "Persistence": 0, "Compactness": 1, "Locomotion": 2, "Periodicity": 3, "Complexity": 4,
+    "MassConcentration": 5, "GliderScore": 6, "OscillatorScore": 7, "Stability": 8, "Custom": 9,
+}
+
 FloatPP = POINTER(POINTER(c_float))
 FP = POINTER(c_float)
 
@@ -148,6 +173,9 @@ EXPORTS = [
     ("hp_batch_set_plan", c_int, [c_void_p, POINTER(HpPlan)]),
     ("hp_batch_autotune", c_int, [c_void_p, c_uint32, POINTER(HpPlan), c_uint32, POINTER(HpPlan),
                                   POINTER(c_double), POINTER(c_uint32)]),
+    ("hp_eval_sample_count", c_int, [POINTER(HpEvalConfig), POINTER(c_uint32)]),
+    ("hp_batch_evaluate", c_int, [c_void_p, POINTER(HpEvalConfig), POINTER(HpMetric), c_uint32, c_int, FP, FP,
+                                  POINTER(HpBehavior), POINTER(HpSample)]),
     ("hp_batch_theta_count", c_int, [c_void_p, c_uint32, POINTER(c_uint32)]),
     ("hp_batch_tape_forward", c_int, [c_void_p, c_uint64, c_uint32, POINTER(c_uint32)]),
     ("hp_batch_tape_backward", c_int, [c_void_p, FloatPP, FloatPP, POINTER(c_double)]),

@@ -40,6 +40,7 @@ SOURCES = [
     ("wave_fft.hip", DEVICE_FLAGS + ["-ffp-contract=fast-honor-pragmas"]),
     ("stencil_kernels.hip", DEVICE_FLAGS + ["-ffp-contract=off"]),
     ("noalloc.hip", DEVICE_FLAGS + ["-ffp-contract=off"]),
+    ("fitness.hip", DEVICE_FLAGS + ["-ffp-contract=off"]),
     ("adjoint_stencil.hip", DEVICE_FLAGS + ["-ffp-contract=off"]),
     ("adjoint_fft.hip", DEVICE_FLAGS + ["-ffp-contract=fast-honor-pragmas"]),
     ("host.cpp", HOST_FLAGS),

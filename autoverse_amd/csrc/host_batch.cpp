@@ -1106,6 +1106,325 @@ hp_status guarded(hp_batch* b, F&& f) {
     }
 }
 
+// ---- fitness evaluation (FitnessEvaluator::evaluate, fitness.rs:26-86) ----------------------
+// Device buffers of one evaluation, freed when it ends.
+struct EvalBuffers {
+    std::vector<void*> p;
+    ~EvalBuffers() {
+        for (void* q : p) (void)hipFree(q);
+    }
+    template <typename T>
+    hp_status get(hp_batch* b, T** out, size_t n) {
+        void* v = nullptr;
+        BTRY(b, hipMalloc(&v, std::max<size_t>(1, n) * sizeof(T)));
+        p.push_back(v);
+        *out = (T*)v;
+        return HP_OK;
+    }
+};
+
+// One sample of a member in the reference's f32 terms (record_sample, fitness.rs:146-160;
+// centre of mass and radius, fitness.rs:196-244), from its FP64 sums.
+struct FitSample {
+    float cx = 0, cy = 0, radius = 0, mass = 0, maxv = 0;
+    uint32_t active = 0;
+    double sumsq = 0;
+};
+
+FitSample fit_sample(const double* st, int W, int H) {
+    FitSample o;
+    o.mass = (float)st[0];
+    const double pos = st[1];
+    if ((float)pos > 1e-6f) {
+        o.cx = (float)(st[2] / pos);
+        o.cy = (float)(st[3] / pos);
+        // second moment about the f32 centre: sum m ((x - cx)^2 + (y - cy)^2) expanded
+        const double cx = o.cx, cy = o.cy;
+        const double mom = st[4] - 2.0 * cx * st[2] - 2.0 * cy * st[3] + (cx * cx + cy * cy) * pos;
+        o.radius = (float)sqrt(std::max(0.0, mom) / pos);
+    } else {
+        o.cx = (float)W / 2.0f;
+        o.cy = (float)H / 2.0f;
+        o.radius = 0.0f;
+    }
+    o.maxv = (float)st[5];
+    o.active = (uint32_t)st[6];
+    o.sumsq = st[8];
+    return o;
+}
+
+float clamp01f(float v) { return v != v ? v : std::min(std::max(v, 0.0f), 1.0f); }
+
+// compute_state_similarity (fitness.rs:353-366) from a dot product and two sums of squares.
+float similarity_of(double dot, double qa, double qb) {
+    const float na = (float)sqrt(qa), nb = (float)sqrt(qb);
+    if (na > 1e-6f && nb > 1e-6f) return (float)(dot / (sqrt(qa) * sqrt(qb)));
+    return 0.0f;
+}
+
+struct MemberEval {
+    int W = 0, H = 0;
+    float initial_mass = 0;
+    std::vector<FitSample> s;    // samples 0 .. S-1
+    std::vector<uint64_t> step;  // their steps
+    const double* fin = nullptr; // final-state sums
+    double init_sumsq = 0;
+    std::map<std::pair<int, int>, double> dot;  // (slot a, slot b) -> dot product; slot 0 = initial
+};
+
+float displacement(const MemberEval& e) {
+    if (e.s.size() < 2) return 0.0f;
+    const float dx = e.s.back().cx - e.s[0].cx, dy = e.s.back().cy - e.s[0].cy;
+    return sqrtf(dx * dx + dy * dy);
+}
+
+float m_persistence(const MemberEval& e) {  // fitness.rs:271-291
+    if (e.initial_mass < 1e-6f) return 0.0f;
+    const float final_mass = e.s.empty() ? 0.0f : e.s.back().mass;
+    const float ratio = final_mass / e.initial_mass;
+    const float fmax_ = e.s.empty() ? 0.0f : e.s.back().maxv;
+    float conc = 0.0f;
+    if (final_mass > 1e-6f) conc = fmax_ / (final_mass / (float)(e.s.empty() ? 1u : e.s.back().active));
+    return clamp01f(ratio * std::min(conc, 10.0f) / 10.0f);
+}
+
+float m_compactness(const MemberEval& e) {  // fitness.rs:294-306
+    const float r = e.s.empty() ? 3.40282347e38f : e.s.back().radius;
+    const float maxr = (float)std::min(e.W, e.H) / 2.0f;
+    return clamp01f(1.0f - r / maxr);
+}
+
+float m_locomotion(const MemberEval& e) {  // fitness.rs:309-326
+    if (e.s.size() < 2) return 0.0f;
+    const float maxd = sqrtf((float)((uint64_t)e.W * e.W + (uint64_t)e.H * e.H));
+    return clamp01f(displacement(e) / maxd);
+}
+
+float m_periodicity(const MemberEval& e, uint64_t period, float tol) {  // fitness.rs:329-350
+    for (size_t j = 0; j < e.s.size(); j++)
+        if (e.step[j] == period) {
+            const float sim = similarity_of(e.dot.at({0, (int)j + 1}), e.init_sumsq, e.s[j].sumsq);
+            if (sim > 1.0f - tol) return sim;
+        }
+    float best = 0.0f;
+    for (size_t j = 0; j < e.s.size(); j++)
+        if (e.step[j] > 0) best = std::max(best, similarity_of(e.dot.at({0, (int)j + 1}), e.init_sumsq, e.s[j].sumsq));
+    return best;
+}
+
+float m_complexity(const MemberEval& e) {  // fitness.rs:369-398
+    const double n = (double)e.W * e.H;
+    const float mean = (float)(e.fin[0] / n);
+    if (mean < 1e-6f) return 0.0f;
+    const double md = e.fin[0] / n;
+    const float var = (float)std::max(0.0, e.fin[8] / n - md * md);
+    const float gmean = (float)(e.fin[9] / (2.0 * n));
+    return std::min((sqrtf(var) + gmean) / 2.0f, 1.0f);
+}
+
+float m_mass_concentration(const MemberEval& e) {  // fitness.rs:401-413
+    const float mean = (float)(e.fin[0] / ((double)e.W * e.H));
+    if (mean < 1e-6f) return 0.0f;
+    return std::min(((float)e.fin[5] / mean) / 100.0f, 1.0f);
+}
+
+float m_glider(const MemberEval& e, float min_disp) {  // fitness.rs:416-453
+    const float loco = m_locomotion(e), comp = m_compactness(e);
+    if (displacement(e) < min_disp) return 0.0f;
+    float cons = 0.5f;
+    if (e.s.size() >= 2) {  // f32 sums in sample order, as the reference's iterators
+        float sum = 0.0f;
+        for (const FitSample& x : e.s) sum += x.radius;
+        const float mean = sum / (float)e.s.size();
+        float v = 0.0f;
+        for (const FitSample& x : e.s) {
+            const float d = x.radius - mean;
+            v += d * d;
+        }
+        v /= (float)e.s.size();
+        const float cv = sqrtf(v) / std::max(mean, 1e-6f);
+        cons = clamp01f(1.0f - cv);
+    }
+    return loco * comp * cons * m_persistence(e);
+}
+
+float m_oscillator(const MemberEval& e, uint64_t max_period, float thr) {  // fitness.rs:456-483
+    float best = 0.0f;
+    for (uint64_t period = 1; period <= max_period; period++)
+        for (size_t i = 0; i < e.s.size(); i++)
+            for (size_t j = i + 1; j < e.s.size(); j++)
+                if (e.step[j] - e.step[i] == period) {
+                    const float sim = similarity_of(e.dot.at({(int)i + 1, (int)j + 1}), e.s[i].sumsq, e.s[j].sumsq);
+                    if (sim > thr) best = std::max(best, sim * (1.0f - (float)period / (float)max_period * 0.5f));
+                }
+    return best;
+}
+
+float m_stability(const MemberEval& e) {  // fitness.rs:486-511
+    const double n = (double)e.W * e.H;
+    const float neg = std::min((float)e.fin[7] / (float)n, 1.0f);
+    const float mx = (float)e.fin[5];
+    const float extreme = mx > 10.0f ? (mx - 10.0f) / mx : 0.0f;
+    const float final_mass = e.s.empty() ? 0.0f : e.s.back().mass;
+    const float err = e.initial_mass > 1e-6f ? fabsf((final_mass - e.initial_mass) / e.initial_mass) : 0.0f;
+    return clamp01f(1.0f - neg - extreme - err);
+}
+
+float metric_score(const hp_metric& mt, const MemberEval& e) {
+    switch (mt.kind) {
+    case HP_METRIC_PERSISTENCE: return m_persistence(e);
+    case HP_METRIC_COMPACTNESS: return m_compactness(e);
+    case HP_METRIC_LOCOMOTION: return m_locomotion(e);
+    case HP_METRIC_PERIODICITY: return m_periodicity(e, mt.period, mt.tolerance);
+    case HP_METRIC_COMPLEXITY: return m_complexity(e);
+    case HP_METRIC_MASS_CONCENTRATION: return m_mass_concentration(e);
+    case HP_METRIC_GLIDER_SCORE: return m_glider(e, mt.min_displacement);
+    case HP_METRIC_OSCILLATOR_SCORE: return m_oscillator(e, mt.max_period, mt.threshold);
+    case HP_METRIC_STABILITY: return m_stability(e);
+    default: return 0.0f;  // Custom: scored outside the evaluator
+    }
+}
+
+hp_status evaluate(hp_batch* b, const hp_eval_config& ev, const hp_metric* metrics, uint32_t nm, bool normalize,
+                   float* fitness, float* scores, hp_behavior* behavior, hp_sample* samples) {
+    for (uint32_t k = 0; k < nm; k++)
+        if (metrics[k].kind < 0 || metrics[k].kind > HP_METRIC_CUSTOM)
+            return b->fail(HP_ERR_SHAPE, "unknown metric kind " + std::to_string(metrics[k].kind));
+    const size_t M = b->members.size();
+    const uint64_t every = std::max<uint64_t>(1, ev.sample_interval);
+    const uint64_t S = 1 + ev.steps / every;
+    if (S > 100000) return b->fail(HP_ERR_SHAPE, "more than 100000 samples");
+    bool need_snap = false;
+    uint64_t max_period = 0;
+    for (uint32_t k = 0; k < nm; k++) {
+        if (metrics[k].kind == HP_METRIC_PERIODICITY) need_snap = true;
+        if (metrics[k].kind == HP_METRIC_OSCILLATOR_SCORE) {
+            need_snap = true;
+            max_period = std::max(max_period, metrics[k].max_period);
+        }
+    }
+    EvalBuffers eb;
+    // work items: per member parts of at least 8192 cells, at most 32
+    std::vector<int4> items;
+    std::vector<int> first(M + 1, 0);
+    for (size_t i = 0; i < M; i++) {
+        first[i] = (int)items.size();
+        const long long n = (long long)b->desc[i].g.W * b->desc[i].g.H;
+        const int parts = (int)std::max<long long>(1, std::min<long long>(32, n / 8192));
+        for (int q = 0; q < parts; q++) items.push_back(make_int4((int)i, q, parts, 0));
+    }
+    first[M] = (int)items.size();
+    int4* d_items = nullptr;
+    int* d_first = nullptr;
+    double *d_part = nullptr, *d_stats = nullptr;
+    const size_t rows = (size_t)S + 2;  // initial, S samples, final
+    STRY(eb.get(b, &d_items, items.size()));
+    STRY(eb.get(b, &d_first, first.size()));
+    STRY(eb.get(b, &d_part, items.size() * hp::kFitStats));
+    STRY(eb.get(b, &d_stats, rows * M * hp::kFitStats));
+    BTRY(b, hipMemcpy(d_items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+    BTRY(b, hipMemcpy(d_first, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice));
+    // snapshot slots: 0 the initial channel sum, 1 .. S the samples
+    float** d_snap_base = nullptr;
+    if (need_snap) {
+        std::vector<float*> base(M);
+        for (size_t i = 0; i < M; i++) {
+            const size_t n = (size_t)b->desc[i].g.W * b->desc[i].g.H;
+            STRY(eb.get(b, &base[i], (S + 1) * n));
+        }
+        STRY(eb.get(b, &d_snap_base, M));
+        BTRY(b, hipMemcpy(d_snap_base, base.data(), M * sizeof(float*), hipMemcpyHostToDevice));
+    }
+    auto sample = [&](size_t row, int slot, bool grad) -> hp_status {
+        hp::launch_fit_stats(b->stream, b->d_desc, d_items, (int)items.size(), b->cur, grad,
+                             slot >= 0 ? d_snap_base : nullptr, slot, d_part);
+        hp::launch_fit_reduce(b->stream, d_first, (int)M, d_part, d_stats + row * M * hp::kFitStats);
+        BTRY(b, hipGetLastError());
+        return HP_OK;
+    };
+    b->tape_recorded = false;
+    STRY(sample(0, need_snap ? 0 : -1, false));  // EvaluationTrajectory::new, before the warmup
+    STRY(advance(b, ev.warmup_steps));
+    STRY(sample(1, need_snap ? 1 : -1, false));
+    uint64_t done = 0;
+    for (uint64_t j = 1; j < S; j++) {
+        STRY(advance(b, every));
+        done += every;
+        STRY(sample(1 + j, need_snap ? (int)(1 + j) : -1, false));
+    }
+    STRY(advance(b, ev.steps - done));
+    STRY(sample(S + 1, -1, true));
+    // similarity dot products: periodicity (initial vs each sample), oscillator (sample pairs
+    // whose steps differ by 1 .. max_period)
+    std::vector<int4> pairs;
+    if (need_snap)
+        for (size_t i = 0; i < M; i++) {
+            for (uint64_t j = 0; j < S; j++) pairs.push_back(make_int4((int)i, 0, (int)(1 + j), 0));
+            for (uint64_t a = 0; a < S; a++)
+                for (uint64_t c = a + 1; c < S && (c - a) * every <= max_period; c++)
+                    pairs.push_back(make_int4((int)i, (int)(1 + a), (int)(1 + c), 0));
+        }
+    std::vector<double> dots(pairs.size());
+    if (!pairs.empty()) {
+        int4* d_pairs = nullptr;
+        double* d_dots = nullptr;
+        STRY(eb.get(b, &d_pairs, pairs.size()));
+        STRY(eb.get(b, &d_dots, pairs.size()));
+        BTRY(b, hipMemcpyAsync(d_pairs, pairs.data(), pairs.size() * sizeof(int4), hipMemcpyHostToDevice, b->stream));
+        hp::launch_fit_dots(b->stream, b->d_desc, d_pairs, (int)pairs.size(), d_snap_base, d_dots);
+        BTRY(b, hipGetLastError());
+        BTRY(b, hipMemcpyAsync(dots.data(), d_dots, dots.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    }
+    std::vector<double> st(rows * M * hp::kFitStats);
+    BTRY(b, hipMemcpyAsync(st.data(), d_stats, st.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    BTRY(b, hipStreamSynchronize(b->stream));
+    std::vector<MemberEval> E(M);
+    for (size_t i = 0; i < M; i++) {
+        MemberEval& e = E[i];
+        e.W = b->desc[i].g.W;
+        e.H = b->desc[i].g.H;
+        const double* r0 = &st[(0 * M + i) * hp::kFitStats];
+        e.initial_mass = (float)r0[0];
+        e.init_sumsq = r0[8];
+        for (uint64_t j = 0; j < S; j++) {
+            e.s.push_back(fit_sample(&st[((1 + j) * M + i) * hp::kFitStats], e.W, e.H));
+            e.step.push_back(j * every);
+        }
+        e.fin = &st[((S + 1) * M + i) * hp::kFitStats];
+    }
+    for (size_t q = 0; q < pairs.size(); q++) E[pairs[q].x].dot[{pairs[q].y, pairs[q].z}] = dots[q];
+    float total_w = 0.0f;
+    for (uint32_t k = 0; k < nm; k++) total_w += metrics[k].weight;
+    for (size_t i = 0; i < M; i++) {
+        const MemberEval& e = E[i];
+        float comb = 0.0f;
+        for (uint32_t k = 0; k < nm; k++) {
+            const float sc = metric_score(metrics[k], e);
+            if (scores) scores[i * nm + k] = sc;
+            comb += (normalize ? clamp01f(sc) : sc) * metrics[k].weight;
+        }
+        if (fitness) fitness[i] = comb / std::max(total_w, 1e-6f);
+        if (behavior) {  // to_behavior_stats (fitness.rs:163-191)
+            hp_behavior& bh = behavior[i];
+            bh.final_mass = e.s.back().mass;
+            bh.initial_mass = e.initial_mass;
+            bh.total_displacement = displacement(e);
+            bh.final_radius = e.s.back().radius;
+            bh.active_cells = e.s.back().active;
+            float mx = 0.0f;
+            for (const FitSample& x : e.s) mx = std::max(mx, x.maxv);
+            bh.max_activation = mx;
+        }
+        if (samples)
+            for (uint64_t j = 0; j < S; j++) {
+                hp_sample& o = samples[i * S + j];
+                const FitSample& x = e.s[j];
+                o = hp_sample{x.cx, x.cy, x.radius, x.mass, x.maxv, x.active};
+            }
+    }
+    return HP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1246,6 +1565,21 @@ hp_status hp_batch_autotune(hp_batch* b, uint32_t steps, hp_plan* best, uint32_t
                             double* us_per_step, uint32_t* n_tried) {
     return guarded(b, [&]() -> hp_status {
         return batch_autotune(b, steps, best, max_results, tried, us_per_step, n_tried);
+    });
+}
+
+hp_status hp_eval_sample_count(const hp_eval_config* ev, uint32_t* count) {
+    if (!ev || !count) return HP_ERR_NULL;
+    *count = (uint32_t)(1 + ev->steps / std::max<uint64_t>(1, ev->sample_interval));
+    return HP_OK;
+}
+
+hp_status hp_batch_evaluate(hp_batch* b, const hp_eval_config* ev, const hp_metric* metrics, uint32_t num_metrics,
+                            int normalize, float* fitness, float* scores, hp_behavior* behavior,
+                            hp_sample* samples) {
+    return guarded(b, [&]() -> hp_status {
+        if (!ev || (num_metrics && !metrics)) return b->fail(HP_ERR_NULL, "evaluation or metric list is NULL");
+        return evaluate(b, *ev, metrics, num_metrics, normalize != 0, fitness, scores, behavior, samples);
     });
 }
 

@@ -169,6 +169,21 @@ void launch_mass_f64(hipStream_t st, const float* A, int C, long long n, double*
 // Clear a small int buffer with a kernel (graph-capture safe, no memset node).
 void launch_clear_ints(hipStream_t st, int* p, int n);
 
+// ---- population fitness statistics: fitness.hip ---------------------------------------------
+// Per (sample, member): sum of the channel sum over all cells; over cells with m > 0 the sums
+// of m, x*m, y*m, (x^2 + y^2)*m; the largest value (fold from 0); cells above 1e-6; cells below
+// 0; the sum of squares; the sum of left and upper neighbour differences (final state only).
+constexpr int kFitStats = 10;
+// items = (member, part, nparts, -); part receives nitems * kFitStats sums.  snap_base (may be
+// null): per member the base of its snapshot slots (W*H floats each), written at `slot`.
+void launch_fit_stats(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cur, bool with_grad,
+                      float* const* snap_base, int slot, double* part);
+// out[m * kFitStats + k]: member m's items first[m] .. first[m + 1] - 1 reduced in order.
+void launch_fit_reduce(hipStream_t st, const int* first, int nmem, const double* part, double* out);
+// pairs = (member, slot a, slot b, -): out[i] = sum over the member's cells of a * b (FP64).
+void launch_fit_dots(hipStream_t st, const MemberDesc* mem, const int4* pairs, int npairs, float* const* snap_base,
+                     double* out);
+
 // ---- gradient tape: adjoint_stencil.hip ----------------------------------------------------
 // items = (member, tile) over adj_tile_w() x adj_tile_h() tiles.
 void launch_adj_flow(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cur, int li);

@@ -175,6 +175,12 @@ class HipPopulation:
             ti += counts[i]
         return g0, th
 
+    def evaluate(self, fitness_config, eval_config):
+        """FitnessEvaluator::evaluate (fitness.rs:26-86) for every member from its current state:
+        per member (combined fitness, [MetricResult], BehaviorStats); see autoverse_amd.fitness."""
+        from .fitness import evaluate_population
+        return evaluate_population(self, fitness_config, eval_config)
+
     def evaluate_displacement(self, seeds: Optional[Sequence[Seed]], warmup: int, steps: int) -> np.ndarray:
         """Centroid displacement of every member over `steps` steps after `warmup` steps
         (FitnessEvaluator::evaluate with the trajectory's total_displacement)."""

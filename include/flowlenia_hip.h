@@ -323,6 +323,77 @@ hp_status hp_batch_centroids(hp_batch* b, double* out_xy, double* mass);
 hp_status hp_batch_mass_f64(hp_batch* b, double* per_member);
 hp_status hp_batch_stats(const hp_batch* b, uint32_t* max_gather_radius, uint64_t* general_path_steps);
 
+/* ---- fitness evaluation (src/compute/evolution/fitness.rs) ------------------------------
+ * hp_batch_evaluate replaces FitnessEvaluator::evaluate (fitness.rs:26-86) for every member at
+ * once: from each member's current state it records the trajectory's initial mass and
+ * snapshot, runs the warmup, samples at step 0 and every sample_interval steps
+ * (EvaluationTrajectory::record_sample, fitness.rs:146-160), runs the remaining steps, then
+ * computes each metric (compute_metric, fitness.rs:247-268) and the weighted combination.  The
+ * sums behind every sample (mass, centre of mass, radius, peak, active cells, similarity dot
+ * products) are FP64 reductions on the device; the metric formulas are the reference's, in f32,
+ * on the host.  The members are left holding their final states. */
+/* EvaluationConfig (src/schema/evolution.rs:479-495); sample_interval 0 is taken as 1. */
+typedef struct hp_eval_config {
+    uint64_t steps;
+    uint64_t warmup_steps;
+    uint64_t sample_interval;
+} hp_eval_config;
+
+/* FitnessMetric (src/schema/evolution.rs:319-375). */
+typedef enum hp_metric_kind {
+    HP_METRIC_PERSISTENCE = 0,
+    HP_METRIC_COMPACTNESS = 1,
+    HP_METRIC_LOCOMOTION = 2,
+    HP_METRIC_PERIODICITY = 3,        /* period, tolerance */
+    HP_METRIC_COMPLEXITY = 4,
+    HP_METRIC_MASS_CONCENTRATION = 5,
+    HP_METRIC_GLIDER_SCORE = 6,       /* min_displacement */
+    HP_METRIC_OSCILLATOR_SCORE = 7,   /* max_period, threshold */
+    HP_METRIC_STABILITY = 8,
+    HP_METRIC_CUSTOM = 9              /* scored outside the evaluator: 0 */
+} hp_metric_kind;
+
+/* WeightedMetric (src/schema/evolution.rs:309-314) with the variant's parameters. */
+typedef struct hp_metric {
+    int32_t kind;
+    float weight;
+    uint64_t period;
+    float tolerance;
+    float min_displacement;
+    uint64_t max_period;
+    float threshold;
+} hp_metric;
+
+/* BehaviorStats (src/schema/evolution.rs:1015-1032) without its two per-sample series, which
+ * hp_sample carries. */
+typedef struct hp_behavior {
+    float final_mass;
+    float initial_mass;
+    float total_displacement;
+    float final_radius;
+    uint64_t active_cells;
+    float max_activation;
+} hp_behavior;
+
+/* One recorded sample of a member's channel sum. */
+typedef struct hp_sample {
+    float center_x;
+    float center_y;
+    float radius;
+    float mass;
+    float max_value;
+    uint32_t active_cells;
+} hp_sample;
+
+/* Samples one evaluation records: 1 + steps / max(sample_interval, 1). */
+hp_status hp_eval_sample_count(const hp_eval_config* ev, uint32_t* count);
+/* fitness[M] receives the combined fitness (FitnessConfig::normalize clamps each score to
+ * [0, 1] before weighting), scores[M * num_metrics] the raw metric scores; behavior[M] and
+ * samples[M * count] may be NULL. */
+hp_status hp_batch_evaluate(hp_batch* b, const hp_eval_config* ev, const hp_metric* metrics, uint32_t num_metrics,
+                            int normalize, float* fitness, float* scores, hp_behavior* behavior,
+                            hp_sample* samples);
+
 /* ---- gradient tape (2D members) ---------------------------------------------------------
  * No counterpart in the reference, which has no derivative.  A tape records T steps of the
  * whole population from its current state (the state advances to A_T, as hp_batch_advance),

@@ -25,3 +25,27 @@ def oracle():
     from oracle import oracle as O
     O.lib()
     return O
+
+
+# Measurement lines the GPU tests print (errors against the f32 and FP64 oracles, mass drift)
+# are repeated in the terminal summary, so that a quiet run (-q, output captured) still states
+# them next to the pass/fail record.
+_MEASURED = []
+
+
+def pytest_runtest_logreport(report):
+    if report.when == "call" and report.capstdout:
+        lines = [l.strip() for l in report.capstdout.splitlines()
+                 if any(k in l for k in ("max|", "drift", "err", "relative"))]
+        if lines:
+            _MEASURED.append((report.nodeid, lines))
+
+
+def pytest_terminal_summary(terminalreporter):
+    if not _MEASURED:
+        return
+    terminalreporter.section("measured errors")
+    for nodeid, lines in _MEASURED:
+        terminalreporter.write_line(nodeid)
+        for l in lines:
+            terminalreporter.write_line("    " + l)
