@@ -26,12 +26,13 @@ HP_ERR_SHAPE = 8
 HP_ERR_NULL = 9
 HP_ERR_NONFINITE = 10
 HP_ERR_UNSUPPORTED_SIZE = 11
+HP_ERR_IO = 12
 
 STATUS_NAMES = {
     0: "HP_OK", 1: "HP_ERR_INVALID_DIMENSIONS", 2: "HP_ERR_INVALID_CHANNELS",
     3: "HP_ERR_INVALID_TIME_STEP", 4: "HP_ERR_INVALID_KERNEL_RADIUS",
     5: "HP_ERR_INVALID_CHANNEL_INDEX", 6: "HP_ERR_NO_DEVICE", 7: "HP_ERR_HIP", 8: "HP_ERR_SHAPE",
-    9: "HP_ERR_NULL", 10: "HP_ERR_NONFINITE", 11: "HP_ERR_UNSUPPORTED_SIZE",
+    9: "HP_ERR_NULL", 10: "HP_ERR_NONFINITE", 11: "HP_ERR_UNSUPPORTED_SIZE", 12: "HP_ERR_IO",
 }
 
 # hp_pattern_kind
@@ -118,6 +119,20 @@ class HpSample(ctypes.Structure):
                 ("max_value", c_float), ("active_cells", c_uint32)]
 
 
+class HpRecorderConfig(ctypes.Structure):
+    _fields_ = [("compression", c_int32), ("frame_skip", c_uint32), ("max_frames", c_uint64)]
+
+
+class HpAnimationStats(ctypes.Structure):
+    _fields_ = [("frame_count", c_uint64), ("total_bytes", c_uint64), ("average_frame_size", c_uint64),
+                ("compression", c_int32)]
+
+
+class HpAnimationInfo(ctypes.Structure):
+    _fields_ = [("width", c_uint32), ("height", c_uint32), ("depth", c_uint32), ("channels", c_uint32),
+                ("frame_count", c_uint64), ("dt", c_float), ("compression", c_int32), ("delta_encoding", c_int32)]
+
+
 # hp_metric_kind, by the serde tag of FitnessMetric (src/schema/evolution.rs:319-375)
 METRIC_KINDS = {
     "Persistence": 0, "Compactness": 1, "Locomotion": 2, "Periodicity": 3, "Complexity": 4,
@@ -176,6 +191,17 @@ EXPORTS = [
     ("hp_eval_sample_count", c_int, [POINTER(HpEvalConfig), POINTER(c_uint32)]),
     ("hp_batch_evaluate", c_int, [c_void_p, POINTER(HpEvalConfig), POINTER(HpMetric), c_uint32, c_int, FP, FP,
                                   POINTER(HpBehavior), POINTER(HpSample)]),
+    ("hp_recorder_open", c_int, [c_char_p, POINTER(HpConfig), POINTER(HpRecorderConfig), POINTER(c_void_p)]),
+    ("hp_recorder_record_ctx", c_int, [c_void_p, c_void_p, POINTER(c_int)]),
+    ("hp_recorder_record_member", c_int, [c_void_p, c_void_p, c_uint32, POINTER(c_int)]),
+    ("hp_recorder_record_host", c_int, [c_void_p, FloatPP, POINTER(c_int)]),
+    ("hp_recorder_finalize", c_int, [c_void_p, POINTER(HpAnimationStats)]),
+    ("hp_animation_last_error", c_char_p, []),
+    ("hp_player_open", c_int, [c_char_p, POINTER(c_void_p)]),
+    ("hp_player_info", c_int, [c_void_p, POINTER(HpAnimationInfo)]),
+    ("hp_player_read_frame", c_int, [c_void_p, c_uint64, FloatPP]),
+    ("hp_player_load_ctx", c_int, [c_void_p, c_uint64, c_void_p, c_float, c_uint64]),
+    ("hp_player_close", None, [c_void_p]),
     ("hp_batch_theta_count", c_int, [c_void_p, c_uint32, POINTER(c_uint32)]),
     ("hp_batch_tape_forward", c_int, [c_void_p, c_uint64, c_uint32, POINTER(c_uint32)]),
     ("hp_batch_tape_backward", c_int, [c_void_p, FloatPP, FloatPP, POINTER(c_double)]),

@@ -46,6 +46,7 @@ SOURCES = [
     ("host.cpp", HOST_FLAGS),
     ("host_seed.cpp", HOST_FLAGS),
     ("host_batch.cpp", HOST_FLAGS),
+    ("host_animation.cpp", HOST_FLAGS),
 ]
 
 

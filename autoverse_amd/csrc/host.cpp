@@ -789,6 +789,23 @@ void hp_destroy(hp_ctx* ctx) {
 const char* hp_last_error(const hp_ctx* ctx) { return ctx ? ctx->err.c_str() : "NULL context"; }
 const char* hp_last_global_error(void) { return g_global_error.c_str(); }
 
+}  // extern "C"
+
+hp_status hp::ctx_device_state(hp_ctx* ctx, hp::DeviceState* out) {
+    if (!ctx || !out) return HP_ERR_NULL;
+    out->data = ctx->A[ctx->cur];
+    out->floats = (size_t)ctx->C * (size_t)ctx->g.N;
+    out->stream = ctx->stream;
+    out->device = ctx->device;
+    out->W = (uint32_t)ctx->g.W;
+    out->H = (uint32_t)ctx->g.H;
+    out->D = (uint32_t)ctx->g.D;
+    out->C = (uint32_t)ctx->C;
+    return HP_OK;
+}
+
+extern "C" {
+
 hp_status hp_get_config(const hp_ctx* ctx, hp_config* out) {
     if (!ctx || !out) return HP_ERR_NULL;
     *out = ctx->cfg;

@@ -1568,6 +1568,25 @@ hp_status hp_batch_autotune(hp_batch* b, uint32_t steps, hp_plan* best, uint32_t
     });
 }
 
+}  // extern "C"
+
+hp_status hp::batch_device_state(hp_batch* b, uint32_t member, hp::DeviceState* out) {
+    if (!b || !out) return HP_ERR_NULL;
+    if (member >= b->members.size()) return b->fail(HP_ERR_SHAPE, "member index out of range");
+    const hp::MemberDesc& m = b->desc[member];
+    out->data = m.A[b->cur];
+    out->floats = (size_t)m.C * (size_t)m.g.N;
+    out->stream = b->stream;
+    out->device = b->device;
+    out->W = (uint32_t)m.g.W;
+    out->H = (uint32_t)m.g.H;
+    out->D = 1;
+    out->C = (uint32_t)m.C;
+    return HP_OK;
+}
+
+extern "C" {
+
 hp_status hp_eval_sample_count(const hp_eval_config* ev, uint32_t* count) {
     if (!ev || !count) return HP_ERR_NULL;
     *count = (uint32_t)(1 + ev->steps / std::max<uint64_t>(1, ev->sample_interval));

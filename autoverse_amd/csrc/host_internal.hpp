@@ -36,6 +36,18 @@ hp_status plan_member(const hp_config* cfg, bool allow_wave, int group, MemberPl
 hp_status validate_config(const hp_config* cfg, uint32_t* bad_kernel, uint32_t* bad_channel);
 // The message hp_last_global_error returns on this thread (calls that have no context).
 void set_global_error(const std::string& msg);
+
+// The device-resident state of a context or of one population member: C*N floats laid out
+// [C][D*H*W] (the order of an FLWA frame), the stream its steps run on, and its shape.
+struct DeviceState {
+    const float* data = nullptr;
+    size_t floats = 0;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    uint32_t W = 0, H = 0, D = 0, C = 0;
+};
+hp_status ctx_device_state(hp_ctx* ctx, DeviceState* out);
+hp_status batch_device_state(hp_batch* b, uint32_t member, DeviceState* out);
 uint32_t actual_radius(float radius, uint32_t kernel_radius);
 std::vector<float> build_ring_kernel(const hp_kernel& k, const hp_ring* rings, uint32_t kernel_radius, int dim,
                                      uint32_t* size_out);

@@ -63,7 +63,9 @@ typedef enum hp_status {
     HP_ERR_SHAPE = 8,                 /* buffer count / size mismatch              */
     HP_ERR_NULL = 9,                  /* a required pointer is NULL                */
     HP_ERR_NONFINITE = 10,            /* the state holds NaN or Inf                */
-    HP_ERR_UNSUPPORTED_SIZE = 11      /* a size outside what the kernels support   */
+    HP_ERR_UNSUPPORTED_SIZE = 11,     /* a size outside what the kernels support   */
+    HP_ERR_IO = 12                    /* a file could not be opened, read or written,
+                                         or is not a valid FLWA animation            */
 } hp_status;
 
 /* RingConfig (config.rs:92-99). */
@@ -418,6 +420,70 @@ hp_status hp_batch_autotune(hp_batch* b, uint32_t steps, hp_plan* best, uint32_t
 hp_status hp_batch_tape_forward(hp_batch* b, uint64_t steps, uint32_t checkpoint_every, uint32_t* used_every);
 hp_status hp_batch_tape_backward(hp_batch* b, const float* const* dl_dfinal, float* const* dl_dinitial,
                                  double* dtheta);
+
+/* ---- Animation files (src/animation/): the FLWA format -----------------------------------
+ * Layout (format.rs:60-185), little-endian: a 52-byte header -- "FLWA", version u16 = 1, flags
+ * u16 (compression in bits 0-3: 0 none, 1 LZ4 with a 4-byte size prefix; bit 4 delta encoding),
+ * width, height, depth, channels (u32 each), frame_count u64, dt f32, 16 zero bytes -- then the
+ * frames, each the state's channels back to back as f32 ([C][D*H*W], the device layout of a
+ * context's state), then one 16-byte index entry (offset u64, size u64) per frame. */
+typedef enum hp_compression { HP_COMPRESSION_NONE = 0, HP_COMPRESSION_LZ4 = 1 } hp_compression;
+
+/* RecorderConfig (recorder.rs:14-33): record every frame_skip-th call (0 taken as 1), at most
+ * max_frames frames (0 = no limit). */
+typedef struct hp_recorder_config {
+    int32_t compression;
+    uint32_t frame_skip;
+    uint64_t max_frames;
+} hp_recorder_config;
+
+/* AnimationStats (recorder.rs:214-224). */
+typedef struct hp_animation_stats {
+    uint64_t frame_count;
+    uint64_t total_bytes;
+    uint64_t average_frame_size;
+    int32_t compression;
+} hp_animation_stats;
+
+/* The header fields (AnimationHeader, format.rs:60-76). */
+typedef struct hp_animation_info {
+    uint32_t width, height, depth, channels;
+    uint64_t frame_count;
+    float dt;
+    int32_t compression;
+    int32_t delta_encoding;
+} hp_animation_info;
+
+typedef struct hp_recorder hp_recorder;
+/* AnimationRecorder::new (recorder.rs:63-114): create `path` and write the placeholder header
+ * for a configuration's grid. */
+hp_status hp_recorder_open(const char* path, const hp_config* cfg, const hp_recorder_config* rc, hp_recorder** out);
+/* AnimationRecorder::record_frame (recorder.rs:119-170).  The _ctx and _member forms take the
+ * frame straight from device memory: one device-to-host copy into a pinned staging buffer,
+ * queued on the context's (population's) stream behind its steps, and written to the file by
+ * a background thread while the device goes on stepping.  *recorded (may be NULL) is 1 when
+ * this call produced a frame (frame_skip / max_frames). */
+hp_status hp_recorder_record_ctx(hp_recorder* r, hp_ctx* ctx, int* recorded);
+hp_status hp_recorder_record_member(hp_recorder* r, hp_batch* b, uint32_t member, int* recorded);
+/* The same from host buffers (one pointer per channel, W*H*D floats each). */
+hp_status hp_recorder_record_host(hp_recorder* r, const float* const* channels, int* recorded);
+/* AnimationRecorder::finalize (recorder.rs:175-205): drain the writer, write the index table,
+ * rewrite the header with the frame count, close.  The recorder is freed (also on error). */
+hp_status hp_recorder_finalize(hp_recorder* r, hp_animation_stats* stats);
+/* Message of the last error of a recorder or player call on this thread. */
+const char* hp_animation_last_error(void);
+
+typedef struct hp_player hp_player;
+/* AnimationPlayer::open (player.rs:36-64): header and index table. */
+hp_status hp_player_open(const char* path, hp_player** out);
+hp_status hp_player_info(const hp_player* p, hp_animation_info* out);
+/* AnimationPlayer::read_frame_into (player.rs:130-180): frame `index` into `channels` host
+ * buffers (one per channel, W*H*D floats each). */
+hp_status hp_player_read_frame(hp_player* p, uint64_t index, float* const* channels);
+/* Resume: frame `index` becomes the device-resident state of a context built for the same grid
+ * and channel count, with the given time and step (hp_set_state semantics). */
+hp_status hp_player_load_ctx(hp_player* p, uint64_t index, hp_ctx* ctx, float time, uint64_t step);
+void hp_player_close(hp_player* p);
 
 #ifdef __cplusplus
 }
