@@ -27,12 +27,14 @@ HP_ERR_NULL = 9
 HP_ERR_NONFINITE = 10
 HP_ERR_UNSUPPORTED_SIZE = 11
 HP_ERR_IO = 12
+HP_ERR_PARSE = 13
 
 STATUS_NAMES = {
     0: "HP_OK", 1: "HP_ERR_INVALID_DIMENSIONS", 2: "HP_ERR_INVALID_CHANNELS",
     3: "HP_ERR_INVALID_TIME_STEP", 4: "HP_ERR_INVALID_KERNEL_RADIUS",
     5: "HP_ERR_INVALID_CHANNEL_INDEX", 6: "HP_ERR_NO_DEVICE", 7: "HP_ERR_HIP", 8: "HP_ERR_SHAPE",
     9: "HP_ERR_NULL", 10: "HP_ERR_NONFINITE", 11: "HP_ERR_UNSUPPORTED_SIZE", 12: "HP_ERR_IO",
+    13: "HP_ERR_PARSE",
 }
 
 # hp_pattern_kind
@@ -119,6 +121,10 @@ class HpSample(ctypes.Structure):
                 ("max_value", c_float), ("active_cells", c_uint32)]
 
 
+class HpEmbedding(ctypes.Structure):
+    _fields_ = [("enabled", c_int32), ("mixing_temperature", c_float), ("linear_mixing", c_int32)]
+
+
 class HpRecorderConfig(ctypes.Structure):
     _fields_ = [("compression", c_int32), ("frame_skip", c_uint32), ("max_frames", c_uint64)]
 
@@ -191,6 +197,15 @@ EXPORTS = [
     ("hp_eval_sample_count", c_int, [POINTER(HpEvalConfig), POINTER(c_uint32)]),
     ("hp_batch_evaluate", c_int, [c_void_p, POINTER(HpEvalConfig), POINTER(HpMetric), c_uint32, c_int, FP, FP,
                                   POINTER(HpBehavior), POINTER(HpSample)]),
+    ("hp_config_from_json", c_int, [c_char_p, POINTER(c_void_p)]),
+    ("hp_seed_from_json", c_int, [c_char_p, POINTER(c_void_p)]),
+    ("hp_config_doc_config", POINTER(HpConfig), [c_void_p]),
+    ("hp_config_doc_embedding", c_int, [c_void_p, POINTER(HpEmbedding)]),
+    ("hp_config_doc_free", None, [c_void_p]),
+    ("hp_seed_doc_seed", POINTER(HpSeed), [c_void_p]),
+    ("hp_seed_doc_free", None, [c_void_p]),
+    ("hp_load_config_and_seed", c_int, [c_char_p, POINTER(c_void_p), POINTER(c_void_p)]),
+    ("hp_json_last_error", c_char_p, []),
     ("hp_recorder_open", c_int, [c_char_p, POINTER(HpConfig), POINTER(HpRecorderConfig), POINTER(c_void_p)]),
     ("hp_recorder_record_ctx", c_int, [c_void_p, c_void_p, POINTER(c_int)]),
     ("hp_recorder_record_member", c_int, [c_void_p, c_void_p, c_uint32, POINTER(c_int)]),

@@ -47,6 +47,7 @@ SOURCES = [
     ("host_seed.cpp", HOST_FLAGS),
     ("host_batch.cpp", HOST_FLAGS),
     ("host_animation.cpp", HOST_FLAGS),
+    ("host_json.cpp", HOST_FLAGS),
 ]
 
 

@@ -251,3 +251,111 @@ class CSeed:
                                 p.major_radius, p.minor_radius, p.amplitude, ch, p.seed,
                                 len(blobs), ctypes.cast(self.blobs, ctypes.POINTER(_abi.HpBlob)),
                                 len(values), ctypes.cast(self.values, ctypes.POINTER(_abi.HpCustomValue)))
+
+
+# ---------------------------------------------------------------------------------------
+# The native JSON loader (hp_config_from_json / hp_seed_from_json / hp_load_config_and_seed):
+# serde's reading of config.rs / seed.rs restated in C++; these helpers turn its C structs back
+# into the Python types above.
+# ---------------------------------------------------------------------------------------
+_KIND_NAMES = {v: k for k, v in _abi.PATTERN_KINDS.items()}
+
+
+def config_from_c(c) -> SimulationConfig:
+    rings = [c.rings[i] for i in range(c.num_rings)]
+    ks = []
+    for i in range(c.num_kernels):
+        k = c.kernels[i]
+        rs = [RingConfig(r.amplitude, r.distance, r.width) for r in rings[k.ring_offset:k.ring_offset + k.ring_count]]
+        ks.append(KernelConfig(k.radius, rs, k.weight, k.mu, k.sigma, k.source_channel, k.target_channel))
+    return SimulationConfig(width=c.width, height=c.height, depth=c.depth, channels=c.channels, dt=c.dt,
+                            kernel_radius=c.kernel_radius, kernels=ks,
+                            flow=FlowConfig(c.flow.beta_a, c.flow.n, c.flow.distribution_size))
+
+
+def seed_from_c(s) -> Seed:
+    t = _KIND_NAMES[s.kind]
+    blobs = [s.blobs[i] for i in range(s.num_blobs)]
+    vals = [s.values[i] for i in range(s.num_values)]
+    if t in ("GaussianBlob", "GaussianSphere"):
+        n = 2 if t == "GaussianBlob" else 3
+        p = Pattern(t, center=tuple(s.center[:n]), radius=s.radius, amplitude=s.amplitude, channel=s.channel)
+    elif t == "MultiBlob":
+        p = Pattern(t, blobs=[BlobSpec(tuple(b.center[:2]), b.radius, b.amplitude, b.channel) for b in blobs])
+    elif t == "MultiSphere":
+        p = Pattern(t, spheres=[SphereSpec(tuple(b.center[:3]), b.radius, b.amplitude, b.channel) for b in blobs])
+    elif t in ("Ring", "Shell"):
+        n = 2 if t == "Ring" else 3
+        p = Pattern(t, center=tuple(s.center[:n]), inner_radius=s.inner_radius, outer_radius=s.outer_radius,
+                    amplitude=s.amplitude, channel=s.channel)
+    elif t == "Torus3D":
+        p = Pattern(t, center=tuple(s.center[:3]), major_radius=s.major_radius, minor_radius=s.minor_radius,
+                    amplitude=s.amplitude, channel=s.channel)
+    elif t == "Noise":
+        p = Pattern(t, amplitude=s.amplitude, channel=None if s.channel < 0 else s.channel, seed=s.noise_seed)
+    elif t == "Custom":
+        p = Pattern(t, values=[(v.x, v.y, v.channel, v.value) for v in vals])
+    else:
+        p = Pattern(t, values=[(v.x, v.y, v.z, v.channel, v.value) for v in vals])
+    return Seed(p)
+
+
+def _native_error(st: int) -> Exception:
+    msg = _abi.load_library().hp_json_last_error().decode()
+    return _abi.HipError(st, msg)
+
+
+def config_from_json_native(text: str) -> SimulationConfig:
+    """serde_json::from_str::<SimulationConfig>, by the native loader."""
+    lib = _abi.load_library()
+    d = ctypes.c_void_p()
+    st = lib.hp_config_from_json(text.encode(), ctypes.byref(d))
+    if st != _abi.HP_OK:
+        raise _native_error(st)
+    try:
+        return config_from_c(lib.hp_config_doc_config(d).contents)
+    finally:
+        lib.hp_config_doc_free(d)
+
+
+def embedding_from_json_native(text: str) -> dict:
+    """The config's EmbeddingConfig (embedding.rs:326-345) as the native loader reads it."""
+    lib = _abi.load_library()
+    d = ctypes.c_void_p()
+    st = lib.hp_config_from_json(text.encode(), ctypes.byref(d))
+    if st != _abi.HP_OK:
+        raise _native_error(st)
+    try:
+        e = _abi.HpEmbedding()
+        lib.hp_config_doc_embedding(d, ctypes.byref(e))
+        return {"enabled": bool(e.enabled), "mixing_temperature": e.mixing_temperature,
+                "linear_mixing": bool(e.linear_mixing)}
+    finally:
+        lib.hp_config_doc_free(d)
+
+
+def seed_from_json_native(text: str) -> Seed:
+    """serde_json::from_str::<Seed>, by the native loader."""
+    lib = _abi.load_library()
+    d = ctypes.c_void_p()
+    st = lib.hp_seed_from_json(text.encode(), ctypes.byref(d))
+    if st != _abi.HP_OK:
+        raise _native_error(st)
+    try:
+        return seed_from_c(lib.hp_seed_doc_seed(d).contents)
+    finally:
+        lib.hp_seed_doc_free(d)
+
+
+def load_config_and_seed(config_path: str):
+    """load_config_and_seed (main.rs:409-436), by the native loader: (SimulationConfig, Seed)."""
+    lib = _abi.load_library()
+    c, s = ctypes.c_void_p(), ctypes.c_void_p()
+    st = lib.hp_load_config_and_seed(str(config_path).encode(), ctypes.byref(c), ctypes.byref(s))
+    if st != _abi.HP_OK:
+        raise _native_error(st)
+    try:
+        return config_from_c(lib.hp_config_doc_config(c).contents), seed_from_c(lib.hp_seed_doc_seed(s).contents)
+    finally:
+        lib.hp_config_doc_free(c)
+        lib.hp_seed_doc_free(s)

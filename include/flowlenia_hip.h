@@ -64,8 +64,9 @@ typedef enum hp_status {
     HP_ERR_NULL = 9,                  /* a required pointer is NULL                */
     HP_ERR_NONFINITE = 10,            /* the state holds NaN or Inf                */
     HP_ERR_UNSUPPORTED_SIZE = 11,     /* a size outside what the kernels support   */
-    HP_ERR_IO = 12                    /* a file could not be opened, read or written,
+    HP_ERR_IO = 12,                   /* a file could not be opened, read or written,
                                          or is not a valid FLWA animation            */
+    HP_ERR_PARSE = 13                 /* JSON text that is not a valid config / seed */
 } hp_status;
 
 /* RingConfig (config.rs:92-99). */
@@ -165,6 +166,37 @@ typedef struct hp_seed {
  * are overwritten (zeroed first).  Pure host code, no device needed. */
 hp_status hp_seed_generate(const hp_seed* seed, uint32_t width, uint32_t height, uint32_t depth,
                            uint32_t channels, float* const* out_channels);
+
+/* ---- JSON files (serde_json's reading of the reference's schema) ------------------------
+ * hp_config_from_json / hp_seed_from_json replace serde_json::from_str::<SimulationConfig> and
+ * ::<Seed> (config.rs:14-120, seed.rs:13-152): the same required fields and defaults (depth 1,
+ * embedding disabled), the "type"-tagged Pattern, tuples as arrays of their exact length,
+ * integers where the reference has usize / u64, unknown fields ignored.  The documents own the
+ * arrays their hp_config / hp_seed point into.  Failures return HP_ERR_PARSE with a message
+ * naming the field (hp_json_last_error). */
+typedef struct hp_config_doc hp_config_doc;
+typedef struct hp_seed_doc hp_seed_doc;
+
+/* EmbeddingConfig (src/schema/embedding.rs:326-345). */
+typedef struct hp_embedding {
+    int32_t enabled;
+    float mixing_temperature;
+    int32_t linear_mixing;
+} hp_embedding;
+
+hp_status hp_config_from_json(const char* text, hp_config_doc** out);
+hp_status hp_seed_from_json(const char* text, hp_seed_doc** out);
+const hp_config* hp_config_doc_config(const hp_config_doc* doc);
+hp_status hp_config_doc_embedding(const hp_config_doc* doc, hp_embedding* out);
+void hp_config_doc_free(hp_config_doc* doc);
+const hp_seed* hp_seed_doc_seed(const hp_seed_doc* doc);
+void hp_seed_doc_free(hp_seed_doc* doc);
+/* load_config_and_seed (main.rs:409-436): the config from config_path; the seed from the same
+ * path with its extension replaced by "seed.json", or Seed::default() (a GaussianBlob at
+ * (0.5, 0.5), radius 0.1, amplitude 1, channel 0) when that file does not exist.  HP_ERR_IO if a
+ * file cannot be read. */
+hp_status hp_load_config_and_seed(const char* config_path, hp_config_doc** config, hp_seed_doc** seed);
+const char* hp_json_last_error(void);
 
 /* ---- Propagator ----------------------------------------------------------- */
 typedef struct hp_ctx hp_ctx;
