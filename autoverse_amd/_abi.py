@@ -164,6 +164,8 @@ EXPORTS = [
     ("hp_advance", c_int, [c_void_p, c_uint64]),
     ("hp_mass_f64", c_int, [c_void_p, POINTER(c_double), POINTER(c_double)]),
     ("hp_stats", c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint64)]),
+    ("hp_set_conservation", c_int, [c_void_p, c_int]),
+    ("hp_get_conservation", c_int, [c_void_p, POINTER(c_int)]),
     ("hp_profile_enable", c_int, [c_void_p, c_int]),
     ("hp_profile_read", c_int, [c_void_p, POINTER(c_double), POINTER(c_uint64)]),
     ("hp_profile_repeats", c_int, [c_void_p, c_uint32]),

@@ -93,6 +93,7 @@ struct hp_ctx {
     hp_plan plan{};        // resolved execution plan
     hp::FftVariant fv;     // LDS FFT variant of the plan
     int tile = 0;          // fused stencil tile shape of the plan
+    bool exact = false;    // exact-conservation reintegration (HP_CONSERVATION_EXACT)
     std::string err;
 
     hp_status fail(hp_status st, const std::string& msg) {
@@ -275,10 +276,10 @@ hp_status enqueue_fast_stencil(hp_ctx* c, const float* A, const float* U, float*
     timed_pass(c, HP_K_STENCIL, true, [&] {  // A, U -> out (and an idempotent flag OR)
         if (c->is3d)
             hp::launch_stencil3d(c->stream, A, U, out, c->C, c->g, c->cfg.dt, c->cfg.flow.beta_a, c->cfg.flow.n,
-                                 c->cfg.flow.distribution_size, c->d_flags);
+                                 c->cfg.flow.distribution_size, c->d_flags, c->exact);
         else
             hp::launch_stencil2d(c->stream, A, U, out, c->C, c->g, c->cfg.dt, c->cfg.flow.beta_a, c->cfg.flow.n,
-                                 c->cfg.flow.distribution_size, c->d_flags, c->tile);
+                                 c->cfg.flow.distribution_size, c->d_flags, c->tile, c->exact);
     });
     HIP_TRY(c, hipGetLastError());
     return HP_OK;
@@ -307,7 +308,8 @@ hp_status general_stencil(hp_ctx* c, const float* A, const float* U, float* out)
                                                     " cells is beyond the general gather's work bound");
     {
         Timed t(c, HP_K_GENERAL);
-        hp::launch_gather_general(c->stream, c->box_lo, c->box_hi, out, c->C, c->g, c->cfg.flow.distribution_size, R);
+        hp::launch_gather_general(c->stream, c->box_lo, c->box_hi, out, c->C, c->g, c->cfg.flow.distribution_size, R,
+                                  c->exact);
     }
     HIP_TRY(c, hipGetLastError());
     c->general_steps++;
@@ -376,6 +378,19 @@ hp_status build_graphs(hp_ctx* c) {
         if (st != HP_OK) return st;
     }
     return HP_OK;
+}
+
+// Switch the reintegration arithmetic; the replay graphs are captured again for it.
+hp_status set_exact(hp_ctx* c, bool exact) {
+    if (c->exact == exact) return HP_OK;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int b = 0; b < 2; b++) {
+        if (c->graph[b]) (void)hipGraphExecDestroy(c->graph[b]);
+        if (c->graph_run[b]) (void)hipGraphExecDestroy(c->graph_run[b]);
+        c->graph[b] = c->graph_run[b] = nullptr;
+    }
+    c->exact = exact;
+    return fast_stencil_ok(c) ? build_graphs(c) : HP_OK;
 }
 
 hp_status advance(hp_ctx* c, uint64_t steps) {
@@ -891,6 +906,20 @@ hp_status hp_mass_f64(hp_ctx* ctx, double* per_channel, double* total) {
     });
 }
 
+hp_status hp_set_conservation(hp_ctx* ctx, int mode) {
+    return guarded(ctx, [&]() -> hp_status {
+        if (mode != HP_CONSERVATION_REFERENCE && mode != HP_CONSERVATION_EXACT)
+            return ctx->fail(HP_ERR_SHAPE, "conservation mode must be HP_CONSERVATION_REFERENCE or _EXACT");
+        return set_exact(ctx, mode == HP_CONSERVATION_EXACT);
+    });
+}
+
+hp_status hp_get_conservation(const hp_ctx* ctx, int* mode) {
+    if (!ctx || !mode) return HP_ERR_NULL;
+    *mode = ctx->exact ? HP_CONSERVATION_EXACT : HP_CONSERVATION_REFERENCE;
+    return HP_OK;
+}
+
 hp_status hp_stats(const hp_ctx* ctx, uint32_t* max_gather_radius, uint64_t* general_path_steps) {
     if (!ctx) return HP_ERR_NULL;
     if (max_gather_radius) *max_gather_radius = ctx->max_radius;
@@ -921,6 +950,14 @@ hp_status set_plan(hp_ctx* c, const hp_plan& p) {
     n->general_steps = c->general_steps;
     n->profiling = c->profiling;
     n->prof_reps = c->prof_reps;
+    if (c->exact) {  // the new plan's graphs were captured for the reference arithmetic
+        st = set_exact(n, true);
+        if (st != HP_OK) {
+            c->err = n->err;
+            release(n);
+            return st;
+        }
+    }
     std::swap(*c, *n);
     release(n);  // the old device plan
     return HP_OK;

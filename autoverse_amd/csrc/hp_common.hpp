@@ -143,17 +143,18 @@ int stencil2d_tile_w(int tile = 0);
 // Per member (sum m, sum x m, sum y m) of the channel sum over cells with m > 0, FP64.
 void launch_batch_moments(hipStream_t st, const MemberDesc* mem, int nmem, int cur, double* out);
 int stencil2d_tile_h(int tile = 0);
+// exact: the exact-conservation mode (FP64 contributions and sums; not the reference's arithmetic)
 bool launch_stencil2d(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g,
-                      float dt, float beta_a, float nexp, float s, int* flag, int tile = 0);
+                      float dt, float beta_a, float nexp, float s, int* flag, int tile = 0, bool exact = false);
 bool stencil3d_supported(const Geo& g, float s);
 bool launch_stencil3d(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g,
-                      float dt, float beta_a, float nexp, float s, int* flag);
+                      float dt, float beta_a, float nexp, float s, int* flag, bool exact = false);
 // General path: per-source destination boxes (+ needed radius via atomicMax)...
 void launch_boxes(hipStream_t st, const float* A, const float* U, int C, const Geo& g, float dt, float beta_a,
                   float nexp, float s, float4* box_lo, float4* box_hi, int* need_radius);
 // ...then a gather over any radius with full periodic-image handling.
 void launch_gather_general(hipStream_t st, const float4* box_lo, const float4* box_hi, float* Aout, int C,
-                           const Geo& g, float s, int radius);
+                           const Geo& g, float s, int radius, bool exact = false);
 // Stage kernels exposed for the stage entry points.
 void launch_sobel2d(hipStream_t st, const float* grid, float* gx, float* gy, int W, int H);
 void launch_flow2d(hipStream_t st, const float* gux, const float* guy, const float* gax, const float* gay,

@@ -101,7 +101,14 @@ constexpr int kStencilTiles = 6;
 // sources in the tile's region passes that test sends no mass into the tile and its outputs are
 // +0, as the reference's.  Such a channel skips its source and gather phases (the test is
 // uniform over the workgroup); a tile with no active channel stores zeros after its loads.
-template <int CC, bool POW2, class TL>
+//
+// CONS (exact-conservation mode, no reference counterpart): each source's mass is scaled once to
+// q = m / (Sx * Sy), Sx and Sy the sums of its overlap widths along x and y (ascending cells, in
+// double), rounded to f32; every destination adds q * (ow * oh) in double, in the same source
+// order, and rounds its total once.  A source's contributions then sum to its mass up to two
+// roundings, and the step conserves mass to FP64-oracle level; the reference's own arithmetic
+// drifts by about 2e-6 per 1000 steps.
+template <int CC, bool POW2, bool CONS, class TL>
 __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, const float* __restrict__ U,
                                                float* __restrict__ Aout, int Cdyn, int W, int H, long long N,
                                                int tiles_x, int tile, float dt, float beta_a, float nexp, float s,
@@ -268,17 +275,28 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
             const float xm = dxp - s, xM = dxp + s, ym = dyp - s, yM = dyp + s;
             // overlaps with destination cells g + 1 - j, j = o + 1 (o = source - dest): cell
             // [c + 1 - j, c + 2 - j)
+            float ovx[4], ovy[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                sOX[j][i] = overlap_clamped(cx + (float)(1 - j), cx + (float)(2 - j), xm, xM);
-                sOY[j][i] = overlap_clamped(cy + (float)(1 - j), cy + (float)(2 - j), ym, yM);
+                ovx[j] = overlap_clamped(cx + (float)(1 - j), cx + (float)(2 - j), xm, xM);
+                ovy[j] = overlap_clamped(cy + (float)(1 - j), cy + (float)(2 - j), ym, yM);
+                sOX[j][i] = ovx[j];
+                sOY[j][i] = ovy[j];
             }
             // With (2s)^2 a power of two the scaling by its inverse is exact (the masses skipped
             // above are the only ones below 1e-10, every nonzero overlap is a multiple of an ulp
             // of s or of a coordinate, so no product is subnormal before the last one): m * ((ow*oh)
             // * inv) equals (m * inv) * (ow*oh) bit for bit, and the gather saves a multiply per
             // pair.  A scaled mass that overflows is sent to the general path below.
-            const float sm = POW2 ? m * inv_ta : m;
+            float sm;
+            if (CONS) {  // widths summed over cells c-2 .. c+1, ascending
+                const double wx = (((0.0 + (double)ovx[3]) + (double)ovx[2]) + (double)ovx[1]) + (double)ovx[0];
+                const double wy = (((0.0 + (double)ovy[3]) + (double)ovy[2]) + (double)ovy[1]) + (double)ovy[0];
+                const double pw = wx * wy;
+                sm = pw > 0.0 ? (float)((double)m / pw) : 0.0f;
+            } else {
+                sm = POW2 ? m * inv_ta : m;
+            }
             sM[i] = sm;
             // every source is checked by the one tile that owns it
             const int ux = x0 - 1 + sx, uy = y0 - 1 + sy;
@@ -286,7 +304,7 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
                 // floor(xm) < c - 2  <=>  xm < c - 2 for integer c (likewise ceil); a non-finite
                 // value fails the comparisons and is classified on the slow side
                 const bool inside = xm >= cx - 2.0f && xM <= cx + 2.0f && ym >= cy - 2.0f && yM <= cy + 2.0f &&
-                                    fabsf(sm) <= 3.40282347e38f;
+                                    (CONS || fabsf(sm) <= 3.40282347e38f);
                 if (!inside)
                     local_flag |= (isfinite(xm) && isfinite(xM) && isfinite(ym) && isfinite(yM) && isfinite(m)) ? 1 : 2;
             }
@@ -309,8 +327,11 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
         // Waves with no periodic edge in reach take the unrotated path, whose LDS addresses are
         // one base plus constants; waves touching an edge visit in rotated order.
         float acc[SR];
+        double accd[CONS ? SR : 1];
 #pragma unroll
         for (int r = 0; r < SR; ++r) acc[r] = 0.0f;
+#pragma unroll
+        for (int r = 0; r < (CONS ? SR : 1); ++r) accd[r] = 0.0;
         if (!live) {
         } else if (__all(rotx == 0 && roty == 0 && gdx < W)) {
             const int base = ty0 * SW + tx;  // source (row 0, column o = -1) of the strip
@@ -324,8 +345,12 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
 #pragma unroll
                     for (int r = 0; r < SR; ++r) {
                         if (a - r >= 0 && a - r <= 3) {
-                            const float area = ow * sOY[a - r][si];
-                            acc[r] += POW2 ? m * area : m * (area / ta);
+                            if (CONS) {
+                                accd[CONS ? r : 0] += (double)m * ((double)ow * (double)sOY[a - r][si]);
+                            } else {
+                                const float area = ow * sOY[a - r][si];
+                                acc[r] += POW2 ? m * area : m * (area / ta);
+                            }
                         }
                     }
                 }
@@ -346,12 +371,20 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
                     for (int r = 0; r < SR; ++r) {
                         const int jy = a - r;  // o_y + 1
                         if (jy >= 0 && jy <= 3) {
-                            const float area = ow * sOY[jy][si];
-                            acc[r] += POW2 ? m * area : m * (area / ta);
+                            if (CONS) {
+                                accd[CONS ? r : 0] += (double)m * ((double)ow * (double)sOY[jy][si]);
+                            } else {
+                                const float area = ow * sOY[jy][si];
+                                acc[r] += POW2 ? m * area : m * (area / ta);
+                            }
                         }
                     }
                 }
             }
+        }
+        if (CONS) {
+#pragma unroll
+            for (int r = 0; r < SR; ++r) acc[r] = (float)accd[CONS ? r : 0];
         }
         if (gdx < W) {
             float* Oc = Aout + (long long)c * N;
@@ -364,14 +397,15 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
     if (local_flag) atomicOr(flag, local_flag);
 }
 
-template <int CC, bool POW2, class TL>
+template <int CC, bool POW2, bool CONS, class TL>
 __global__ __launch_bounds__(TL::NT, TL::WAVES_PER_EU) void k_stencil2d(const float* __restrict__ A, const float* __restrict__ U,
                                                       float* __restrict__ Aout, int Cdyn, int W, int H, long long N,
                                                       int tiles_x, int ntiles, float dt, float beta_a, float nexp,
                                                       float s, float ta, float inv_ta, int* __restrict__ flag) {
     const int tile = xcd_row_tile(blockIdx.x, tiles_x, ntiles / tiles_x);
     if (tile >= ntiles) return;
-    stencil2d_body<CC, POW2, TL>(A, U, Aout, Cdyn, W, H, N, tiles_x, tile, dt, beta_a, nexp, s, ta, inv_ta, flag);
+    stencil2d_body<CC, POW2, CONS, TL>(A, U, Aout, Cdyn, W, H, N, tiles_x, tile, dt, beta_a, nexp, s, ta, inv_ta,
+                                       flag);
 }
 
 // batched: item = (member, tile); every member of one launch has the same channel count class
@@ -383,8 +417,8 @@ __global__ __launch_bounds__(TL::NT, TL::WAVES_PER_EU) void kb_stencil2d(const M
     const MemberDesc& m = mem[it.x];
     const float ta = (2.0f * m.s) * (2.0f * m.s);
     const int tiles_x = (m.g.W + TL::TX - 1) / TL::TX;
-    stencil2d_body<CC, POW2, TL>(m.A[cur], m.U, m.A[1 - cur], m.C, m.g.W, m.g.H, m.g.N, tiles_x, it.y, m.dt,
-                                 m.beta_a, m.nexp, m.s, ta, 1.0f / ta, flag);
+    stencil2d_body<CC, POW2, false, TL>(m.A[cur], m.U, m.A[1 - cur], m.C, m.g.W, m.g.H, m.g.N, tiles_x, it.y, m.dt,
+                                        m.beta_a, m.nexp, m.s, ta, 1.0f / ta, flag);
 }
 
 // HP_TILE binds the run-time tile variant to the compile-time tile type TL of the call.
@@ -455,15 +489,18 @@ int stencil2d_tile_h(int tile) {
 int stencil2d_tile_count() { return kStencilTiles; }
 
 template <int CC, class TL>
-static void launch_st2(hipStream_t st, bool pow2, dim3 grid, const float* A, const float* U, float* Aout, int C,
-                       const Geo& g, int tiles_x, int ntiles, float dt, float beta_a, float nexp, float s,
+static void launch_st2(hipStream_t st, bool pow2, bool cons, dim3 grid, const float* A, const float* U, float* Aout,
+                       int C, const Geo& g, int tiles_x, int ntiles, float dt, float beta_a, float nexp, float s,
                        float ta, float inv_ta, int* flag) {
-    if (pow2)
-        hipLaunchKernelGGL((k_stencil2d<CC, true, TL>), grid, dim3(TL::NT), 0, st, A, U, Aout, C, g.W, g.H, g.N,
-                           tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag);
+    if (cons)
+        hipLaunchKernelGGL((k_stencil2d<CC, false, true, TL>), grid, dim3(TL::NT), 0, st, A, U, Aout, C, g.W, g.H,
+                           g.N, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag);
+    else if (pow2)
+        hipLaunchKernelGGL((k_stencil2d<CC, true, false, TL>), grid, dim3(TL::NT), 0, st, A, U, Aout, C, g.W, g.H,
+                           g.N, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag);
     else
-        hipLaunchKernelGGL((k_stencil2d<CC, false, TL>), grid, dim3(TL::NT), 0, st, A, U, Aout, C, g.W, g.H, g.N,
-                           tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag);
+        hipLaunchKernelGGL((k_stencil2d<CC, false, false, TL>), grid, dim3(TL::NT), 0, st, A, U, Aout, C, g.W, g.H,
+                           g.N, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag);
 }
 
 bool stencil2d_supported(const Geo& g, float s, int tile) {
@@ -478,7 +515,7 @@ bool stencil2d_supported(const Geo& g, float s, int tile) {
 
 template <class TL>
 static void launch_k_st2(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g, float dt,
-                         float beta_a, float nexp, float s, int* flag) {
+                         float beta_a, float nexp, float s, int* flag, bool cons) {
     const float ta = (2.0f * s) * (2.0f * s);
     int e;
     const float fr = frexpf(ta, &e);
@@ -488,18 +525,18 @@ static void launch_k_st2(hipStream_t st, const float* A, const float* U, float* 
     const int ntiles = tiles_x * tiles_y;
     const dim3 grid(xcd_row_grid(tiles_x, ntiles / tiles_x));
     switch (C) {
-    case 1: launch_st2<1, TL>(st, pow2, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
-    case 2: launch_st2<2, TL>(st, pow2, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
-    case 3: launch_st2<3, TL>(st, pow2, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
-    case 4: launch_st2<4, TL>(st, pow2, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
-    default: launch_st2<0, TL>(st, pow2, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag);
+    case 1: launch_st2<1, TL>(st, pow2, cons, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
+    case 2: launch_st2<2, TL>(st, pow2, cons, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
+    case 3: launch_st2<3, TL>(st, pow2, cons, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
+    case 4: launch_st2<4, TL>(st, pow2, cons, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
+    default: launch_st2<0, TL>(st, pow2, cons, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag);
     }
 }
 
 bool launch_stencil2d(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g, float dt,
-                      float beta_a, float nexp, float s, int* flag, int tile) {
+                      float beta_a, float nexp, float s, int* flag, int tile, bool exact) {
     if (!stencil2d_supported(g, s, tile)) return false;
-    HP_TILE(tile, launch_k_st2<TL>(st, A, U, Aout, C, g, dt, beta_a, nexp, s, flag));
+    HP_TILE(tile, launch_k_st2<TL>(st, A, U, Aout, C, g, dt, beta_a, nexp, s, flag, exact));
     return true;
 }
 
@@ -577,7 +614,8 @@ __device__ __forceinline__ int wrap_rotation(int g, int n) {
     return g < 1 ? 1 - g : (g + 2 >= n ? 4 - (g + 2 - (n - 1)) : 0);
 }
 
-template <bool POW2, class TL>
+// CONS: the exact-conservation mode of the 2D kernel, with volumes ((ow * oh) * od) in double.
+template <bool POW2, bool CONS, class TL>
 __global__ __launch_bounds__(TL::NT) void k_stencil3d(const float* __restrict__ A, const float* __restrict__ U,
                                                     float* __restrict__ Aout, int C, int W, int H, int D, int N,
                                                     int tiles_x, int tiles_y, int ntiles, float dt, float beta_a,
@@ -734,18 +772,33 @@ __global__ __launch_bounds__(TL::NT) void k_stencil3d(const float* __restrict__ 
             const float dzp = cz + dt * fz;
             const float xm = dxp - s, xM = dxp + s, ym = dyp - s, yM = dyp + s, zm = dzp - s, zM = dzp + s;
             // overlaps with destination cells g + 1 - j, j = o + 1 (o = source - dest)
+            double wx = 0.0, wy = 0.0, wz = 0.0;  // width sums, cells c-2 .. c+1 ascending (CONS)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                sOX[j * NSRC + i] = overlap_clamped(cx + (float)(1 - j), cx + (float)(2 - j), xm, xM);
-                sOY[j * NSRC + i] = overlap_clamped(cy + (float)(1 - j), cy + (float)(2 - j), ym, yM);
-                sOZ[j * NSRC + i] = overlap_clamped(cz + (float)(1 - j), cz + (float)(2 - j), zm, zM);
+            for (int j = 3; j >= 0; --j) {
+                const float ox = overlap_clamped(cx + (float)(1 - j), cx + (float)(2 - j), xm, xM);
+                const float oy = overlap_clamped(cy + (float)(1 - j), cy + (float)(2 - j), ym, yM);
+                const float oz = overlap_clamped(cz + (float)(1 - j), cz + (float)(2 - j), zm, zM);
+                sOX[j * NSRC + i] = ox;
+                sOY[j * NSRC + i] = oy;
+                sOZ[j * NSRC + i] = oz;
+                if (CONS) {
+                    wx += (double)ox;
+                    wy += (double)oy;
+                    wz += (double)oz;
+                }
             }
             // With (2s)^3 a power of two the scaling by its inverse is exact (the masses skipped
             // above are the only ones below 1e-10, every nonzero overlap is a multiple of an ulp
             // of s or of a coordinate, so no product is subnormal before the last one): m * (vol *
             // inv) equals (m * inv) * vol bit for bit, and the gather saves a multiply per pair.
             // A scaled mass that overflows is sent to the general path below.
-            const float sm = POW2 ? m * inv_tv : m;
+            float sm;
+            if (CONS) {
+                const double pw = (wx * wy) * wz;
+                sm = pw > 0.0 ? (float)((double)m / pw) : 0.0f;
+            } else {
+                sm = POW2 ? m * inv_tv : m;
+            }
             sM[i] = sm;
             // every source is checked by the one tile that owns it
             const int ux = x0 - 1 + sx, uy = y0 - 1 + sy, uz = z0 - 1 + sz;
@@ -754,7 +807,7 @@ __global__ __launch_bounds__(TL::NT) void k_stencil3d(const float* __restrict__ 
                 // floor(xm) < c - 2  <=>  xm < c - 2 for integer c (likewise ceil); a non-finite
                 // value fails the comparisons and is classified on the slow side
                 const bool ok = xm >= cx - 2.0f && xM <= cx + 2.0f && ym >= cy - 2.0f && yM <= cy + 2.0f &&
-                                zm >= cz - 2.0f && zM <= cz + 2.0f && fabsf(sm) <= 3.40282347e38f;
+                                zm >= cz - 2.0f && zM <= cz + 2.0f && (CONS || fabsf(sm) <= 3.40282347e38f);
                 if (!ok)
                     local_flag |= (isfinite(xm) && isfinite(xM) && isfinite(ym) && isfinite(yM) && isfinite(zm) &&
                                    isfinite(zM) && isfinite(m))
@@ -784,8 +837,11 @@ __global__ __launch_bounds__(TL::NT) void k_stencil3d(const float* __restrict__ 
         // Waves with no periodic edge in reach take the unrotated path, whose LDS addresses are
         // one base plus constants; waves touching an edge visit in rotated order.
         float acc[SR];
+        double accd[CONS ? SR : 1];
 #pragma unroll
         for (int r = 0; r < SR; ++r) acc[r] = 0.0f;
+#pragma unroll
+        for (int r = 0; r < (CONS ? SR : 1); ++r) accd[r] = 0.0;
         if (!live) {
         } else if (__all(rx == 0 && ry == 0 && rz == 0 && inside)) {
 #pragma unroll
@@ -796,6 +852,14 @@ __global__ __launch_bounds__(TL::NT) void k_stencil3d(const float* __restrict__ 
                     for (int jx = 0; jx < 4; ++jx) {
                         const int si = base + (a * SY + jy) * SX + jx;
                         const float m = sM[si];
+                        if (CONS) {
+                            const double ad = (double)sOX[jx * NSRC + si] * (double)sOY[jy * NSRC + si];
+#pragma unroll
+                            for (int r = 0; r < SR; ++r)
+                                if (a - r >= 0 && a - r <= 3)
+                                    accd[CONS ? r : 0] += (double)m * (ad * (double)sOZ[(a - r) * NSRC + si]);
+                            continue;
+                        }
                         const float area = sOX[jx * NSRC + si] * sOY[jy * NSRC + si];
 #pragma unroll
                         for (int r = 0; r < SR; ++r) {
@@ -828,6 +892,15 @@ __global__ __launch_bounds__(TL::NT) void k_stencil3d(const float* __restrict__ 
                         const int jx = xo[kx];
                         const int si = rb + jx;
                         const float m = sM[si];
+                        if (CONS) {
+                            const double ad = (double)sOX[jx * NSRC + si] * (double)sOY[jy * NSRC + si];
+#pragma unroll
+                            for (int r = 0; r < SR; ++r) {
+                                const int jz = a - r;
+                                if (jz >= 0 && jz <= 3) accd[CONS ? r : 0] += (double)m * (ad * (double)sOZ[jz * NSRC + si]);
+                            }
+                            continue;
+                        }
                         const float area = sOX[jx * NSRC + si] * sOY[jy * NSRC + si];
 #pragma unroll
                         for (int r = 0; r < SR; ++r) {
@@ -840,6 +913,10 @@ __global__ __launch_bounds__(TL::NT) void k_stencil3d(const float* __restrict__ 
                     }
                 }
             }
+        }
+        if (CONS) {
+#pragma unroll
+            for (int r = 0; r < SR; ++r) acc[r] = (float)accd[CONS ? r : 0];
         }
         if (inside) {
             float* Oc = Aout + (long long)c * N + gdy * W + gdx;
@@ -860,7 +937,7 @@ bool stencil3d_supported(const Geo& g, float s) {
 }
 
 bool launch_stencil3d(hipStream_t st, const float* A, const float* U, float* Aout, int C, const Geo& g, float dt,
-                      float beta_a, float nexp, float s, int* flag) {
+                      float beta_a, float nexp, float s, int* flag, bool exact) {
     using TL = st3::Tile3;
     if (!stencil3d_supported(g, s)) return false;
     const float tv = (2.0f * s) * (2.0f * s) * (2.0f * s);  // reintegration3d.rs:134
@@ -871,11 +948,14 @@ bool launch_stencil3d(hipStream_t st, const float* A, const float* U, float* Aou
     const int ntiles = tiles_x * tiles_y * tiles_z;
     const dim3 grid(xcd_row_grid(tiles_x, ntiles / tiles_x));
     const int N = (int)g.N;
-    if (pow2)
-        hipLaunchKernelGGL((k_stencil3d<true, TL>), grid, dim3(TL::NT), 0, st, A, U, Aout, C, g.W, g.H, g.D, N, tiles_x,
-                           tiles_y, ntiles, dt, beta_a, nexp, s, tv, 1.0f / tv, flag);
+    if (exact)
+        hipLaunchKernelGGL((k_stencil3d<false, true, TL>), grid, dim3(TL::NT), 0, st, A, U, Aout, C, g.W, g.H, g.D, N,
+                           tiles_x, tiles_y, ntiles, dt, beta_a, nexp, s, tv, 1.0f / tv, flag);
+    else if (pow2)
+        hipLaunchKernelGGL((k_stencil3d<true, false, TL>), grid, dim3(TL::NT), 0, st, A, U, Aout, C, g.W, g.H, g.D, N,
+                           tiles_x, tiles_y, ntiles, dt, beta_a, nexp, s, tv, 1.0f / tv, flag);
     else
-        hipLaunchKernelGGL((k_stencil3d<false, TL>), grid, dim3(TL::NT), 0, st, A, U, Aout, C, g.W, g.H, g.D, N,
+        hipLaunchKernelGGL((k_stencil3d<false, false, TL>), grid, dim3(TL::NT), 0, st, A, U, Aout, C, g.W, g.H, g.D, N,
                            tiles_x, tiles_y, ntiles, dt, beta_a, nexp, s, tv, 1.0f / tv, flag);
     return true;
 }
@@ -1040,8 +1120,19 @@ __device__ __forceinline__ int win_at(int i, int d, int R, int n) {
     return i < a ? i : lo + (i - a);
 }
 
+// Sum of a box's overlap widths over its clamped index range [i0, i1] along one axis, ascending,
+// in double (the exact-conservation scale).
+__device__ double width_sum(int i0, int i1, float lo, float hi) {
+    double acc = 0.0;
+    for (int i = i0; i <= i1; ++i) acc += (double)overlap1((float)i, (float)(i + 1), lo, hi);
+    return acc;
+}
+
+// exact: the exact-conservation mode of the fused kernels (q = m / (Sx Sy [Sz]) per source,
+// double contributions and sum, one rounding per destination).
 __global__ void k_gather_general(const float4* __restrict__ box_lo, const float4* __restrict__ box_hi,
-                                 float* __restrict__ Aout, int C, int W, int H, int D, long long N, float s, int R) {
+                                 float* __restrict__ Aout, int C, int W, int H, int D, long long N, float s, int R,
+                                 int exact) {
     const long long total = (long long)C * N;
     const bool is3 = D > 1;
     const float ta = is3 ? (2.0f * s) * (2.0f * s) * (2.0f * s) : (2.0f * s) * (2.0f * s);
@@ -1056,6 +1147,7 @@ __global__ void k_gather_general(const float4* __restrict__ box_lo, const float4
         const float4* hi_c = box_hi + (long long)c * N;
         const int nzw = is3 ? win_count(R, D) : 1, nyw = win_count(R, H), nxw = win_count(R, W);
         float acc = 0.0f;
+        double accd = 0.0;
         for (int a = 0; a < nzw; ++a) {
             const int sz = is3 ? win_at(a, dz, R, D) : 0;
             for (int b = 0; b < nyw; ++b) {
@@ -1071,7 +1163,10 @@ __global__ void k_gather_general(const float4* __restrict__ box_lo, const float4
                         const int nx = wrapi(sat_i32(roundf(lo.x)), W);
                         const int ny = wrapi(sat_i32(roundf(lo.y)), H);
                         const int nz = is3 ? wrapi(sat_i32(roundf(lo.z)), D) : 0;
-                        if (nx == dx && ny == dy && nz == dz) acc += m;
+                        if (nx == dx && ny == dy && nz == dz) {
+                            if (exact) accd += (double)m;
+                            else acc += m;
+                        }
                         continue;
                     }
                     const int ixl = max(sat_i32(floorf(lo.x)), -W), ixh = min(sat_i32(ceilf(hi.x)), 2 * W);
@@ -1080,6 +1175,13 @@ __global__ void k_gather_general(const float4* __restrict__ box_lo, const float4
                     if (is3) {
                         izl = max(sat_i32(floorf(lo.z)), -D);
                         izh = min(sat_i32(ceilf(hi.z)), 2 * D);
+                    }
+                    double q = 0.0;
+                    if (exact) {
+                        double pw = width_sum(ixl, ixh, lo.x, hi.x) * width_sum(iyl, iyh, lo.y, hi.y);
+                        if (is3) pw = pw * width_sum(izl, izh, lo.z, hi.z);
+                        if (!(pw > 0.0)) continue;
+                        q = (double)(float)((double)m / pw);
                     }
                     for (int kz = (is3 ? -1 : 0); kz <= (is3 ? 2 : 0); ++kz) {
                         const int iz = dz + kz * D;
@@ -1094,6 +1196,12 @@ __global__ void k_gather_general(const float4* __restrict__ box_lo, const float4
                                 const int ix = dx + kx * W;
                                 if (ix < ixl || ix > ixh) continue;
                                 const float ow = overlap1((float)ix, (float)(ix + 1), lo.x, hi.x);
+                                if (exact) {
+                                    const double ad = (double)ow * (double)oh;
+                                    const double vd = is3 ? ad * (double)od : ad;
+                                    if (vd > 0.0) accd += q * vd;
+                                    continue;
+                                }
                                 const float vol = is3 ? ow * oh * od : ow * oh;
                                 if (vol > 0.0f) {
                                     const float frac = vol / ta;
@@ -1105,7 +1213,7 @@ __global__ void k_gather_general(const float4* __restrict__ box_lo, const float4
                 }
             }
         }
-        Aout[t] = acc;
+        Aout[t] = exact ? (float)accd : acc;
     }
 }
 
@@ -1125,11 +1233,11 @@ void launch_boxes_from_flow(hipStream_t st, const float* A, const float* fx, con
 }
 
 void launch_gather_general(hipStream_t st, const float4* box_lo, const float4* box_hi, float* Aout, int C,
-                           const Geo& g, float s, int radius) {
+                           const Geo& g, float s, int radius, bool exact) {
     const long long total = (long long)C * g.N;
     const int blocks = (int)std::min<long long>((total + 255) / 256, 65535);
     hipLaunchKernelGGL(k_gather_general, dim3(blocks), dim3(256), 0, st, box_lo, box_hi, Aout, C, g.W, g.H, g.D,
-                       g.N, s, radius);
+                       g.N, s, radius, exact ? 1 : 0);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -199,6 +199,17 @@ class HipPropagator:
         self._check(self._lib.hp_mass_f64(self._ptr, per, ctypes.byref(tot)), "hp_mass_f64")
         return list(per), tot.value
 
+    def set_conservation(self, mode: str) -> None:
+        """'reference' (default: the reference's f32 reintegration, bit for bit) or 'exact' (FP64
+        contributions and sums per destination; mass conserved to FP64-reference level)."""
+        m = {"reference": 0, "exact": 1}[mode]
+        self._check(self._lib.hp_set_conservation(self._ptr, m), "hp_set_conservation")
+
+    def conservation(self) -> str:
+        m = ctypes.c_int(0)
+        self._check(self._lib.hp_get_conservation(self._ptr, ctypes.byref(m)), "hp_get_conservation")
+        return "exact" if m.value else "reference"
+
     def stats(self):
         r, g = ctypes.c_uint32(), ctypes.c_uint64()
         self._check(self._lib.hp_stats(self._ptr, ctypes.byref(r), ctypes.byref(g)), "hp_stats")

@@ -149,6 +149,22 @@ def main() -> int:
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                 "per_kernel": per_kernel}
 
+    # the opt-in exact-conservation reintegration over the same K steps from the same start
+    # (reported beside `value`, never as it): its step time and its FP64 mass drift
+    prop.set_conservation("exact")
+    prop.set_state(start)
+    prop.advance(34)
+    prop.set_state(start)
+    t0 = time.perf_counter()
+    prop.advance(args.steps)
+    t1 = time.perf_counter()
+    _, mass_ex = prop.mass_f64()
+    prop.set_conservation("reference")
+    exact_mode = {"ms_per_step": round((t1 - t0) * 1e3 / args.steps, 5),
+                  "cell_steps_per_s": N * args.steps / (t1 - t0),
+                  "mass_rel_drift": (mass_ex - mass0) / mass0 if mass0 else None,
+                  "steps": args.steps}
+
     # the boundary with host buffers: hp_step and hp_run copy the state in from the caller's
     # (pageable) per-channel arrays and back out on every call.  PCIe-inclusive rates, reported
     # beside `value`, never as it.
@@ -215,6 +231,7 @@ def main() -> int:
         "roofline": roofline,
         "host_boundary": host_boundary,
         "step_model": step_model,
+        "exact_conservation": exact_mode,
         "fp64_error": fp64,
         "cpu_baseline": cpu,
         "population": population,

@@ -75,6 +75,11 @@ int32_t orc_sat_i32_f64(double v);
     void orc_prop_destroy_##S(orc_prop_##S* p);                                                \
     void orc_prop_set_threads_##S(orc_prop_##S* p, int threads);                               \
     void orc_prop_set_line_threads_##S(orc_prop_##S* p, int threads);                          \
+    void orc_prop_set_conserve_##S(orc_prop_##S* p, int conserve);                             \
+    void orc_advect2d_mode_##S(const T* current, const T* fx, const T* fy, T* next, uint32_t w, \
+                               uint32_t h, T dt, T s, int conserve);                           \
+    void orc_advect3d_mode_##S(const T* current, const T* fx, const T* fy, const T* fz, T* next,\
+                               uint32_t w, uint32_t h, uint32_t d, T dt, T s, int conserve);   \
     const T* orc_prop_kernel_freq_##S(const orc_prop_##S* p, uint32_t k);                      \
     void orc_prop_affinity_##S(orc_prop_##S* p, T* const* channels, T* const* affinity);        \
     void orc_prop_stencil_##S(orc_prop_##S* p, T* const* channels, T* const* affinity,          \

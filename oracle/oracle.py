@@ -64,6 +64,9 @@ def lib() -> ctypes.CDLL:
                 f"orc_prop_destroy_{S}": (None, [c_void_p]),
                 f"orc_prop_set_threads_{S}": (None, [c_void_p, c_int]),
                 f"orc_prop_set_line_threads_{S}": (None, [c_void_p, c_int]),
+                f"orc_prop_set_conserve_{S}": (None, [c_void_p, c_int]),
+                f"orc_advect2d_mode_{S}": (None, [FPx, FPx, FPx, FPx, c_uint32, c_uint32, T, T, c_int]),
+                f"orc_advect3d_mode_{S}": (None, [FPx] * 5 + [c_uint32, c_uint32, c_uint32, T, T, c_int]),
                 f"orc_prop_kernel_freq_{S}": (FPx, [c_void_p, c_uint32]),
                 f"orc_prop_affinity_{S}": (None, [c_void_p, PP, PP]),
                 f"orc_prop_stencil_{S}": (None, [c_void_p, PP, PP, PP]),
@@ -203,20 +206,22 @@ def flow2d(gux, guy, gax, gay, mass, beta_a, n):
     return fx, fy
 
 
-def advect2d(current: np.ndarray, fx: np.ndarray, fy: np.ndarray, dt: float, s: float) -> np.ndarray:
-    """advect_mass (reintegration.rs:21-43): scatter into a zeroed grid."""
+def advect2d(current: np.ndarray, fx: np.ndarray, fy: np.ndarray, dt: float, s: float,
+             conserve: bool = False) -> np.ndarray:
+    """advect_mass (reintegration.rs:21-43): scatter into a zeroed grid.  conserve: the
+    exact-conservation variant (each source's mass rescaled by (2s)^2 / its overlap-width product)."""
     h, w = current.shape
     nxt = np.zeros_like(current)
-    getattr(lib(), f"orc_advect2d_{_sfx(current.dtype)}")(_p(np.ascontiguousarray(current)), _p(fx), _p(fy),
-                                                           _p(nxt), w, h, dt, s)
+    getattr(lib(), f"orc_advect2d_mode_{_sfx(current.dtype)}")(_p(np.ascontiguousarray(current)), _p(fx), _p(fy),
+                                                                _p(nxt), w, h, dt, s, int(conserve))
     return nxt
 
 
-def advect3d(current, fx, fy, fz, dt, s):
+def advect3d(current, fx, fy, fz, dt, s, conserve: bool = False):
     d, h, w = current.shape
     nxt = np.zeros_like(current)
-    getattr(lib(), f"orc_advect3d_{_sfx(current.dtype)}")(_p(np.ascontiguousarray(current)), _p(fx), _p(fy),
-                                                           _p(fz), _p(nxt), w, h, d, dt, s)
+    getattr(lib(), f"orc_advect3d_mode_{_sfx(current.dtype)}")(_p(np.ascontiguousarray(current)), _p(fx), _p(fy),
+                                                                _p(fz), _p(nxt), w, h, d, dt, s, int(conserve))
     return nxt
 
 
@@ -271,7 +276,8 @@ class Propagator:
     """CpuPropagator / CpuPropagator3D restated (propagator.rs:124-427), f32 or f64.
     `theta` (float64, layout of theta_of) overrides the config's kernel and ring values."""
 
-    def __init__(self, cfg, dtype=np.float32, threads: int = 1, theta=None, line_threads: int = 1):
+    def __init__(self, cfg, dtype=np.float32, threads: int = 1, theta=None, line_threads: int = 1,
+                 conserve: bool = False):
         self.cfg = cfg
         self.dtype = np.dtype(dtype)
         self.S = _sfx(dtype)
@@ -287,6 +293,8 @@ class Propagator:
         self.n = cfg.width * cfg.height * cfg.depth
         if line_threads > 1:
             self.set_line_threads(line_threads)
+        if conserve:
+            getattr(lib(), f"orc_prop_set_conserve_{self.S}")(self.ptr, 1)
 
     def close(self):
         if self.ptr:
