@@ -125,6 +125,10 @@ class HpEmbedding(ctypes.Structure):
     _fields_ = [("enabled", c_int32), ("mixing_temperature", c_float), ("linear_mixing", c_int32)]
 
 
+class HpCellParams(ctypes.Structure):
+    _fields_ = [("mu", c_float), ("sigma", c_float), ("weight", c_float), ("beta_a", c_float), ("n", c_float)]
+
+
 class HpRecorderConfig(ctypes.Structure):
     _fields_ = [("compression", c_int32), ("frame_skip", c_uint32), ("max_frames", c_uint64)]
 
@@ -219,6 +223,24 @@ EXPORTS = [
     ("hp_player_read_frame", c_int, [c_void_p, c_uint64, FloatPP]),
     ("hp_player_load_ctx", c_int, [c_void_p, c_uint64, c_void_p, c_float, c_uint64]),
     ("hp_player_close", None, [c_void_p]),
+    # parameter grids travel as one float pointer per channel (W*H cells of 5 floats)
+    ("hp_emb_create", c_int, [POINTER(HpConfig), POINTER(HpEmbedding), c_int, POINTER(c_void_p)]),
+    ("hp_emb_destroy", None, [c_void_p]),
+    ("hp_emb_last_error", c_char_p, [c_void_p]),
+    ("hp_emb_default_params", c_int, [POINTER(HpConfig), POINTER(HpCellParams)]),
+    ("hp_emb_set_state", c_int, [c_void_p, FloatPP, FloatPP, c_float, c_uint64]),
+    ("hp_emb_get_state", c_int, [c_void_p, FloatPP, FloatPP, POINTER(c_float), POINTER(c_uint64)]),
+    ("hp_emb_step", c_int, [c_void_p, FloatPP, FloatPP, POINTER(c_float), POINTER(c_uint64)]),
+    ("hp_emb_run", c_int, [c_void_p, FloatPP, FloatPP, POINTER(c_float), POINTER(c_uint64), c_uint64]),
+    ("hp_emb_advance", c_int, [c_void_p, c_uint64]),
+    ("hp_emb_mass_f64", c_int, [c_void_p, POINTER(c_double), POINTER(c_double)]),
+    ("hp_emb_get_next_params", c_int, [c_void_p, FloatPP]),
+    ("hp_emb_set_next_params", c_int, [c_void_p, FloatPP]),
+    ("hp_emb_last_radius", c_int, [c_void_p, POINTER(c_int32)]),
+    ("hp_emb_affinity", c_int, [c_void_p, FloatPP, FloatPP, FloatPP]),
+    ("hp_emb_stencil", c_int, [c_void_p, FloatPP, FloatPP, FloatPP, FloatPP, FloatPP]),
+    ("hp_advect_mass_and_params", c_int, [FP, FP, FP, FP, POINTER(HpEmbedding), c_float, c_float, c_uint32,
+                                          c_uint32, FP, FP, c_int]),
     ("hp_batch_theta_count", c_int, [c_void_p, c_uint32, POINTER(c_uint32)]),
     ("hp_batch_tape_forward", c_int, [c_void_p, c_uint64, c_uint32, POINTER(c_uint32)]),
     ("hp_batch_tape_backward", c_int, [c_void_p, FloatPP, FloatPP, POINTER(c_double)]),

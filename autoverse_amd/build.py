@@ -3,7 +3,8 @@
     python -m autoverse_amd.build
 
 Per-file floating-point flags:
-  * stencil_kernels.hip, host*.cpp: -ffp-contract=off -- these reproduce the
+  * stencil_kernels.hip, embedded.hip, fitness.hip, noalloc.hip, adjoint_stencil.hip,
+    host*.cpp: -ffp-contract=off -- these reproduce the
     reference's f32 arithmetic operation for operation (Rust never fuses a*b+c);
   * fft_*.hip, wave_fft.hip, adjoint_fft.hip: -ffp-contract=fast-honor-pragmas -- the FFT is not
     bitwise anyway, FMAs make it faster and more accurate; the LDS path's growth epilogue
@@ -41,6 +42,7 @@ SOURCES = [
     ("stencil_kernels.hip", DEVICE_FLAGS + ["-ffp-contract=off"]),
     ("noalloc.hip", DEVICE_FLAGS + ["-ffp-contract=off"]),
     ("fitness.hip", DEVICE_FLAGS + ["-ffp-contract=off"]),
+    ("embedded.hip", DEVICE_FLAGS + ["-ffp-contract=off"]),
     ("adjoint_stencil.hip", DEVICE_FLAGS + ["-ffp-contract=off"]),
     ("adjoint_fft.hip", DEVICE_FLAGS + ["-ffp-contract=fast-honor-pragmas"]),
     ("host.cpp", HOST_FLAGS),
@@ -48,6 +50,7 @@ SOURCES = [
     ("host_batch.cpp", HOST_FLAGS),
     ("host_animation.cpp", HOST_FLAGS),
     ("host_json.cpp", HOST_FLAGS),
+    ("host_embedded.cpp", HOST_FLAGS),
 ]
 
 

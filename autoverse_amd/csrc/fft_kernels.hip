@@ -13,6 +13,8 @@ namespace hp {
                              const int*, const Geo&, const float2*, const FftVariant&);                          \
     bool launch_rows_inv_##G(int, hipStream_t, const float2*, float*, const TgtWork*, int, const int*,            \
                              const GrowthParams*, const Geo&, const float2*, const FftVariant&);                 \
+    bool launch_rows_inv_conv_##G(int, hipStream_t, const float2*, float*, const TgtWork*, int, const int*,       \
+                                  const Geo&, const float2*, const FftVariant&);                                \
     bool launch_batch_rows_fwd_##G(int, hipStream_t, const MemberDesc*, const int4*, int, int, const FftVariant&); \
     bool launch_batch_spectral_##G(int, hipStream_t, const MemberDesc*, const int4*, int, const FftVariant&);    \
     bool launch_batch_rows_inv_##G(int, hipStream_t, const MemberDesc*, const int4*, int, bool, int,             \
@@ -62,6 +64,13 @@ bool launch_rows_inv(int L, hipStream_t st, const float2* Y, float* U, const Tgt
                      const int* klist, const GrowthParams* gp, const Geo& g, const float2* tw, const FftVariant& v) {
 #define ARGS (L, st, Y, U, work, nwork, klist, gp, g, tw, v)
     return HP_ANY(launch_rows_inv);
+#undef ARGS
+}
+
+bool launch_rows_inv_conv(int L, hipStream_t st, const float2* Y, float* conv, const TgtWork* work, int nwork,
+                          const int* klist, const Geo& g, const float2* tw, const FftVariant& v) {
+#define ARGS (L, st, Y, conv, work, nwork, klist, g, tw, v)
+    return HP_ANY(launch_rows_inv_conv);
 #undef ARGS
 }
 

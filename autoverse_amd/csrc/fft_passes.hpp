@@ -211,7 +211,8 @@ __device__ __forceinline__ float growth_term(float c, const GrowthParams& p) {
 }
 
 // CONV: also store every kernel's convolution rows (conv + k*N), for the adjoint's recompute.
-template <int L, int NT, bool CONV, int RP>
+// GROW false: the convolutions only (no growth, no U), for growth that reads per-cell values.
+template <int L, int NT, bool CONV, int RP, bool GROW = true>
 __device__ __forceinline__ void rows_inv_body(const float2* __restrict__ Y, float* __restrict__ U,
                                               const TgtWork* __restrict__ work, const int* __restrict__ klist,
                                               const GrowthParams* __restrict__ gp, int rows, int Wp, long long N,
@@ -248,7 +249,8 @@ __device__ __forceinline__ void rows_inv_body(const float2* __restrict__ Y, floa
         }
         __syncthreads();
         lds_fft<L, +1, 1, NT, L, RP>(buf, tw);
-        const GrowthParams p = gp[k];
+        GrowthParams p{};
+        if (GROW) p = gp[k];
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int m = threadIdx.x + q * NT;
@@ -263,12 +265,15 @@ __device__ __forceinline__ void rows_inv_body(const float2* __restrict__ Y, floa
                         cv[0] = c0;
                         if (has1) cv[L] = c1;
                     }
-                    acc0[q] += growth_term(c0, p);
-                    acc1[q] += growth_term(c1, p);
+                    if (GROW) {
+                        acc0[q] += growth_term(c0, p);
+                        acc1[q] += growth_term(c1, p);
+                    }
                 }
             }
         }
     }
+    if (!GROW) return;
     float* u0 = U + (long long)wk.target * N + (long long)y0 * L;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -287,6 +292,16 @@ __global__ __launch_bounds__(NT) void k_rows_inv(const float2* __restrict__ Y, f
                                                  long long N, long long spec, float scale,
                                                  const float2* __restrict__ twg) {
     rows_inv_body<L, NT, false, RP>(Y, U, work, klist, gp, rows, Wp, N, spec, scale, twg, blockIdx.x, blockIdx.y);
+}
+
+// every kernel's convolution into conv + k*N, no growth (grid and arguments as k_rows_inv)
+template <int L, int NT, int RP>
+__global__ __launch_bounds__(NT) void k_rows_inv_conv(const float2* __restrict__ Y, float* __restrict__ conv,
+                                                      const TgtWork* __restrict__ work, const int* __restrict__ klist,
+                                                      int rows, int Wp, long long N, long long spec, float scale,
+                                                      const float2* __restrict__ twg) {
+    rows_inv_body<L, NT, true, RP, false>(Y, nullptr, work, klist, nullptr, rows, Wp, N, spec, scale, twg, blockIdx.x,
+                                          blockIdx.y, conv);
 }
 
 // batched: item = (member, row pair, target work item); CONV: convolutions into conv slot cslot
@@ -370,6 +385,16 @@ inline void rows_inv(hipStream_t st, const float2* Y, float* U, const TgtWork* w
     hipLaunchKernelGGL((k_rows_inv<L, Cfg<L>::ROW_NT, RP>), grid, dim3(Cfg<L>::ROW_NT), 0, st, Y, U, work, klist, gp,
                        rows, g.Wp, g.N, g.spec, scale, tw);
 }
+
+template <int L, int RP>
+inline void rows_inv_conv(hipStream_t st, const float2* Y, float* conv, const TgtWork* work, int nwork,
+                          const int* klist, const Geo& g, const float2* tw) {
+    const int rows = g.H * g.D;
+    dim3 grid((rows + 1) / 2, nwork);
+    const float scale = 1.0f / (float)(g.N);
+    hipLaunchKernelGGL((k_rows_inv_conv<L, Cfg<L>::ROW_NT, RP>), grid, dim3(Cfg<L>::ROW_NT), 0, st, Y, conv, work,
+                       klist, rows, g.Wp, g.N, g.spec, scale, tw);
+}
 }  // namespace detail
 
 // Launchers for the lengths of one group G (SIZES(X) lists them); each returns false for a
@@ -397,6 +422,12 @@ inline void rows_inv(hipStream_t st, const float2* Y, float* U, const TgtWork* w
         SIZES(HP_X_ROWS_INV)                                                                                    \
         return false;                                                                                           \
     }                                                                                                           \
+    bool launch_rows_inv_conv_##G(int L, hipStream_t st, const float2* Y, float* conv, const TgtWork* work,    \
+                                  int nwork, const int* klist, const Geo& g, const float2* tw,                 \
+                                  const FftVariant& v) {                                                       \
+        SIZES(HP_X_ROWS_INV_CONV)                                                                               \
+        return false;                                                                                           \
+    }                                                                                                           \
     bool launch_batch_rows_fwd_##G(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, \
                                    int cur, const FftVariant& v) {                                              \
         SIZES(HP_X_B_ROWS_FWD)                                                                                  \
@@ -417,6 +448,7 @@ inline void rows_inv(hipStream_t st, const float2* Y, float* U, const TgtWork* w
 #define HP_X_COLS(n) if (L == n) { HP_RP(v, HP_CS(v, detail::cols<n, RP, CS>(st, dir, X, nspec, g, tw))); return true; }
 #define HP_X_SPECTRAL(n) if (L == n) { HP_RP(v, HP_CS(v, detail::spectral<n, RP, CS>(dim, st, X, Y, Kq, work, nwork, klist, g, tw))); return true; }
 #define HP_X_ROWS_INV(n) if (L == n) { HP_RP(v, detail::rows_inv<n, RP>(st, Y, U, work, nwork, klist, gp, g, tw)); return true; }
+#define HP_X_ROWS_INV_CONV(n) if (L == n) { HP_RP(v, detail::rows_inv_conv<n, RP>(st, Y, conv, work, nwork, klist, g, tw)); return true; }
 #define HP_X_B_ROWS_FWD(n) if (L == n) { HP_RP(v, hipLaunchKernelGGL((kb_rows_fwd<n, Cfg<n>::ROW_NT, RP>), dim3(nitems), dim3(Cfg<n>::ROW_NT), 0, st, mem, items, cur)); return true; }
 #define HP_X_B_SPECTRAL(n) if (L == n) { HP_RP(v, HP_CS(v, hipLaunchKernelGGL((kb_spectral<n, Cfg<n, CS>::CB, Cfg<n, CS>::COL_NT, RP>), dim3(nitems), dim3(Cfg<n, CS>::COL_NT), 0, st, mem, items))); return true; }
 #define HP_X_B_ROWS_INV(n) if (L == n) { HP_RP(v, if (conv) hipLaunchKernelGGL((kb_rows_inv<n, Cfg<n>::ROW_NT, true, RP>), dim3(nitems), dim3(Cfg<n>::ROW_NT), 0, st, mem, items, cslot); \

@@ -819,6 +819,47 @@ hp_status hp::ctx_device_state(hp_ctx* ctx, hp::DeviceState* out) {
     return HP_OK;
 }
 
+hp_status hp::ctx_create_lds(const hp_config* cfg, int device, hp_ctx** out) {
+    if (!cfg || !out) return HP_ERR_NULL;
+    hp_plan p = default_plan(cfg);
+    p.fft_path = 1;
+    return create(cfg, device, &p, out);
+}
+
+hp_status hp::ctx_view(hp_ctx* ctx, hp::CtxView* out) {
+    if (!ctx || !out) return HP_ERR_NULL;
+    out->stream = ctx->stream;
+    out->device = ctx->device;
+    out->g = ctx->g;
+    out->C = ctx->C;
+    out->K = ctx->K;
+    out->A[0] = ctx->A[0];
+    out->A[1] = ctx->A[1];
+    out->cur = &ctx->cur;
+    out->time = &ctx->time;
+    out->step = &ctx->step;
+    out->box_lo = ctx->box_lo;
+    out->box_hi = ctx->box_hi;
+    out->snapshot = ctx->snapshot;
+    return HP_OK;
+}
+
+hp_status hp::ctx_enqueue_conv(hp_ctx* c, const float* A, float* conv) {
+    if (!c || !A || !conv) return HP_ERR_NULL;
+    if (c->wave || c->is3d) return c->fail(HP_ERR_UNSUPPORTED_SIZE, "convolutions need a 2D context on the LDS path");
+    if (c->K == 0) return HP_OK;
+    const hp::Geo& g = c->g;
+    hp::launch_rows_fwd(g.W, c->stream, A, c->X, c->d_src_chan, (int)c->src_chan.size(), g, c->tw[g.W], c->fv);
+    hp::launch_spectral(g.H, 2, c->stream, c->X, c->Y, c->Kq, c->d_spec_work, c->n_spec_work, c->d_klist_spec, g,
+                        c->tw[g.H], c->fv);
+    hp::launch_rows_inv_conv(g.W, c->stream, c->Y, conv, c->d_tgt_work, c->n_tgt_work, c->d_klist_tgt, g, c->tw[g.W],
+                             c->fv);
+    HIP_TRY(c, hipGetLastError());
+    return HP_OK;
+}
+
+const char* hp::ctx_error(const hp_ctx* ctx) { return ctx ? ctx->err.c_str() : ""; }
+
 extern "C" {
 
 hp_status hp_get_config(const hp_ctx* ctx, hp_config* out) {

@@ -47,6 +47,30 @@ struct DeviceState {
     uint32_t W = 0, H = 0, D = 0, C = 0;
 };
 hp_status ctx_device_state(hp_ctx* ctx, DeviceState* out);
+
+// A context's device buffers and step bookkeeping, for a caller that drives its own step on
+// them (the embedded-parameter context): the state ping-pong A[*cur], the general-path boxes,
+// the snapshot buffer, and the time and step the context reports.
+struct CtxView {
+    hipStream_t stream = nullptr;
+    int device = 0;
+    Geo g{};
+    int C = 0, K = 0;
+    float* A[2] = {nullptr, nullptr};
+    int* cur = nullptr;
+    float* time = nullptr;
+    uint64_t* step = nullptr;
+    float4* box_lo = nullptr;
+    float4* box_hi = nullptr;
+    float* snapshot = nullptr;
+};
+// A context whose convolutions run on the LDS FFT passes (the path ctx_enqueue_conv needs).
+hp_status ctx_create_lds(const hp_config* cfg, int device, hp_ctx** out);
+hp_status ctx_view(hp_ctx* ctx, CtxView* out);
+// Every kernel's convolution of its source channel of A (2D) into conv + k*N (no growth).
+hp_status ctx_enqueue_conv(hp_ctx* ctx, const float* A, float* conv);
+// Message of the context's last error (also what hp_last_error returns).
+const char* ctx_error(const hp_ctx* ctx);
 hp_status batch_device_state(hp_batch* b, uint32_t member, DeviceState* out);
 uint32_t actual_radius(float radius, uint32_t kernel_radius);
 std::vector<float> build_ring_kernel(const hp_kernel& k, const hp_ring* rings, uint32_t kernel_radius, int dim,

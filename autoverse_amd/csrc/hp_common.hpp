@@ -102,6 +102,9 @@ bool launch_spectral(int L, int dim, hipStream_t st, const float2* X, float2* Y,
 bool launch_rows_inv(int L, hipStream_t st, const float2* Y, float* U, const TgtWork* work, int nwork,
                      const int* klist, const GrowthParams* gp, const Geo& g, const float2* tw,
                      const FftVariant& v = FftVariant());
+// The same inverse pass writing every kernel's convolution to conv + k*N, with no growth.
+bool launch_rows_inv_conv(int L, hipStream_t st, const float2* Y, float* conv, const TgtWork* work, int nwork,
+                          const int* klist, const Geo& g, const float2* tw, const FftVariant& v = FftVariant());
 
 // Batched 2D passes over a size class: items[i] = (member, block, slot/work, -).
 bool launch_batch_rows_fwd(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cur,
@@ -152,9 +155,12 @@ bool launch_stencil3d(hipStream_t st, const float* A, const float* U, float* Aou
 // General path: per-source destination boxes (+ needed radius via atomicMax)...
 void launch_boxes(hipStream_t st, const float* A, const float* U, int C, const Geo& g, float dt, float beta_a,
                   float nexp, float s, float4* box_lo, float4* box_hi, int* need_radius);
-// ...then a gather over any radius with full periodic-image handling.
+// ...then a gather over any radius with full periodic-image handling.  With rdev the radius is
+// read on the device from rdev[0]; a flag in rdev[1] or a radius above rmax (which sets bit 2
+// of rdev[1]) leaves Aout unwritten.
 void launch_gather_general(hipStream_t st, const float4* box_lo, const float4* box_hi, float* Aout, int C,
-                           const Geo& g, float s, int radius, bool exact = false);
+                           const Geo& g, float s, int radius, bool exact = false, int* rdev = nullptr,
+                           int rmax = 0);
 // Stage kernels exposed for the stage entry points.
 void launch_sobel2d(hipStream_t st, const float* grid, float* gx, float* gy, int W, int H);
 void launch_flow2d(hipStream_t st, const float* gux, const float* guy, const float* gax, const float* gay,
@@ -169,6 +175,23 @@ void launch_mass_f64(hipStream_t st, const float* A, int C, long long n, double*
                      double* out);
 // Clear a small int buffer with a kernel (graph-capture safe, no memset node).
 void launch_clear_ints(hipStream_t st, int* p, int n);
+
+// ---- embedded-parameter step: embedded.hip ------------------------------------------------
+// Parameter planes P[c][f][N], f = mu, sigma, weight, beta_a, n; AoS = 5 floats per cell.
+void launch_emb_split(hipStream_t st, const float* aos, float* planes, int C, long long N);
+void launch_emb_join(hipStream_t st, const float* planes, float* aos, int C, long long N);
+// U[t] from the convolutions conv[k] (K x N) and the targets' per-cell growth parameters.
+void launch_emb_growth(hipStream_t st, const float* conv, const float* P, const int* ktarget, float* U, int C, int K,
+                       long long N);
+// F[c][2][N] and the per-channel largest |F| as float bits (maxbits cleared to 0 first).
+void launch_emb_flow(hipStream_t st, const float* A, const float* U, const float* P, float* F, unsigned* maxbits,
+                     int C, int W, int H);
+// The largest |F| per channel of given flows F[c][2][N] (maxbits cleared to 0 first).
+void launch_emb_maxmag(hipStream_t st, const float* F, unsigned* maxbits, int C, long long N);
+// Gather with parameter mixing; a radius of rmax or more sets err[0] bit 0 and writes nothing.
+void launch_emb_advect(hipStream_t st, const float* A, const float* P, const float* F, const unsigned* maxbits,
+                       float* Aout, float* Pout, int C, int W, int H, float dt, float s, bool linear,
+                       float temperature, int rmax, int* err);
 
 // ---- population fitness statistics: fitness.hip ---------------------------------------------
 // Per (sample, member): sum of the channel sum over all cells; over cells with m > 0 the sums

@@ -1130,9 +1130,20 @@ __device__ double width_sum(int i0, int i1, float lo, float hi) {
 
 // exact: the exact-conservation mode of the fused kernels (q = m / (Sx Sy [Sz]) per source,
 // double contributions and sum, one rounding per destination).
+// rdev (may be null): the radius is read on the device from rdev[0] (the boxes' atomicMax);
+// nothing is written when rdev[1] holds a flag, or when the radius exceeds rmax (then bit 2 of
+// rdev[1] is set for the host to see).
 __global__ void k_gather_general(const float4* __restrict__ box_lo, const float4* __restrict__ box_hi,
                                  float* __restrict__ Aout, int C, int W, int H, int D, long long N, float s, int R,
-                                 int exact) {
+                                 int exact, int* __restrict__ rdev, int rmax) {
+    if (rdev) {
+        if (rdev[1] != 0) return;
+        R = max(rdev[0], 1);
+        if (R > rmax) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(rdev + 1, 4);
+            return;
+        }
+    }
     const long long total = (long long)C * N;
     const bool is3 = D > 1;
     const float ta = is3 ? (2.0f * s) * (2.0f * s) * (2.0f * s) : (2.0f * s) * (2.0f * s);
@@ -1233,11 +1244,11 @@ void launch_boxes_from_flow(hipStream_t st, const float* A, const float* fx, con
 }
 
 void launch_gather_general(hipStream_t st, const float4* box_lo, const float4* box_hi, float* Aout, int C,
-                           const Geo& g, float s, int radius, bool exact) {
+                           const Geo& g, float s, int radius, bool exact, int* rdev, int rmax) {
     const long long total = (long long)C * g.N;
     const int blocks = (int)std::min<long long>((total + 255) / 256, 65535);
     hipLaunchKernelGGL(k_gather_general, dim3(blocks), dim3(256), 0, st, box_lo, box_hi, Aout, C, g.W, g.H, g.D,
-                       g.N, s, radius, exact ? 1 : 0);
+                       g.N, s, radius, exact ? 1 : 0, rdev, rmax);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -63,8 +63,32 @@ class FlowConfig:
 
 
 @dataclass
+class EmbeddingConfig:
+    """EmbeddingConfig (embedding.rs:326-367): parameter embedding on/off, the softmax
+    temperature of the mixing, and linear mass-weighted mixing instead of softmax."""
+    enabled: bool = False
+    mixing_temperature: float = 1.0
+    linear_mixing: bool = False
+
+    @classmethod
+    def enabled_softmax(cls) -> "EmbeddingConfig":
+        """EmbeddingConfig::enabled (embedding.rs:349-356)."""
+        return cls(True, 1.0, False)
+
+    @classmethod
+    def enabled_linear(cls) -> "EmbeddingConfig":
+        """EmbeddingConfig::enabled_linear (embedding.rs:358-366)."""
+        return cls(True, 1.0, True)
+
+    def to_c(self) -> "_abi.HpEmbedding":
+        return _abi.HpEmbedding(int(bool(self.enabled)), float(self.mixing_temperature),
+                                int(bool(self.linear_mixing)))
+
+
+@dataclass
 class SimulationConfig:
-    """SimulationConfig (config.rs:14-51).  The embedding block is not part of the plain step."""
+    """SimulationConfig (config.rs:14-51).  `embedding` is read by the embedded-parameter
+    propagator only."""
     width: int = 256
     height: int = 256
     depth: int = 1
@@ -73,6 +97,7 @@ class SimulationConfig:
     kernel_radius: int = 13
     kernels: List[KernelConfig] = field(default_factory=lambda: [KernelConfig()])
     flow: FlowConfig = field(default_factory=FlowConfig)
+    embedding: EmbeddingConfig = field(default_factory=EmbeddingConfig)
 
     def is_3d(self) -> bool:
         return self.depth > 1
@@ -87,9 +112,11 @@ class SimulationConfig:
                                 weight=k["weight"], mu=k["mu"], sigma=k["sigma"],
                                 source_channel=k["source_channel"], target_channel=k["target_channel"])
                    for k in d["kernels"]]
+        emb = d.get("embedding")
         return cls(width=d["width"], height=d["height"], depth=d.get("depth", 1), channels=d["channels"],
                    dt=d["dt"], kernel_radius=d["kernel_radius"], kernels=kernels,
-                   flow=FlowConfig(**d["flow"]))
+                   flow=FlowConfig(**d["flow"]),
+                   embedding=EmbeddingConfig(**emb) if emb is not None else EmbeddingConfig())
 
     @classmethod
     def from_json(cls, text: str) -> "SimulationConfig":
@@ -104,6 +131,7 @@ class SimulationConfig:
                          "source_channel": k.source_channel, "target_channel": k.target_channel}
                         for k in self.kernels],
             "flow": self.flow.__dict__.copy(),
+            "embedding": self.embedding.__dict__.copy(),
         }
 
     # ---- lowering to the C ABI ------------------------------------------------------
@@ -313,7 +341,11 @@ def config_from_json_native(text: str) -> SimulationConfig:
     if st != _abi.HP_OK:
         raise _native_error(st)
     try:
-        return config_from_c(lib.hp_config_doc_config(d).contents)
+        cfg = config_from_c(lib.hp_config_doc_config(d).contents)
+        e = _abi.HpEmbedding()
+        lib.hp_config_doc_embedding(d, ctypes.byref(e))
+        cfg.embedding = EmbeddingConfig(bool(e.enabled), float(e.mixing_temperature), bool(e.linear_mixing))
+        return cfg
     finally:
         lib.hp_config_doc_free(d)
 

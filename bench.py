@@ -59,6 +59,11 @@ def main() -> int:
     ap.add_argument("--grad-every", type=int, default=8)
     ap.add_argument("--sphere3d", action="store_true",
                     help="3D mode: 128^3, one channel, one shell kernel (prints its own JSON line)")
+    ap.add_argument("--embedded", action="store_true",
+                    help="embedded-parameter mode: multispecies --emb-size^2 x 3 with embedding enabled "
+                         "(prints its own JSON line)")
+    ap.add_argument("--emb-size", type=int, default=1024)
+    ap.add_argument("--emb-linear", action="store_true", help="embedded mode: linear mixing instead of softmax")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of 256 MiB ten times (calibrates rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE against a kernel of known traffic)")
@@ -75,6 +80,8 @@ def main() -> int:
         return grad_bench(args)
     if args.sphere3d:
         return sphere3d_bench(args)
+    if args.embedded:
+        return embedded_bench(args)
 
     import numpy as np
 
@@ -490,6 +497,57 @@ def sphere3d_bench(args) -> int:
            "config": {"workload": "sphere 128^3, 1 channel, shell kernel R=10, dt=0.1", "plan": tune,
                       "max_gather_radius": radius, "general_path_steps": general, "mass_f64_after": mass},
            "per_kernel": per_kernel}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def embedded_bench(args) -> int:
+    """Embedded-parameter mode: the 1024^2 x 3 multispecies workload with embedding enabled
+    (softmax mixing, temperature 1), every cell starting at EmbeddedState::from_seed's
+    parameters; the device step against the oracle's (direct convolution, the reference's
+    algorithm) on a bounded sample of steps on the host."""
+    import numpy as np
+
+    from autoverse_amd import workloads
+    from autoverse_amd.embedded import EmbeddedState, HipEmbeddedPropagator
+    from autoverse_amd.schema import EmbeddingConfig
+
+    size = args.emb_size
+    cfg, seed = workloads.multispecies(size=size)
+    cfg.embedding = EmbeddingConfig(True, 1.0, args.emb_linear)
+    N = cfg.width * cfg.height
+    K = args.steps if args.steps != 1000 else 100
+    st = EmbeddedState.from_seed(seed, cfg)
+    hp = HipEmbeddedPropagator(cfg)
+    hp.set_state(st)
+    hp.advance(args.warmup)
+    start = hp.get_state()
+    t0 = time.perf_counter()
+    hp.advance(K)
+    secs = time.perf_counter() - t0
+    radius = hp.last_radius()
+    _, mass = hp.mass_f64()
+    hp.close()
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = min(os.cpu_count() or 1, 16)
+        o = O.EmbeddedPropagator(cfg, cfg.embedding.__dict__, threads=threads)
+        ch = [c.copy() for c in start.channels]
+        ps = [p.data.copy() for p in start.params]
+        n_cpu = 1
+        t0 = time.perf_counter()
+        o.run(ch, ps, n_cpu)
+        cs = time.perf_counter() - t0
+        cpu = {"value": N * n_cpu / cs, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+               "sample": f"{n_cpu} step of the same workload, f32, direct convolution as the reference, "
+                         f"{cs * 1e3:.0f} ms/step"}
+    out = {"metric": f"cell-steps/s at {size}^2x3ch (embedded-parameter step, softmax mixing)",
+           "value": N * K / secs, "unit": "cell-steps/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+           "ms_per_step": secs * 1e3 / K, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"multispecies {size}x{size}x3ch, 9 kernels, R=15, dt=0.2, embedding enabled",
+                      "linear_mixing": args.emb_linear, "search_radius": radius, "mass_f64_after": mass},
+           "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -531,6 +531,65 @@ hp_status hp_player_read_frame(hp_player* p, uint64_t index, float* const* chann
 hp_status hp_player_load_ctx(hp_player* p, uint64_t index, hp_ctx* ctx, float time, uint64_t step);
 void hp_player_close(hp_player* p);
 
+/* ---- embedded-parameter mode (src/compute/embedded_propagator.rs) -----------------------
+ * The state carries per-cell parameters that move with the mass (EmbeddedState,
+ * embedded_propagator.rs:24-37): per channel W*H CellParams (embedding.rs:20-31), five floats
+ * each, laid out as the reference's Vec<CellParams>.  The grid is 2D; depth is not read.
+ * Differences from the reference, all in the convolution stage: the convolutions run as FFTs
+ * (the reference sums the kernel taps directly, direct_convolution.rs:129-174; the two agree
+ * to the FFT's rounding), and exp is evaluated in FP64 and rounded once (libm's expf differs
+ * from that by one ulp in about 7 of 10000 arguments).  Given the affinity, the flow, the
+ * gather and the mixing follow the reference's f32 arithmetic operation for operation. */
+typedef struct hp_cell_params {
+    float mu, sigma, weight, beta_a, n;
+} hp_cell_params;
+
+typedef struct hp_emb hp_emb;
+/* EmbeddedPropagator::new (embedded_propagator.rs:165-220); emb NULL is EmbeddingConfig's
+ * default (disabled).  The propagator's next parameter grids start as CellParams::default(). */
+hp_status hp_emb_create(const hp_config* cfg, const hp_embedding* emb, int device, hp_emb** out);
+void hp_emb_destroy(hp_emb* e);
+/* Message of the last failed call on e (e NULL: of the last failed hp_emb_create or
+ * hp_advect_mass_and_params on this thread). */
+const char* hp_emb_last_error(const hp_emb* e);
+/* The parameters EmbeddedState::from_seed gives every cell (embedded_propagator.rs:50-57). */
+hp_status hp_emb_default_params(const hp_config* cfg, hp_cell_params* out);
+/* The device-resident state: channels (W*H floats each) and parameter grids (W*H cells each). */
+hp_status hp_emb_set_state(hp_emb* e, const float* const* channels, const hp_cell_params* const* params, float time,
+                           uint64_t step);
+/* params may be NULL (mass only). */
+hp_status hp_emb_get_state(hp_emb* e, float* const* channels, hp_cell_params* const* params, float* time,
+                           uint64_t* step);
+/* EmbeddedPropagator::step and ::run (embedded_propagator.rs:222-338) on host buffers, in
+ * place.  A step whose search radius would exceed the gather's work bound (2e10 source visits)
+ * returns HP_ERR_UNSUPPORTED_SIZE and leaves the state as it was. */
+hp_status hp_emb_step(hp_emb* e, float* const* channels, hp_cell_params* const* params, float* time,
+                      uint64_t* step);
+hp_status hp_emb_run(hp_emb* e, float* const* channels, hp_cell_params* const* params, float* time, uint64_t* step,
+                     uint64_t steps);
+/* `steps` steps of the device-resident state. */
+hp_status hp_emb_advance(hp_emb* e, uint64_t steps);
+/* FP64 mass of the device-resident state (as hp_mass_f64). */
+hp_status hp_emb_mass_f64(hp_emb* e, double* per_channel, double* total);
+/* The propagator's next parameter grids: what a destination that receives no mix keeps. */
+hp_status hp_emb_get_next_params(hp_emb* e, hp_cell_params* const* params);
+hp_status hp_emb_set_next_params(hp_emb* e, const hp_cell_params* const* params);
+/* Search radius of the last step per channel (param_advection.rs:99-100); 0 with embedding
+ * disabled. */
+hp_status hp_emb_last_radius(hp_emb* e, int32_t* radius);
+/* Stage 1 (embedded_propagator.rs:229-249) for the given state: the affinity per channel. */
+hp_status hp_emb_affinity(hp_emb* e, const float* const* channels, const hp_cell_params* const* params,
+                          float* const* affinity);
+/* Stages 2-5 (embedded_propagator.rs:251-330) with the affinity given; next_params is read and
+ * written (destinations that receive no mix keep its values). */
+hp_status hp_emb_stencil(hp_emb* e, const float* const* channels, const hp_cell_params* const* params,
+                         const float* const* affinity, float* const* next, hp_cell_params* const* next_params);
+/* advect_mass_and_params_into (param_advection.rs:83-166), one channel on host buffers;
+ * next_params is read and written as above; emb NULL is EmbeddingConfig::enabled(). */
+hp_status hp_advect_mass_and_params(const float* mass, const hp_cell_params* params, const float* fx,
+                                    const float* fy, const hp_embedding* emb, float dt, float s, uint32_t w,
+                                    uint32_t h, float* next_mass, hp_cell_params* next_params, int device);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,6 +24,9 @@
  *   config validation      src/schema/config.rs:136-164
  *   centre of mass         src/compute/evolution/fitness.rs:196-218, 309-326
  *   adjoint (oracle_grad.inc)  no reference counterpart: the transpose of the steps above
+ *   embedded parameters (oracle_embedded.inc)  src/compute/direct_convolution.rs:29-174,
+ *                          src/compute/param_advection.rs:83-357, src/schema/embedding.rs:60-167,
+ *                          src/compute/embedded_propagator.rs:161-362
  *
  * Parity pins (which check covers which function) are listed in the tests; the
  * only stage with no bit-level pin is the FFT (rustfft's exact rounding cannot
@@ -90,7 +93,31 @@ int32_t orc_sat_i32_f64(double v);
     orc_prop_##S* orc_prop_create_theta_##S(const hp_config* cfg, int threads, const double* theta); \
     uint32_t orc_theta_count_##S(const orc_prop_##S* p);                                       \
     int orc_grad_##S(orc_prop_##S* p, T* const* A0, uint64_t steps, T* const* gAT, T* const* AT, \
-                     T* const* gA0, double* dtheta);
+                     T* const* gA0, double* dtheta);                                            \
+    /* embedded-parameter step (oracle_embedded.inc); params: 5 T per cell */                  \
+    void orc_conv_direct2d_##S(const T* in, const T* k, uint32_t ksize, uint32_t w, uint32_t h, T* out); \
+    void orc_conv_growth_emb_##S(const T* in, const T* k, uint32_t ksize, const T* params, uint32_t w, \
+                                 uint32_t h, T* target, int accumulate, int threads);           \
+    void orc_flow_emb_##S(const T* gux, const T* guy, const T* gax, const T* gay, const T* mass, \
+                          const T* params, uint64_t n, T* fx, T* fy);                           \
+    T orc_overlap_emb_##S(T dest_x, T dest_y, T s, T cell_x, T cell_y);                        \
+    T orc_max_flow_mag_##S(const T* fx, const T* fy, uint64_t n);                              \
+    void orc_mix_softmax_##S(const T* params, const T* mass, uint32_t count, T temperature, T* out); \
+    void orc_mix_linear_##S(const T* params, const T* mass, uint32_t count, T* out);           \
+    int32_t orc_advect_params_##S(const T* mass, const T* params, const T* fx, const T* fy, int linear, \
+                                  T temperature, T dt, T s, uint32_t w, uint32_t h, T* next_mass, \
+                                  T* next_params);                                              \
+    typedef struct orc_emb_##S orc_emb_##S;                                                    \
+    orc_emb_##S* orc_emb_create_##S(const hp_config* cfg, const hp_embedding* emb, int threads); \
+    void orc_emb_destroy_##S(orc_emb_##S* p);                                                  \
+    T* orc_emb_next_params_##S(orc_emb_##S* p, uint32_t c);                                    \
+    int32_t orc_emb_last_radius_##S(const orc_emb_##S* p, uint32_t c);                         \
+    void orc_emb_affinity_##S(orc_emb_##S* p, T* const* channels, T* const* params, T* const* affinity); \
+    void orc_emb_stencil_##S(orc_emb_##S* p, T* const* channels, T* const* params, T* const* affinity, \
+                             T* const* next, T* const* next_params);                            \
+    void orc_emb_step_##S(orc_emb_##S* p, T* const* channels, T* const* params, float* time, uint64_t* step); \
+    void orc_emb_run_##S(orc_emb_##S* p, T* const* channels, T* const* params, float* time,      \
+                         uint64_t* step, uint64_t steps);
 
 ORC_DECLARE(float, f32)
 ORC_DECLARE(double, f64)

@@ -75,6 +75,24 @@ def lib() -> ctypes.CDLL:
                 f"orc_prop_create_theta_{S}": (c_void_p, [POINTER(_abi.HpConfig), c_int, FPd]),
                 f"orc_theta_count_{S}": (c_uint32, [c_void_p]),
                 f"orc_grad_{S}": (c_int, [c_void_p, PP, c_uint64, PP, PP, PP, FPd]),
+                f"orc_conv_direct2d_{S}": (None, [FPx, FPx, c_uint32, c_uint32, c_uint32, FPx]),
+                f"orc_conv_growth_emb_{S}": (None, [FPx, FPx, c_uint32, FPx, c_uint32, c_uint32, FPx, c_int,
+                                                    c_int]),
+                f"orc_flow_emb_{S}": (None, [FPx] * 6 + [c_uint64, FPx, FPx]),
+                f"orc_overlap_emb_{S}": (T, [T, T, T, T, T]),
+                f"orc_max_flow_mag_{S}": (T, [FPx, FPx, c_uint64]),
+                f"orc_mix_softmax_{S}": (None, [FPx, FPx, c_uint32, T, FPx]),
+                f"orc_mix_linear_{S}": (None, [FPx, FPx, c_uint32, FPx]),
+                f"orc_advect_params_{S}": (c_int32, [FPx, FPx, FPx, FPx, c_int, T, T, T, c_uint32, c_uint32, FPx,
+                                                     FPx]),
+                f"orc_emb_create_{S}": (c_void_p, [POINTER(_abi.HpConfig), POINTER(_abi.HpEmbedding), c_int]),
+                f"orc_emb_destroy_{S}": (None, [c_void_p]),
+                f"orc_emb_next_params_{S}": (FPx, [c_void_p, c_uint32]),
+                f"orc_emb_last_radius_{S}": (c_int32, [c_void_p, c_uint32]),
+                f"orc_emb_affinity_{S}": (None, [c_void_p, PP, PP, PP]),
+                f"orc_emb_stencil_{S}": (None, [c_void_p, PP, PP, PP, PP, PP]),
+                f"orc_emb_step_{S}": (None, [c_void_p, PP, PP, POINTER(c_float), POINTER(c_uint64)]),
+                f"orc_emb_run_{S}": (None, [c_void_p, PP, PP, POINTER(c_float), POINTER(c_uint64), c_uint64]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, name)
@@ -365,3 +383,158 @@ def loss(cfg, channels, steps: int, seed, theta=None, dtype=np.float64) -> float
     p.close()
     return float(sum(np.dot(np.asarray(a, np.float64), np.asarray(s, np.float64).reshape(-1))
                      for a, s in zip(at, seed)))
+
+
+# ---- embedded-parameter mode (oracle_embedded.inc) ------------------------------------------
+# Parameter grids are float arrays of shape (N, 5): mu, sigma, weight, beta_a, n per cell, the
+# layout of the reference's Vec<CellParams>.
+CELL_DEFAULT = (0.15, 0.015, 1.0, 1.0, 2.0)  # CellParams::default (embedding.rs:35-45)
+
+
+def _params(a, n, dtype):
+    a = np.ascontiguousarray(a, dtype=dtype).reshape(n, 5)
+    return a
+
+
+def conv_direct2d(inp: np.ndarray, k: np.ndarray, w: int, h: int) -> np.ndarray:
+    """convolve_direct (direct_convolution.rs:20-60)."""
+    dt = inp.dtype
+    inp = np.ascontiguousarray(inp).reshape(-1)
+    k = np.ascontiguousarray(k, dtype=dt)
+    out = np.zeros(w * h, dtype=dt)
+    getattr(lib(), f"orc_conv_direct2d_{_sfx(dt)}")(_p(inp), _p(k.reshape(-1)), k.shape[0], w, h, _p(out))
+    return out
+
+
+def conv_growth_emb(inp, k, params, w, h, target=None, threads: int = 1):
+    """convolve_growth_embedded (target None) or convolve_growth_accumulate_embedded
+    (direct_convolution.rs:66-174)."""
+    dt = inp.dtype
+    inp = np.ascontiguousarray(inp).reshape(-1)
+    k = np.ascontiguousarray(k, dtype=dt)
+    pr = _params(params, w * h, dt)
+    acc = target is not None
+    out = target if acc else np.zeros(w * h, dtype=dt)
+    getattr(lib(), f"orc_conv_growth_emb_{_sfx(dt)}")(_p(inp), _p(k.reshape(-1)), k.shape[0], _p(pr.reshape(-1)),
+                                                    w, h, _p(out), int(acc), threads)
+    return out
+
+
+def flow_emb(gux, guy, gax, gay, mass, params):
+    dt = gux.dtype
+    n = gux.size
+    fx = np.zeros(n, dtype=dt)
+    fy = np.zeros(n, dtype=dt)
+    pr = _params(params, n, dt)
+    getattr(lib(), f"orc_flow_emb_{_sfx(dt)}")(_p(gux), _p(guy), _p(gax), _p(gay), _p(mass), _p(pr.reshape(-1)), n,
+                                             _p(fx), _p(fy))
+    return fx, fy
+
+
+def overlap_emb(dest_x, dest_y, s, cell_x, cell_y, dtype=np.float32) -> float:
+    return getattr(lib(), f"orc_overlap_emb_{_sfx(dtype)}")(dest_x, dest_y, s, cell_x, cell_y)
+
+
+def mix_softmax(params, masses, temperature, dtype=np.float32):
+    pr = np.ascontiguousarray(params, dtype=dtype).reshape(-1, 5)
+    m = np.ascontiguousarray(masses, dtype=dtype).reshape(-1)
+    out = np.zeros(5, dtype=dtype)
+    getattr(lib(), f"orc_mix_softmax_{_sfx(dtype)}")(_p(pr.reshape(-1)), _p(m), m.size, temperature, _p(out))
+    return out
+
+
+def mix_linear(params, masses, dtype=np.float32):
+    pr = np.ascontiguousarray(params, dtype=dtype).reshape(-1, 5)
+    m = np.ascontiguousarray(masses, dtype=dtype).reshape(-1)
+    out = np.zeros(5, dtype=dtype)
+    getattr(lib(), f"orc_mix_linear_{_sfx(dtype)}")(_p(pr.reshape(-1)), _p(m), m.size, _p(out))
+    return out
+
+
+def advect_params(mass, params, fx, fy, w, h, dt, s, linear=False, temperature=1.0, next_params=None):
+    """advect_mass_and_params_into (param_advection.rs:83-166).  next_params (N x 5) is the
+    grid written into (cells that receive no mix keep its values; default CellParams::default).
+    Returns (next_mass, next_params, R)."""
+    d = mass.dtype
+    n = w * h
+    pr = _params(params, n, d)
+    nm = np.zeros(n, dtype=d)
+    if next_params is None:
+        nps = np.tile(np.array(CELL_DEFAULT, dtype=d), (n, 1))
+    else:
+        nps = _params(next_params, n, d).copy()
+    R = getattr(lib(), f"orc_advect_params_{_sfx(d)}")(_p(np.ascontiguousarray(mass)), _p(pr.reshape(-1)),
+                                                     _p(np.ascontiguousarray(fx)), _p(np.ascontiguousarray(fy)),
+                                                     int(linear), temperature, dt, s, w, h, _p(nm),
+                                                     _p(nps.reshape(-1)))
+    return nm, nps, R
+
+
+class EmbeddedPropagator:
+    """EmbeddedPropagator restated (embedded_propagator.rs:144-340), f32 or f64.  `embedding`
+    is a dict with enabled / mixing_temperature / linear_mixing (EmbeddingConfig)."""
+
+    def __init__(self, cfg, embedding=None, dtype=np.float32, threads: int = 1):
+        self.cfg = cfg
+        self.dtype = np.dtype(dtype)
+        self.S = _sfx(dtype)
+        self._c = cfg.to_c()
+        e = embedding or {}
+        self._e = _abi.HpEmbedding(int(bool(e.get("enabled", False))), float(e.get("mixing_temperature", 1.0)),
+                                   int(bool(e.get("linear_mixing", False))))
+        self.ptr = getattr(lib(), f"orc_emb_create_{self.S}")(ctypes.byref(self._c.cfg), ctypes.byref(self._e),
+                                                              threads)
+        if not self.ptr:
+            raise ValueError("invalid configuration")
+        self.n = cfg.width * cfg.height
+
+    def close(self):
+        if self.ptr:
+            getattr(lib(), f"orc_emb_destroy_{self.S}")(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chans(self, channels):
+        return [np.ascontiguousarray(c, dtype=self.dtype).reshape(-1) for c in channels]
+
+    def _pars(self, params):
+        return [np.ascontiguousarray(p, dtype=self.dtype).reshape(-1) for p in params]
+
+    def next_params(self, c: int) -> np.ndarray:
+        p = getattr(lib(), f"orc_emb_next_params_{self.S}")(self.ptr, c)
+        return np.ctypeslib.as_array(p, shape=(self.n * 5,)).copy().reshape(self.n, 5)
+
+    def set_next_params(self, c: int, values) -> None:
+        p = getattr(lib(), f"orc_emb_next_params_{self.S}")(self.ptr, c)
+        np.ctypeslib.as_array(p, shape=(self.n * 5,))[:] = np.asarray(values, dtype=self.dtype).reshape(-1)
+
+    def last_radius(self, c: int) -> int:
+        return getattr(lib(), f"orc_emb_last_radius_{self.S}")(self.ptr, c)
+
+    def affinity(self, channels, params):
+        ch, pr = self._chans(channels), self._pars(params)
+        aff = [np.zeros(self.n, dtype=self.dtype) for _ in ch]
+        getattr(lib(), f"orc_emb_affinity_{self.S}")(self.ptr, _pp(ch), _pp(pr), _pp(aff))
+        return aff
+
+    def stencil(self, channels, params, affinity, next_params):
+        """Stages 2-5 into fresh mass grids and copies of next_params (N x 5 per channel)."""
+        ch, pr, af = self._chans(channels), self._pars(params), self._chans(affinity)
+        nxt = [np.zeros(self.n, dtype=self.dtype) for _ in ch]
+        nps = [np.array(p, dtype=self.dtype).reshape(-1) for p in next_params]
+        getattr(lib(), f"orc_emb_stencil_{self.S}")(self.ptr, _pp(ch), _pp(pr), _pp(af), _pp(nxt), _pp(nps))
+        return nxt, [p.reshape(self.n, 5) for p in nps]
+
+    def run(self, channels, params, steps: int, time: float = 0.0, step: int = 0):
+        """Returns (channels, params, time, step); inputs are not modified (the propagator's
+        own next grids are)."""
+        ch = [c.copy() for c in self._chans(channels)]
+        pr = [p.copy() for p in self._pars(params)]
+        t, s = c_float(time), c_uint64(step)
+        getattr(lib(), f"orc_emb_run_{self.S}")(self.ptr, _pp(ch), _pp(pr), ctypes.byref(t), ctypes.byref(s), steps)
+        return ch, [p.reshape(self.n, 5) for p in pr], t.value, s.value
