@@ -9,8 +9,9 @@
 //                  (direct_convolution.rs:129-174);
 //   k_emb_flow   : Sobel of the channel sum and of U, per-cell alpha and the flow
 //                  (embedded_propagator.rs:251-302, 343-362), and the largest |F| per channel;
-//   k_emb_advect : the reference's destination gather with parameter mixing
-//                  (param_advection.rs:83-166, embedding.rs:60-167).
+//   k_emb_advect_tile / k_emb_advect : the reference's destination gather with parameter
+//                  mixing (param_advection.rs:83-166, embedding.rs:60-167), tiled through LDS
+//                  for the usual radii, straight from global memory otherwise.
 // Compiled with -ffp-contract=off: every f32 operation rounds as the reference's does.  The
 // growth and softmax exponentials are evaluated in FP64 and rounded once (the reference calls
 // libm's expf; the two agree to the last bit except in about 7 of 10000 arguments, by one ulp).
@@ -95,48 +96,60 @@ __global__ __launch_bounds__(kEmbThreads) void k_emb_growth(const float* __restr
     }
 }
 
-// Flow of channel blockIdx.y from Sobel(channel sum), Sobel(U_c) and the cell's alpha; the
-// largest |F| of the channel goes to maxbits[c] (non-negative floats order as their bits).
+// Flows of channels c0 .. c0 + kFlowChunk - 1 from Sobel(channel sum) (once per cell), Sobel(U_c)
+// and the cell's alpha; the largest |F| of each channel goes to maxbits[c] (non-negative floats
+// order as their bits), one atomic per workgroup and channel.
+constexpr int kFlowChunk = 8;
 __global__ __launch_bounds__(kEmbThreads) void k_emb_flow(const float* __restrict__ A, const float* __restrict__ U,
                                                           const float* __restrict__ P, float* __restrict__ F,
-                                                          unsigned* __restrict__ maxbits, int C, int W, int H,
+                                                          unsigned* __restrict__ maxbits, int C, int c0, int W, int H,
                                                           long long N) {
-    __shared__ float red[kEmbThreads];
-    const int c = blockIdx.y;
-    const float* Uc = U + (long long)c * N;
-    const float* beta = P + ((long long)c * 5 + 3) * N;
-    const float* nexp = P + ((long long)c * 5 + 4) * N;
-    float* fxo = F + (long long)c * 2 * N;
-    float* fyo = fxo + N;
-    float local = 0.0f;
+    __shared__ float red[kEmbThreads / 64][kFlowChunk];
+    const int nc = min(C - c0, kFlowChunk);
+    float lmax[kFlowChunk];
+#pragma unroll
+    for (int k = 0; k < kFlowChunk; ++k) lmax[k] = 0.0f;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
          i += (long long)gridDim.x * blockDim.x) {
         const int x = (int)(i % W), y = (int)(i / W);
         const int xp = x == 0 ? W - 1 : x - 1, xn = x + 1 == W ? 0 : x + 1;
         const int yp = y == 0 ? H - 1 : y - 1, yn = y + 1 == H ? 0 : y + 1;
         const long long rp = (long long)yp * W, rc = (long long)y * W, rn = (long long)yn * W;
-        float gax, gay, gux, guy;
+        float gax, gay;
         sobel9(emb_asum(A, C, N, rp + xp), emb_asum(A, C, N, rp + x), emb_asum(A, C, N, rp + xn),
                emb_asum(A, C, N, rc + xp), emb_asum(A, C, N, rc + xn), emb_asum(A, C, N, rn + xp),
                emb_asum(A, C, N, rn + x), emb_asum(A, C, N, rn + xn), gax, gay);
-        sobel9(Uc[rp + xp], Uc[rp + x], Uc[rp + xn], Uc[rc + xp], Uc[rc + xn], Uc[rn + xp], Uc[rn + x], Uc[rn + xn],
-               gux, guy);
-        const float al = alpha_of(emb_asum(A, C, N, i), beta[i], nexp[i]);
-        const float oma = 1.0f - al;
-        const float fx = oma * gux - al * gax;
-        const float fy = oma * guy - al * gay;
-        fxo[i] = fx;
-        fyo[i] = fy;
-        const float mag = sqrtf(fx * fx + fy * fy);
-        local = fmaxf(local, mag);  // f32::max: a NaN magnitude is passed over
+        const float asum = emb_asum(A, C, N, i);
+#pragma unroll
+        for (int k = 0; k < kFlowChunk; ++k) {
+            if (k >= nc) break;
+            const int c = c0 + k;
+            const float* Uc = U + (long long)c * N;
+            float gux, guy;
+            sobel9(Uc[rp + xp], Uc[rp + x], Uc[rp + xn], Uc[rc + xp], Uc[rc + xn], Uc[rn + xp], Uc[rn + x],
+                   Uc[rn + xn], gux, guy);
+            const float al = alpha_of(asum, P[((long long)c * 5 + 3) * N + i], P[((long long)c * 5 + 4) * N + i]);
+            const float oma = 1.0f - al;
+            const float fx = oma * gux - al * gax;
+            const float fy = oma * guy - al * gay;
+            F[(long long)c * 2 * N + i] = fx;
+            F[((long long)c * 2 + 1) * N + i] = fy;
+            lmax[k] = fmaxf(lmax[k], sqrtf(fx * fx + fy * fy));  // f32::max: a NaN magnitude is passed over
+        }
     }
-    red[threadIdx.x] = local;
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+#pragma unroll
+    for (int k = 0; k < kFlowChunk; ++k) {
+        float v = lmax[k];
+        for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+        if (lane == 0) red[wave][k] = v;
+    }
     __syncthreads();
-    for (int w = kEmbThreads / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
-        __syncthreads();
+    if ((int)threadIdx.x < nc) {
+        float v = red[0][threadIdx.x];
+        for (int w = 1; w < kEmbThreads / 64; ++w) v = fmaxf(v, red[w][threadIdx.x]);
+        atomicMax(maxbits + c0 + threadIdx.x, __float_as_uint(v));
     }
-    if (threadIdx.x == 0) atomicMax(maxbits + c, __float_as_uint(red[0]));
 }
 
 // compute_max_flow_magnitude (param_advection.rs:351-357) of given flows F[c][2][N].
@@ -159,6 +172,188 @@ __global__ __launch_bounds__(kEmbThreads) void k_emb_maxmag(const float* __restr
     if (threadIdx.x == 0) atomicMax(maxbits + c, __float_as_uint(red[0]));
 }
 
+// The tiled gather stages a destination tile's source window in LDS: radii up to kTileR, sides of
+// at least kTile cells, and boxes of area above the point threshold.  k_emb_advect covers the
+// rest; both kernels are launched and one of them returns at once.
+constexpr int kTile = 16, kTileR = 4, kRegion = kTile + 2 * kTileR, kCap = 10;
+__device__ __forceinline__ bool tiled_gather(int R, int W, int H, float s) {
+    return R <= kTileR && W >= kTile && H >= kTile && (2.0f * s) * (2.0f * s) > 1e-10f;
+}
+
+// Destination gather of channel blockIdx.z over one 16 x 16 tile (one destination per thread),
+// the same arithmetic as k_emb_advect: each window source's box [x + dt F - s, x + dt F + s] and
+// mass are staged once (the values every destination would compute), and each destination keeps
+// its contributions in LDS (up to kCap; more are recomputed) so the softmax weights are formed
+// once per contribution.  A tile whose window holds no active source writes mass 0.
+__global__ __launch_bounds__(kTile * kTile) void k_emb_advect_tile(
+    const float* __restrict__ A, const float* __restrict__ P, const float* __restrict__ F,
+    const unsigned* __restrict__ maxbits, float* __restrict__ Aout, float* __restrict__ Pout, int W, int H,
+    long long N, float dt, float s, int linear, float temperature, int rmax) {
+    __shared__ float4 sbox[kRegion * kRegion];
+    __shared__ float smass[kRegion * kRegion];
+    __shared__ float spar[5][kRegion * kRegion];
+    __shared__ float lc[kCap][kTile * kTile];
+    __shared__ short li[kCap][kTile * kTile];
+    const int c = blockIdx.z;
+    const float mag = __uint_as_float(maxbits[c]);
+    const int r0 = sat_i32(ceilf(mag * dt + s));
+    if (r0 >= rmax) return;  // k_emb_advect raises the flag
+    const int R = r0 + 1;
+    if (!tiled_gather(R, W, H, s)) return;
+    const float* Ac = A + (long long)c * N;
+    const float* fxv = F + (long long)c * 2 * N;
+    const float* fyv = fxv + N;
+    const float* Pc = P + (long long)c * 5 * N;
+    const int tx0 = blockIdx.x * kTile, ty0 = blockIdx.y * kTile;
+    const int RW = kTile + 2 * R;
+    const int tid = threadIdx.x;
+    bool any = false;
+    for (int e = tid; e < RW * RW; e += kTile * kTile) {
+        const int ry = e / RW, rx = e - ry * RW;
+        const int sx = wrapi(tx0 - R + rx, W), sy = wrapi(ty0 - R + ry, H);
+        const long long si = (long long)sy * W + sx;
+        const float m = Ac[si];
+        float4 b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float mm = 0.0f;  // 0 marks a skipped source (an active mass has |m| >= 1e-10)
+        if (!(fabsf(m) < 1e-10f)) {
+            const float dfx = (float)sx + dt * fxv[si];
+            const float dfy = (float)sy + dt * fyv[si];
+            b = make_float4(dfx - s, dfx + s, dfy - s, dfy + s);
+            mm = m;
+            any = true;
+#pragma unroll
+            for (int f = 0; f < 5; ++f) spar[f][e] = Pc[f * N + si];
+        }
+        sbox[e] = b;
+        smass[e] = mm;
+    }
+    any = __syncthreads_or(any);
+    const int lx = tid % kTile, ly = tid / kTile;
+    const int x = tx0 + lx, y = ty0 + ly;
+    if (x >= W || y >= H) return;
+    const long long di = (long long)y * W + x;
+    float* Ao = Aout + (long long)c * N;
+    if (!any) {
+        Ao[di] = 0.0f;
+        return;
+    }
+    float* Po = Pout + (long long)c * 5 * N;
+    const float cx = (float)x, cy = (float)y, cxm = cx + 1.0f, cym = cy + 1.0f;
+    const float ta = (2.0f * s) * (2.0f * s);
+    // a power-of-two area divides exactly as a multiplication by its reciprocal (both round the
+    // same exact quotient)
+    int ex;
+    const bool pow2 = frexpf(ta, &ex) == 0.5f;
+    const float inv_ta = 1.0f / ta;
+    // the contribution of window entry e to this destination; false when it contributes nothing.
+    // An empty overlap along either axis makes the reference's area / total zero, so those
+    // entries are passed over before the division.
+    auto contribution = [&](int e, float& cm) -> bool {
+        const float m = smass[e];
+        if (m == 0.0f) return false;
+        const float4 b = sbox[e];
+        const float oh = fmaxf(fminf(b.w, cym) - fmaxf(b.z, cy), 0.0f);
+        if (!(oh > 0.0f)) return false;
+        const float ow = fmaxf(fminf(b.y, cxm) - fmaxf(b.x, cx), 0.0f);
+        if (!(ow > 0.0f)) return false;
+        const float area = ow * oh;
+        const float ov = pow2 ? area * inv_ta : area / ta;
+        if (!(ov > 0.0f)) return false;
+        cm = m * ov;
+        return true;
+    };
+    float total = 0.0f, mx = -INFINITY;
+    int count = 0, first = 0;
+    for (int dy = -R; dy <= R; ++dy) {
+        const int row = (ly + R + dy) * RW + lx + R;
+        for (int dx = -R; dx <= R; ++dx) {
+            float cm;
+            if (!contribution(row + dx, cm)) continue;
+            total += cm;
+            mx = fmaxf(mx, cm);
+            if (count < kCap) {
+                lc[count][tid] = cm;
+                li[count][tid] = (short)(row + dx);
+            }
+            if (count == 0) first = row + dx;
+            ++count;
+        }
+    }
+    Ao[di] = total;
+    if (count == 0 || !(total > 1e-10f)) return;
+    float acc[5];
+    if (count == 1) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) Po[f * N + di] = spar[f][first];
+        return;
+    }
+#pragma unroll
+    for (int f = 0; f < 5; ++f) acc[f] = 0.0f;
+    const bool cached = count <= kCap;
+    if (linear) {
+        if (cached) {
+            for (int k = 0; k < count; ++k) {
+                const float w = lc[k][tid] / total;
+                const int e = li[k][tid];
+#pragma unroll
+                for (int f = 0; f < 5; ++f) acc[f] += spar[f][e] * w;
+            }
+        } else {
+            for (int dy = -R; dy <= R; ++dy) {
+                const int row = (ly + R + dy) * RW + lx + R;
+                for (int dx = -R; dx <= R; ++dx) {
+                    float cm;
+                    if (!contribution(row + dx, cm)) continue;
+                    const float w = cm / total;
+#pragma unroll
+                    for (int f = 0; f < 5; ++f) acc[f] += spar[f][row + dx] * w;
+                }
+            }
+        }
+    } else {
+        float wsum = 0.0f;
+        if (cached) {
+            for (int k = 0; k < count; ++k) {
+                const float e = exp_rn((lc[k][tid] - mx) / temperature);
+                lc[k][tid] = e;
+                wsum += e;
+            }
+        } else {
+            for (int dy = -R; dy <= R; ++dy) {
+                const int row = (ly + R + dy) * RW + lx + R;
+                for (int dx = -R; dx <= R; ++dx) {
+                    float cm;
+                    if (!contribution(row + dx, cm)) continue;
+                    wsum += exp_rn((cm - mx) / temperature);
+                }
+            }
+        }
+        const bool uniform = !(wsum > 0.0f);
+        const float u = 1.0f / (float)count;
+        if (cached) {
+            for (int k = 0; k < count; ++k) {
+                const float w = uniform ? u : lc[k][tid] / wsum;
+                const int e = li[k][tid];
+#pragma unroll
+                for (int f = 0; f < 5; ++f) acc[f] += spar[f][e] * w;
+            }
+        } else {
+            for (int dy = -R; dy <= R; ++dy) {
+                const int row = (ly + R + dy) * RW + lx + R;
+                for (int dx = -R; dx <= R; ++dx) {
+                    float cm;
+                    if (!contribution(row + dx, cm)) continue;
+                    const float w = uniform ? u : exp_rn((cm - mx) / temperature) / wsum;
+#pragma unroll
+                    for (int f = 0; f < 5; ++f) acc[f] += spar[f][row + dx] * w;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < 5; ++f) Po[f * N + di] = acc[f];
+}
+
 // Destination gather of channel blockIdx.y with parameter mixing.  R = ceil(max|F| dt + s) + 1
 // from maxbits[c]; a radius above rmax sets err bit 0 and writes nothing (the host refuses the
 // step).  Pout keeps its content at destinations that receive no mix.
@@ -176,6 +371,7 @@ __global__ __launch_bounds__(kEmbThreads) void k_emb_advect(const float* __restr
         return;
     }
     const int R = r0 + 1;
+    if (tiled_gather(R, W, H, s)) return;  // k_emb_advect_tile covers this step
     const float* Ac = A + (long long)c * N;
     const float* fxv = F + (long long)c * 2 * N;
     const float* fyv = fxv + N;
@@ -279,7 +475,9 @@ void launch_emb_growth(hipStream_t st, const float* conv, const float* P, const 
 void launch_emb_flow(hipStream_t st, const float* A, const float* U, const float* P, float* F, unsigned* maxbits,
                      int C, int W, int H) {
     const long long N = (long long)W * H;
-    hipLaunchKernelGGL(k_emb_flow, dim3(grid_for(N), C), dim3(kEmbThreads), 0, st, A, U, P, F, maxbits, C, W, H, N);
+    const int blocks = (int)std::min<long long>((N + kEmbThreads - 1) / kEmbThreads, 2048);
+    for (int c0 = 0; c0 < C; c0 += kFlowChunk)
+        hipLaunchKernelGGL(k_emb_flow, dim3(blocks), dim3(kEmbThreads), 0, st, A, U, P, F, maxbits, C, c0, W, H, N);
 }
 
 void launch_emb_maxmag(hipStream_t st, const float* F, unsigned* maxbits, int C, long long N) {
@@ -290,6 +488,9 @@ void launch_emb_advect(hipStream_t st, const float* A, const float* P, const flo
                        float* Aout, float* Pout, int C, int W, int H, float dt, float s, bool linear,
                        float temperature, int rmax, int* err) {
     const long long N = (long long)W * H;
+    hipLaunchKernelGGL(k_emb_advect_tile, dim3((W + kTile - 1) / kTile, (H + kTile - 1) / kTile, C),
+                       dim3(kTile * kTile), 0, st, A, P, F, maxbits, Aout, Pout, W, H, N, dt, s, linear ? 1 : 0,
+                       temperature, rmax);
     hipLaunchKernelGGL(k_emb_advect, dim3(grid_for(N), C), dim3(kEmbThreads), 0, st, A, P, F, maxbits, Aout, Pout, W,
                        H, N, dt, s, linear ? 1 : 0, temperature, rmax, err);
 }

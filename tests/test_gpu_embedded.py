@@ -253,3 +253,29 @@ def test_flow_beyond_work_bound_is_refused(mode):
     for c in range(cfg.channels):
         np.testing.assert_array_equal(g.channels[c], ch[c])
         np.testing.assert_array_equal(g.params[c].data, ps[c])
+
+
+@pytest.mark.parametrize("linear", [True, False])
+def test_stage_entry_many_contributors(oracle, linear):
+    """s = 2: a destination receives from up to 25 sources, more than the tiled gather keeps in
+    LDS, so it recomputes them; dense masses on 40 x 32 (radius 4: the tiled path)."""
+    _say(f"[emb many contributors] linear={linear}")
+    rng = np.random.default_rng(51 + linear)
+    w, h = 40, 32
+    n = w * h
+    m = rng.random(n).astype(np.float32)
+    pr = rng.uniform(0.01, 3.0, (n, 5)).astype(np.float32)
+    fx = rng.uniform(-1, 1, n).astype(np.float32)
+    fy = rng.uniform(-1, 1, n).astype(np.float32)
+    sent = np.full((n, 5), -1.0, np.float32)
+    emb = EmbeddingConfig(True, 0.5, linear)
+    want_m, want_p, R = oracle.advect_params(m, pr, fx, fy, w, h, 0.2, 2.0, linear=linear, temperature=0.5,
+                                             next_params=sent)
+    assert R == 4
+    got_m, got_p = advect_mass_and_params(m, ParameterGrid(w, h, data=pr), fx, fy, emb, 0.2, 2.0, w, h,
+                                          next_params=ParameterGrid(w, h, data=sent))
+    np.testing.assert_array_equal(got_m, want_m)
+    if linear:
+        np.testing.assert_array_equal(got_p.data, want_p)
+    else:
+        np.testing.assert_allclose(got_p.data, want_p, rtol=1e-6, atol=0)
