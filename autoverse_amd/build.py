@@ -38,11 +38,13 @@ SOURCES = [
     ("fft_g1.hip", DEVICE_FLAGS + ["-ffp-contract=fast-honor-pragmas"]),
     ("fft_g2.hip", DEVICE_FLAGS + ["-ffp-contract=fast-honor-pragmas"]),
     ("fft_g3.hip", DEVICE_FLAGS + ["-ffp-contract=fast-honor-pragmas"]),
+    ("fft_g4.hip", DEVICE_FLAGS + ["-ffp-contract=fast-honor-pragmas"]),
     ("wave_fft.hip", DEVICE_FLAGS + ["-ffp-contract=fast-honor-pragmas"]),
     ("stencil_kernels.hip", DEVICE_FLAGS + ["-ffp-contract=off"]),
     ("noalloc.hip", DEVICE_FLAGS + ["-ffp-contract=off"]),
     ("fitness.hip", DEVICE_FLAGS + ["-ffp-contract=off"]),
     ("embedded.hip", DEVICE_FLAGS + ["-ffp-contract=off"]),
+    ("direct_conv.hip", DEVICE_FLAGS + ["-ffp-contract=fast-honor-pragmas"]),
     ("adjoint_stencil.hip", DEVICE_FLAGS + ["-ffp-contract=off"]),
     ("adjoint_fft.hip", DEVICE_FLAGS + ["-ffp-contract=fast-honor-pragmas"]),
     ("host.cpp", HOST_FLAGS),

@@ -4,7 +4,7 @@
 
 namespace hp {
 
-#define HP_GROUPS(M) M(g0) M(g1) M(g2) M(g3)
+#define HP_GROUPS(M) M(g0) M(g1) M(g2) M(g3) M(g4)
 #define HP_DECL(G)                                                                                               \
     bool launch_rows_fwd_##G(int, hipStream_t, const float*, float2*, const int*, int, const Geo&, const float2*,  \
                              const FftVariant&);                                                                 \
@@ -24,19 +24,19 @@ HP_GROUPS(HP_DECL)
 
 bool fft_supported(int L) {
 #define X_(n) if (L == n) return true;
-    HP_FFT_G0(X_) HP_FFT_G1(X_) HP_FFT_G2(X_) HP_FFT_G3(X_)
+    HP_FFT_G0(X_) HP_FFT_G1(X_) HP_FFT_G2(X_) HP_FFT_G3(X_) HP_FFT_G4(X_)
 #undef X_
     return false;
 }
 
 int spectral_block_lines(int L, const FftVariant& v) {
 #define X_(n) if (L == n) return v.cs == 2 ? Cfg<n, 2>::CB : Cfg<n, 1>::CB;
-    HP_FFT_G0(X_) HP_FFT_G1(X_) HP_FFT_G2(X_) HP_FFT_G3(X_)
+    HP_FFT_G0(X_) HP_FFT_G1(X_) HP_FFT_G2(X_) HP_FFT_G3(X_) HP_FFT_G4(X_)
 #undef X_
     return 0;
 }
 
-#define HP_ANY(call) (call##_g0 ARGS || call##_g1 ARGS || call##_g2 ARGS || call##_g3 ARGS)
+#define HP_ANY(call) (call##_g0 ARGS || call##_g1 ARGS || call##_g2 ARGS || call##_g3 ARGS || call##_g4 ARGS)
 
 bool launch_rows_fwd(int L, hipStream_t st, const float* A, float2* X, const int* src_chan, int nsrc, const Geo& g,
                      const float2* tw, const FftVariant& v) {

@@ -459,5 +459,6 @@ inline void rows_inv_conv(hipStream_t st, const float2* Y, float* conv, const Tg
 #define HP_FFT_G1(X) X(128) X(256) X(96) X(192)
 #define HP_FFT_G2(X) X(512) X(1024) X(384) X(768)
 #define HP_FFT_G3(X) X(2048) X(1536)
+#define HP_FFT_G4(X) X(4096) X(3072)
 
 }  // namespace hp

@@ -70,6 +70,13 @@ struct hp_ctx {
     int n_tgt_work = 0;
     int* d_klist_tgt = nullptr;
     hp::GrowthParams* d_gp = nullptr;
+    // sides the FFT passes do not cover: the direct convolution's kernels, taps, column ranges
+    bool direct = false;
+    int direct_rmax = 0;
+    float* d_conv = nullptr;  // [K][N] convolutions of the direct path
+    hp::DirectKernel* d_dk = nullptr;
+    float* d_taps = nullptr;
+    int2* d_cols = nullptr;
     int* d_flags = nullptr;  // [0] stencil flag bits, [1] needed radius, [2] non-finite bits
     float4* box_lo = nullptr;
     float4* box_hi = nullptr;
@@ -225,6 +232,14 @@ hp_status make_stage_twiddles(hp_ctx* c, int L) {
 hp_status enqueue_affinity(hp_ctx* c, const float* A) {
     if (c->K == 0) return HP_OK;  // U stays zero (propagator.rs:245-247)
     const hp::Geo& g = c->g;
+    if (c->direct) {
+        Timed t(c, HP_K_ROWS_INV);
+        if (!hp::launch_direct_affinity(c->stream, A, c->U, c->d_conv, c->d_dk, c->d_taps, c->d_cols, c->K, c->C, g,
+                                        c->direct_rmax))
+            return c->fail(HP_ERR_UNSUPPORTED_SIZE, "kernel radius beyond the direct convolution's LDS tile");
+        HIP_TRY(c, hipGetLastError());
+        return HP_OK;
+    }
     if (c->wave) {  // A -> X -> Y -> U, each pass out of place
         timed_pass(c, HP_K_ROWS_FWD, true, [&] {
             hp::launch_rows_fwd_t(c->stream, A, c->X, c->d_src_chan, (int)c->src_chan.size(), g, c->tw[g.W], c->tws[g.W]);
@@ -497,7 +512,7 @@ void release(hp_ctx* c) {
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     void* ptrs[] = {c->A[0], c->A[1], c->U, c->X, c->Y, c->Kq, c->d_src_chan, c->d_spec_work, c->d_klist_spec,
                     c->d_tgt_work, c->d_klist_tgt, c->d_gp, c->d_flags, c->box_lo, c->box_hi, c->d_partials,
-                    c->d_mass, c->snapshot};
+                    c->d_mass, c->snapshot, c->d_dk, c->d_taps, c->d_cols, c->d_conv};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& kv : c->tw) (void)hipFree(kv.second);
@@ -629,9 +644,21 @@ hp_status create(const hp_config* cfg, int device, const hp_plan* req, hp_ctx** 
     TRY(dalloc(c, &c->A[1], CN));
     TRY(dalloc(c, &c->U, CN));
     TRY(dalloc(c, &c->snapshot, CN));
-    TRY(dalloc(c, &c->X, nsrc * g.spec));
-    TRY(dalloc(c, &c->Y, (size_t)std::max(1, c->K) * g.spec));
+    c->direct = plan.direct;
+    c->direct_rmax = plan.rmax;
+    // spectra only for the FFT path
+    TRY(dalloc(c, &c->X, c->direct ? 1 : nsrc * g.spec));
+    TRY(dalloc(c, &c->Y, c->direct ? 1 : (size_t)std::max(1, c->K) * g.spec));
     TRY(dalloc(c, &c->Kq, kq.size()));
+    if (c->direct) {
+        TRY(dalloc(c, &c->d_dk, std::max<size_t>(1, plan.dk.size())));
+        TRY(dalloc(c, &c->d_taps, std::max<size_t>(1, plan.taps.size())));
+        TRY(dalloc(c, &c->d_cols, std::max<size_t>(1, plan.cols.size())));
+        TRY(dalloc(c, &c->d_conv, (size_t)std::max(1, c->K) * N));
+        TRY(upload(c, c->d_dk, plan.dk));
+        TRY(upload(c, c->d_taps, plan.taps));
+        TRY(upload(c, c->d_cols, plan.cols));
+    }
     TRY(dalloc(c, &c->d_src_chan, nsrc));
     TRY(dalloc(c, &c->d_spec_work, std::max<size_t>(1, sw.size())));
     TRY(dalloc(c, &c->d_klist_spec, std::max<size_t>(1, kls.size())));
@@ -848,6 +875,13 @@ hp_status hp::ctx_enqueue_conv(hp_ctx* c, const float* A, float* conv) {
     if (!c || !A || !conv) return HP_ERR_NULL;
     if (c->wave || c->is3d) return c->fail(HP_ERR_UNSUPPORTED_SIZE, "convolutions need a 2D context on the LDS path");
     if (c->K == 0) return HP_OK;
+    if (c->direct) {
+        if (!hp::launch_direct_affinity(c->stream, A, nullptr, conv, c->d_dk, c->d_taps, c->d_cols, c->K, c->C, c->g,
+                                        c->direct_rmax))
+            return c->fail(HP_ERR_UNSUPPORTED_SIZE, "kernel radius beyond the direct convolution's LDS tile");
+        HIP_TRY(c, hipGetLastError());
+        return HP_OK;
+    }
     const hp::Geo& g = c->g;
     hp::launch_rows_fwd(g.W, c->stream, A, c->X, c->d_src_chan, (int)c->src_chan.size(), g, c->tw[g.W], c->fv);
     hp::launch_spectral(g.H, 2, c->stream, c->X, c->Y, c->Kq, c->d_spec_work, c->n_spec_work, c->d_klist_spec, g,

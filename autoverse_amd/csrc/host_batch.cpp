@@ -828,6 +828,12 @@ hp_status create(const hp_config* cfgs, uint32_t count, int device, hp_batch** o
         if (st == HP_OK && cfg->depth != 1) st = HP_ERR_UNSUPPORTED_SIZE;
         std::string perr = "invalid configuration";
         if (st == HP_OK) st = hp::plan_member(cfg, false, 1, &b->members[i].plan, &perr);
+        if (st == HP_OK && b->members[i].plan.direct) {
+            // the batched passes and the gradient tape run on the FFT; such members step on
+            // their own contexts
+            st = HP_ERR_UNSUPPORTED_SIZE;
+            perr = "population members need sides 2^a * 3^b between 8 and 4096";
+        }
         if (st != HP_OK) {
             g_batch_error = "member " + std::to_string(i) + ": " + perr;
             release(b);

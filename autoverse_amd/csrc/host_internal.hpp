@@ -27,6 +27,12 @@ struct MemberPlan {
     std::vector<int> klist_tgt;
     std::vector<int> tgt_chan;
     std::vector<GrowthParams> gp;
+    // sides the FFT passes do not cover: the convolutions are summed directly
+    bool direct = false;
+    std::vector<DirectKernel> dk;
+    std::vector<float> taps;
+    std::vector<int2> cols;
+    int rmax = 0;
 };
 
 // group: 1 = one spectral item per source, 0 = one per kernel, -1 = choose.

@@ -151,10 +151,8 @@ hp_status plan_member(const hp_config* cfg, bool allow_wave, int group, MemberPl
     MemberPlan& p = *out;
     const bool is3d = cfg->depth > 1;
     const uint32_t W = cfg->width, H = cfg->height, D = cfg->depth;
-    if (!fft_supported((int)W) || !fft_supported((int)H) || (is3d && !fft_supported((int)D))) {
-        *err = "grid sides must be 2^k or 3 * 2^k between 8 and 2048";
-        return HP_ERR_UNSUPPORTED_SIZE;
-    }
+    // sides the LDS FFT passes do not cover take the direct convolution
+    p.direct = !fft_supported((int)W) || !fft_supported((int)H) || (is3d && !fft_supported((int)D));
     p.C = (int)cfg->channels;
     p.K = (int)cfg->num_kernels;
     p.is3d = is3d;
@@ -168,9 +166,9 @@ hp_status plan_member(const hp_config* cfg, bool allow_wave, int group, MemberPl
     g.Dq = is3d ? (int)(D / 2 + 1) : 1;
     g.N = (long long)W * H * D;
     g.spec = (long long)D * H * g.Wp;
-    p.wave = allow_wave && !is3d && wave_fft_supported(g.W, g.H);
-    // kernel spectra (host FP64)
-    p.kq.assign((size_t)std::max(1, p.K) * g.Dq * g.Hq * g.Wh, 0.0f);
+    p.wave = !p.direct && allow_wave && !is3d && wave_fft_supported(g.W, g.H);
+    // kernel spectra (host FP64); the direct path keeps the spatial taps instead
+    p.kq.assign(p.direct ? 1 : (size_t)std::max(1, p.K) * g.Dq * g.Hq * g.Wh, 0.0f);
     std::vector<int> slot_of(p.C, -1);
     p.gp.assign(std::max(1, p.K), GrowthParams{0, 0, 0, 0});
     for (int k = 0; k < p.K; k++) {
@@ -181,14 +179,46 @@ hp_status plan_member(const hp_config* cfg, bool allow_wave, int group, MemberPl
             *err = "kernel diameter 2R+1 exceeds a grid side";
             return HP_ERR_UNSUPPORTED_SIZE;
         }
-        std::vector<float> f = folded_spectrum(data, size, is3d ? 3 : 2, W, H, D);
+        if (p.direct) {
+            const int R = (int)size / 2, S = (int)size;
+            if (R > direct_max_radius(is3d ? 3 : 2)) {
+                *err = "kernel radius " + std::to_string(R) + " is beyond the direct convolution's LDS tile (sides " +
+                       "outside 2^a * 3^b up to 4096 take the direct path)";
+                return HP_ERR_UNSUPPORTED_SIZE;
+            }
+            DirectKernel d{};
+            d.src = (int)kc.source_channel;
+            d.tgt = (int)kc.target_channel;
+            d.R = R;
+            d.tap_off = (int)p.taps.size();
+            d.col_off = (int)p.cols.size();
+            const float sigma_sq_2 = 2.0f * kc.sigma * kc.sigma;  // growth.rs:51-52
+            d.weight = kc.weight;
+            d.mu = kc.mu;
+            d.inv = 1.0f / sigma_sq_2;
+            p.taps.insert(p.taps.end(), data.begin(), data.end());
+            // per column of the other axes, the nonzero range along the last axis
+            const int ncol = is3d ? S * S : S, stride = ncol;
+            for (int c = 0; c < ncol; c++) {
+                int lo = S, hi = -1;
+                for (int sl = 0; sl < S; sl++)
+                    if (data[(size_t)sl * stride + c] != 0.0f) {
+                        lo = std::min(lo, sl);
+                        hi = std::max(hi, sl);
+                    }
+                p.cols.push_back(make_int2(lo, hi));
+            }
+            p.dk.push_back(d);
+            p.rmax = std::max(p.rmax, R);
+        }
+        std::vector<float> f = p.direct ? std::vector<float>() : folded_spectrum(data, size, is3d ? 3 : 2, W, H, D);
         if (p.wave) {  // column-major [kx][kyf] for the y pass
             std::vector<float> t(f.size());
             for (int fy = 0; fy < g.Hq; fy++)
                 for (int fx = 0; fx < g.Wh; fx++) t[(size_t)fx * g.Hq + fy] = f[(size_t)fy * g.Wh + fx];
             f.swap(t);
         }
-        std::copy(f.begin(), f.end(), p.kq.begin() + (size_t)k * f.size());
+        if (!p.direct) std::copy(f.begin(), f.end(), p.kq.begin() + (size_t)k * f.size());
         if (slot_of[kc.source_channel] < 0) {
             slot_of[kc.source_channel] = (int)p.src_chan.size();
             p.src_chan.push_back((int)kc.source_channel);

@@ -36,6 +36,14 @@ struct GrowthParams {
     float sigma;
 };
 
+// One kernel of the direct convolution path: source and target channel, actual radius R, its
+// (2R+1)^dim taps at taps[tap_off] (raster order, x fastest), its per-column nonzero ranges along
+// the last axis at cols[col_off] ((2R+1)^(dim-1) entries), and its growth parameters.
+struct DirectKernel {
+    int src, tgt, R, tap_off, col_off;
+    float weight, mu, inv;
+};
+
 // Shapes of one 2D/3D grid and its half spectrum.
 struct Geo {
     int W, H, D;      // grid
@@ -117,6 +125,13 @@ bool launch_batch_rows_inv(int L, hipStream_t st, const MemberDesc* mem, const i
                            bool conv = false, int cslot = 0, const FftVariant& v = FftVariant());
 // columns per spectral workgroup for length-L columns
 int spectral_block_lines(int L, const FftVariant& v = FftVariant());
+
+// ---- direct convolution for sides the FFT passes do not cover: direct_conv.hip -------------
+// Every kernel's convolution into conv + k*N (K*N floats), then, with U non-null, U from them
+// (growth, kernel order).  Returns false if the largest radius Rmax does not fit the LDS tile.
+bool launch_direct_affinity(hipStream_t st, const float* A, float* U, float* conv, const DirectKernel* kd,
+                            const float* taps, const int2* cols, int K, int C, const Geo& g, int Rmax);
+int direct_max_radius(int dim);
 
 // ---- launchers implemented in wave_fft.hip (2D, sides 256..2048, column-major spectra) ----
 bool wave_fft_supported(int W, int H);

@@ -204,7 +204,10 @@ typedef struct hp_ctx hp_ctx;
 /* Build the propagator: validates the config, builds the ring kernels
  * (kernel.rs:32-86 / kernel3d.rs:32-97), their spectra, and every device buffer.
  * `device` is a HIP device ordinal.  Returns HP_ERR_NO_DEVICE when there is no
- * GPU; *out is NULL on any error. */
+ * GPU; *out is NULL on any error.  Any side is accepted: sides 2^a * 3^b from 8 to 4096
+ * convolve by FFT, others by a direct tap sum (the same circular convolution to rounding;
+ * kernel radius up to 85 in 2D, 13 in 3D).  HP_ERR_UNSUPPORTED_SIZE when a kernel's
+ * diameter 2R+1 exceeds a side (the reference panics). */
 hp_status hp_create(const hp_config* cfg, int device, hp_ctx** out);
 void hp_destroy(hp_ctx* ctx);
 
@@ -454,7 +457,7 @@ hp_status hp_batch_evaluate(hp_batch* b, const hp_eval_config* ev, const hp_metr
  * checkpoint_every = 0 picks the interval from free device memory: every step (no
  * recomputation) when the whole trajectory fits in half of it, else about sqrt(steps);
  * *used_every (optional) receives the interval used.
- * Limits: sides 2^k or 3 * 2^k (8 .. 1024); every kernel's support 2R+1 within the grid. */
+ * Limits: sides 2^a * 3^b (8 .. 4096); every kernel's support 2R+1 within the grid. */
 hp_status hp_batch_theta_count(const hp_batch* b, uint32_t member, uint32_t* count);
 /* Batch plans: the LDS FFT variant and the stencil tile shared by all members (fft_path and
  * spectral_group are fixed for populations: LDS FFTs, one spectral item per source).
