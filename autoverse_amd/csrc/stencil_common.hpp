@@ -34,6 +34,15 @@ __device__ __forceinline__ int wrap_tile(int c, int n, bool multi) {
 // |m| < 1e-10 is skipped; NaN is not.
 __device__ __forceinline__ bool active_mass(float m) { return !(fabsf(m) < 1e-10f); }
 
+// Workgroup barrier that orders LDS only: this wave's LDS accesses complete before it, and the
+// workgroup's LDS writes are visible after it.  Unlike __syncthreads() it does not wait for the
+// wave's outstanding global loads and stores (a release fence at workgroup scope would).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // OR of v over the 64 lanes of the wave.
 __device__ __forceinline__ unsigned wave_or(unsigned v) {
 #pragma unroll

@@ -108,6 +108,157 @@ constexpr int kStencilTiles = 6;
 // order, and rounds its total once.  A source's contributions then sum to its mass up to two
 // roundings, and the step conserves mass to FP64-oracle level; the reference's own arithmetic
 // drifts by about 2e-6 per 1000 steps.
+//
+// Gather of one strip of SR destinations.  Source rows are visited in the order ROTY, ROTY + 1,
+// ... wrapping at SROWS (the row rotation depends only on the strip's row, so it is a compile-time
+// instance chosen per wave); source columns in the lane's own order, column i of strip row 0
+// being source cb[i] with x overlap ob[i] (flat indices into the mass and x-overlap planes).  A
+// destination adds its contributions in that order, which is ascending wrapped raster order of
+// the sources.  With cb[i] = base + i every address is one base plus constants.
+template <int ROTY, bool POW2, bool CONS, class TL>
+__device__ __forceinline__ void gather_strip(const float* __restrict__ sM, const float* __restrict__ sOX,
+                                             const float* __restrict__ sOY, const int (&cb)[4], const int (&ob)[4],
+                                             float ta, float (&acc)[TL::SR], double (&accd)[CONS ? TL::SR : 1]) {
+    constexpr int SW = TL::SW, SR = TL::SR, SROWS = TL::SROWS, NSRC = TL::NSRC;
+#pragma unroll
+    for (int it = 0; it < SROWS; ++it) {
+        const int a = (it + ROTY) % SROWS;  // strip source row a <-> grid row gdy0 - 1 + a
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int si = cb[i] + a * SW;
+            const float m = sM[si];
+            const float ow = sOX[ob[i] + a * SW];
+#pragma unroll
+            for (int r = 0; r < SR; ++r) {
+                const int jy = a - r;  // o_y + 1
+                if (jy >= 0 && jy <= 3) {
+                    if (CONS) {
+                        accd[CONS ? r : 0] += (double)m * ((double)ow * (double)sOY[jy * NSRC + si]);
+                    } else {
+                        const float area = ow * sOY[jy * NSRC + si];
+                        acc[r] += POW2 ? m * area : m * (area / ta);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Gather of one strip that touches a periodic edge: source rows in the order roty, roty + 1, ...
+// wrapping at SROWS, source columns in the order rotx, rotx + 1, ... mod 4 (both per lane, roty
+// in [0, SROWS)), base the lane's source (strip row 0, column o = -1).  The loop over rows stays
+// rolled; every (source, destination) pair is formed from in-bounds LDS entries and added only
+// where the source row reaches the destination, so each destination adds exactly the pairs of
+// the unrotated instance, in ascending wrapped raster order of the sources.
+template <bool POW2, bool CONS, class TL>
+__device__ __forceinline__ void gather_strip_edge(const float* __restrict__ sM, const float* __restrict__ sOX,
+                                                  const float* __restrict__ sOY, int base, int rotx, int roty,
+                                                  float ta, float (&acc)[TL::SR],
+                                                  double (&accd)[CONS ? TL::SR : 1]) {
+    constexpr int SW = TL::SW, SR = TL::SR, SROWS = TL::SROWS, NSRC = TL::NSRC;
+    int cb[4], jn[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int jx = (i + rotx) & 3;  // plane jx = o + 1, o = source - dest
+        cb[i] = base + jx;
+        jn[i] = jx * NSRC;
+    }
+#pragma unroll 1
+    for (int it = 0; it < SROWS; ++it) {
+        int a = it + roty;  // strip source row a <-> grid row gdy0 - 1 + a
+        if (a >= SROWS) a -= SROWS;
+        const int ro = a * SW;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int si = cb[i] + ro;
+            const float m = sM[si];
+            const float ow = sOX[jn[i] + si];
+#pragma unroll
+            for (int r = 0; r < SR; ++r) {
+                const int jy = a - r;  // o_y + 1
+                const bool ok = (unsigned)jy <= 3u;
+                const float oy = sOY[(ok ? jy : 0) * NSRC + si];
+                if (CONS) {
+                    const double v = (double)m * ((double)ow * (double)oy);
+                    accd[CONS ? r : 0] = ok ? accd[CONS ? r : 0] + v : accd[CONS ? r : 0];
+                } else {
+                    const float area = ow * oy;
+                    const float v = POW2 ? m * area : m * (area / ta);
+                    acc[r] = ok ? acc[r] + v : acc[r];
+                }
+            }
+        }
+    }
+}
+
+// One source of a 2D tile: its flow from U's Sobel and the alpha terms, its box, its four x and
+// four y overlaps with the destination cells it can reach (in its own wrapped frame, as the
+// scatter computes them) and its scaled mass, into the LDS planes (plane stride NSRC).  i is the
+// source's index in the (TX+3) x (TY+3) source region, (sy, sx) its row and column there, su the
+// channel's affinity over the load region, (gx, gy) the source's wrapped grid cell.  A source the
+// tile owns whose box leaves the 16-source window sets bit 0 of local_flag (bit 1 when a value is
+// not finite).
+template <bool POW2, bool CONS, class TL>
+__device__ __forceinline__ void source_one(int i, int sy, int sx, float m, const float* __restrict__ su, float aOm,
+                                           float aGx, float aGy, int gx, int gy, int x0, int y0, int W, int H,
+                                           float dt, float s, float inv_ta, float* __restrict__ sM,
+                                           float* __restrict__ sOX, float* __restrict__ sOY, int& local_flag) {
+    constexpr int LW = TL::LW, NSRC = TL::NSRC, TX = TL::TX, TY = TL::TY;
+    if (fabsf(m) < 1e-10f) {  // reintegration.rs:76: a skipped source adds nothing
+        sM[i] = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) sOX[j * NSRC + i] = sOY[j * NSRC + i] = 0.0f;
+        return;
+    }
+    const float* r0 = su + sy * LW + sx;
+    const float* r1 = r0 + LW;
+    const float* r2 = r1 + LW;
+    float gux, guy;
+    sobel9(r0[0], r0[1], r0[2], r1[0], r1[2], r2[0], r2[1], r2[2], gux, guy);
+    const float fx = aOm * gux - aGx;
+    const float fy = aOm * guy - aGy;
+    // integer-valued floats below 2^24: (float)(g + k) == (float)g + k exactly
+    const float cx = (float)gx, cy = (float)gy;
+    const float dxp = cx + dt * fx;  // reintegration.rs:81-82
+    const float dyp = cy + dt * fy;
+    const float xm = dxp - s, xM = dxp + s, ym = dyp - s, yM = dyp + s;
+    // overlaps with destination cells g + 1 - j, j = o + 1 (o = source - dest): cell
+    // [c + 1 - j, c + 2 - j)
+    float ovx[4], ovy[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        ovx[j] = overlap_clamped(cx + (float)(1 - j), cx + (float)(2 - j), xm, xM);
+        ovy[j] = overlap_clamped(cy + (float)(1 - j), cy + (float)(2 - j), ym, yM);
+        sOX[j * NSRC + i] = ovx[j];
+        sOY[j * NSRC + i] = ovy[j];
+    }
+    // With (2s)^2 a power of two the scaling by its inverse is exact (the masses skipped above are
+    // the only ones below 1e-10, every nonzero overlap is a multiple of an ulp of s or of a
+    // coordinate, so no product is subnormal before the last one): m * ((ow*oh) * inv) equals
+    // (m * inv) * (ow*oh) bit for bit, and the gather saves a multiply per pair.  A scaled mass
+    // that overflows is sent to the general path below.
+    float sm;
+    if (CONS) {  // widths summed over cells c-2 .. c+1, ascending
+        const double wx = (((0.0 + (double)ovx[3]) + (double)ovx[2]) + (double)ovx[1]) + (double)ovx[0];
+        const double wy = (((0.0 + (double)ovy[3]) + (double)ovy[2]) + (double)ovy[1]) + (double)ovy[0];
+        const double pw = wx * wy;
+        sm = pw > 0.0 ? (float)((double)m / pw) : 0.0f;
+    } else {
+        sm = POW2 ? m * inv_ta : m;
+    }
+    sM[i] = sm;
+    // every source is checked by the one tile that owns it
+    const int ux = x0 - 1 + sx, uy = y0 - 1 + sy;
+    if (ux >= x0 && ux < x0 + TX && uy >= y0 && uy < y0 + TY && ux < W && uy < H) {
+        // floor(xm) < c - 2  <=>  xm < c - 2 for integer c (likewise ceil); a non-finite value
+        // fails the comparisons and is classified on the slow side
+        const bool inside = xm >= cx - 2.0f && xM <= cx + 2.0f && ym >= cy - 2.0f && yM <= cy + 2.0f &&
+                            (CONS || fabsf(sm) <= 3.40282347e38f);
+        if (!inside)
+            local_flag |= (isfinite(xm) && isfinite(xM) && isfinite(ym) && isfinite(yM) && isfinite(m)) ? 1 : 2;
+    }
+}
+
 template <int CC, bool POW2, bool CONS, class TL>
 __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, const float* __restrict__ U,
                                                float* __restrict__ Aout, int Cdyn, int W, int H, long long N,
@@ -184,7 +335,17 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
             sU[0][i] = upre[q];
         }
     }
-    __syncthreads();
+    // the next channel's affinity travels in registers ahead of its use: channel c + 1's is
+    // loaded before channel c's source phase (its latency hides behind that phase and the
+    // gather before it) and goes into the other LDS buffer after channel c's first barrier.  The
+    // loads are unconditional (past the last channel they reread it) and are issued before the
+    // previous gather's stores, so that nothing waits for the stores.
+    {
+        const float* Un = U + (long long)(C > 1 ? 1 : 0) * N;
+#pragma unroll
+        for (int q = 0; q < LPT; ++q) upre[q] = Un[goff[q]];
+    }
+    lds_barrier();
     unsigned act = 0;
 #pragma unroll
     for (int w = 0; w < NT / 64; ++w) act |= sAct[w];
@@ -233,7 +394,8 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
     const int gdx = x0 + tx, gdy0 = y0 + ty0;
     // ascending wrapped order: start with the first source column / row that does not wrap low
     const int rotx = gdx < 1 ? 1 - gdx : (gdx + 2 >= W ? 4 - (gdx + 2 - (W - 1)) : 0);
-    const int roty = gdy0 < 1 ? 1 - gdy0 : (gdy0 + SR + 1 >= H ? SROWS - (gdy0 + SR + 1 - (H - 1)) : 0);
+    int roty = gdy0 < 1 ? 1 - gdy0 : (gdy0 + SR + 1 >= H ? SROWS - (gdy0 + SR + 1 - (H - 1)) : 0);
+    if (roty < 0 || roty >= SROWS) roty = 0;  // strips below the grid's last row store nothing
 
     int local_flag = 0;
 #pragma unroll 1
@@ -255,67 +417,10 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
             } else {
                 m = A[(long long)c * N + (long long)sgy[q] * W + sgx[q]];
             }
-            if (fabsf(m) < 1e-10f) {  // reintegration.rs:76: a skipped source adds nothing
-                sM[i] = 0.0f;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) sOX[j][i] = sOY[j][i] = 0.0f;
-                continue;
-            }
-            const float* r0 = su + sy * LW + sx;
-            const float* r1 = r0 + LW;
-            const float* r2 = r1 + LW;
-            float gux, guy;
-            sobel9(r0[0], r0[1], r0[2], r1[0], r1[2], r2[0], r2[1], r2[2], gux, guy);
-            const float fx = aOm[q] * gux - aGx[q];
-            const float fy = aOm[q] * guy - aGy[q];
-            // integer-valued floats below 2^24: (float)(g + k) == (float)g + k exactly
-            const float cx = (float)sgx[q], cy = (float)sgy[q];
-            const float dxp = cx + dt * fx;  // reintegration.rs:81-82
-            const float dyp = cy + dt * fy;
-            const float xm = dxp - s, xM = dxp + s, ym = dyp - s, yM = dyp + s;
-            // overlaps with destination cells g + 1 - j, j = o + 1 (o = source - dest): cell
-            // [c + 1 - j, c + 2 - j)
-            float ovx[4], ovy[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                ovx[j] = overlap_clamped(cx + (float)(1 - j), cx + (float)(2 - j), xm, xM);
-                ovy[j] = overlap_clamped(cy + (float)(1 - j), cy + (float)(2 - j), ym, yM);
-                sOX[j][i] = ovx[j];
-                sOY[j][i] = ovy[j];
-            }
-            // With (2s)^2 a power of two the scaling by its inverse is exact (the masses skipped
-            // above are the only ones below 1e-10, every nonzero overlap is a multiple of an ulp
-            // of s or of a coordinate, so no product is subnormal before the last one): m * ((ow*oh)
-            // * inv) equals (m * inv) * (ow*oh) bit for bit, and the gather saves a multiply per
-            // pair.  A scaled mass that overflows is sent to the general path below.
-            float sm;
-            if (CONS) {  // widths summed over cells c-2 .. c+1, ascending
-                const double wx = (((0.0 + (double)ovx[3]) + (double)ovx[2]) + (double)ovx[1]) + (double)ovx[0];
-                const double wy = (((0.0 + (double)ovy[3]) + (double)ovy[2]) + (double)ovy[1]) + (double)ovy[0];
-                const double pw = wx * wy;
-                sm = pw > 0.0 ? (float)((double)m / pw) : 0.0f;
-            } else {
-                sm = POW2 ? m * inv_ta : m;
-            }
-            sM[i] = sm;
-            // every source is checked by the one tile that owns it
-            const int ux = x0 - 1 + sx, uy = y0 - 1 + sy;
-            if (ux >= x0 && ux < x0 + TX && uy >= y0 && uy < y0 + TY && ux < W && uy < H) {
-                // floor(xm) < c - 2  <=>  xm < c - 2 for integer c (likewise ceil); a non-finite
-                // value fails the comparisons and is classified on the slow side
-                const bool inside = xm >= cx - 2.0f && xM <= cx + 2.0f && ym >= cy - 2.0f && yM <= cy + 2.0f &&
-                                    (CONS || fabsf(sm) <= 3.40282347e38f);
-                if (!inside)
-                    local_flag |= (isfinite(xm) && isfinite(xM) && isfinite(ym) && isfinite(yM) && isfinite(m)) ? 1 : 2;
-            }
+            source_one<POW2, CONS, TL>(i, sy, sx, m, su, aOm[q], aGx[q], aGy[q], sgx[q], sgy[q], x0, y0, W, H, dt, s,
+                                       inv_ta, sM, sOX[0], sOY[0], local_flag);
         }
-        // next channel's affinity: into registers now, into the other LDS buffer after the barrier
-        if (c + 1 < C) {
-            const float* Un = U + (long long)(c + 1) * N;
-#pragma unroll
-            for (int q = 0; q < LPT; ++q) upre[q] = Un[goff[q]];
-        }
-        __syncthreads();
+        lds_barrier();
         if (c + 1 < C) {
 #pragma unroll
             for (int q = 0; q < LPT; ++q)
@@ -324,67 +429,34 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
         // ---- gather: strip of SR destinations ----
         // A zero-area pair contributes m * 0 = +-0; added to an accumulator that started at +0 it
         // changes no bit, so every pair is added without the reference's `area > 0` test.
-        // Waves with no periodic edge in reach take the unrotated path, whose LDS addresses are
-        // one base plus constants; waves touching an edge visit in rotated order.
+        // Waves with no periodic edge in reach take the unrotated instance, whose LDS addresses
+        // are one base plus constants; a wave touching an edge takes the instance of its row
+        // rotation with per-lane column bases.  (Lanes past the grid's right edge gather
+        // in-bounds LDS entries and store nothing.)
         float acc[SR];
         double accd[CONS ? SR : 1];
 #pragma unroll
         for (int r = 0; r < SR; ++r) acc[r] = 0.0f;
 #pragma unroll
         for (int r = 0; r < (CONS ? SR : 1); ++r) accd[r] = 0.0;
-        if (!live) {
-        } else if (__all(rotx == 0 && roty == 0 && gdx < W)) {
-            const int base = ty0 * SW + tx;  // source (row 0, column o = -1) of the strip
-#pragma unroll
-            for (int a = 0; a < SROWS; ++a) {  // strip source row a <-> grid row gdy0 - 1 + a
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {  // plane j = o + 1, o = source - dest
-                    const int si = base + a * SW + j;
-                    const float m = sM[si];
-                    const float ow = sOX[j][si];
-#pragma unroll
-                    for (int r = 0; r < SR; ++r) {
-                        if (a - r >= 0 && a - r <= 3) {
-                            if (CONS) {
-                                accd[CONS ? r : 0] += (double)m * ((double)ow * (double)sOY[a - r][si]);
-                            } else {
-                                const float area = ow * sOY[a - r][si];
-                                acc[r] += POW2 ? m * area : m * (area / ta);
-                            }
-                        }
-                    }
-                }
-            }
-        } else if (gdx < W) {
-#pragma unroll 1
-            for (int it = 0; it < SROWS; ++it) {
-                int a = it + roty;
-                if (a >= SROWS) a -= SROWS;
-                const int srow = (ty0 + a) * SW + tx + 1;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int jx = (i + rotx) & 3;
-                    const int si = srow + jx - 1;
-                    const float m = sM[si];
-                    const float ow = sOX[jx][si];
-#pragma unroll
-                    for (int r = 0; r < SR; ++r) {
-                        const int jy = a - r;  // o_y + 1
-                        if (jy >= 0 && jy <= 3) {
-                            if (CONS) {
-                                accd[CONS ? r : 0] += (double)m * ((double)ow * (double)sOY[jy][si]);
-                            } else {
-                                const float area = ow * sOY[jy][si];
-                                acc[r] += POW2 ? m * area : m * (area / ta);
-                            }
-                        }
-                    }
-                }
+        if (live) {
+            const int base = ty0 * SW + tx;  // source (strip row 0, column o = -1) of the strip
+            if (__all(rotx == 0 && roty == 0 && gdx < W)) {
+                const int cb[4] = {base, base + 1, base + 2, base + 3};
+                const int ob[4] = {base, NSRC + base + 1, 2 * NSRC + base + 2, 3 * NSRC + base + 3};
+                gather_strip<0, POW2, CONS, TL>(sM, sOX[0], sOY[0], cb, ob, ta, acc, accd);
+            } else {
+                gather_strip_edge<POW2, CONS, TL>(sM, sOX[0], sOY[0], base, rotx, roty, ta, acc, accd);
             }
         }
         if (CONS) {
 #pragma unroll
             for (int r = 0; r < SR; ++r) acc[r] = (float)accd[CONS ? r : 0];
+        }
+        {  // channel c + 2's affinity (see above)
+            const float* Un = U + (long long)(c + 2 < C ? c + 2 : C - 1) * N;
+#pragma unroll
+            for (int q = 0; q < LPT; ++q) upre[q] = Un[goff[q]];
         }
         if (gdx < W) {
             float* Oc = Aout + (long long)c * N;
@@ -392,7 +464,7 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
             for (int r = 0; r < SR; ++r)
                 if (gdy0 + r < H) Oc[(long long)(gdy0 + r) * W + gdx] = acc[r];
         }
-        __syncthreads();  // the next channel's source phase overwrites the overlaps
+        lds_barrier();  // the next channel's source phase overwrites the overlaps
     }
     if (local_flag) atomicOr(flag, local_flag);
 }
