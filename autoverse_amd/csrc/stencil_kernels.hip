@@ -764,7 +764,7 @@ __global__ __launch_bounds__(TL::NT) void k_stencil3d(const float* __restrict__ 
             sU[i] = upre[q];
         }
     }
-    __syncthreads();
+    lds_barrier();
     unsigned act = 0;
 #pragma unroll
     for (int w = 0; w < NT / 64; ++w) act |= sAct[w];
@@ -805,7 +805,7 @@ __global__ __launch_bounds__(TL::NT) void k_stencil3d(const float* __restrict__ 
         aGy[q] = ok ? sOX[2 * NSRC + i] : 0.0f;
         aGz[q] = ok ? sOX[3 * NSRC + i] : 0.0f;
     }
-    __syncthreads();  // the channel sum (in the z overlap planes) has been read
+    lds_barrier();  // the channel sum (in the z overlap planes) has been read
 
     const int rx = wrap_rotation(gdx, W), ry = wrap_rotation(gdy, H);
     // ascending wrapped order of the strip's planes: start with the first that does not wrap low
@@ -897,7 +897,7 @@ __global__ __launch_bounds__(TL::NT) void k_stencil3d(const float* __restrict__ 
 #pragma unroll
             for (int q = 0; q < SPT; ++q) msrc[q] = An[soff[q]];
         }
-        __syncthreads();
+        lds_barrier();
         if (c + 1 < C) {
 #pragma unroll
             for (int q = 0; q < LPT; ++q)
@@ -944,11 +944,17 @@ __global__ __launch_bounds__(TL::NT) void k_stencil3d(const float* __restrict__ 
                 }
             }
         } else if (inside) {
-            int xo[4], yo[4];  // overlap index visited in slot k
+            // a strip at a periodic edge: the same pairs in rotated order (per lane in x and y,
+            // per strip in z), the loop over planes rolled, every pair formed from in-bounds LDS
+            // entries and added only where the source plane reaches the destination
+            int xo[4], yoS[4], xoN[4], yoN[4];  // slot k -> column jx (and jx * NSRC), row jy * SX (jy * NSRC)
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                xo[k] = (k + rx) & 3;
-                yo[k] = (k + ry) & 3;
+                const int jx = (k + rx) & 3, jy = (k + ry) & 3;
+                xo[k] = jx;
+                xoN[k] = jx * NSRC;
+                yoS[k] = jy * SX;
+                yoN[k] = jy * NSRC;
             }
 #pragma unroll 1
             for (int it = 0; it < SROWS; ++it) {
@@ -957,30 +963,31 @@ __global__ __launch_bounds__(TL::NT) void k_stencil3d(const float* __restrict__ 
                 const int pb = base + a * SY * SX;
 #pragma unroll
                 for (int ky = 0; ky < 4; ++ky) {
-                    const int jy = yo[ky];
-                    const int rb = pb + jy * SX;
+                    const int rb = pb + yoS[ky];
 #pragma unroll
                     for (int kx = 0; kx < 4; ++kx) {
-                        const int jx = xo[kx];
-                        const int si = rb + jx;
+                        const int si = rb + xo[kx];
                         const float m = sM[si];
+                        const float ox = sOX[xoN[kx] + si], oy = sOY[yoN[ky] + si];
                         if (CONS) {
-                            const double ad = (double)sOX[jx * NSRC + si] * (double)sOY[jy * NSRC + si];
+                            const double ad = (double)ox * (double)oy;
 #pragma unroll
                             for (int r = 0; r < SR; ++r) {
                                 const int jz = a - r;
-                                if (jz >= 0 && jz <= 3) accd[CONS ? r : 0] += (double)m * (ad * (double)sOZ[jz * NSRC + si]);
+                                const bool ok = (unsigned)jz <= 3u;
+                                const double v = (double)m * (ad * (double)sOZ[(ok ? jz : 0) * NSRC + si]);
+                                accd[CONS ? r : 0] = ok ? accd[CONS ? r : 0] + v : accd[CONS ? r : 0];
                             }
                             continue;
                         }
-                        const float area = sOX[jx * NSRC + si] * sOY[jy * NSRC + si];
+                        const float area = ox * oy;
 #pragma unroll
                         for (int r = 0; r < SR; ++r) {
                             const int jz = a - r;
-                            if (jz >= 0 && jz <= 3) {
-                                const float vol = area * sOZ[jz * NSRC + si];
-                                acc[r] += POW2 ? m * vol : m * (vol / tv);
-                            }
+                            const bool ok = (unsigned)jz <= 3u;
+                            const float vol = area * sOZ[(ok ? jz : 0) * NSRC + si];
+                            const float v = POW2 ? m * vol : m * (vol / tv);
+                            acc[r] = ok ? acc[r] + v : acc[r];
                         }
                     }
                 }
@@ -996,7 +1003,7 @@ __global__ __launch_bounds__(TL::NT) void k_stencil3d(const float* __restrict__ 
             for (int r = 0; r < SR; ++r)
                 if (gdz0 + r < D) Oc[(gdz0 + r) * HW] = acc[r];
         }
-        if (c + 1 < C) __syncthreads();  // the next channel's source phase overwrites the overlaps
+        if (c + 1 < C) lds_barrier();  // the next channel's source phase overwrites the overlaps
     }
     if (local_flag) atomicOr(flag, local_flag);
 }
