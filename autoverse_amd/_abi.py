@@ -198,6 +198,8 @@ EXPORTS = [
                             POINTER(c_uint32)]),
     ("hp_batch_get_plan", c_int, [c_void_p, POINTER(HpPlan)]),
     ("hp_batch_set_plan", c_int, [c_void_p, POINTER(HpPlan)]),
+    ("hp_batch_set_step_mode", c_int, [c_void_p, c_int32]),
+    ("hp_batch_step_mode", c_int, [c_void_p, POINTER(c_int32), POINTER(c_uint64)]),
     ("hp_batch_autotune", c_int, [c_void_p, c_uint32, POINTER(HpPlan), c_uint32, POINTER(HpPlan),
                                   POINTER(c_double), POINTER(c_uint32)]),
     ("hp_eval_sample_count", c_int, [POINTER(HpEvalConfig), POINTER(c_uint32)]),

@@ -6,7 +6,7 @@ Per-file floating-point flags:
   * stencil_kernels.hip, embedded.hip, fitness.hip, noalloc.hip, adjoint_stencil.hip,
     host*.cpp: -ffp-contract=off -- these reproduce the
     reference's f32 arithmetic operation for operation (Rust never fuses a*b+c);
-  * fft_*.hip, wave_fft.hip, adjoint_fft.hip: -ffp-contract=fast-honor-pragmas -- the FFT is not
+  * fft_*.hip, wave_fft.hip, adjoint_fft.hip, batch_step.hip: -ffp-contract=fast-honor-pragmas -- the FFT is not
     bitwise anyway, FMAs make it faster and more accurate; the LDS path's growth epilogue
     (growth_term) turns contraction back off with a pragma, the wave path's is contracted like
     the rest of its pass (it uses the hardware exp2 and is tolerance-checked either way).
@@ -40,6 +40,7 @@ SOURCES = [
     ("fft_g3.hip", DEVICE_FLAGS + ["-ffp-contract=fast-honor-pragmas"]),
     ("fft_g4.hip", DEVICE_FLAGS + ["-ffp-contract=fast-honor-pragmas"]),
     ("wave_fft.hip", DEVICE_FLAGS + ["-ffp-contract=fast-honor-pragmas"]),
+    ("batch_step.hip", DEVICE_FLAGS + ["-ffp-contract=fast-honor-pragmas"]),
     ("stencil_kernels.hip", DEVICE_FLAGS + ["-ffp-contract=off"]),
     ("noalloc.hip", DEVICE_FLAGS + ["-ffp-contract=off"]),
     ("fitness.hip", DEVICE_FLAGS + ["-ffp-contract=off"]),

@@ -35,12 +35,13 @@ namespace hp {
 
 // ---- pass 1: R2C along x on pairs of real rows ----------------------------------------
 // grid (ceil(rows/2), nsrc); rows = H*D; src_chan[s] = channel of source slot s.
+// The pass bodies take their LDS arrays as pointers (the *_core functions), so that a kernel that
+// runs several passes can lay them over one buffer; the *_body wrappers own static arrays.
 template <int L, int NT, int RP>
-__device__ __forceinline__ void rows_fwd_body(const float* __restrict__ A, float2* __restrict__ X,
+__device__ __forceinline__ void rows_fwd_core(float2* __restrict__ buf, float2* __restrict__ tw,
+                                              const float* __restrict__ A, float2* __restrict__ X,
                                               const int* __restrict__ src_chan, int rows, int Wp, long long N,
                                               long long spec, const float2* __restrict__ twg, int pair, int s) {
-    __shared__ float2 buf[L];
-    __shared__ float2 tw[L];
     const int y0 = 2 * pair;
     const bool has1 = (y0 + 1) < rows;
     const float* a = A + (long long)src_chan[s] * N + (long long)y0 * L;
@@ -56,6 +57,15 @@ __device__ __forceinline__ void rows_fwd_body(const float* __restrict__ A, float
         x0[k] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
         if (has1) x1[k] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
     }
+}
+
+template <int L, int NT, int RP>
+__device__ __forceinline__ void rows_fwd_body(const float* __restrict__ A, float2* __restrict__ X,
+                                              const int* __restrict__ src_chan, int rows, int Wp, long long N,
+                                              long long spec, const float2* __restrict__ twg, int pair, int s) {
+    __shared__ float2 buf[L];
+    __shared__ float2 tw[L];
+    rows_fwd_core<L, NT, RP>(buf, tw, A, X, src_chan, rows, Wp, N, spec, twg, pair, s);
 }
 
 template <int L, int NT, int RP>
@@ -104,17 +114,16 @@ __global__ __launch_bounds__(NT) void k_cols(float2* __restrict__ X, int Wh, int
 // DIM 3: lines are ky*Wp + kx (element stride H*Wp, L = D).
 // grid (ceil(lines/CB), nwork); work[w] names the source slot and its kernels.
 template <int L, int CB, int NT, int DIM, int RP>
-__device__ __forceinline__ void spectral_body(const float2* __restrict__ X, float2* __restrict__ Y,
+__device__ __forceinline__ void spectral_core(float2* __restrict__ buf, float2* __restrict__ tw,
+                                              const float2* __restrict__ X, float2* __restrict__ Y,
                                               const float* __restrict__ Kq, const SpecWork* __restrict__ work,
                                               const int* __restrict__ klist, int Wh, int Wp, int H, int Hq,
                                               long long spec, long long kq_stride, const float2* __restrict__ twg,
                                               int blk, int wi) {
-    constexpr int LP = L + 1;
+    constexpr int LP = L + 1;  // buf holds CB * LP elements
     constexpr int TOT = CB * L;
     constexpr int PER = (TOT + NT - 1) / NT;  // elements per thread; EVEN: no thread runs past TOT
     constexpr bool EVEN = TOT % NT == 0;
-    __shared__ float2 buf[CB * LP];
-    __shared__ float2 tw[L];
     const SpecWork wk = work[wi];
     const long long nlines = DIM == 2 ? (long long)Wh : (long long)H * Wp;
     const long long es = DIM == 2 ? (long long)Wp : (long long)H * Wp;
@@ -181,6 +190,17 @@ __device__ __forceinline__ void spectral_body(const float2* __restrict__ X, floa
 }
 
 template <int L, int CB, int NT, int DIM, int RP>
+__device__ __forceinline__ void spectral_body(const float2* __restrict__ X, float2* __restrict__ Y,
+                                              const float* __restrict__ Kq, const SpecWork* __restrict__ work,
+                                              const int* __restrict__ klist, int Wh, int Wp, int H, int Hq,
+                                              long long spec, long long kq_stride, const float2* __restrict__ twg,
+                                              int blk, int wi) {
+    __shared__ float2 buf[CB * (L + 1)];
+    __shared__ float2 tw[L];
+    spectral_core<L, CB, NT, DIM, RP>(buf, tw, X, Y, Kq, work, klist, Wh, Wp, H, Hq, spec, kq_stride, twg, blk, wi);
+}
+
+template <int L, int CB, int NT, int DIM, int RP>
 __global__ __launch_bounds__(NT) void k_spectral(const float2* __restrict__ X, float2* __restrict__ Y,
                                                  const float* __restrict__ Kq, const SpecWork* __restrict__ work,
                                                  const int* __restrict__ klist, int Wh, int Wp, int H, int Hq,
@@ -213,14 +233,13 @@ __device__ __forceinline__ float growth_term(float c, const GrowthParams& p) {
 // CONV: also store every kernel's convolution rows (conv + k*N), for the adjoint's recompute.
 // GROW false: the convolutions only (no growth, no U), for growth that reads per-cell values.
 template <int L, int NT, bool CONV, int RP, bool GROW = true>
-__device__ __forceinline__ void rows_inv_body(const float2* __restrict__ Y, float* __restrict__ U,
+__device__ __forceinline__ void rows_inv_core(float2* __restrict__ buf, float2* __restrict__ tw,
+                                              const float2* __restrict__ Y, float* __restrict__ U,
                                               const TgtWork* __restrict__ work, const int* __restrict__ klist,
                                               const GrowthParams* __restrict__ gp, int rows, int Wp, long long N,
                                               long long spec, float scale, const float2* __restrict__ twg, int pair,
                                               int wi, float* __restrict__ conv = nullptr) {
     constexpr int PER = (L + NT - 1) / NT;
-    __shared__ float2 buf[L];
-    __shared__ float2 tw[L];
     const TgtWork wk = work[wi];
     const int y0 = 2 * pair;
     const bool has1 = (y0 + 1) < rows;
@@ -283,6 +302,18 @@ __device__ __forceinline__ void rows_inv_body(const float2* __restrict__ Y, floa
             if (has1) u0[L + m] = acc1[q];
         }
     }
+}
+
+template <int L, int NT, bool CONV, int RP, bool GROW = true>
+__device__ __forceinline__ void rows_inv_body(const float2* __restrict__ Y, float* __restrict__ U,
+                                              const TgtWork* __restrict__ work, const int* __restrict__ klist,
+                                              const GrowthParams* __restrict__ gp, int rows, int Wp, long long N,
+                                              long long spec, float scale, const float2* __restrict__ twg, int pair,
+                                              int wi, float* __restrict__ conv = nullptr) {
+    __shared__ float2 buf[L];
+    __shared__ float2 tw[L];
+    rows_inv_core<L, NT, CONV, RP, GROW>(buf, tw, Y, U, work, klist, gp, rows, Wp, N, spec, scale, twg, pair, wi,
+                                         conv);
 }
 
 template <int L, int NT, int RP>

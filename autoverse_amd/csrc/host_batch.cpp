@@ -88,6 +88,13 @@ struct hp_batch {
     double* d_mass = nullptr;
     int cur = 0;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // step mode: one launch per pass and class (default) or one persistent launch per step
+    bool persistent = false;
+    bool persistent_ok = false;  // every member's sides are compiled lengths of the persistent step
+    ItemList pitems;             // the persistent step's item segments, back to back
+    int pstart[32] = {0};        // segment starts (batch_step_persistent_segments() + 1 used)
+    unsigned* d_bar = nullptr;   // its grid-barrier counters (zero between launches)
+    uint64_t barrier_timeouts = 0;
     uint32_t max_radius = 0;
     uint64_t general_steps = 0;
     // plan: LDS FFT variant and stencil tile shared by every member
@@ -184,9 +191,19 @@ void enqueue_stencils(hp_batch* b, int cur) {
                                    b->tile);
 }
 
+// Bound on a grid barrier's polls (a few tenths of a second of sleeps): far past any phase's
+// length, short enough that a grid that cannot be resident ends the launch quickly.
+constexpr int kBarrierSpins = 1 << 20;
+
 hp_status enqueue_step(hp_batch* b, int cur) {
-    enqueue_fft_passes(b, cur, false);
-    enqueue_stencils(b, cur);
+    if (b->persistent) {
+        hp::launch_batch_step_persistent(b->stream, b->d_desc, b->pitems.d, b->pstart, cur, b->d_flags, b->d_bar,
+                                         std::min(hp::batch_step_persistent_grid(), std::max(1, b->pitems.n)),
+                                         kBarrierSpins);
+    } else {
+        enqueue_fft_passes(b, cur, false);
+        enqueue_stencils(b, cur);
+    }
     BTRY(b, hipGetLastError());
     return HP_OK;
 }
@@ -282,6 +299,7 @@ hp_status advance(hp_batch* b, uint64_t steps) {
     }
     int f = 0;
     STRY(read_flag(b, &f));
+    if (f & 4) b->barrier_timeouts++;  // a persistent step's grid barrier timed out: recompute
     if (f != 0) {  // replay from the snapshot, one checked step at a time
         b->cur = start;
         copy_states(b, start, -1, false);
@@ -797,7 +815,72 @@ hp_status build_items(hp_batch* b) {
         gr.items.n = (int)it.size();
         b->stencil_groups.push_back(gr);
     }
+    // the persistent step's phases, when every member's sides are among its lengths
+    b->persistent_ok = true;
+    for (const HostMember& hm : b->members)
+        if (!hp::batch_step_persistent_size(hm.plan.g.W) || !hp::batch_step_persistent_size(hm.plan.g.H) ||
+            hm.plan.g.D != 1)
+            b->persistent_ok = false;
+    b->pitems = ItemList{};
+    const int nseg = hp::batch_step_persistent_segments();
+    if (b->persistent_ok && nseg + 1 <= (int)(sizeof(b->pstart) / sizeof(b->pstart[0]))) {
+        std::vector<std::vector<int4>> seg(nseg);
+        const int PTX = hp::batch_step_persistent_tile_w(), PTY = hp::batch_step_persistent_tile_h();
+        for (uint32_t i = 0; i < (uint32_t)b->members.size(); i++) {
+            const hp::MemberPlan& p = b->members[i].plan;
+            const HostMember& hm = b->members[i];
+            const int pairs = (p.g.H + 1) / 2;
+            const int CB = hp::batch_step_persistent_block_lines(p.g.H);
+            if (p.K > 0) {
+                auto& rf = seg[hp::batch_step_persistent_segment(0, p.g.W, p.C, false)];
+                for (int sl = 0; sl < (int)p.src_chan.size(); sl++)
+                    for (int pr = 0; pr < pairs; pr++) rf.push_back(make_int4((int)i, pr, sl, 0));
+                auto& sp = seg[hp::batch_step_persistent_segment(1, p.g.H, p.C, false)];
+                for (int wi = 0; wi < (int)p.spec_work.size(); wi++)
+                    for (int blk = 0; blk < (p.g.Wh + CB - 1) / CB; blk++) sp.push_back(make_int4((int)i, blk, wi, 0));
+                auto& ri = seg[hp::batch_step_persistent_segment(2, p.g.W, p.C, false)];
+                for (int wi = 0; wi < (int)p.tgt_work.size(); wi++)
+                    for (int pr = 0; pr < pairs; pr++) ri.push_back(make_int4((int)i, pr, wi, 0));
+            }
+            const float s = hm.cfg.flow.distribution_size, ta = (2.0f * s) * (2.0f * s);
+            int e;
+            const bool pow2 = frexpf(ta, &e) == 0.5f && ta >= 1.17549435e-38f && ta < INFINITY;
+            auto& st = seg[hp::batch_step_persistent_segment(3, p.g.W, p.C, pow2)];
+            const int nt = ((p.g.W + PTX - 1) / PTX) * ((p.g.H + PTY - 1) / PTY);
+            for (int t = 0; t < nt; t++) st.push_back(make_int4((int)i, t, 0, 0));
+        }
+        std::vector<int4> all;
+        for (int k = 0; k < nseg; k++) {
+            b->pstart[k] = (int)all.size();
+            all.insert(all.end(), seg[k].begin(), seg[k].end());
+        }
+        b->pstart[nseg] = (int)all.size();
+        STRY(bupload(b, &b->pitems.d, all, L));
+        b->pitems.n = (int)all.size();
+    } else {
+        b->persistent_ok = false;
+    }
+    if (!b->persistent_ok) b->persistent = false;
     return HP_OK;
+}
+
+// Switches between the multi-launch step and the persistent step; the graphs are recaptured.
+hp_status batch_set_step_mode(hp_batch* b, int mode) {
+    if (mode != HP_STEP_MULTI_LAUNCH && mode != HP_STEP_PERSISTENT) return b->fail(HP_ERR_SHAPE, "unknown step mode");
+    if (mode == HP_STEP_PERSISTENT) {
+        if (!b->persistent_ok)
+            return b->fail(HP_ERR_UNSUPPORTED_SIZE,
+                           "the persistent step covers member sides 64, 128, 192 and 256 only");
+        if (b->fv.rp != 0) return b->fail(HP_ERR_SHAPE, "the persistent step runs radix order 0");
+    }
+    BTRY(b, hipStreamSynchronize(b->stream));
+    for (int p = 0; p < 2; p++)
+        if (b->graph[p]) {
+            (void)hipGraphExecDestroy(b->graph[p]);
+            b->graph[p] = nullptr;
+        }
+    b->persistent = mode == HP_STEP_PERSISTENT;
+    return build_graphs(b);
 }
 
 void release(hp_batch* b) {
@@ -943,6 +1026,8 @@ hp_status create(const hp_config* cfgs, uint32_t count, int device, hp_batch** o
         return HP_ERR_UNSUPPORTED_SIZE;
     }
     TRY(balloc(b, &b->d_flags, 4));
+    TRY(balloc(b, &b->d_bar, 2));
+    BTRY(b, hipMemset(b->d_bar, 0, 2 * sizeof(unsigned)));
     TRY(balloc(b, &b->box_lo, max_cn));
     TRY(balloc(b, &b->box_hi, max_cn));
     TRY(balloc(b, &b->d_moments, 3 * (size_t)count));
@@ -960,6 +1045,7 @@ hp_status batch_set_plan(hp_batch* b, const hp_plan& p) {
     if (p.radix_order < 0 || p.radix_order > 1 || p.column_split < 1 || p.column_split > 2 || p.stencil_tile < 0 ||
         p.stencil_tile >= hp::stencil2d_tile_count())
         return b->fail(HP_ERR_SHAPE, "plan field out of range");
+    if (b->persistent && p.radix_order != 0) return b->fail(HP_ERR_SHAPE, "the persistent step runs radix order 0");
     for (const HostMember& hm : b->members)
         if (!hp::stencil2d_supported(hm.plan.g, hm.cfg.flow.distribution_size, p.stencil_tile))
             return b->fail(HP_ERR_UNSUPPORTED_SIZE, "stencil tile does not fit every member's grid");
@@ -1028,6 +1114,7 @@ hp_status batch_autotune(hp_batch* b, uint32_t steps, hp_plan* best_out, uint32_
     const hp_plan start = batch_plan(b);
     hp_plan best = start;
     auto measure = [&](const hp_plan& p) -> hp_status {
+        if (b->persistent && p.radix_order != 0) return HP_OK;  // the persistent step runs radix order 0
         const hp_status ps = batch_set_plan(b, p);
         if (ps == HP_ERR_UNSUPPORTED_SIZE) return HP_OK;  // a candidate these members cannot take
         if (ps != HP_OK) return ps;  // anything else leaves the plan unusable: stop tuning
@@ -1546,6 +1633,17 @@ hp_status hp_batch_stats(const hp_batch* b, uint32_t* max_gather_radius, uint64_
     if (!b) return HP_ERR_NULL;
     if (max_gather_radius) *max_gather_radius = b->max_radius;
     if (general_path_steps) *general_path_steps = b->general_steps;
+    return HP_OK;
+}
+
+hp_status hp_batch_set_step_mode(hp_batch* b, int32_t mode) {
+    return guarded(b, [&]() -> hp_status { return batch_set_step_mode(b, mode); });
+}
+
+hp_status hp_batch_step_mode(const hp_batch* b, int32_t* mode, uint64_t* barrier_timeouts) {
+    if (!b) return HP_ERR_NULL;
+    if (mode) *mode = b->persistent ? HP_STEP_PERSISTENT : HP_STEP_MULTI_LAUNCH;
+    if (barrier_timeouts) *barrier_timeouts = b->barrier_timeouts;
     return HP_OK;
 }
 

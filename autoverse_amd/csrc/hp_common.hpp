@@ -158,6 +158,22 @@ bool launch_batch_stencil2d(hipStream_t st, int C, bool pow2, const MemberDesc* 
 // Tile shapes of the fused stencil: variant t in [0, stencil2d_tile_count()).
 int stencil2d_tile_count();
 int stencil2d_tile_w(int tile = 0);
+// ---- the persistent population step (batch_step.hip) --------------------------------
+// One launch per step over four phases' item lists (rows_fwd (member, pair, source), spectral
+// (member, column block, work), rows_inv (member, pair, work), stencil (member, tile)), each
+// phase's items sorted into segments by class (batch_step_persistent_segment), starts[0 ..
+// segments] bounding the segments, with a bounded grid barrier between phases (a barrier that
+// waits past spin_limit polls sets bit 2 of flags[0]).  Members' sides must be among the
+// compiled lengths; the FFT passes run with radix order 0.
+bool batch_step_persistent_size(int L);
+int batch_step_persistent_segment(int phase, int L, int C, bool pow2);
+int batch_step_persistent_segments();
+int batch_step_persistent_tile_w();
+int batch_step_persistent_tile_h();
+int batch_step_persistent_block_lines(int L);  // spectral column block of length L
+int batch_step_persistent_grid();              // workgroups the device holds at once
+void launch_batch_step_persistent(hipStream_t st, const MemberDesc* mem, const int4* items, const int* starts,
+                                  int cur, int* flags, unsigned* bar, int grid, int spin_limit);
 // Per member (sum m, sum x m, sum y m) of the channel sum over cells with m > 0, FP64.
 void launch_batch_moments(hipStream_t st, const MemberDesc* mem, int nmem, int cur, double* out);
 int stencil2d_tile_h(int tile = 0);

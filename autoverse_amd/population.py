@@ -125,6 +125,20 @@ class HipPopulation:
         p = _abi.HpPlan.from_dict({**self.plan(), **plan})
         self._check(self._lib.hp_batch_set_plan(self._ptr, ctypes.byref(p)), "hp_batch_set_plan")
 
+    STEP_MODES = {"multi_launch": 0, "persistent": 1}
+
+    def set_step_mode(self, mode: str) -> None:
+        """'multi_launch' (default: one launch per FFT pass and size class, one stencil launch per
+        channel-count group) or 'persistent' (one launch per step with grid barriers between the
+        passes; member sides 64, 128, 192 or 256 and radix order 0).  Trajectories are identical."""
+        self._check(self._lib.hp_batch_set_step_mode(self._ptr, self.STEP_MODES[mode]), "hp_batch_set_step_mode")
+
+    def step_mode(self) -> Tuple[str, int]:
+        """(mode, grid-barrier timeouts so far: runs recomputed through the checked path)."""
+        m, t = ctypes.c_int32(0), ctypes.c_uint64(0)
+        self._check(self._lib.hp_batch_step_mode(self._ptr, ctypes.byref(m), ctypes.byref(t)), "hp_batch_step_mode")
+        return {v: k for k, v in self.STEP_MODES.items()}[m.value], t.value
+
     def autotune(self, steps: int = 20):
         """Time the LDS FFT variants and stencil tiles on the population from its current states
         (restored afterwards); keeps the fastest.  Returns (best plan, [(plan, us per step)])."""

@@ -340,6 +340,27 @@ def population_measure(K: int, warmup: int, tune: bool = True, lone: bool = True
     c1, _ = pop.centroids()
     m1 = pop.mass_f64()
     radius, general = pop.stats()
+    # the persistent step (one launch per step, grid barriers between the passes) over the same
+    # K steps from the same start states; it runs FFT radix order 0
+    persistent = None
+    from autoverse_amd._abi import HipError
+    try:
+        if pop.plan()["radix_order"] != 0:
+            pop.set_plan({"radix_order": 0})
+        pop.set_step_mode("persistent")
+        for rep in range(2):  # untimed first: both step graphs launched once
+            for i, st in enumerate(start):
+                pop.set_state(i, st)
+            t0 = time.perf_counter()
+            pop.advance(2 if rep == 0 else K)
+            secs_p = time.perf_counter() - t0
+        _, timeouts = pop.step_mode()
+        persistent = {"value": cells * K / secs_p, "unit": "cell-steps/s", "ms_per_step": secs_p * 1e3 / K,
+                      "barrier_timeouts": timeouts, "plan": pop.plan(),
+                      "path": "one persistent launch per step (grid barriers between the passes), hipGraph replay"}
+        pop.set_step_mode("multi_launch")
+    except HipError as e:
+        persistent = {"error": str(e)}
     pop.close()
     disp = np.hypot(c1[:, 0] - c0[:, 0], c1[:, 1] - c0[:, 1])
     drift = float(np.max(np.abs(m1 - m0) / np.maximum(m0, 1e-30)))
@@ -395,6 +416,7 @@ def population_measure(K: int, warmup: int, tune: bool = True, lone: bool = True
         "max_gather_radius": radius, "general_path_steps": general, "plan": plan,
         "batched_forward": {"value": value, "unit": "cell-steps/s", "ms_per_step": secs_b * 1e3 / K,
                             "path": "one launch per pass per size class, hipGraph replay"},
+        "batched_forward_persistent": persistent,
         "per_instance_forward": None if per_inst is None else {
             "value": per_inst, "unit": "cell-steps/s", "ms_per_step": secs_i * 1e3 / K,
             "path": "one HipPropagator per member, advanced in turn"},

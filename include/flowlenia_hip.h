@@ -466,6 +466,20 @@ hp_status hp_batch_get_plan(const hp_batch* b, hp_plan* out);
 hp_status hp_batch_set_plan(hp_batch* b, const hp_plan* plan);
 hp_status hp_batch_autotune(hp_batch* b, uint32_t steps, hp_plan* best, uint32_t max_results, hp_plan* tried,
                             double* us_per_step, uint32_t* n_tried);
+/* Step mode of a population (no reference counterpart: the reference steps each candidate's
+ * CpuPropagator on its own, fitness.rs:26-54).  HP_STEP_MULTI_LAUNCH (the default): a step is one
+ * launch per FFT pass and size class and one stencil launch per channel-count group, replayed
+ * from hipGraphs.  HP_STEP_PERSISTENT: a step is one launch, replayed from a hipGraph, whose
+ * workgroups (as many as the device holds at once) walk every member's forward row pass, then
+ * every spectral pass, inverse row pass and stencil tile, with a bounded grid barrier between
+ * phases; every member's sides must be 64, 128, 192 or 256 (else HP_ERR_UNSUPPORTED_SIZE) and
+ * the plan's radix order 0.  Both modes give every member the same trajectory bit for bit.  A
+ * barrier that waits past its bound makes the run recompute its steps through the checked path
+ * and counts one *barrier_timeouts. */
+#define HP_STEP_MULTI_LAUNCH 0
+#define HP_STEP_PERSISTENT 1
+hp_status hp_batch_set_step_mode(hp_batch* b, int32_t mode);
+hp_status hp_batch_step_mode(const hp_batch* b, int32_t* mode, uint64_t* barrier_timeouts);
 hp_status hp_batch_tape_forward(hp_batch* b, uint64_t steps, uint32_t checkpoint_every, uint32_t* used_every);
 hp_status hp_batch_tape_backward(hp_batch* b, const float* const* dl_dfinal, float* const* dl_dinitial,
                                  double* dtheta);
