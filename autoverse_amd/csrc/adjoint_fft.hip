@@ -212,22 +212,29 @@ __global__ __launch_bounds__(NT) void kb_adj_rows_inv(const MemberDesc* __restri
     }
 }
 
-// item = (member, kernel).  dkwin[off + (jy+R)(2R+1) + (jx+R)] = sum over the full spectrum
-// of P_k e^{+2 pi i (kx jx / W + ky jy / H)} (unscaled; times 1/N on the host), evaluated from
-// the half spectrum as sum_ky sum_{kx <= W/2} m_kx Re(...), m_kx = 1 at kx = 0, W/2 else 2;
-// separable: a chunk of 8 spectrum rows is reduced over kx into LDS first.  FP64 throughout.
+// dkwin[off + (jy+R)(2R+1) + (jx+R)] = sum over the full spectrum of P_k e^{+2 pi i (kx jx / W +
+// ky jy / H)} (unscaled; times 1/N on the host), evaluated from the half spectrum as sum_ky
+// sum_{kx <= W/2} m_kx Re(...), m_kx = 1 at kx = 0, W/2 else 2; separable: each block of 8
+// spectrum rows is reduced over kx into LDS first, then over its rows into the window.  FP64
+// throughout.  Two kernels: kb_window_dft_rows, one workgroup per (member, kernel, row block),
+// writes each block's window to part[poff + rb (2R+1)^2 + o]; kb_window_dft_sum adds the blocks
+// into dkwin in ascending block order (the order a single pass over the blocks would use), so
+// the result does not depend on how the blocks were spread over workgroups.
 constexpr int WDFT_NT = 256, WDFT_ROWS = 8, WDFT_MAXS = 129, WDFT_MAXL = 1024;
-__global__ __launch_bounds__(WDFT_NT) void kb_window_dft(const MemberDesc* __restrict__ mem,
-                                                         const int4* __restrict__ items) {
+
+// item = (member, kernel, row block rb, poff)
+__global__ __launch_bounds__(WDFT_NT) void kb_window_dft_rows(const MemberDesc* __restrict__ mem,
+                                                              const int4* __restrict__ items,
+                                                              double* __restrict__ part) {
     __shared__ double2 twW[WDFT_MAXL];
     __shared__ double2 twH[WDFT_MAXL];
     __shared__ double2 Q[WDFT_ROWS][WDFT_MAXS];
     const int4 it = items[blockIdx.x];
     const MemberDesc& m = mem[it.x];
-    const int k = it.y;
+    const int k = it.y, ky0 = it.z * WDFT_ROWS;
     const int W = m.g.W, H = m.g.H, Wh = m.g.Wh, Wp = m.g.Wp;
     const int R = m.dkw[k].y, S = 2 * R + 1;
-    double* out = m.dkwin + m.dkw[k].x;
+    double* out = part + it.w + (long long)it.z * S * S;
     const float2* P = m.P + (long long)k * m.g.spec;
     for (int i = threadIdx.x; i < W; i += WDFT_NT) {
         double sn, cs;
@@ -240,44 +247,58 @@ __global__ __launch_bounds__(WDFT_NT) void kb_window_dft(const MemberDesc* __res
         twH[i] = make_double2(cs, sn);
     }
     __syncthreads();
-    for (int ky0 = 0; ky0 < H; ky0 += WDFT_ROWS) {
-        for (int t = threadIdx.x; t < WDFT_ROWS * S; t += WDFT_NT) {
-            const int r = t / S, jxi = t - r * S;
-            const int ky = ky0 + r;
-            double re = 0.0, im = 0.0;
-            if (ky < H) {
-                const int jx = jxi - R;
-                const int step = ((jx % W) + W) % W;
-                int ph = 0;
-                const float2* row = P + (long long)ky * Wp;
-                for (int kx = 0; kx < Wh; ++kx) {
-                    const double2 w = twW[ph];
-                    const float2 p = row[kx];
-                    const double mk = (kx == 0 || 2 * kx == W) ? 1.0 : 2.0;
-                    re += mk * ((double)p.x * w.x - (double)p.y * w.y);
-                    im += mk * ((double)p.x * w.y + (double)p.y * w.x);
-                    ph += step;
-                    if (ph >= W) ph -= W;
-                }
-            }
-            Q[r][jxi] = make_double2(re, im);
-        }
-        __syncthreads();
-        for (int o = threadIdx.x; o < S * S; o += WDFT_NT) {
-            const int jyi = o / S, jxi = o - jyi * S;
-            const int jy = jyi - R;
-            const int step = ((jy % H) + H) % H;
-            int ph = (int)(((long long)ky0 * step) % H);
-            double a = 0.0;
-            for (int r = 0; r < WDFT_ROWS && ky0 + r < H; ++r) {
-                const double2 w = twH[ph];
-                a += Q[r][jxi].x * w.x - Q[r][jxi].y * w.y;
+    for (int t = threadIdx.x; t < WDFT_ROWS * S; t += WDFT_NT) {
+        const int r = t / S, jxi = t - r * S;
+        const int ky = ky0 + r;
+        double re = 0.0, im = 0.0;
+        if (ky < H) {
+            const int jx = jxi - R;
+            const int step = ((jx % W) + W) % W;
+            int ph = 0;
+            const float2* row = P + (long long)ky * Wp;
+            for (int kx = 0; kx < Wh; ++kx) {
+                const double2 w = twW[ph];
+                const float2 p = row[kx];
+                const double mk = (kx == 0 || 2 * kx == W) ? 1.0 : 2.0;
+                re += mk * ((double)p.x * w.x - (double)p.y * w.y);
+                im += mk * ((double)p.x * w.y + (double)p.y * w.x);
                 ph += step;
-                if (ph >= H) ph -= H;
+                if (ph >= W) ph -= W;
             }
-            out[o] += a;
         }
-        __syncthreads();
+        Q[r][jxi] = make_double2(re, im);
+    }
+    __syncthreads();
+    for (int o = threadIdx.x; o < S * S; o += WDFT_NT) {
+        const int jyi = o / S, jxi = o - jyi * S;
+        const int jy = jyi - R;
+        const int step = ((jy % H) + H) % H;
+        int ph = (int)(((long long)ky0 * step) % H);
+        double a = 0.0;
+        for (int r = 0; r < WDFT_ROWS && ky0 + r < H; ++r) {
+            const double2 w = twH[ph];
+            a += Q[r][jxi].x * w.x - Q[r][jxi].y * w.y;
+            ph += step;
+            if (ph >= H) ph -= H;
+        }
+        out[o] = a;
+    }
+}
+
+// item = (member, kernel, row blocks, poff): dkwin += the blocks' windows, block by block
+__global__ __launch_bounds__(WDFT_NT) void kb_window_dft_sum(const MemberDesc* __restrict__ mem,
+                                                             const int4* __restrict__ items,
+                                                             const double* __restrict__ part) {
+    const int4 it = items[blockIdx.x];
+    const MemberDesc& m = mem[it.x];
+    const int k = it.y, nrb = it.z;
+    const int S = 2 * m.dkw[k].y + 1, SS = S * S;
+    double* out = m.dkwin + m.dkw[k].x;
+    const double* pk = part + it.w;
+    for (int o = threadIdx.x; o < SS; o += WDFT_NT) {
+        double acc = out[o];
+        for (int rb = 0; rb < nrb; ++rb) acc += pk[(long long)rb * SS + o];
+        out[o] = acc;
     }
 }
 
@@ -320,10 +341,16 @@ bool adj_fft_supported(int L) {
     return false;
 }
 
-void launch_window_dft(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems) {
-    hipLaunchKernelGGL(kb_window_dft, dim3(nitems), dim3(WDFT_NT), 0, st, mem, items);
+void launch_window_dft(hipStream_t st, const MemberDesc* mem, const int4* row_items, int nrow_items,
+                       const int4* sum_items, int nsum_items, double* part) {
+    if (nrow_items > 0)
+        hipLaunchKernelGGL(kb_window_dft_rows, dim3(nrow_items), dim3(WDFT_NT), 0, st, mem, row_items, part);
+    if (nsum_items > 0)
+        hipLaunchKernelGGL(kb_window_dft_sum, dim3(nsum_items), dim3(WDFT_NT), 0, st, mem, sum_items,
+                           (const double*)part);
 }
 
+int window_dft_rows() { return WDFT_ROWS; }
 int window_dft_max_side() { return WDFT_MAXL; }
 int window_dft_max_taps() { return WDFT_MAXS; }
 

@@ -16,10 +16,12 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <mutex>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/flowlenia_hip.h"
@@ -57,6 +59,7 @@ struct hp_ctx {
     std::vector<int> tgt_chan;  // target work item -> channel
     float* A[2] = {nullptr, nullptr};
     float* U = nullptr;
+    float* h_stage = nullptr;  // pinned host staging of one state (C*N floats), made on first use
     float2* X = nullptr;
     float2* Y = nullptr;
     float* Kq = nullptr;
@@ -478,24 +481,86 @@ hp_status check_host_state(hp_ctx* c, const float* const* ch) {
     return HP_OK;
 }
 
-// Every channel is checked before any is copied, so a refused state leaves the device state as
-// it was.
+// The host side of a state transfer: the channels move between the caller's (pageable) arrays
+// and a pinned staging buffer of the context on a few host threads, and between the staging
+// buffer and the device in one DMA copy.  Host threads for states of at least 1 Mi values.
+hp_status ensure_host_stage(hp_ctx* c) {
+    if (c->h_stage) return HP_OK;
+    HIP_TRY(c, hipHostMalloc((void**)&c->h_stage, std::max<size_t>(1, (size_t)c->C * c->g.N) * sizeof(float),
+                             hipHostMallocDefault));
+    return HP_OK;
+}
+
+// f(lo, hi) over [0, n) in contiguous parts on up to 8 threads (the caller runs the first part).
+template <class F>
+void host_parallel(size_t n, const F& f) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t T = n < (1u << 20) ? 1 : std::min<size_t>(std::min(8u, hw), n >> 18);
+    if (T <= 1) {
+        f((size_t)0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const size_t part = (n + T - 1) / T;
+    for (size_t t = 1; t < T; t++) {
+        const size_t lo = std::min(n, t * part), hi = std::min(n, lo + part);
+        try {
+            th.emplace_back([&f, lo, hi] { f(lo, hi); });
+        } catch (...) {  // no thread: run the part here
+            f(lo, hi);
+        }
+    }
+    f((size_t)0, std::min(n, part));
+    for (auto& t : th) t.join();
+}
+
+// Every channel is checked (in the same pass that stages it) before anything is copied to the
+// device, so a refused state leaves the device state as it was.
 hp_status upload_state(hp_ctx* c, const float* const* ch, float* dst) {
-    for (int i = 0; i < c->C; i++)
-        for (long long j = 0; j < c->g.N; j++)
-            if (!isfinite(ch[i][j])) return c->fail(HP_ERR_NONFINITE, "input state holds NaN/Inf");
-    for (int i = 0; i < c->C; i++)
-        HIP_TRY(c, hipMemcpyAsync(dst + (size_t)i * c->g.N, ch[i], c->g.N * sizeof(float), hipMemcpyHostToDevice,
-                                  c->stream));
+    hp_status st = ensure_host_stage(c);
+    if (st != HP_OK) return st;
+    const size_t N = (size_t)c->g.N, total = (size_t)c->C * N;
+    std::vector<char> bad(64, 0);  // one flag per possible part, written by its thread only
+    std::atomic<int> part_id{0};
+    host_parallel(total, [&](size_t lo, size_t hi) {
+        const int me = part_id.fetch_add(1);
+        unsigned acc = 0;
+        for (size_t k = lo; k < hi;) {
+            const size_t i = k / N, j0 = k - i * N, j1 = std::min(N, j0 + (hi - k));
+            const float* s = ch[i] + j0;
+            float* d = c->h_stage + k;
+            const size_t m = j1 - j0;
+            for (size_t j = 0; j < m; j++) {
+                const float v = s[j];
+                d[j] = v;
+                unsigned u;
+                memcpy(&u, &v, 4);
+                acc |= (unsigned)((u & 0x7f800000u) == 0x7f800000u);  // Inf or NaN
+            }
+            k += m;
+        }
+        bad[me & 63] = (char)acc;
+    });
+    for (char b : bad)
+        if (b) return c->fail(HP_ERR_NONFINITE, "input state holds NaN/Inf");
+    HIP_TRY(c, hipMemcpyAsync(dst, c->h_stage, total * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return HP_OK;
 }
 
 hp_status download(hp_ctx* c, const float* src, float* const* ch) {
-    for (int i = 0; i < c->C; i++)
-        HIP_TRY(c, hipMemcpyAsync(ch[i], src + (size_t)i * c->g.N, c->g.N * sizeof(float), hipMemcpyDeviceToHost,
-                                  c->stream));
+    hp_status st = ensure_host_stage(c);
+    if (st != HP_OK) return st;
+    const size_t N = (size_t)c->g.N, total = (size_t)c->C * N;
+    HIP_TRY(c, hipMemcpyAsync(c->h_stage, src, total * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    host_parallel(total, [&](size_t lo, size_t hi) {
+        for (size_t k = lo; k < hi;) {
+            const size_t i = k / N, j0 = k - i * N, m = std::min(N - j0, hi - k);
+            memcpy(ch[i] + j0, c->h_stage + k, m * sizeof(float));
+            k += m;
+        }
+    });
     return HP_OK;
 }
 
@@ -517,6 +582,7 @@ void release(hp_ctx* c) {
         if (p) (void)hipFree(p);
     for (auto& kv : c->tw) (void)hipFree(kv.second);
     for (auto& kv : c->tws) (void)hipFree(kv.second);
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

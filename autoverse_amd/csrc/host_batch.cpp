@@ -76,7 +76,9 @@ struct hp_batch {
     std::vector<StencilGroup> stencil_groups;
     ItemList copy_items;  // (member, part, nparts) over each member's C*N floats
     ItemList adj_tiles;   // (member, tile) of the adjoint stencil passes
-    ItemList wdft_items;  // (member, kernel)
+    ItemList wdft_rows;   // (member, kernel, row block, partial offset) of the tape's layout
+    ItemList wdft_sums;   // (member, kernel, row blocks, partial offset)
+    double* d_wpart = nullptr;  // the kernel-tap windows of every row block
     std::map<int, float2*> tw;
     std::vector<void*> allocs;
     std::vector<void*> tape_allocs;
@@ -388,6 +390,29 @@ hp_status ensure_tape(hp_batch* b, uint64_t T, uint32_t k, bool store_conv) {
         pacc_n += (size_t)std::max(p.K, 0) * ((p.g.H + 1) / 2) * 3;
         for (int r : hm.radius) dkwin_n += (size_t)(2 * r + 1) * (2 * r + 1);
         stage_n += (size_t)p.C * p.g.N;
+    }
+    // the window DFT's row-block items and partial windows
+    {
+        std::vector<int4> rows, sums;
+        size_t poff = 0;
+        const int RB = hp::window_dft_rows();
+        for (size_t i = 0; i < b->members.size(); i++) {
+            const HostMember& hm = b->members[i];
+            const int nrb = (hm.plan.g.H + RB - 1) / RB;
+            for (size_t kk = 0; kk < hm.radius.size(); kk++) {
+                const size_t SS = (size_t)(2 * hm.radius[kk] + 1) * (2 * hm.radius[kk] + 1);
+                if (poff + nrb * SS > (size_t)0x7fffffff)
+                    return b->fail(HP_ERR_UNSUPPORTED_SIZE, "kernel-tap windows beyond 2^31 values");
+                for (int rb = 0; rb < nrb; rb++) rows.push_back(make_int4((int)i, (int)kk, rb, (int)poff));
+                sums.push_back(make_int4((int)i, (int)kk, nrb, (int)poff));
+                poff += nrb * SS;
+            }
+        }
+        STRY(bupload(b, &b->wdft_rows.d, rows, &b->tape_allocs));
+        b->wdft_rows.n = (int)rows.size();
+        STRY(bupload(b, &b->wdft_sums.d, sums, &b->tape_allocs));
+        b->wdft_sums.n = (int)sums.size();
+        STRY(balloc(b, &b->d_wpart, std::max<size_t>(1, poff), &b->tape_allocs));
     }
     STRY(balloc(b, &b->d_pacc, pacc_n, &b->tape_allocs));
     STRY(balloc(b, &b->d_dkwin, dkwin_n, &b->tape_allocs));
@@ -702,7 +727,8 @@ hp_status tape_backward(hp_batch* b, const float* const* seed, float* const* gA0
         (void)hipStreamSynchronize(b->stream);
         return lst;
     }
-    hp::launch_window_dft(b->stream, b->d_desc, b->wdft_items.d, b->wdft_items.n);
+    hp::launch_window_dft(b->stream, b->d_desc, b->wdft_rows.d, b->wdft_rows.n, b->wdft_sums.d, b->wdft_sums.n,
+                          b->d_wpart);
     BTRY(b, hipGetLastError());
     // results: one copy each for dL/dA_0, the growth sums and the kernel-tap windows
     hp::launch_stage_lam(b->stream, b->d_desc, b->copy_items.d, b->copy_items.n, b->d_stage, li, true);
@@ -949,7 +975,7 @@ hp_status create(const hp_config* cfgs, uint32_t count, int device, hp_batch** o
     b->desc.resize(count);
     std::map<std::pair<int, int>, std::vector<uint32_t>>& by_size = b->by_size;
     std::map<std::pair<int, bool>, std::vector<uint32_t>>& by_stencil = b->by_stencil;
-    std::vector<int4> copy_it, tiles, wdft;
+    std::vector<int4> copy_it, tiles;
     const int ATX = hp::adj_tile_w(), ATY = hp::adj_tile_h();
     for (uint32_t i = 0; i < count; i++) {
         HostMember& hm = b->members[i];
@@ -1008,15 +1034,12 @@ hp_status create(const hp_config* cfgs, uint32_t count, int device, hp_batch** o
         for (int q = 0; q < nparts; q++) copy_it.push_back(make_int4((int)i, q, nparts, 0));
         const int nt = ((p.g.W + ATX - 1) / ATX) * ((p.g.H + ATY - 1) / ATY);
         for (int t = 0; t < nt; t++) tiles.push_back(make_int4((int)i, t, 0, 0));
-        for (int kk = 0; kk < p.K; kk++) wdft.push_back(make_int4((int)i, kk, 0, 0));
     }
     TRY(bupload(b, &b->d_desc, b->desc));
     TRY(bupload(b, &b->copy_items.d, copy_it));
     b->copy_items.n = (int)copy_it.size();
     TRY(bupload(b, &b->adj_tiles.d, tiles));
     b->adj_tiles.n = (int)tiles.size();
-    TRY(bupload(b, &b->wdft_items.d, wdft));
-    b->wdft_items.n = (int)wdft.size();
     TRY(build_items(b));
     size_t fused = 0;
     for (auto& kv : by_stencil) fused += kv.second.size();

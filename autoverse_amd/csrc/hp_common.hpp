@@ -268,7 +268,11 @@ int adj_spectral_block_lines(int L);
 // items as the forward row pass (member, row pair, source slot)
 bool launch_adj_rows_inv(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int li);
 // items = (member, kernel)
-void launch_window_dft(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems);
+// The kernel-tap windows: row_items (member, kernel, row block, poff) write each block's window to
+// part + poff + block * (2R+1)^2; sum_items (member, kernel, blocks, poff) add them into dkwin.
+void launch_window_dft(hipStream_t st, const MemberDesc* mem, const int4* row_items, int nrow_items,
+                       const int4* sum_items, int nsum_items, double* part);
+int window_dft_rows();  // spectrum rows per block
 int window_dft_max_side();
 int window_dft_max_taps();
 

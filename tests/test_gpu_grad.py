@@ -279,3 +279,25 @@ def test_stored_convolution_tape_matches_recompute(dt):
     dt = max(float(np.abs(a - b).max()) / (1e-12 + float(np.abs(b).max())) for a, b in zip(out[0][2], out[1][2]))
     _say(f"  states max|d| {ds:.2e}; dL/dA0 max|d|/(1+max|g|) {dg:.2e}; dL/dtheta max|d|/max|th| {dt:.2e}")
     assert ds <= 1e-5 and dg <= 1e-4 and dt <= 1e-3
+
+
+def test_flow_lenia_steps_autograd_function_matches_grad(oracle):
+    """FlowLeniaSteps (the torch.autograd.Function over the tape) gives grad()'s final state,
+    dL/dA_0 and dL/dtheta bit for bit for L = <w, A_T>, with theta at the config's own values."""
+    import torch
+
+    cfg, st = _case(oracle, size=48, channels=2, seed=11)
+    steps, k = 6, 3
+    w = _seed(cfg.width * cfg.height, cfg.channels, s=5)
+    final, g0, th = _gpu_grad(cfg, st, steps, w, k=k)
+    _say(f"[FlowLeniaSteps] 48^2 x 2, {steps} steps, checkpoint every {k}")
+    a0 = torch.from_numpy(np.stack(st).reshape(cfg.channels, cfg.height, cfg.width)).requires_grad_(True)
+    theta = torch.from_numpy(G.theta_of(cfg)).requires_grad_(True)
+    aT = G.FlowLeniaSteps.apply(a0, theta, cfg, steps, k)
+    wt = torch.from_numpy(np.stack(w).reshape(cfg.channels, cfg.height, cfg.width))
+    (aT * wt).sum().backward()
+    for c in range(cfg.channels):
+        np.testing.assert_array_equal(aT[c].detach().numpy().reshape(-1), final.channels[c])
+        np.testing.assert_array_equal(a0.grad[c].numpy().reshape(-1), g0[c])
+    np.testing.assert_array_equal(theta.grad.numpy(), th)
+    assert theta.grad.dtype == torch.float64
