@@ -285,7 +285,8 @@ __global__ __launch_bounds__(WDFT_NT) void kb_window_dft_rows(const MemberDesc* 
     }
 }
 
-// item = (member, kernel, row blocks, poff): dkwin += the blocks' windows, block by block
+// item = (member, kernel, row blocks, poff), blockIdx.y = a chunk of WDFT_NT window entries:
+// dkwin += the blocks' windows, block by block (the loads of eight blocks in flight at once)
 __global__ __launch_bounds__(WDFT_NT) void kb_window_dft_sum(const MemberDesc* __restrict__ mem,
                                                              const int4* __restrict__ items,
                                                              const double* __restrict__ part) {
@@ -293,13 +294,21 @@ __global__ __launch_bounds__(WDFT_NT) void kb_window_dft_sum(const MemberDesc* _
     const MemberDesc& m = mem[it.x];
     const int k = it.y, nrb = it.z;
     const int S = 2 * m.dkw[k].y + 1, SS = S * S;
+    const int o = (int)blockIdx.y * WDFT_NT + (int)threadIdx.x;
+    if (o >= SS) return;
     double* out = m.dkwin + m.dkw[k].x;
-    const double* pk = part + it.w;
-    for (int o = threadIdx.x; o < SS; o += WDFT_NT) {
-        double acc = out[o];
-        for (int rb = 0; rb < nrb; ++rb) acc += pk[(long long)rb * SS + o];
-        out[o] = acc;
+    const double* pk = part + it.w + o;
+    double acc = out[o];
+    int rb = 0;
+    for (; rb + 8 <= nrb; rb += 8) {
+        double v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = pk[(long long)(rb + j) * SS];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc += v[j];
     }
+    for (; rb < nrb; ++rb) acc += pk[(long long)rb * SS];
+    out[o] = acc;
 }
 
 // ---- launchers -------------------------------------------------------------------------------
@@ -346,8 +355,8 @@ void launch_window_dft(hipStream_t st, const MemberDesc* mem, const int4* row_it
     if (nrow_items > 0)
         hipLaunchKernelGGL(kb_window_dft_rows, dim3(nrow_items), dim3(WDFT_NT), 0, st, mem, row_items, part);
     if (nsum_items > 0)
-        hipLaunchKernelGGL(kb_window_dft_sum, dim3(nsum_items), dim3(WDFT_NT), 0, st, mem, sum_items,
-                           (const double*)part);
+        hipLaunchKernelGGL(kb_window_dft_sum, dim3(nsum_items, (WDFT_MAXS * WDFT_MAXS + WDFT_NT - 1) / WDFT_NT),
+                           dim3(WDFT_NT), 0, st, mem, sum_items, (const double*)part);
 }
 
 int window_dft_rows() { return WDFT_ROWS; }
