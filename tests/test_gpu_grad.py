@@ -301,3 +301,59 @@ def test_flow_lenia_steps_autograd_function_matches_grad(oracle):
         np.testing.assert_array_equal(a0.grad[c].numpy().reshape(-1), g0[c])
     np.testing.assert_array_equal(theta.grad.numpy(), th)
     assert theta.grad.dtype == torch.float64
+
+
+def test_grad_at_differentiation_member_size(oracle):
+    """One member of the differentiation workload (512^2 x 2 channels, 4 kernels, R = 15, seed as
+    in the bench), T = 8 with a checkpoint every 4, against the FP64 oracle adjoint.
+
+    At this size f32 state arithmetic limits every f32 adjoint: the oracle's own f32 adjoint is
+    about 3e-3 from its FP64 adjoint in the parameter vector (box coordinates near x ~ 512 carry
+    ulp(512) = 6e-5, and G' reaches ~1000).  The bars: dL/dA0 within 1e-3 of the FP64 adjoint's
+    largest entry; every dL/dtheta component within 1e-3 relative or 3x the f32 oracle's own
+    deviation (as in the smaller cases); the parameter vector's relative error at most twice the
+    f32 oracle's; and the FP64 adjoint itself checked by FP64 central differences on four
+    parameters (mu, sigma of kernel 0; amplitude and width of its first ring) to 1e-5.  The
+    differences use eps = 1e-6 |p|: at this size box edges cross cell edges within any larger
+    eps (the quotient for mu_0 reads 28.60, 28.41, 28.48 at eps = 1e-3, 1e-4, 1e-5 |p| against
+    28.376 at 1e-6, where it equals the adjoint to 1e-7), so 1e-4 pins nothing here.  dL/dA0 is
+    held to the same rule per cell: the f32 oracle's own dL/dA0 is up to 1.7e-2 of the largest
+    entry from the FP64 one here (about 0.5% of the cells beyond 1e-3)."""
+    cfg, sd = workloads.differentiation(seed=0x5EED4)
+    st = [np.asarray(a, np.float32).reshape(-1) for a in oracle.seed_generate(sd, cfg.width, cfg.height, 1,
+                                                                               cfg.channels)]
+    T = 8
+    _say(f"[grad_512] 512^2 x 2, 4 kernels, T={T}, checkpoint every 4")
+    seed = _seed(cfg.width * cfg.height, cfg.channels, 9)
+    final, g0, th = _gpu_grad(cfg, st, T, seed, k=4)
+    p64 = oracle.Propagator(cfg, np.float64)
+    st64 = [a.astype(np.float64) for a in st]
+    sd64 = [a.astype(np.float64) for a in seed]
+    at, o0, oth = p64.grad([a.copy() for a in st64], T, sd64)
+    _, o032, oth32 = oracle.Propagator(cfg, np.float32).grad(st, T, seed)
+    for c in range(cfg.channels):
+        assert np.abs(final.channels[c] - at[c]).max() < 1e-4
+        m = np.abs(o0[c]).max()
+        d, d32 = np.abs(g0[c] - o0[c]), np.abs(o032[c] - o0[c])
+        _say(f"  dL/dA0[{c}] max rel err {d.max() / m:.2e} (oracle f32: {d32.max() / m:.2e}); "
+             f"cells beyond 1e-3: {(d > REL * m).sum()} (oracle f32: {(d32 > REL * m).sum()})")
+        assert not (d > np.maximum(REL * m, 3.0 * d32) + 0.1 * REL * m).any()
+    scale = np.abs(oth).max()
+    err = np.abs(th - oth)
+    bad = err > np.maximum(REL * np.abs(oth), 3.0 * np.abs(oth32 - oth)) + 0.1 * REL * scale
+    assert not bad.any(), f"dtheta components {np.nonzero(bad)[0]}"
+    vec = np.linalg.norm(th - oth) / np.linalg.norm(oth)
+    vec32 = np.linalg.norm(oth32 - oth) / np.linalg.norm(oth)
+    _say(f"  dtheta: |g - g64| / |g64| = {vec:.2e} (oracle f32: {vec32:.2e})")
+    assert vec <= max(REL, 2.0 * vec32)
+    theta = G.theta_of(cfg).astype(np.float32).astype(np.float64)
+    for j in (1, 2, 12, 14):  # kernel 0's mu and sigma; its first ring's amplitude and width
+        e = 1e-6 * max(1e-2, abs(theta[j]))
+        tp, tm = theta.copy(), theta.copy()
+        tp[j] += e
+        tm[j] -= e
+        fd = (oracle.loss(cfg, st64, T, sd64, tp) - oracle.loss(cfg, st64, T, sd64, tm)) / (2 * e)
+        _say(f"  theta[{j}]: FP64 adjoint {oth[j]:.6e}, FP64 central difference {fd:.6e}, gpu {th[j]:.6e} "
+             f"(gpu vs FP64 {abs(th[j] - oth[j]) / abs(oth[j]):.1e}, oracle f32 vs FP64 "
+             f"{abs(oth32[j] - oth[j]) / abs(oth[j]):.1e})")
+        assert abs(oth[j] - fd) <= 1e-5 * abs(fd) + 1e-7 * scale
