@@ -70,18 +70,31 @@ __global__ __launch_bounds__(adj::NT) void kb_adj_flow(const MemberDesc* __restr
     const long long N = m.g.N;
     const int tiles_x = (W + TX - 1) / TX;
     const int tx0 = (it.y % tiles_x) * TX, ty0 = (it.y / tiles_x) * TY;
-    const float* A = m.A[cur];
-    const float* lamn = m.lam[li];
-    float* lamo = m.lam[1 - li];
+    const auto A = gpc(m.A[cur]);
+    const auto lamn = gpc(m.lam[li]);
+    const auto lamo = gp(m.lam[1 - li]);
+    const auto gscr = gp(m.gscr);
     const float dt = m.dt, s = m.s, beta_a = m.beta_a, nexp = m.nexp;
     const float ta = (2.0f * s) * (2.0f * s);
     const float inv_ta = 1.0f / ta;
-    for (int t = threadIdx.x; t < HH * HW; t += NT) {
+    // The halo tile's cells this thread loads (at most two), wrapped once for every channel.
+    static_assert(HH * HW <= 2 * NT, "two halo cells per thread");
+    long long toff[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int t = threadIdx.x + k * NT;
         const int r = t / HW, c = t - r * HW;
-        const long long idx = (long long)wrap_near(ty0 + r - 1, H) * W + wrap_near(tx0 + c - 1, W);
+        toff[k] = t < HH * HW ? (long long)wrap_near(ty0 + r - 1, H) * W + wrap_near(tx0 + c - 1, W) : 0;
+    }
+    float* const sAf = &sA[0][0];
+    float* const sUf = &sU[0][0];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int t = threadIdx.x + k * NT;
+        if (t >= HH * HW) break;
         float v = 0.0f;
-        for (int ch = 0; ch < C; ++ch) v += A[(long long)ch * N + idx];  // 0 + A_0 + A_1 + ...
-        sA[r][c] = v;
+        for (int ch = 0; ch < C; ++ch) v += A[(long long)ch * N + toff[k]];  // 0 + A_0 + A_1 + ...
+        sAf[t] = v;
     }
     __syncthreads();
     const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
@@ -98,11 +111,13 @@ __global__ __launch_bounds__(adj::NT) void kb_adj_flow(const MemberDesc* __restr
     const float oma = 1.0f - al;
     float gGAx = 0.0f, gGAy = 0.0f, gal = 0.0f;
     for (int c = 0; c < C; ++c) {
-        const float* Uc = m.U + (long long)c * N;
+        const auto Uc = gpc(m.U) + (long long)c * N;
         __syncthreads();  // previous channel's sU reads are done
-        for (int t = threadIdx.x; t < HH * HW; t += NT) {
-            const int r = t / HW, cc = t - r * HW;
-            sU[r][cc] = Uc[(long long)wrap_near(ty0 + r - 1, H) * W + wrap_near(tx0 + cc - 1, W)];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int t = threadIdx.x + k * NT;
+            if (t >= HH * HW) break;
+            sUf[t] = Uc[toff[k]];
         }
         __syncthreads();
         if (!inside) continue;
@@ -115,7 +130,7 @@ __global__ __launch_bounds__(adj::NT) void kb_adj_flow(const MemberDesc* __restr
         const bool active = !(fabsf(mass) < 1e-10f);
         const float dxp = (float)x + dt * fx;  // reintegration.rs:81-82
         const float dyp = (float)y + dt * fy;
-        const float* ln = lamn + (long long)c * N;
+        const auto ln = lamn + (long long)c * N;
         float dm = 0.0f, gx = 0.0f, gy = 0.0f;
         if (ta < 1e-10f) {  // point fallback (reintegration.rs:117-124)
             dm = ln[(long long)wrapi(sat_i32(roundf(dyp)), H) * W + wrapi(sat_i32(roundf(dxp)), W)];
@@ -123,7 +138,48 @@ __global__ __launch_bounds__(adj::NT) void kb_adj_flow(const MemberDesc* __restr
             const float xm = dxp - s, xM = dxp + s, ym = dyp - s, yM = dyp + s;
             const int ixl = max(sat_i32(floorf(xm)), -W), ixh = min(sat_i32(ceilf(xM)), 2 * W);
             const int iyl = max(sat_i32(floorf(ym)), -H), iyh = min(sat_i32(ceilf(yM)), 2 * H);
-            if (ixh - ixl <= 4) {
+            if (ixl > ixh || iyl > iyh) {
+                // an empty box range (a coordinate past 2W or 2H): no cell overlaps
+            } else if (ixh - ixl <= 3 && iyh - iyl <= 3 && W > 2 && H > 2) {
+                // At most three columns and three rows (every box with s <= 1): cell ixl + 3 starts
+                // at or past xM, so it and every later cell have zero overlap, as do the rows past
+                // iyl + 2.  The nine values of lam' are loaded at once (wrapped, always in bounds)
+                // and the zero-overlap cells are skipped in the order of the loops below.  With
+                // -W <= ixl <= ixh <= 2W every index lies in [-W, 3W), so wrap3 applies.
+                float ow[3], dow[3], oh[3], doh[3];
+                int cw[3];
+                long long rw[3];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const int ix = ixl + j, iy = iyl + j;
+                    const float cx0 = (float)ix, cx1 = (float)(ix + 1);
+                    const float cy0 = (float)iy, cy1 = (float)(iy + 1);
+                    ow[j] = j <= ixh - ixl ? overlap1(cx0, cx1, xm, xM) : 0.0f;
+                    dow[j] = (float)(xM < cx1) - (float)(xm > cx0);
+                    cw[j] = wrap3(ix, W);
+                    oh[j] = j <= iyh - iyl ? overlap1(cy0, cy1, ym, yM) : 0.0f;
+                    doh[j] = (float)(yM < cy1) - (float)(ym > cy0);
+                    rw[j] = (long long)wrap3(iy, H) * W;
+                }
+                float l[3][3];
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) l[r][j] = ln[rw[r] + cw[j]];
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        const bool on = ow[j] * oh[r] > 0.0f;
+                        const float a = dm + l[r][j] * (ow[j] * oh[r]);
+                        const float b = gx + l[r][j] * (dow[j] * oh[r]);
+                        const float d = gy + l[r][j] * (ow[j] * doh[r]);
+                        dm = on ? a : dm;
+                        gx = on ? b : gx;
+                        gy = on ? d : gy;
+                    }
+                }
+            } else if (ixh - ixl <= 4) {
                 // At most five columns (every box with s <= 1): their overlaps, derivatives and
                 // wrapped indices once, reused on every row.  Columns past ixh have zero overlap
                 // and are skipped as the loop below skips them; the order is the loop's.
@@ -142,7 +198,7 @@ __global__ __launch_bounds__(adj::NT) void kb_adj_flow(const MemberDesc* __restr
                     const float oh = overlap1(cy0, cy1, ym, yM);
                     if (!(oh > 0.0f)) continue;
                     const float doh = (float)(yM < cy1) - (float)(ym > cy0);
-                    const float* lrow = ln + (long long)wrap_near(iy, H) * W;
+                    const auto lrow = ln + (long long)wrap_near(iy, H) * W;
 #pragma unroll
                     for (int j = 0; j < 5; ++j) {
                         if (!(ow[j] * oh > 0.0f)) continue;
@@ -158,7 +214,7 @@ __global__ __launch_bounds__(adj::NT) void kb_adj_flow(const MemberDesc* __restr
                     const float oh = overlap1(cy0, cy1, ym, yM);
                     if (!(oh > 0.0f)) continue;
                     const float doh = (float)(yM < cy1) - (float)(ym > cy0);
-                    const float* lrow = ln + (long long)wrap_near(iy, H) * W;
+                    const auto lrow = ln + (long long)wrap_near(iy, H) * W;
                     for (int ix = ixl; ix <= ixh; ++ix) {
                         const float cx0 = (float)ix, cx1 = (float)(ix + 1);
                         const float ow = overlap1(cx0, cx1, xm, xM);
@@ -177,21 +233,21 @@ __global__ __launch_bounds__(adj::NT) void kb_adj_flow(const MemberDesc* __restr
             gy *= k;
         }
         lamo[(long long)c * N + i] = dm;
-        m.gscr[(long long)(2 * c) * N + i] = oma * gx;
-        m.gscr[(long long)(2 * c + 1) * N + i] = oma * gy;
+        gscr[(long long)(2 * c) * N + i] = oma * gx;
+        gscr[(long long)(2 * c + 1) * N + i] = oma * gy;
         gGAx -= al * gx;
         gGAy -= al * gy;
         gal -= gx * (gux + gax) + gy * (guy + gay);
     }
     if (inside) {
-        m.gscr[(long long)(2 * C) * N + i] = gGAx;
-        m.gscr[(long long)(2 * C + 1) * N + i] = gGAy;
-        m.gscr[(long long)(2 * C + 2) * N + i] = gal * dalpha_of(asum, beta_a, nexp);
+        gscr[(long long)(2 * C) * N + i] = gGAx;
+        gscr[(long long)(2 * C + 1) * N + i] = gGAy;
+        gscr[(long long)(2 * C + 2) * N + i] = gal * dalpha_of(asum, beta_a, nexp);
     }
 }
 
 // Sobel transpose at one cell: -(Sobel_x(ax) + Sobel_y(ay)) from the 3x3 neighbourhood.
-__device__ __forceinline__ float sobel_adj(const float* ax, const float* ay, long long rp, long long rc,
+__device__ __forceinline__ float sobel_adj(gptr<const float> ax, gptr<const float> ay, long long rp, long long rc,
                                            long long rn, int xp, int x, int xn) {
     float sx, sy, dummy;
     sobel9(ax[rp + xp], ax[rp + x], ax[rp + xn], ax[rc + xp], ax[rc + xn], ax[rn + xp], ax[rn + x], ax[rn + xn], sx,
@@ -217,13 +273,14 @@ __global__ __launch_bounds__(adj::NT) void kb_adj_div(const MemberDesc* __restri
     const int yp = y == 0 ? H - 1 : y - 1, yn = y == H - 1 ? 0 : y + 1;
     const long long rp = (long long)yp * W, rc = (long long)y * W, rn = (long long)yn * W;
     const long long i = rc + x;
-    const float* g = m.gscr;
+    const auto g = gpc(m.gscr);
     const float lsum = sobel_adj(g + (long long)(2 * C) * N, g + (long long)(2 * C + 1) * N, rp, rc, rn, xp, x, xn) +
                        g[(long long)(2 * C + 2) * N + i];
-    float* lamo = m.lam[1 - li];
+    const auto lamo = gp(m.lam[1 - li]);
+    const auto lamU = gp(m.lamU);
     for (int c = 0; c < C; ++c) {
         lamo[(long long)c * N + i] += lsum;
-        m.lamU[(long long)c * N + i] =
+        lamU[(long long)c * N + i] =
             sobel_adj(g + (long long)(2 * c) * N, g + (long long)(2 * c + 1) * N, rp, rc, rn, xp, x, xn);
     }
 }

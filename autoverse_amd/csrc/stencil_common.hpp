@@ -9,6 +9,21 @@
 
 namespace hp {
 
+// Pointers into global memory (HBM).  A pointer read from a descriptor in memory is generic to
+// the compiler, which then accesses it with flat loads and stores: those count against the LDS
+// counter as well, so every wait for an LDS read also waits for them, and they take a full
+// 64-bit address per lane.  Accesses through gptr compile to global loads and stores.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> gp(T* p) {
+    return (gptr<T>)p;
+}
+template <class T>
+__device__ __forceinline__ gptr<const T> gpc(const T* p) {
+    return (gptr<const T>)p;
+}
+
 __device__ __forceinline__ int wrapi(int c, int n) {
     int r = c % n;
     return r < 0 ? r + n : r;
@@ -19,6 +34,13 @@ __device__ __forceinline__ int wrap_near(int c, int n) {
     while (c < 0) c += n;
     while (c >= n) c -= n;
     return c;
+}
+
+// Wrap c in [-n, 3n) into [0, n): at most two steps, no loop and no division.
+__device__ __forceinline__ int wrap3(int c, int n) {
+    c = c < 0 ? c + n : c;
+    c = c >= n ? c - n : c;
+    return c >= n ? c - n : c;
 }
 
 // Wrap c into [0, n).  `multi` is uniform over the workgroup: false when the caller knows
@@ -50,12 +72,13 @@ __device__ __forceinline__ unsigned wave_or(unsigned v) {
     return v;
 }
 
-// Rust `f as i32`: saturating, NaN -> 0.
+// Rust `f as i32`: saturating, NaN -> 0.  Branch-free: the clamp keeps the conversion in range
+// (2147483520 is the largest float below 2^31; v >= 2147483647.0f means v >= 2^31), a NaN
+// clamps to -2^31 and is then replaced by 0.
 __device__ __forceinline__ int sat_i32(float v) {
-    if (v != v) return 0;
-    if (v >= 2147483647.0f) return 2147483647;
-    if (v <= -2147483648.0f) return (int)0x80000000;
-    return (int)v;
+    const int t = (int)fminf(fmaxf(v, -2147483648.0f), 2147483520.0f);
+    const int r = v >= 2147483647.0f ? 2147483647 : t;
+    return v != v ? 0 : r;
 }
 
 // compute_alpha (flow.rs:14-32): integer fast paths for n = 1..4, else powf; clamp keeps NaN.
