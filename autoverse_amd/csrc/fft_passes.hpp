@@ -210,7 +210,9 @@ __global__ __launch_bounds__(NT) void k_spectral(const float2* __restrict__ X, f
                                       blockIdx.y);
 }
 
-// batched (2D): item = (member, column block, work item)
+// batched (2D): item = (member, column block, work item).  Blocks of 16 or more columns
+// share few lines with their neighbours: launch order (the XCD-aware order of the adjoint's
+// narrower blocks measured slower here at sides 128 and 256).
 template <int L, int CB, int NT, int RP>
 __global__ __launch_bounds__(NT) void kb_spectral(const MemberDesc* __restrict__ mem, const int4* __restrict__ items) {
     const int4 it = items[blockIdx.x];
