@@ -100,7 +100,7 @@ __global__ __launch_bounds__(NT) void kb_adj_spectral(const MemberDesc* __restri
     static_assert((CB * L) % NT == 0, "tile must split evenly over the threads");
     __shared__ float2 buf[CB * LP];
     __shared__ float2 tw[L];
-    const int lin = xcd_chunk_item(nitems);
+    const int lin = xcd_chunk_item<16>(nitems);
     if (lin < 0) return;  // before any barrier: the whole workgroup leaves
     const int4 it = items[lin];
     const MemberDesc& m = mem[it.x];
@@ -330,7 +330,7 @@ bool launch_adj_rows_fwd(int L, hipStream_t st, const MemberDesc* mem, const int
 }
 
 bool launch_adj_spectral(int L, hipStream_t st, const MemberDesc* mem, const int4* items, int nitems) {
-#define X_(n) if (L == n) { hipLaunchKernelGGL((kb_adj_spectral<n, AdjCfg<n>::CB, AdjCfg<n>::NT>), dim3(xcd_chunk_grid(nitems)), dim3(AdjCfg<n>::NT), 0, st, mem, items, nitems); return true; }
+#define X_(n) if (L == n) { hipLaunchKernelGGL((kb_adj_spectral<n, AdjCfg<n>::CB, AdjCfg<n>::NT>), dim3(xcd_chunk_grid<16>(nitems)), dim3(AdjCfg<n>::NT), 0, st, mem, items, nitems); return true; }
     HP_ADJ_SIZES(X_)
 #undef X_
     return false;

@@ -33,6 +33,11 @@ constexpr int TX = 64, TY = 4, NT = TX * TY;
 constexpr int HW = TX + 2, HH = TY + 2;
 }  // namespace adj
 
+// Tiles are items in row-major tile order; a tile reads one or two rows of the tiles above and
+// below as halo.  Chunks of 64 consecutive tiles (eight tile rows of a 512-wide member) run on
+// one XCD (xcd_chunk_item), so most of those rows are read from memory once.
+constexpr int kTileChunk = 64;
+
 // alpha'(m) on the open interval where clamp((m/beta)^n, 0, 1) is inactive.
 __device__ __forceinline__ float dalpha_of(float mass, float beta_a, float n) {
     const float r = mass / beta_a;
@@ -60,11 +65,13 @@ __device__ __forceinline__ float dalpha_of(float mass, float beta_a, float n) {
 // items[b] = (member, tile); tiles of TX x TY cells, row-major over ceil(W/TX) columns.
 // li: lam[li] holds dL/dA_{t+1}; lam[1 - li] receives the first term of dL/dA_t.
 __global__ __launch_bounds__(adj::NT) void kb_adj_flow(const MemberDesc* __restrict__ mem,
-                                                      const int4* __restrict__ items, int cur, int li) {
+                                                      const int4* __restrict__ items, int nitems, int cur, int li) {
     using namespace adj;
     __shared__ float sA[HH][HW];
     __shared__ float sU[HH][HW];
-    const int4 it = items[blockIdx.x];
+    const int lin = xcd_chunk_item<kTileChunk>(nitems);
+    if (lin < 0) return;  // before any barrier: the whole workgroup leaves
+    const int4 it = items[lin];
     const MemberDesc& m = mem[it.x];
     const int W = m.g.W, H = m.g.H, C = m.C;
     const long long N = m.g.N;
@@ -259,9 +266,11 @@ __device__ __forceinline__ float sobel_adj(gptr<const float> ax, gptr<const floa
 
 // items as kb_adj_flow.  lam[1 - li] += dL/dA_sum (every channel); lamU_c = dL/dU_c.
 __global__ __launch_bounds__(adj::NT) void kb_adj_div(const MemberDesc* __restrict__ mem,
-                                                     const int4* __restrict__ items, int li) {
+                                                     const int4* __restrict__ items, int nitems, int li) {
     using namespace adj;
-    const int4 it = items[blockIdx.x];
+    const int lin = xcd_chunk_item<kTileChunk>(nitems);
+    if (lin < 0) return;
+    const int4 it = items[lin];
     const MemberDesc& m = mem[it.x];
     const int W = m.g.W, H = m.g.H, C = m.C;
     const long long N = m.g.N;
@@ -286,11 +295,13 @@ __global__ __launch_bounds__(adj::NT) void kb_adj_div(const MemberDesc* __restri
 }
 
 void launch_adj_flow(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int cur, int li) {
-    hipLaunchKernelGGL(kb_adj_flow, dim3(nitems), dim3(adj::NT), 0, st, mem, items, cur, li);
+    hipLaunchKernelGGL(kb_adj_flow, dim3(xcd_chunk_grid<kTileChunk>(nitems)), dim3(adj::NT), 0, st, mem, items, nitems,
+                       cur, li);
 }
 
 void launch_adj_div(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, int li) {
-    hipLaunchKernelGGL(kb_adj_div, dim3(nitems), dim3(adj::NT), 0, st, mem, items, li);
+    hipLaunchKernelGGL(kb_adj_div, dim3(xcd_chunk_grid<kTileChunk>(nitems)), dim3(adj::NT), 0, st, mem, items, nitems,
+                       li);
 }
 
 int adj_tile_w() { return adj::TX; }

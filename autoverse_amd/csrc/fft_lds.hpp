@@ -154,20 +154,6 @@ __device__ __forceinline__ void lds_fft(float2* buf, const float2* tw) {
     fft_stages<L, 0, DIR, NFFT, NT, LP, RP>(buf, tw);
 }
 
-// XCD-aware item order for the batched column passes.  Workgroup b runs on XCD b % 8, and
-// neighbouring column blocks (consecutive items) share 128-byte lines of the spectrum rows and
-// of the kernel spectra.  Items go to the XCDs in chunks of XCD_CHUNK consecutive items, dealt
-// round-robin: a chunk runs on one XCD at about the same time, so its lines are fetched into
-// one L2 once, and the XCDs get near-equal shares of the list.  The grid is
-// xcd_chunk_grid(nitems) workgroups; a workgroup past the last item gets -1.
-constexpr int XCD_CHUNK = 16;
-__device__ __forceinline__ int xcd_chunk_item(int nitems) {
-    const int b = (int)blockIdx.x, x = b % 8, t = b / 8;
-    const int lin = ((t / XCD_CHUNK) * 8 + x) * XCD_CHUNK + t % XCD_CHUNK;
-    return lin < nitems ? lin : -1;
-}
-inline int xcd_chunk_grid(int nitems) { return (nitems + 8 * XCD_CHUNK - 1) / (8 * XCD_CHUNK) * (8 * XCD_CHUNK); }
-
 template <int NT>
 __device__ __forceinline__ void stage_twiddles(float2* tw, const float2* __restrict__ g, int L) {
     for (int i = threadIdx.x; i < L; i += NT) tw[i] = g[i];

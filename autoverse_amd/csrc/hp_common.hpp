@@ -97,6 +97,23 @@ struct MemberDesc {
     long long stage_off;         // this member's offset in the contiguous host-transfer staging
 };
 
+// XCD-aware item order for batched passes whose neighbouring items share cache lines (column
+// blocks narrower than a 128-byte line, tiles that read their neighbours' rows as halo).
+// Workgroup b runs on XCD b % 8.  Items go to the XCDs in chunks of CH consecutive items,
+// dealt round-robin: a chunk runs on one XCD at about the same time, so the lines it shares
+// are fetched into one L2 once, and the XCDs get near-equal shares of the list.  The grid is
+// xcd_chunk_grid<CH>(nitems) workgroups; a workgroup past the last item gets -1.
+template <int CH>
+__device__ __forceinline__ int xcd_chunk_item(int nitems) {
+    const int b = (int)blockIdx.x, x = b % 8, t = b / 8;
+    const int lin = ((t / CH) * 8 + x) * CH + t % CH;
+    return lin < nitems ? lin : -1;
+}
+template <int CH>
+inline int xcd_chunk_grid(int nitems) {
+    return (nitems + 8 * CH - 1) / (8 * CH) * (8 * CH);
+}
+
 // ---- launchers implemented in fft_kernels.hip -------------------------------------
 // Each returns false if the length is not one of the compiled FFT sizes.
 bool fft_supported(int L);
