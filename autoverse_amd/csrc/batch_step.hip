@@ -94,9 +94,9 @@ __device__ __forceinline__ void p_stencil_cc(unsigned char* lds, const MemberDes
     St2Lds<PTile>& S = *reinterpret_cast<St2Lds<PTile>*>(lds);
     const float ta = (2.0f * m.s) * (2.0f * m.s);
     const int tiles_x = (m.g.W + PTile::TX - 1) / PTile::TX;
-    stencil2d_core<CC, POW2, false, PTile>(S.sU, S.sM, S.sOX, S.sOY, S.sAct, m.A[cur], m.U, m.A[1 - cur], m.C, m.g.W,
-                                           m.g.H, m.g.N, tiles_x, it.y, m.dt, m.beta_a, m.nexp, m.s, ta, 1.0f / ta,
-                                           flags);
+    stencil2d_core<CC, POW2, false, PTile, gptr<const float>, gptr<float>>(
+        S.sU, S.sM, S.sOX, S.sOY, S.sAct, gpc(m.A[cur]), gpc(m.U), gp(m.A[1 - cur]), m.C, m.g.W, m.g.H, m.g.N, tiles_x,
+        it.y, m.dt, m.beta_a, m.nexp, m.s, ta, 1.0f / ta, flags);
 }
 
 }  // namespace

@@ -261,11 +261,15 @@ struct St2Lds {
     unsigned sAct[TL::NT / 64];
 };
 
-template <int CC, bool POW2, bool CONS, class TL>
+// PC / PW: the pointer types of the state and affinity reads and of the output writes -- plain
+// pointers for kernel arguments, gptr<const float> / gptr<float> for pointers read from a
+// member descriptor (global loads and stores rather than flat ones, whose outstanding count
+// the LDS waits of the gather would otherwise include).
+template <int CC, bool POW2, bool CONS, class TL, class PC = const float*, class PW = float*>
 __device__ __forceinline__ void stencil2d_core(float (*__restrict__ sU)[TL::NLOAD], float* __restrict__ sM,
                                                float (*__restrict__ sOX)[TL::NSRC], float (*__restrict__ sOY)[TL::NSRC],
-                                               unsigned* __restrict__ sAct, const float* __restrict__ A,
-                                               const float* __restrict__ U, float* __restrict__ Aout, int Cdyn, int W,
+                                               unsigned* __restrict__ sAct, PC __restrict__ A,
+                                               PC __restrict__ U, PW __restrict__ Aout, int Cdyn, int W,
                                                int H, long long N, int tiles_x, int tile, float dt, float beta_a,
                                                float nexp, float s, float ta, float inv_ta, int* __restrict__ flag) {
     constexpr int TX = TL::TX, TY = TL::TY, NT = TL::NT, SR = TL::SR, SW = TL::SW;
@@ -340,7 +344,7 @@ __device__ __forceinline__ void stencil2d_core(float (*__restrict__ sU)[TL::NLOA
     // loads are unconditional (past the last channel they reread it) and are issued before the
     // previous gather's stores, so that nothing waits for the stores.
     {
-        const float* Un = U + (long long)(C > 1 ? 1 : 0) * N;
+        const auto Un = U + (long long)(C > 1 ? 1 : 0) * N;
 #pragma unroll
         for (int q = 0; q < LPT; ++q) upre[q] = Un[goff[q]];
     }
@@ -353,7 +357,7 @@ __device__ __forceinline__ void stencil2d_core(float (*__restrict__ sU)[TL::NLOA
         if (gdx < W) {
 #pragma unroll
             for (int c = 0; c < (CC > 0 ? CC : 1); ++c) {
-                float* Oc = Aout + (long long)c * N;
+                const auto Oc = Aout + (long long)c * N;
 #pragma unroll
                 for (int r = 0; r < SR; ++r)
                     if (gdy0 + r < H) Oc[(long long)(gdy0 + r) * W + gdx] = 0.0f;
@@ -453,12 +457,12 @@ __device__ __forceinline__ void stencil2d_core(float (*__restrict__ sU)[TL::NLOA
             for (int r = 0; r < SR; ++r) acc[r] = (float)accd[CONS ? r : 0];
         }
         {  // channel c + 2's affinity (see above)
-            const float* Un = U + (long long)(c + 2 < C ? c + 2 : C - 1) * N;
+            const auto Un = U + (long long)(c + 2 < C ? c + 2 : C - 1) * N;
 #pragma unroll
             for (int q = 0; q < LPT; ++q) upre[q] = Un[goff[q]];
         }
         if (gdx < W) {
-            float* Oc = Aout + (long long)c * N;
+            const auto Oc = Aout + (long long)c * N;
 #pragma unroll
             for (int r = 0; r < SR; ++r)
                 if (gdy0 + r < H) Oc[(long long)(gdy0 + r) * W + gdx] = acc[r];

@@ -28,9 +28,8 @@
 
 namespace hp {
 
-template <int CC, bool POW2, bool CONS, class TL>
-__device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, const float* __restrict__ U,
-                                               float* __restrict__ Aout, int Cdyn, int W, int H, long long N,
+template <int CC, bool POW2, bool CONS, class TL, class PC = const float*, class PW = float*>
+__device__ __forceinline__ void stencil2d_body(PC A, PC U, PW Aout, int Cdyn, int W, int H, long long N,
                                                int tiles_x, int tile, float dt, float beta_a, float nexp, float s,
                                                float ta, float inv_ta, int* __restrict__ flag) {
     __shared__ float sU[2][TL::NLOAD];
@@ -38,8 +37,8 @@ __device__ __forceinline__ void stencil2d_body(const float* __restrict__ A, cons
     __shared__ float sOX[4][TL::NSRC];
     __shared__ float sOY[4][TL::NSRC];
     __shared__ unsigned sAct[TL::NT / 64];
-    stencil2d_core<CC, POW2, CONS, TL>(sU, sM, sOX, sOY, sAct, A, U, Aout, Cdyn, W, H, N, tiles_x, tile, dt, beta_a,
-                                       nexp, s, ta, inv_ta, flag);
+    stencil2d_core<CC, POW2, CONS, TL, PC, PW>(sU, sM, sOX, sOY, sAct, A, U, Aout, Cdyn, W, H, N, tiles_x, tile, dt,
+                                               beta_a, nexp, s, ta, inv_ta, flag);
 }
 
 template <int CC, bool POW2, bool CONS, class TL>
@@ -62,8 +61,8 @@ __global__ __launch_bounds__(TL::NT, TL::WAVES_PER_EU) void kb_stencil2d(const M
     const MemberDesc& m = mem[it.x];
     const float ta = (2.0f * m.s) * (2.0f * m.s);
     const int tiles_x = (m.g.W + TL::TX - 1) / TL::TX;
-    stencil2d_body<CC, POW2, false, TL>(m.A[cur], m.U, m.A[1 - cur], m.C, m.g.W, m.g.H, m.g.N, tiles_x, it.y, m.dt,
-                                        m.beta_a, m.nexp, m.s, ta, 1.0f / ta, flag);
+    stencil2d_body<CC, POW2, false, TL>(gpc(m.A[cur]), gpc(m.U), gp(m.A[1 - cur]), m.C, m.g.W, m.g.H, m.g.N, tiles_x,
+                                        it.y, m.dt, m.beta_a, m.nexp, m.s, ta, 1.0f / ta, flag);
 }
 
 // HP_TILE binds the run-time tile variant to the compile-time tile type TL of the call.
