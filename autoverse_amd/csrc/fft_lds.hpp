@@ -95,6 +95,19 @@ __device__ __forceinline__ void dft(float2* v) {
     }
 }
 
+// The first stage writes its butterflies' outputs R0 apart (R0 its radix): in the plain
+// layout the 16 lanes of a quarter-wave then share two or four of the 32 banks.  Between the
+// first and the second stage a transform's element i is kept at local index
+// i ^ ((i >> 4) & (R0 - 1)) instead, a permutation inside each aligned group of R0 elements
+// (so inside the transform) that spreads those writes over all banks; the second stage reads
+// through the same map, and every other stage, the caller's input and the output use the plain
+// layout.  Only the LDS addresses change: every element takes part in the same arithmetic.
+template <int L, int RP>
+__host__ __device__ constexpr int swizzle_mask() {
+    constexpr int R0 = radix_at(L, 0, RP);
+    return num_stages(L, RP) >= 2 && (R0 == 2 || R0 == 4 || R0 == 8) ? R0 - 1 : 0;
+}
+
 // One Stockham stage over NFFT transforms of length L at buf[f*LP + i], run by NT threads.
 template <int L, int STAGE, int DIR, int NFFT, int NT, int LP, int RP>
 __device__ __forceinline__ void fft_stage(float2* buf, const float2* tw) {
@@ -104,15 +117,20 @@ __device__ __forceinline__ void fft_stage(float2* buf, const float2* tw) {
     constexpr int TOT = NFFT * NB;
     constexpr int Q = (TOT + NT - 1) / NT;
     constexpr int TSTEP = L / (NS * R);
+    constexpr int SWM = swizzle_mask<L, RP>();
+    constexpr bool SW_IN = SWM != 0 && STAGE == 1, SW_OUT = SWM != 0 && STAGE == 0;
     float2 v[Q][R];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int b = threadIdx.x + q * NT;
         if (TOT % NT == 0 || b < TOT) {
             const int f = b / NB, j = b - f * NB;
-            const float2* src = buf + f * LP + j;
+            const float2* src = buf + f * LP;
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = src[r * NB];
+            for (int r = 0; r < R; ++r) {
+                const int i = j + r * NB;
+                v[q][r] = src[SW_IN ? i ^ ((i >> 4) & SWM) : i];
+            }
         }
     }
     __syncthreads();
@@ -131,9 +149,13 @@ __device__ __forceinline__ void fft_stage(float2* buf, const float2* tw) {
                 }
             }
             dft<R, DIR>(v[q]);
-            float2* dst = buf + f * LP + (j / NS) * NS * R + k;
+            const int o = (j / NS) * NS * R + k;
+            float2* dst = buf + f * LP;
 #pragma unroll
-            for (int r = 0; r < R; ++r) dst[r * NS] = v[q][r];
+            for (int r = 0; r < R; ++r) {
+                const int i = o + r * NS;
+                dst[SW_OUT ? i ^ ((i >> 4) & SWM) : i] = v[q][r];
+            }
         }
     }
     __syncthreads();
