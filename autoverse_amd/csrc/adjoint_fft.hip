@@ -35,17 +35,35 @@ __global__ __launch_bounds__(NT) void kb_adj_rows_fwd(const MemberDesc* __restri
     const bool has1 = (y0 + 1) < rows;
     const GrowthParams p = m.gp[k];
     const int2 km = m.kmap[k];
-    const float* cv = m.conv + ((long long)cslot * m.K + k) * N + (long long)y0 * L;
-    const float* lu = m.lamU + (long long)km.y * N + (long long)y0 * L;
+    const auto cv = gpc(m.conv) + ((long long)cslot * m.K + k) * N + (long long)y0 * L;
+    const auto lu = gpc(m.lamU) + (long long)km.y * N + (long long)y0 * L;
     const float inv_s3 = 1.0f / (p.sigma * p.sigma * p.sigma);
-    stage_twiddles<NT>(tw, m.tw_w, L);
+    stage_twiddles<NT, L>(tw, gpc(m.tw_w));
+    // every load of the row pair first (global loads, issued together), then the arithmetic in
+    // the order of the element loop below
+    constexpr int PER = (L + NT - 1) / NT;
+    float uv[PER][2], lv[PER][2];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int i = threadIdx.x + q * NT;
+        const bool ok = L % NT == 0 || i < L;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const bool in = ok && (r == 0 || has1);
+            uv[q][r] = in ? cv[r * L + (in ? i : 0)] : 0.0f;
+            lv[q][r] = in ? lu[r * L + (in ? i : 0)] : 0.0f;
+        }
+    }
     double sw = 0.0, smu = 0.0, ssig = 0.0;
-    for (int i = threadIdx.x; i < L; i += NT) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int i = threadIdx.x + q * NT;
+        if (!(L % NT == 0 || i < L)) break;
         float g[2] = {0.0f, 0.0f};
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
             if (r == 1 && !has1) break;
-            const float u = cv[r * L + i], l = lu[r * L + i];
+            const float u = uv[q][r], l = lv[q][r];
             const float d = u - p.mu;
             const float e = expf(-d * d * p.inv);
             g[r] = l * p.weight * (-4.0f * d * p.inv * e);
@@ -60,13 +78,13 @@ __global__ __launch_bounds__(NT) void kb_adj_rows_fwd(const MemberDesc* __restri
     red[2][threadIdx.x] = ssig;
     __syncthreads();
     lds_fft<L, -1, 1, NT, L>(buf, tw);
-    float2* x0 = m.Y + (long long)k * m.g.spec + (long long)y0 * m.g.Wp;
-    float2* x1 = x0 + m.g.Wp;
+    const auto x0 = gp(m.Y) + (long long)k * m.g.spec + (long long)y0 * m.g.Wp;
+    const auto x1 = x0 + m.g.Wp;
     for (int kx = threadIdx.x; kx <= L / 2; kx += NT) {
         const float2 zk = buf[kx];
         const float2 zm = buf[kx == 0 ? 0 : L - kx];
-        x0[kx] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-        if (has1) x1[kx] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+        st2(x0 + kx, make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y)));
+        if (has1) st2(x1 + kx, make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x)));
     }
     if (threadIdx.x < 3) {  // fixed-order block sum, then into this row pair's own slot
         double t = 0.0;
@@ -93,7 +111,7 @@ struct AdjCfg {
 // xcd_chunk_item, so each line is fetched into one L2 once, not once per block (the pass read
 // 4.3 times its bytes from memory with one item per workgroup in launch order).
 template <int L, int CB, int NT>
-__global__ __launch_bounds__(NT) void kb_adj_spectral(const MemberDesc* __restrict__ mem,
+__global__ __launch_bounds__(NT, 4) void kb_adj_spectral(const MemberDesc* __restrict__ mem,
                                                       const int4* __restrict__ items, int nitems) {
     constexpr int LP = L + 1;
     constexpr int PER = CB * L / NT;
@@ -108,20 +126,26 @@ __global__ __launch_bounds__(NT) void kb_adj_spectral(const MemberDesc* __restri
     const int Wh = m.g.Wh, Wp = m.g.Wp;
     const long long spec = m.g.spec, kq_stride = (long long)m.g.Hq * Wh;
     const long long line0 = (long long)it.y * CB;
-    float2* xs = m.X + (long long)wk.src_slot * spec;
-    stage_twiddles<NT>(tw, m.tw_h, L);
+    const auto xs = gp(m.X) + (long long)wk.src_slot * spec;
+    stage_twiddles<NT, L>(tw, gpc(m.tw_h));
     float2 xr[PER], acc[PER];
     int koff[PER];
+    // every memory read is issued before the LDS stores and transforms that follow it
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int i = threadIdx.x + q * NT;
         const int c = i % CB, e = i / CB;
         const long long kx = line0 + c;
         const bool valid = kx < Wh;
-        buf[c * LP + e] = valid ? xs[(long long)e * Wp + kx] : make_float2(0.f, 0.f);
+        xr[q] = valid ? ld2(xs + (long long)e * Wp + kx) : make_float2(0.f, 0.f);
         const int kyf = e <= L / 2 ? e : L - e;
         koff[q] = valid ? kyf * Wh + (int)kx : -1;
         acc[q] = make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int i = threadIdx.x + q * NT;
+        buf[(i % CB) * LP + i / CB] = xr[q];
     }
     __syncthreads();
     lds_fft<L, -1, CB, NT, LP>(buf, tw);
@@ -132,31 +156,47 @@ __global__ __launch_bounds__(NT) void kb_adj_spectral(const MemberDesc* __restri
     }
     for (int j = 0; j < wk.k_count; ++j) {
         const int k = m.klist_spec[wk.k_first + j];
-        const float* kq = m.Kq + (long long)k * kq_stride;
-        const float2* gs = m.Y + (long long)k * spec;
-        float2* pk = m.P + (long long)k * spec;
+        const auto kq = gpc(m.Kq) + (long long)k * kq_stride;
+        const auto gs = gpc(m.Y) + (long long)k * spec;
+        const auto pk = gp(m.P) + (long long)k * spec;
+        float2 gv[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int i = threadIdx.x + q * NT;
+            const long long o = (long long)(i / CB) * Wp + line0 + i % CB;
+            gv[q] = koff[q] >= 0 ? ld2(gs + o) : make_float2(0.f, 0.f);
+        }
         __syncthreads();  // previous reads of buf are done
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int i = threadIdx.x + q * NT;
-            const int c = i % CB, e = i / CB;
-            buf[c * LP + e] = koff[q] >= 0 ? gs[(long long)e * Wp + line0 + c] : make_float2(0.f, 0.f);
+            buf[(i % CB) * LP + i / CB] = gv[q];
         }
         __syncthreads();
         lds_fft<L, -1, CB, NT, LP>(buf, tw);
+        // P and the kernel spectrum after the transform (issued while it was still in flight
+        // they held the registers of a fourth wave per SIMD), all before the arithmetic
+        float2 pv[PER];
+        float kv[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int i = threadIdx.x + q * NT;
+            const long long o = (long long)(i / CB) * Wp + line0 + i % CB;
+            const bool ok = koff[q] >= 0;
+            pv[q] = ok ? ld2(pk + o) : make_float2(0.f, 0.f);
+            kv[q] = ok ? kq[koff[q]] : 0.0f;
+        }
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int i = threadIdx.x + q * NT;
             const int c = i % CB, e = i / CB;
             if (koff[q] < 0) continue;
             const float2 gq = buf[c * LP + e];
-            const float kv = kq[koff[q]];
-            float2* pp = pk + (long long)e * Wp + line0 + c;
-            const float2 pv = *pp;
             // g * conj(x)
-            *pp = make_float2(pv.x + (gq.x * xr[q].x + gq.y * xr[q].y), pv.y + (gq.y * xr[q].x - gq.x * xr[q].y));
-            acc[q].x += kv * gq.x;
-            acc[q].y += kv * gq.y;
+            st2(pk + (long long)e * Wp + line0 + c,
+                make_float2(pv[q].x + (gq.x * xr[q].x + gq.y * xr[q].y), pv[q].y + (gq.y * xr[q].x - gq.x * xr[q].y)));
+            acc[q].x += kv[q] * gq.x;
+            acc[q].y += kv[q] * gq.y;
         }
     }
     __syncthreads();
@@ -171,7 +211,7 @@ __global__ __launch_bounds__(NT) void kb_adj_spectral(const MemberDesc* __restri
     for (int q = 0; q < PER; ++q) {
         const int i = threadIdx.x + q * NT;
         const int c = i % CB, e = i / CB;
-        if (koff[q] >= 0) xs[(long long)e * Wp + line0 + c] = buf[c * LP + e];
+        if (koff[q] >= 0) st2(xs + (long long)e * Wp + line0 + c, buf[c * LP + e]);
     }
 }
 
@@ -190,31 +230,41 @@ __global__ __launch_bounds__(NT) void kb_adj_rows_inv(const MemberDesc* __restri
     const bool has1 = (y0 + 1) < rows;
     const long long N = m.g.N;
     const float scale = 1.0f / (float)N;
-    stage_twiddles<NT>(tw, m.tw_w, L);
-    const float2* r0 = m.X + (long long)it.z * m.g.spec + (long long)y0 * m.g.Wp;
-    const float2* r1 = r0 + m.g.Wp;
-    for (int n = threadIdx.x; n < L; n += NT) {
-        float2 a, b;
-        if (n <= L / 2) {
-            a = r0[n];
-            b = has1 ? r1[n] : make_float2(0.f, 0.f);
-            buf[n] = make_float2(a.x - b.y, a.y + b.x);
-        } else {
-            a = r0[L - n];
-            b = has1 ? r1[L - n] : make_float2(0.f, 0.f);
-            buf[n] = make_float2(a.x + b.y, b.x - a.y);
+    stage_twiddles<NT, L>(tw, m.tw_w);
+    const auto r0 = gpc(m.X) + (long long)it.z * m.g.spec + (long long)y0 * m.g.Wp;
+    const auto r1 = r0 + m.g.Wp;
+    const auto out = gp(m.lam[1 - li]) + (long long)m.src_chan[it.z] * N + (long long)y0 * L;
+    // every load first (the two spectrum rows and the accumulated rows), then the LDS stores
+    float2 ra[PER], rb[PER];
+    float o0[PER], o1[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int n = threadIdx.x + q * NT;
+        if (L % NT == 0 || n < L) {
+            const int mm = n <= L / 2 ? n : L - n;
+            ra[q] = ld2(r0 + mm);
+            rb[q] = has1 ? ld2(r1 + mm) : make_float2(0.f, 0.f);
+            o0[q] = out[n];
+            o1[q] = has1 ? out[L + n] : 0.0f;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int n = threadIdx.x + q * NT;
+        if (L % NT == 0 || n < L) {
+            const float2 a = ra[q], b = rb[q];
+            buf[n] = n <= L / 2 ? make_float2(a.x - b.y, a.y + b.x) : make_float2(a.x + b.y, b.x - a.y);
         }
     }
     __syncthreads();
     lds_fft<L, +1, 1, NT, L>(buf, tw);
-    float* out = m.lam[1 - li] + (long long)m.src_chan[it.z] * N + (long long)y0 * L;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int n = threadIdx.x + q * NT;
         if (L % NT == 0 || n < L) {
             const float2 z = buf[n];
-            out[n] += z.x * scale;
-            if (has1) out[L + n] += z.y * scale;
+            out[n] = o0[q] + z.x * scale;
+            if (has1) out[L + n] = o1[q] + z.y * scale;
         }
     }
 }

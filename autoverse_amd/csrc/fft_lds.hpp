@@ -176,9 +176,23 @@ __device__ __forceinline__ void lds_fft(float2* buf, const float2* tw) {
     fft_stages<L, 0, DIR, NFFT, NT, LP, RP>(buf, tw);
 }
 
-template <int NT>
-__device__ __forceinline__ void stage_twiddles(float2* tw, const float2* __restrict__ g, int L) {
-    for (int i = threadIdx.x; i < L; i += NT) tw[i] = g[i];
+// Copy the length-L twiddle table into LDS, workgroup-cooperative: every load first, then the
+// stores (in one loop the compiler reuses one register pair and waits for each load in turn).
+// PT: a plain pointer (kernel argument) or gptr<const float2> (read from a member descriptor).
+template <int NT, int L, class PT>
+__device__ __forceinline__ void stage_twiddles(float2* tw, PT g) {
+    constexpr int PER = (L + NT - 1) / NT;
+    float2 v[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int i = threadIdx.x + q * NT;
+        if (L % NT == 0 || i < L) v[q] = ld2(g + i);
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int i = threadIdx.x + q * NT;
+        if (L % NT == 0 || i < L) tw[i] = v[q];
+    }
 }
 
 // Launch shapes per length: threads of a row pass, columns per spectral workgroup and

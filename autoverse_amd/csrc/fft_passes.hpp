@@ -45,8 +45,24 @@ __device__ __forceinline__ void rows_fwd_core(float2* __restrict__ buf, float2* 
     const int y0 = 2 * pair;
     const bool has1 = (y0 + 1) < rows;
     const float* a = A + (long long)src_chan[s] * N + (long long)y0 * L;
-    stage_twiddles<NT>(tw, twg, L);
-    for (int i = threadIdx.x; i < L; i += NT) buf[i] = make_float2(a[i], has1 ? a[L + i] : 0.0f);
+    stage_twiddles<NT, L>(tw, twg);
+    // the row pair's loads first, then the LDS stores: in one loop a flat load (the batched
+    // kernels' pointers come from a descriptor) waits behind the previous LDS store
+    constexpr int PER = (L + NT - 1) / NT;
+    float va[PER], vb[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int i = threadIdx.x + q * NT;
+        if (L % NT == 0 || i < L) {
+            va[q] = a[i];
+            vb[q] = has1 ? a[L + i] : 0.0f;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int i = threadIdx.x + q * NT;
+        if (L % NT == 0 || i < L) buf[i] = make_float2(va[q], vb[q]);
+    }
     __syncthreads();
     lds_fft<L, -1, 1, NT, L, RP>(buf, tw);
     float2* x0 = X + (long long)s * spec + (long long)y0 * Wp;
@@ -94,7 +110,7 @@ __global__ __launch_bounds__(NT) void k_cols(float2* __restrict__ X, int Wh, int
     __shared__ float2 tw[L];
     const int kx0 = blockIdx.x * CB;
     float2* base = X + (long long)blockIdx.z * spec + (long long)blockIdx.y * L * Wp;
-    stage_twiddles<NT>(tw, twg, L);
+    stage_twiddles<NT, L>(tw, twg);
     for (int i = threadIdx.x; i < CB * L; i += NT) {
         const int c = i % CB, e = i / CB;
         const int kx = kx0 + c;
@@ -129,7 +145,7 @@ __device__ __forceinline__ void spectral_core(float2* __restrict__ buf, float2* 
     const long long es = DIM == 2 ? (long long)Wp : (long long)H * Wp;
     const long long line0 = (long long)blk * CB;
     const float2* xs = X + (long long)wk.src_slot * spec;
-    stage_twiddles<NT>(tw, twg, L);
+    stage_twiddles<NT, L>(tw, twg);
     float2 reg[PER];
     int koff[PER];  // offset of this element's folded kernel-spectrum value, or -1 if the line is padding
 #pragma unroll
@@ -148,8 +164,7 @@ __device__ __forceinline__ void spectral_core(float2* __restrict__ buf, float2* 
             kx = (int)(line - (long long)ky * Wp);
         }
         const bool valid = line < nlines && kx < Wh;
-        const float2 v = valid ? xs[(long long)e * es + line] : make_float2(0.f, 0.f);
-        buf[c * LP + e] = v;
+        reg[q] = valid ? xs[(long long)e * es + line] : make_float2(0.f, 0.f);
         if (DIM == 2) {
             const int kyf = ky <= L / 2 ? ky : L - ky;
             koff[q] = valid ? kyf * Wh + kx : -1;
@@ -158,6 +173,11 @@ __device__ __forceinline__ void spectral_core(float2* __restrict__ buf, float2* 
             const int kzf = e <= L / 2 ? e : L - e;
             koff[q] = valid ? (kzf * Hq + kyf) * Wh + kx : -1;
         }
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {  // the LDS stores after every load (see rows_fwd_core)
+        const int i = threadIdx.x + q * NT;
+        if (EVEN || i < TOT) buf[(i % CB) * LP + i / CB] = reg[q];
     }
     __syncthreads();
     lds_fft<L, -1, CB, NT, LP, RP>(buf, tw);
@@ -170,12 +190,14 @@ __device__ __forceinline__ void spectral_core(float2* __restrict__ buf, float2* 
         const int k = klist[wk.k_first + j];
         const float* kq = Kq + (long long)k * kq_stride;
         __syncthreads();  // previous kernel's outputs have been read
+        float kv[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) kv[q] = koff[q] >= 0 ? kq[koff[q]] : 0.0f;
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int i = threadIdx.x + q * NT;
             if (!EVEN && i >= TOT) continue;
-            const float kv = koff[q] >= 0 ? kq[koff[q]] : 0.0f;
-            buf[(i % CB) * LP + i / CB] = make_float2(reg[q].x * kv, reg[q].y * kv);
+            buf[(i % CB) * LP + i / CB] = make_float2(reg[q].x * kv[q], reg[q].y * kv[q]);
         }
         __syncthreads();
         lds_fft<L, +1, CB, NT, LP, RP>(buf, tw);
@@ -245,7 +267,7 @@ __device__ __forceinline__ void rows_inv_core(float2* __restrict__ buf, float2* 
     const TgtWork wk = work[wi];
     const int y0 = 2 * pair;
     const bool has1 = (y0 + 1) < rows;
-    stage_twiddles<NT>(tw, twg, L);
+    stage_twiddles<NT, L>(tw, twg);
     float acc0[PER], acc1[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) acc0[q] = acc1[q] = 0.0f;
@@ -253,19 +275,26 @@ __device__ __forceinline__ void rows_inv_core(float2* __restrict__ buf, float2* 
         const int k = klist[wk.k_first + j];
         const float2* r0 = Y + (long long)k * spec + (long long)y0 * Wp;
         const float2* r1 = r0 + Wp;
+        // Hermitian completion: X0[n] = conj(X0[L-n]) for n > L/2, likewise X1; the packed
+        // line is Z = X0 + i X1, whose inverse is x0 + i x1.  Every load first, then the LDS
+        // stores (see rows_fwd_core).
+        float2 ra[PER], rb[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int n = threadIdx.x + q * NT;
+            if (L % NT == 0 || n < L) {
+                const int m = n <= L / 2 ? n : L - n;
+                ra[q] = r0[m];
+                rb[q] = has1 ? r1[m] : make_float2(0.f, 0.f);
+            }
+        }
         __syncthreads();  // previous kernel's results have been read
-        for (int n = threadIdx.x; n < L; n += NT) {
-            // Hermitian completion: X0[n] = conj(X0[L-n]) for n > L/2, likewise X1;
-            // the packed line is Z = X0 + i X1, whose inverse is x0 + i x1.
-            float2 a, b;
-            if (n <= L / 2) {
-                a = r0[n];
-                b = has1 ? r1[n] : make_float2(0.f, 0.f);
-                buf[n] = make_float2(a.x - b.y, a.y + b.x);
-            } else {
-                a = r0[L - n];
-                b = has1 ? r1[L - n] : make_float2(0.f, 0.f);
-                buf[n] = make_float2(a.x + b.y, b.x - a.y);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int n = threadIdx.x + q * NT;
+            if (L % NT == 0 || n < L) {
+                const float2 a = ra[q], b = rb[q];
+                buf[n] = n <= L / 2 ? make_float2(a.x - b.y, a.y + b.x) : make_float2(a.x + b.y, b.x - a.y);
             }
         }
         __syncthreads();

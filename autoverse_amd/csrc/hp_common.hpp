@@ -97,6 +97,40 @@ struct MemberDesc {
     long long stage_off;         // this member's offset in the contiguous host-transfer staging
 };
 
+// Pointers into global memory (HBM).  A pointer read from a descriptor in memory is generic to
+// the compiler, which then accesses it with flat loads and stores: those count against the LDS
+// counter as well, so every wait for an LDS read also waits for them; they take a full 64-bit
+// address per lane; and since a flat access may touch LDS, the compiler keeps each such load
+// behind the LDS stores before it (a loop that loads from memory and stores to LDS then waits
+// for every load in turn).  Accesses through gptr compile to global loads and stores.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> gp(T* p) {
+    return (gptr<T>)p;
+}
+template <class T>
+__device__ __forceinline__ gptr<const T> gpc(const T* p) {
+    return (gptr<const T>)p;
+}
+
+// float2 through a gptr: HIP's float2 is a class whose assignment operators take a generic
+// `this`, so its values move through the two-float vector type.  Plain pointers as usual.
+using f2v = float __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 ld2(const float2* p) { return *p; }
+__device__ __forceinline__ float2 ld2(gptr<const float2> p) {
+    const f2v v = *(gptr<const f2v>)p;
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ float2 ld2(gptr<float2> p) { return ld2((gptr<const float2>)p); }
+__device__ __forceinline__ void st2(float2* p, float2 a) { *p = a; }
+__device__ __forceinline__ void st2(gptr<float2> p, float2 a) {
+    f2v v;
+    v.x = a.x;
+    v.y = a.y;
+    *(gptr<f2v>)p = v;
+}
+
 // XCD-aware item order for batched passes whose neighbouring items share cache lines (column
 // blocks narrower than a 128-byte line, tiles that read their neighbours' rows as halo).
 // Workgroup b runs on XCD b % 8.  Items go to the XCDs in chunks of CH consecutive items,

@@ -9,21 +9,6 @@
 
 namespace hp {
 
-// Pointers into global memory (HBM).  A pointer read from a descriptor in memory is generic to
-// the compiler, which then accesses it with flat loads and stores: those count against the LDS
-// counter as well, so every wait for an LDS read also waits for them, and they take a full
-// 64-bit address per lane.  Accesses through gptr compile to global loads and stores.
-template <class T>
-using gptr = __attribute__((address_space(1))) T*;
-template <class T>
-__device__ __forceinline__ gptr<T> gp(T* p) {
-    return (gptr<T>)p;
-}
-template <class T>
-__device__ __forceinline__ gptr<const T> gpc(const T* p) {
-    return (gptr<const T>)p;
-}
-
 __device__ __forceinline__ int wrapi(int c, int n) {
     int r = c % n;
     return r < 0 ? r + n : r;
