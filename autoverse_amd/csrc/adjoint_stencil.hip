@@ -310,6 +310,20 @@ int adj_tile_h() { return adj::TY; }
 // ---- tape copies and clears -----------------------------------------------------------------
 // items[b] = (member, part, nparts): block b strides over the member's C*N floats.
 // to_tape: tape[slot] <- A[cur]; else A[cur] <- tape[slot].
+// dst[j] = src[j] for j = j0, j0 + stride, ... below n (the buffers do not overlap), with four
+// loads in flight per thread: one load and its store per iteration waited for every load.
+__device__ __forceinline__ void copy_strided(gptr<float> dst, gptr<const float> src, long long j, long long n,
+                                             long long stride) {
+    for (; j + 3 * stride < n; j += 4 * stride) {
+        const float v0 = src[j], v1 = src[j + stride], v2 = src[j + 2 * stride], v3 = src[j + 3 * stride];
+        dst[j] = v0;
+        dst[j + stride] = v1;
+        dst[j + 2 * stride] = v2;
+        dst[j + 3 * stride] = v3;
+    }
+    for (; j < n; j += stride) dst[j] = src[j];
+}
+
 __global__ __launch_bounds__(256) void kb_tape_copy(const MemberDesc* __restrict__ mem, const int4* __restrict__ items,
                                                    int cur, int slot, int to_tape) {
     const int4 it = items[blockIdx.x];
@@ -317,9 +331,8 @@ __global__ __launch_bounds__(256) void kb_tape_copy(const MemberDesc* __restrict
     const long long n = (long long)m.C * m.g.N;
     float* a = m.A[cur];
     float* t = slot < 0 ? m.snap : m.tape + (long long)slot * n;
-    const float* src = to_tape ? a : t;
-    float* dst = to_tape ? t : a;
-    for (long long j = (long long)it.y * 256 + threadIdx.x; j < n; j += (long long)it.z * 256) dst[j] = src[j];
+    copy_strided(gp(to_tape ? t : a), gpc(to_tape ? a : t), (long long)it.y * 256 + threadIdx.x, n,
+                 (long long)it.z * 256);
 }
 
 // U of a taped step to / from its tape slot (items as kb_tape_copy).
@@ -329,9 +342,8 @@ __global__ __launch_bounds__(256) void kb_u_copy(const MemberDesc* __restrict__ 
     const MemberDesc& m = mem[it.x];
     const long long n = (long long)m.C * m.g.N;
     float* t = m.tape_u + (long long)slot * n;
-    const float* src = to_tape ? m.U : t;
-    float* dst = to_tape ? t : m.U;
-    for (long long j = (long long)it.y * 256 + threadIdx.x; j < n; j += (long long)it.z * 256) dst[j] = src[j];
+    copy_strided(gp(to_tape ? t : m.U), gpc(to_tape ? m.U : t), (long long)it.y * 256 + threadIdx.x, n,
+                 (long long)it.z * 256);
 }
 
 // Point each member's A[0] at tape slot `slot`, and its U and source spectra X at tape_u and
@@ -392,10 +404,8 @@ __global__ __launch_bounds__(256) void kb_stage_lam(const MemberDesc* __restrict
     const long long n = (long long)m.C * m.g.N;
     float* l = m.lam[li];
     float* s = stage + m.stage_off;
-    for (long long j = (long long)it.y * 256 + threadIdx.x; j < n; j += (long long)it.z * 256) {
-        if (to_stage) s[j] = l[j];
-        else l[j] = s[j];
-    }
+    copy_strided(gp(to_stage ? s : l), gpc(to_stage ? l : s), (long long)it.y * 256 + threadIdx.x, n,
+                 (long long)it.z * 256);
 }
 
 void launch_stage_lam(hipStream_t st, const MemberDesc* mem, const int4* items, int nitems, float* stage, int li,
