@@ -448,6 +448,19 @@ __device__ __forceinline__ void stencil2d_core(float (*__restrict__ sU)[TL::NLOA
                 const int cb[4] = {base, base + 1, base + 2, base + 3};
                 const int ob[4] = {base, NSRC + base + 1, 2 * NSRC + base + 2, 3 * NSRC + base + 3};
                 gather_strip<0, POW2, CONS, TL>(sM, sOX[0], sOY[0], cb, ob, ta, acc, accd);
+            } else if (!CONS && CC >= 1 && CC <= 3 && __all(roty == 0 && gdx < W)) {
+                // an x edge only (every wave of an edge tile column; all of them on grids up to
+                // two tiles wide): the unrolled gather with the lane's rotated column bases (the
+                // exact-conservation and four-or-more-channel bodies, already at 128 registers,
+                // would spill and keep the rolled gather)
+                int cb[4], ob[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int jx = (i + rotx) & 3;
+                    cb[i] = base + jx;
+                    ob[i] = jx * NSRC + base + jx;
+                }
+                gather_strip<0, POW2, CONS, TL>(sM, sOX[0], sOY[0], cb, ob, ta, acc, accd);
             } else {
                 gather_strip_edge<POW2, CONS, TL>(sM, sOX[0], sOY[0], base, rotx, roty, ta, acc, accd);
             }
