@@ -111,10 +111,20 @@ __global__ __launch_bounds__(NT) void k_cols(float2* __restrict__ X, int Wh, int
     const int kx0 = blockIdx.x * CB;
     float2* base = X + (long long)blockIdx.z * spec + (long long)blockIdx.y * L * Wp;
     stage_twiddles<NT, L>(tw, twg);
-    for (int i = threadIdx.x; i < CB * L; i += NT) {
+    constexpr int PER = (CB * L + NT - 1) / NT;
+    float2 v[PER];  // every load first, then the LDS stores (see rows_fwd_core)
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int i = threadIdx.x + q * NT;
         const int c = i % CB, e = i / CB;
         const int kx = kx0 + c;
-        buf[c * LP + e] = kx < Wh ? base[(long long)e * Wp + kx] : make_float2(0.f, 0.f);
+        if ((CB * L) % NT == 0 || i < CB * L)
+            v[q] = kx < Wh ? base[(long long)e * Wp + kx] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int i = threadIdx.x + q * NT;
+        if ((CB * L) % NT == 0 || i < CB * L) buf[(i % CB) * LP + i / CB] = v[q];
     }
     __syncthreads();
     lds_fft<L, DIR, CB, NT, LP, RP>(buf, tw);
