@@ -300,12 +300,26 @@ __device__ __forceinline__ void stencil2d_core(float (*__restrict__ sU)[TL::NLOA
         sgx[q] = i < NSRC ? wrap_tile(x0 - 1 + sx, W, multi) : 0;
         sgy[q] = i < NSRC ? wrap_tile(y0 - 1 + sy, H, multi) : 0;
     }
+    // Every global read of the load phase is issued before any of them is used: the owned
+    // sources' masses, the first affinity and each channel's state over the load region (for a
+    // channel count known at compile time; the run-time count reads the state in the sum loop).
+    // Used as soon as loaded, they cost a memory latency per group.
     float msrc[CC > 0 ? CC : 1][SPT];
+    float aval[CC > 0 ? CC : 1][LPT];
+    float upre[LPT];
     if (CC > 0) {
 #pragma unroll
         for (int c = 0; c < (CC > 0 ? CC : 1); ++c)
 #pragma unroll
             for (int q = 0; q < SPT; ++q) msrc[c][q] = A[(long long)c * N + (long long)sgy[q] * W + sgx[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < LPT; ++q) upre[q] = U[goff[q]];
+    if (CC > 0) {
+#pragma unroll
+        for (int c = 0; c < (CC > 0 ? CC : 1); ++c)
+#pragma unroll
+            for (int q = 0; q < LPT; ++q) aval[c][q] = A[(long long)c * N + goff[q]];
     }
     // bit c: channel c has a source in the region that the reference does not skip.  A channel
     // without one receives no mass anywhere in the tile.  (With the channel count known only at
@@ -320,15 +334,12 @@ __device__ __forceinline__ void stencil2d_core(float (*__restrict__ sU)[TL::NLOA
         act_t = wave_or(act_t);
     }
     if ((tid & 63) == 0) sAct[tid >> 6] = act_t;
-    float upre[LPT];
-#pragma unroll
-    for (int q = 0; q < LPT; ++q) upre[q] = U[goff[q]];
 #pragma unroll
     for (int q = 0; q < LPT; ++q) {  // channel sum (propagator.rs:304-310): 0 + A_0 + A_1 + ...
         float sum = 0.0f;
         if (CC > 0) {
 #pragma unroll
-            for (int c = 0; c < (CC > 0 ? CC : 1); ++c) sum += A[(long long)c * N + goff[q]];
+            for (int c = 0; c < (CC > 0 ? CC : 1); ++c) sum += aval[c][q];
         } else {
             for (int c = 0; c < C; ++c) sum += A[(long long)c * N + goff[q]];
         }
