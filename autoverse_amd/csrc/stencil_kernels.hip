@@ -306,9 +306,17 @@ __global__ __launch_bounds__(TL::NT) void k_stencil3d(const float* __restrict__ 
                                  wrap_tile(x0 - 1 + sx, W, multi)
                            : 0;
     }
+    // the owned sources' masses, the first affinity and channel 0's state over the load region
+    // are all requested before any of them is used (used as loaded, each group cost a memory
+    // latency, and the channel sum waited for every load in turn)
     float msrc[SPT];
 #pragma unroll
     for (int q = 0; q < SPT; ++q) msrc[q] = A[soff[q]];
+    float upre[LPT], a0[LPT];
+#pragma unroll
+    for (int q = 0; q < LPT; ++q) upre[q] = U[goff[q]];
+#pragma unroll
+    for (int q = 0; q < LPT; ++q) a0[q] = A[goff[q]];
     // bit c: channel c has a source in the region that the reference does not skip (channels
     // past 31 count as active).  A channel without one receives no mass anywhere in the tile.
     unsigned act_t = 0;
@@ -323,13 +331,10 @@ __global__ __launch_bounds__(TL::NT) void k_stencil3d(const float* __restrict__ 
     }
     act_t = wave_or(act_t);
     if ((tid & 63) == 0) sAct[tid >> 6] = act_t;
-    float upre[LPT];
-#pragma unroll
-    for (int q = 0; q < LPT; ++q) upre[q] = U[goff[q]];
 #pragma unroll
     for (int q = 0; q < LPT; ++q) {  // channel sum (propagator3d.rs: 0 + A_0 + A_1 + ...)
-        float sum = 0.0f;
-        for (int c = 0; c < C; ++c) sum += A[(long long)c * N + goff[q]];
+        float sum = 0.0f + a0[q];
+        for (int c = 1; c < C; ++c) sum += A[(long long)c * N + goff[q]];
         const int i = tid + q * NT;
         if (i < NLOAD) {
             sSum[i] = sum;
