@@ -148,11 +148,28 @@ __host__ __device__ constexpr int stage_twiddle_src(int L, int idx) {
     return r * k * tstep;
 }
 
-// Copy the per-stage tables of length L (built once on the host) into LDS, workgroup-cooperative.
+// LDS entries of the per-stage tables of length L staged by NTH threads: tw_total rounded up to
+// a multiple of NTH, so every thread writes every trip (the entries past tw_total hold copies
+// of the last one and are never read).
+__host__ __device__ constexpr int tw_alloc(int L, int NTH) { return (tw_total(L) + NTH - 1) / NTH * NTH; }
+
+// Copy the per-stage tables of length L (built once on the host) into LDS, workgroup-cooperative,
+// in two halves: load() issues the reads into registers, store() writes them to LDS.  A kernel
+// calls load() first, its own reads next and store() after them, so no read waits for the
+// twiddles' LDS writes and the twiddles' reads share one memory latency with the kernel's.
 template <int L, int NTH>
-__device__ __forceinline__ void stage_twiddles(float2* tw, const float2* __restrict__ tws) {
-    for (int idx = threadIdx.x; idx < tw_total(L); idx += NTH) tw[idx] = tws[idx];
-}
+struct TwiddleStage {
+    static constexpr int T = tw_total(L), IT = (T + NTH - 1) / NTH;
+    float2 v[IT];
+    __device__ __forceinline__ void load(const float2* __restrict__ tws) {
+#pragma unroll
+        for (int i = 0; i < IT; ++i) v[i] = tws[min((int)threadIdx.x + i * NTH, T - 1)];
+    }
+    __device__ __forceinline__ void store(float2* tw) const {  // tw holds tw_alloc(L, NTH) entries
+#pragma unroll
+        for (int i = 0; i < IT; ++i) tw[threadIdx.x + i * NTH] = v[i];
+    }
+};
 
 // One wavefront transforms one line of L elements held as e[i] = x[lane + 64 i].
 // `line` is this wave's private LDS line (padded(L) float2), `tw` the per-stage twiddle
@@ -243,12 +260,13 @@ __global__ __launch_bounds__(NT) void k_rows_fwd_t(const float* __restrict__ A, 
                                                    const float2* __restrict__ tws) {
     constexpr int M = L / 2, E = M / WAVE, LH = L / 2 + 1;
     constexpr int LINE = padded(M);
-    __shared__ float2 tw[tw_total(M)];
+    __shared__ float2 tw[tw_alloc(M, NT)];
     __shared__ float2 lines[WAVES * LINE > LH * TP ? WAVES * LINE : LH * TP];  // FFT lines, then the tile
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     int bx, s;
     if (!xcd_block(H / RB, nblocks, bx, s)) return;
-    stage_twiddles<M, NT>(tw, tws);
+    TwiddleStage<M, NT> stage;
+    stage.load(tws);
     const float2* row =
         reinterpret_cast<const float2*>(A + (long long)src_chan[s] * N + (long long)(bx * RB + wave) * L);
     float2 e[E], w[E];
@@ -257,6 +275,7 @@ __global__ __launch_bounds__(NT) void k_rows_fwd_t(const float* __restrict__ A, 
         e[i] = row[lane + WAVE * i];
         w[i] = twg[lane + WAVE * i];  // W_L^k, k = lane + 64 i
     }
+    stage.store(tw);
     __syncthreads();  // twiddles staged
     float2* line = lines + wave * LINE;
     wave_fft<M, -1>(e, line, tw);
@@ -299,9 +318,11 @@ __global__ __launch_bounds__(NT) void k_spectral_t(const float2* __restrict__ XT
                                                    long long kq_stride, int ncp, int nblocks,
                                                    const float2* __restrict__ twg, const float2* __restrict__ tws) {
     constexpr int M = L / 2, E = M / WAVE, LINE = padded(M), HQ = L / 2 + 1;
-    __shared__ float2 tw[tw_total(M)];
+    __shared__ float2 tw[tw_alloc(M, NT)];
     __shared__ float2 lines[WAVES * LINE];
-    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    // the wave index is uniform over the wavefront: held in a scalar register, the half and
+    // column tests below are scalar branches, not per-lane masks around every LDS read
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / WAVE), lane = threadIdx.x % WAVE;
     const int h = wave & 1;                    // half: 0 -> rows/outputs n, even k; 1 -> n + M, odd k
     // XCD-aware order: the workgroups that share the 128-byte lines of a row block (adjacent
     // column pairs) run on the same XCD, so the lines are completed in one L2
@@ -310,7 +331,8 @@ __global__ __launch_bounds__(NT) void k_spectral_t(const float2* __restrict__ XT
     const SpecWork wk = work[wi];
     const int kx = cp * (WAVES / 2) + (wave >> 1);
     const bool active = kx < Wh;               // both halves of a column agree
-    stage_twiddles<M, NT>(tw, tws);
+    TwiddleStage<M, NT> stage;
+    stage.load(tws);
     float2* line = lines + wave * LINE;
     const float2* other = lines + (wave ^ 1) * LINE;
     // blocked column-major element (kx, y): (y / 4) * Wh * 4 + kx * 4 + y % 4; y = h M + lane + 64 i
@@ -319,9 +341,11 @@ __global__ __launch_bounds__(NT) void k_spectral_t(const float2* __restrict__ XT
     const long long ystep = (long long)(WAVE / RB) * Wh * RB;  // 64 rows further
     float2 f[E], w[E];
     const float2* xs = XT + (long long)wk.src_slot * spec + col;
+    // an inactive column (past Wh) reads column 0 unconditionally: its transforms are never
+    // stored, and both of its waves are inactive, so nothing reads their values
 #pragma unroll
     for (int i = 0; i < E; ++i) {
-        f[i] = active ? xs[i * ystep] : make_float2(0.0f, 0.0f);
+        f[i] = xs[i * ystep];
         w[i] = twg[lane + WAVE * i];  // W_L^n, n = lane + 64 i
     }
     // the next kernel's spectrum values (ky = 2k + h) are loaded while the current one works
@@ -335,14 +359,22 @@ __global__ __launch_bounds__(NT) void k_spectral_t(const float2* __restrict__ XT
         }
     };
     if (wk.k_count > 0) fetch(klist[wk.k_first]);
+    stage.store(tw);
     // forward butterfly across the halves
 #pragma unroll
     for (int i = 0; i < E; ++i) line[lane + WAVE * i] = f[i];
     __syncthreads();  // both halves (and the twiddle tables) are in LDS
+    {
+        float2 o[E];
 #pragma unroll
-    for (int i = 0; i < E; ++i) {
-        const float2 o = other[lane + WAVE * i];
-        f[i] = h == 0 ? cadd(f[i], o) : cmul(csub(o, f[i]), w[i]);
+        for (int i = 0; i < E; ++i) o[i] = other[lane + WAVE * i];
+        if (h == 0) {
+#pragma unroll
+            for (int i = 0; i < E; ++i) f[i] = cadd(f[i], o[i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < E; ++i) f[i] = cmul(csub(o[i], f[i]), w[i]);
+        }
     }
     __syncthreads();  // the partner has read this line before the FFT reuses it
     wave_fft<M, -1>(f, line, tw);
@@ -361,11 +393,17 @@ __global__ __launch_bounds__(NT) void k_spectral_t(const float2* __restrict__ XT
         for (int i = 0; i < E; ++i) line[lane + WAVE * i] = e[i];
         __syncthreads();  // both halves written
         float2* ys = YT + (long long)k * spec + col;
+        float2 o[E];
 #pragma unroll
-        for (int i = 0; i < E; ++i) {
-            const float2 o = other[lane + WAVE * i];
-            const float2 y = h == 0 ? cadd(e[i], o) : csub(o, e[i]);
-            if (active) ys[i * ystep] = y;
+        for (int i = 0; i < E; ++i) o[i] = other[lane + WAVE * i];
+        if (active) {
+            if (h == 0) {
+#pragma unroll
+                for (int i = 0; i < E; ++i) ys[i * ystep] = cadd(e[i], o[i]);
+            } else {
+#pragma unroll
+                for (int i = 0; i < E; ++i) ys[i * ystep] = csub(o[i], e[i]);
+            }
         }
         __syncthreads();  // the partner has read this line before the next FFT reuses it
     }
@@ -386,13 +424,17 @@ __global__ __launch_bounds__(NT) void k_rows_inv_t(const float2* __restrict__ YT
                                                    long long spec, float scale, int nblocks,
                                                    const float2* __restrict__ twg, const float2* __restrict__ tws) {
     constexpr int M = L / 2, E = M / WAVE, LINE = padded(M), LH = L / 2 + 1;
-    __shared__ float2 tw[tw_total(M)];
+    __shared__ float2 tw[tw_alloc(M, NT)];
     __shared__ float2 lines[WAVES * LINE];
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     int bx, wi;
     if (!xcd_block(H / RB, nblocks, bx, wi)) return;
     const TgtWork wk = work[wi];
-    stage_twiddles<M, NT>(tw, tws);
+    {  // staged before the spectrum reads are issued: measured faster here than after them
+        TwiddleStage<M, NT> stage;
+        stage.load(tws);
+        stage.store(tw);
+    }
     float2 w[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) w[i] = twg[lane + WAVE * i];  // W_L^k, k = lane + 64 i
