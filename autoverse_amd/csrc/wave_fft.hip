@@ -320,8 +320,8 @@ __global__ __launch_bounds__(NT) void k_spectral_t(const float2* __restrict__ XT
     constexpr int M = L / 2, E = M / WAVE, LINE = padded(M), HQ = L / 2 + 1;
     __shared__ float2 tw[tw_alloc(M, NT)];
     __shared__ float2 lines[WAVES * LINE];
-    // the wave index is uniform over the wavefront: held in a scalar register, the half and
-    // column tests below are scalar branches, not per-lane masks around every LDS read
+    // the wave index is uniform over the wavefront: held in a scalar register, the half test
+    // below is a scalar branch, not a per-lane mask around every LDS read
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / WAVE), lane = threadIdx.x % WAVE;
     const int h = wave & 1;                    // half: 0 -> rows/outputs n, even k; 1 -> n + M, odd k
     // XCD-aware order: the workgroups that share the 128-byte lines of a row block (adjacent
@@ -330,19 +330,20 @@ __global__ __launch_bounds__(NT) void k_spectral_t(const float2* __restrict__ XT
     if (!xcd_block(ncp, nblocks, cp, wi)) return;  // before any barrier: the whole workgroup leaves
     const SpecWork wk = work[wi];
     const int kx = cp * (WAVES / 2) + (wave >> 1);
-    const bool active = kx < Wh;               // both halves of a column agree
     TwiddleStage<M, NT> stage;
     stage.load(tws);
     float2* line = lines + wave * LINE;
     const float2* other = lines + (wave ^ 1) * LINE;
     // blocked column-major element (kx, y): (y / 4) * Wh * 4 + kx * 4 + y % 4; y = h M + lane + 64 i
-    const int kxc = active ? kx : 0;
+    // W/2 + 1 columns is odd, so the last pair's second column (kx = Wh) does not exist: its two
+    // waves transform column Wh - 1 again and store the same values to the same addresses as
+    // that column's own waves.  Every wave then stores on every trip, which lets the compiler
+    // count the stores still in flight when it waits for the next kernel's spectrum values.
+    const int kxc = min(kx, Wh - 1);
     const long long col = (long long)((h * M) / RB + lane / RB) * Wh * RB + (long long)kxc * RB + (lane % RB);
     const long long ystep = (long long)(WAVE / RB) * Wh * RB;  // 64 rows further
     float2 f[E], w[E];
     const float2* xs = XT + (long long)wk.src_slot * spec + col;
-    // an inactive column (past Wh) reads column 0 unconditionally: its transforms are never
-    // stored, and both of its waves are inactive, so nothing reads their values
 #pragma unroll
     for (int i = 0; i < E; ++i) {
         f[i] = xs[i * ystep];
@@ -359,6 +360,10 @@ __global__ __launch_bounds__(NT) void k_spectral_t(const float2* __restrict__ XT
         }
     };
     if (wk.k_count > 0) fetch(klist[wk.k_first]);
+    else {
+#pragma unroll
+        for (int i = 0; i < E; ++i) kvn[i] = 0.0f;
+    }
     stage.store(tw);
     // forward butterfly across the halves
 #pragma unroll
@@ -378,12 +383,21 @@ __global__ __launch_bounds__(NT) void k_spectral_t(const float2* __restrict__ XT
     }
     __syncthreads();  // the partner has read this line before the FFT reuses it
     wave_fft<M, -1>(f, line, tw);
+    // The first kernel's values arrived during the forward transform; marking them consumed
+    // here (an empty asm that takes and returns them) leaves no load of them pending on the
+    // loop's entry path.  The loop head then waits only as the back edge needs: for the
+    // values fetched in the trip before, with that trip's stores still in flight.
+#pragma unroll
+    for (int i = 0; i < E; ++i) asm volatile("" : "+v"(kvn[i]));
     for (int j = 0; j < wk.k_count; ++j) {
         const int k = klist[wk.k_first + j];
         float2 e[E];
 #pragma unroll
         for (int i = 0; i < E; ++i) e[i] = make_float2(f[i].x * kvn[i], f[i].y * kvn[i]);
-        if (j + 1 < wk.k_count) fetch(klist[wk.k_first + j + 1]);
+        // unconditional (the last trip refetches the current kernel): a conditional fetch
+        // leaves a copy into the loop's registers at the back edge, which waits for this
+        // trip's stores
+        fetch(klist[wk.k_first + min(j + 1, wk.k_count - 1)]);
         wave_fft<M, +1>(e, line, tw);  // own line only: the partner finished reading it
         if (h == 1) {
 #pragma unroll
@@ -396,15 +410,11 @@ __global__ __launch_bounds__(NT) void k_spectral_t(const float2* __restrict__ XT
         float2 o[E];
 #pragma unroll
         for (int i = 0; i < E; ++i) o[i] = other[lane + WAVE * i];
-        if (active) {
-            if (h == 0) {
+        // y = e + o (half 0) or o - e (half 1), as o + sg e with sg = +-1: the product is exact
+        // and the sum rounds once, fused or not, so both halves store from one straight block
+        const float sg = h == 0 ? 1.0f : -1.0f;
 #pragma unroll
-                for (int i = 0; i < E; ++i) ys[i * ystep] = cadd(e[i], o[i]);
-            } else {
-#pragma unroll
-                for (int i = 0; i < E; ++i) ys[i * ystep] = csub(o[i], e[i]);
-            }
-        }
+        for (int i = 0; i < E; ++i) ys[i * ystep] = make_float2(o[i].x + sg * e[i].x, o[i].y + sg * e[i].y);
         __syncthreads();  // the partner has read this line before the next FFT reuses it
     }
 }
