@@ -474,7 +474,7 @@ __global__ __launch_bounds__(NT) void k_rows_inv_t(const float2* __restrict__ YT
             const float2 t = cmul(make_float2(a.x - b.x, a.y + b.y), make_float2(w[i].x, -w[i].y));
             e[i] = make_float2(sm.x - t.y, sm.y + t.x);
         }
-        if (j + 1 < wk.k_count) fetch(klist[wk.k_first + j + 1]);
+        fetch(klist[wk.k_first + min(j + 1, wk.k_count - 1)]);  // the last trip refetches its own
         wave_fft<M, +1>(e, line, tw);
         const GrowthParams p = gp[k];
         // exp(-d^2 inv) as 2^(-d^2 inv log2 e) on the hardware exp2 (v_exp_f32): the growth
