@@ -30,6 +30,21 @@ __device__ __forceinline__ int xcd_row_tile(int b, int tiles_x, int nrows) {
 }
 inline int xcd_row_grid(int tiles_x, int nrows) { return 8 * ((nrows + 7) / 8) * tiles_x; }
 
+// The same with blocks of RB consecutive tile rows dealt to the XCDs in turn, column-major
+// inside a block (the RB vertically adjacent tiles of one column run one after another), so
+// the halo rows that vertically adjacent tiles share are fetched into one L2.  RB = 1 is
+// xcd_row_tile.
+template <int RB>
+__device__ __forceinline__ int xcd_rows_tile(int b, int tiles_x, int nrows) {
+    const int j = b >> 3, per = RB * tiles_x;
+    const int q = j / per, r = j - q * per;
+    const int col = r / RB, sub = r - col * RB;
+    const int row = ((b & 7) + 8 * q) * RB + sub;
+    return row < nrows ? row * tiles_x + col : 0x7fffffff;
+}
+template <int RB>
+inline int xcd_rows_grid(int tiles_x, int nrows) { return 8 * RB * ((nrows + 8 * RB - 1) / (8 * RB)) * tiles_x; }
+
 // ---------------------------------------------------------------------------------------------
 // Fused 2D stencil.
 //

@@ -41,12 +41,18 @@ __device__ __forceinline__ void stencil2d_body(PC A, PC U, PW Aout, int Cdyn, in
                                                beta_a, nexp, s, ta, inv_ta, flag);
 }
 
+// Tile rows per XCD block of the lone 2D stencil.  At 1024^2 x 3 (64 x 16 tiles) blocks of 4
+// rows took 27.8 us against 29.0 us for single rows and 28.6 us for pairs; blocks of 8 (one
+// contiguous band per XCD) varied from run to run (27.7 and 29.8 us).  The 3D stencil gained
+// nothing from blocks of 2 or 4 and deals single rows.
+constexpr int kSt2RowBlock = 4;
+
 template <int CC, bool POW2, bool CONS, class TL>
 __global__ __launch_bounds__(TL::NT, TL::WAVES_PER_EU) void k_stencil2d(const float* __restrict__ A, const float* __restrict__ U,
                                                       float* __restrict__ Aout, int Cdyn, int W, int H, long long N,
                                                       int tiles_x, int ntiles, float dt, float beta_a, float nexp,
                                                       float s, float ta, float inv_ta, int* __restrict__ flag) {
-    const int tile = xcd_row_tile(blockIdx.x, tiles_x, ntiles / tiles_x);
+    const int tile = xcd_rows_tile<kSt2RowBlock>(blockIdx.x, tiles_x, ntiles / tiles_x);
     if (tile >= ntiles) return;
     stencil2d_body<CC, POW2, CONS, TL>(A, U, Aout, Cdyn, W, H, N, tiles_x, tile, dt, beta_a, nexp, s, ta, inv_ta,
                                        flag);
@@ -167,7 +173,7 @@ static void launch_k_st2(hipStream_t st, const float* A, const float* U, float* 
     const float inv_ta = 1.0f / ta;
     const int tiles_x = (g.W + TL::TX - 1) / TL::TX, tiles_y = (g.H + TL::TY - 1) / TL::TY;
     const int ntiles = tiles_x * tiles_y;
-    const dim3 grid(xcd_row_grid(tiles_x, ntiles / tiles_x));
+    const dim3 grid(xcd_rows_grid<kSt2RowBlock>(tiles_x, ntiles / tiles_x));
     switch (C) {
     case 1: launch_st2<1, TL>(st, pow2, cons, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
     case 2: launch_st2<2, TL>(st, pow2, cons, grid, A, U, Aout, C, g, tiles_x, ntiles, dt, beta_a, nexp, s, ta, inv_ta, flag); break;
