@@ -39,8 +39,8 @@ __host__ __device__ constexpr int ns_before(int L, int s) {
     for (int i = 0; i < s; ++i) ns *= radix(L, i);
     return ns;
 }
-__host__ __device__ constexpr int padded(int L) { return L + L / 16; }
-__device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int padded(int L) { return L + L / 8; }
+__device__ __forceinline__ int pad(int i) { return i + (i >> 3); }
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
